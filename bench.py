@@ -1,0 +1,277 @@
+"""Benchmark of the GPU factor-solve path on one MI355X.
+
+Workload (the default line): the PTDF block on the 10,000-bus synthetic grid. B' (buses 1..N-1,
+bus 0 the slack) is factored by AMD-ordered Cholesky on the host, uploaded once, and its 13,500
+pair right-hand sides (+1/-1 at each branch's end buses) are staged in HBM; one step solves the
+whole block forward and backward in FP64 on the device, the right-hand sides built in-kernel.
+The metric is RHS x nnz(L) per second (nnz(L) counting the diagonal).
+
+A second object, "precision", reports the FP32-solve + FP64-refinement path against the plain
+FP64 path on the 25,000-bus AC Newton Jacobian (COLAMD LU, 512 N(0,1) right-hand sides, refined
+to a relative residual of 1e-12).
+
+The CPU baseline is the oracle (the C++ restatement of ybus_sparse's par_solve_many: one
+right-hand side per thread) timed on this host on the same factor. The oracle also supplies the
+factor itself: in the reference path the factor comes from ybus_sparse on the host, and that is
+what the oracle restates; the timed GPU region never calls it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-precision]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RHS·nnz(L)/s & HBM GB/s, FP32+refine vs FP64 at 1e-12 resid, 1×MI355X"
+HBM_PEAK_GBS = 8000.0
+SEED = 20261203
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---- CPU-baseline leg: the reference's host side (ybus_sparse), restated by the oracle ----
+
+def reference_host_pipeline_dc(buses: int, seed: int):
+    """Grid, B' and its AMD Cholesky factor, as ybus_sparse would make them on the host."""
+    import oracle as O
+
+    grid = O.make_grid(buses, seed)
+    bprime, bf, _ = grid.dc()
+    t0 = time.perf_counter()
+    chol = O.Cholesky(bprime)
+    factor_s = time.perf_counter() - t0
+    return grid, bprime, chol, factor_s
+
+
+def reference_host_pipeline_ac(buses: int, seed: int):
+    import oracle as O
+
+    grid = O.make_grid(buses, seed)
+    jac, _ = grid.jacobian(seed)
+    t0 = time.perf_counter()
+    lu = O.Lu(jac, O.COLAMD)
+    factor_s = time.perf_counter() - t0
+    return jac, lu, factor_s
+
+
+def cpu_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_pair_baseline(chol, plus, minus, sample: int):
+    """par_solve_many on `sample` of the pair right-hand sides (dense vectors, as the reference's
+    CPU path takes them), on this host's cores; returns (RHS per second, threads, the block)."""
+    n = chol.n
+    threads = cpu_threads()
+    dense = np.zeros((sample, n))
+    for k in range(sample):
+        if plus[k] >= 0:
+            dense[k, plus[k]] = 1.0
+        if minus[k] >= 0:
+            dense[k, minus[k]] = -1.0
+    chol.solve_many(dense[: min(64, sample)], threads)  # warm-up block
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        out = chol.solve_many(dense, threads)
+        reps += 1
+        if time.perf_counter() - t0 >= 1.0:
+            break
+    elapsed = time.perf_counter() - t0
+    return reps * sample / elapsed, threads, out
+
+
+def cpu_dense_baseline(factor, rhs):
+    threads = cpu_threads()
+    factor.solve_many(rhs[: min(32, len(rhs))], threads)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        out = factor.solve_many(rhs, threads)
+        reps += 1
+        if time.perf_counter() - t0 >= 1.0:
+            break
+    return reps * rhs.shape[0] / (time.perf_counter() - t0), threads, out
+
+
+# ---- GPU legs ----
+
+def pair_block(grid):
+    """One pair per branch: +1 at its from bus, -1 at its to bus, the slack (bus 0) dropped."""
+    plus = np.array([-1 if f == 0 else int(f) - 1 for f in grid.branch_from], dtype=np.int64)
+    minus = np.array([-1 if t == 0 else int(t) - 1 for t in grid.branch_to], dtype=np.int64)
+    return plus, minus
+
+
+def run_ptdf(gpu, args):
+    import ybus_amd as Y
+
+    buses = 10000
+    grid, bprime, chol, factor_s = reference_host_pipeline_dc(buses, SEED + 1)
+    plus, minus = pair_block(grid)
+    rhs_count = len(plus)
+    nnz_l = int(chol.l.nnz)
+    log(f"[ptdf] n={chol.n} nnz(L)={nnz_l} RHS={rhs_count}; host factor {factor_s * 1e3:.1f} ms")
+
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    factor.set_block_width(rhs_count)
+    pairs = [Y.Pair(None if p < 0 else int(p), None if m < 0 else int(m)) for p, m in zip(plus, minus)]
+    factor.stage_pairs(pairs)
+    for _ in range(args.warmup):
+        factor.solve_staged(Y.FP64).free()
+    factor.reset_kernel_stats()
+    # timed region: K device solves of the staged block (inputs already in HBM; each call
+    # returns with its work finished, and each frees the previous call's block first)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        block = factor.solve_staged(Y.FP64)
+    elapsed = time.perf_counter() - t0
+    stats = factor.kernel_stats()
+    log(f"[ptdf] {args.steps} steps in {elapsed * 1e3:.2f} ms; kernel {stats.milliseconds / stats.launches:.3f} ms "
+        f"avg over {stats.launches} launches")
+    solutions = block.solutions()
+    block.free()
+    return dict(grid=grid, bprime=bprime, chol=chol, plus=plus, minus=minus, nnz_l=nnz_l, rhs=rhs_count,
+                elapsed=elapsed, stats=stats, solutions=solutions, factor=factor)
+
+
+def run_precision(gpu, args):
+    import oracle as O
+    import ybus_amd as Y
+
+    jac, lu, factor_s = reference_host_pipeline_ac(25000, SEED + 2)
+    nnz_lu = int(lu.l.nnz + lu.u.nnz)
+    rhs = O.normals(SEED + 2, 9, 512 * jac.rows).reshape(512, jac.rows)
+    log(f"[precision] n={jac.rows} nnz(L+U)={nnz_lu}; host COLAMD LU {factor_s * 1e3:.1f} ms")
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    factor.set_matrix(jac)
+    factor.set_block_width(512)
+    factor.stage(rhs)
+    out = {}
+    for name, mode in [("fp64", Y.FP64), ("fp32_refined", Y.FP32_REFINED)]:
+        for _ in range(max(1, args.warmup)):
+            factor.solve_staged(mode, 1e-12, 10).free()
+        factor.reset_kernel_stats()
+        steps = max(3, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            block = factor.solve_staged(mode, 1e-12, 10)
+            if _ + 1 < steps:
+                block.free()
+        elapsed = (time.perf_counter() - t0) / steps
+        stats = factor.kernel_stats()
+        x = block.solutions()
+        corr, ratio = block.refinement()
+        block.free()
+        if mode == Y.FP64:
+            ratios = np.array([jac.relative_residual(x[k], rhs[k]) for k in range(0, 512, 8)])
+        else:
+            ratios = ratio
+        out[name] = dict(seconds=elapsed, value=512 * nnz_lu / elapsed,
+                         kernel_ms=stats.milliseconds / stats.launches,
+                         gbs=stats.bytes / stats.launches / (stats.milliseconds / stats.launches) / 1e6,
+                         max_relative_residual=float(np.max(ratios)),
+                         max_steps=int(corr.max()) if mode != Y.FP64 else 0,
+                         mean_steps=float(corr.mean()) if mode != Y.FP64 else 0.0)
+        log(f"[precision] {name}: {elapsed * 1e3:.3f} ms/solve, {out[name]['value']:.3e} RHS*nnz/s, "
+            f"residual <= {out[name]['max_relative_residual']:.2e}")
+        if mode == Y.FP64:
+            x64 = x
+    cpu = lu.solve_many(rhs[:64], 4)
+    err = float(np.max(np.abs(x64[:64] - cpu)) / np.max(np.abs(cpu)))
+    out["fp64_vs_cpu_max_rel_diff"] = err
+    out["speedup_fp32_refined_over_fp64"] = out["fp32_refined"]["value"] / out["fp64"]["value"]
+    out["workload"] = "25,000-bus AC Newton Jacobian (n=%d), COLAMD LU, 512 N(0,1) RHS" % jac.rows
+    out["nnz_factor"] = nnz_lu
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the factor-solve path is measured per GPU "
+              "(see DESIGN.md for how it shards); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import ybus_amd as Y
+
+    gpu = Y.Gpu.new()
+    if gpu is None:
+        raise RuntimeError("no HIP device")
+    log(f"[bench] device {gpu.name()}")
+    ptdf = run_ptdf(gpu, args)
+    nnz_l, rhs_count = ptdf["nnz_l"], ptdf["rhs"]
+    ms_per_step = ptdf["elapsed"] / args.steps * 1e3
+    value = rhs_count * nnz_l / (ms_per_step / 1e3)
+    stats = ptdf["stats"]
+    kernel_ms = stats.milliseconds / stats.launches
+    bytes_per_launch = stats.bytes / stats.launches
+    achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
+
+    # parity: sampled columns against the CPU (bit for bit)
+    sample = np.arange(0, rhs_count, max(1, rhs_count // 256))
+    chol = ptdf["chol"]
+    dense = np.zeros((len(sample), chol.n))
+    for j, k in enumerate(sample):
+        if ptdf["plus"][k] >= 0:
+            dense[j, ptdf["plus"][k]] = 1.0
+        if ptdf["minus"][k] >= 0:
+            dense[j, ptdf["minus"][k]] = -1.0
+    cpu_cols = chol.solve_many(dense, cpu_threads())
+    got = ptdf["solutions"][sample]
+    bit_exact = int(np.sum([a.tobytes() == b.tobytes() for a, b in zip(got, cpu_cols)]))
+    max_rel = float(np.max(np.abs(got - cpu_cols)) / max(np.max(np.abs(cpu_cols)), 1e-300))
+
+    cpu_baseline = None
+    if not args.no_cpu:
+        rate, threads, _ = cpu_pair_baseline(chol, ptdf["plus"], ptdf["minus"], min(2000, rhs_count))
+        cpu_baseline = {"value": rate * nnz_l, "unit": "RHS·nnz(L)/s", "cores": threads, "kind": "port",
+                        "sample": "par_solve_many (oracle, one RHS per thread) on the first 2,000 of the 13,500 "
+                                  "pair RHS as dense vectors, repeated for >= 1 s"}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "RHS·nnz(L)/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "PTDF block: 10,000-bus synthetic grid, B' AMD Cholesky, 13,500 pair RHS built "
+                               "on the device, FP64 forward+backward", "buses": 10000, "n": int(chol.n),
+                   "nnz_l": nnz_l, "rhs": rhs_count, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "solve64_kernel", "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu_baseline,
+        "parity": {"sampled_columns": int(len(sample)), "bit_exact_columns": bit_exact,
+                   "max_rel_diff_vs_cpu": max_rel},
+    }
+    if not args.no_precision:
+        line["precision"] = run_precision(gpu, args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,244 @@
+/*
+ * ybus_hip: the GPU factor-solve path of ybus, MI355X-native (HIP, gfx950), behind a C ABI.
+ *
+ * A Cholesky or LU factorization made on the host (by ybus_sparse) is uploaded once and solved
+ * on the device for many right-hand sides: forward and back substitution, transposed, in FP64 or
+ * as FP32 solves refined in FP64, for host vectors or for blocks kept on the device. Each entry
+ * point replaces a method of the reference crate ybus_cuda; the comment above it names the
+ * reference item (file:line in the reference repository) whose meaning, arguments and errors it
+ * keeps. Conventions follow ybus_ffi's C API (ybus_ffi/src/status.rs:9-98,
+ * ybus_ffi/src/lib.rs:54-80):
+ *
+ *  - every fallible call returns an int32 status, YBUS_HIP_OK (0) on success;
+ *  - the message of the calling thread's last failure is read with ybus_hip_last_error_message;
+ *  - handles are opaque, made by a _create/_upload call and released by their _free, and
+ *    _free(NULL) does nothing;
+ *  - what the reference treats as a caller bug (an assert/panic) is YBUS_HIP_INVALID_ARGUMENT.
+ *
+ * Index arrays are uint64 on the host (the reference's usize); the device holds int32 indices,
+ * so a factor or matrix with 2^31 or more entries is YBUS_HIP_TOO_LARGE (sparse.rs:62-68).
+ * Right-hand sides and solutions cross the boundary as `count` vectors of `dim` doubles stored
+ * one after another, the memory of the reference's &[Vec<f64>].
+ *
+ * Threading: a context and everything made from it are used by one thread at a time; work on
+ * a context runs in order on its one stream, and every call returns with that work finished.
+ */
+#ifndef YBUS_HIP_H
+#define YBUS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (ybus_ffi/src/status.rs:9-14; CudaError, ybus_cuda/src/lib.rs:88-124) ---- */
+enum {
+  YBUS_HIP_OK = 0,
+  YBUS_HIP_NO_DEVICE = 1,        /* Gpu::new's Ok(None): no HIP runtime or no device */
+  YBUS_HIP_INVALID_ARGUMENT = 2, /* a caller bug the reference asserts on */
+  YBUS_HIP_SHAPE_MISMATCH = 3,   /* CudaError::ShapeMismatch */
+  YBUS_HIP_TOO_LARGE = 4,        /* CudaError::TooLarge: beyond 32-bit device indices */
+  YBUS_HIP_DEVICE_ERROR = 5,     /* CudaError::Driver: the HIP runtime failed */
+  YBUS_HIP_OUT_OF_MEMORY = 6,    /* a device allocation failed */
+  YBUS_HIP_INTERNAL = 7          /* a bound inside a kernel was exceeded (a library bug) */
+};
+
+/* Precision and direction of a device solve. */
+enum {
+  YBUS_HIP_FP64 = 0,         /* FP64 factor, FP64 arithmetic: the reference's solve, bit for bit */
+  YBUS_HIP_FP32_REFINED = 1, /* FP32 factor and sweeps, FP64 residual and correction */
+  YBUS_HIP_TRANSPOSE = 2     /* A^T x = b with the same factor, FP64 (lu.rs:502-533) */
+};
+
+typedef struct ybus_hip_ctx ybus_hip_ctx;           /* Gpu (lib.rs:166-173) */
+typedef struct ybus_hip_factor ybus_hip_factor;     /* GpuFactor (factor.rs:40-62) */
+typedef struct ybus_hip_block ybus_hip_block;       /* GpuBlock (pairs.rs:88-96) */
+typedef struct ybus_hip_operator ybus_hip_operator; /* GpuOperator (pairs.rs:52-64) */
+typedef struct ybus_hip_batch ybus_hip_batch;       /* many factors solved in one launch (new) */
+
+/* Timings (factor.rs:21-31): seconds spent in solve_many's stages, and blocks solved. */
+typedef struct {
+  double upload;
+  double solve;
+  double download;
+  uint64_t blocks;
+} ybus_hip_timings;
+
+/* ColumnMax (pairs.rs:39-50); row is -1 for none. */
+typedef struct {
+  int64_t row;
+  double value;
+  int32_t not_finite;
+} ybus_hip_column_max;
+
+/* Device kernel statistics of a factor since its last reset, from HIP events on the factor's
+ * stream: launches and summed milliseconds of the solve kernels (one launch solves a block). */
+typedef struct {
+  uint64_t launches;
+  double milliseconds;
+  uint64_t bytes; /* algorithmic bytes of those launches (see DESIGN.md's traffic model) */
+} ybus_hip_kernel_stats;
+
+/* The message of the calling thread's last failure (ybus_ffi/src/lib.rs:54-80): copies at most
+ * len - 1 bytes and a NUL into buf, and returns the length the whole message needs, NUL
+ * included. */
+size_t ybus_hip_last_error_message(char *buf, size_t len);
+
+/* ---- context: Gpu (ybus_cuda/src/lib.rs:175-219) ---- */
+
+/* Gpu::new (lib.rs:179-207): the first device, or YBUS_HIP_NO_DEVICE without one. */
+int32_t ybus_hip_ctx_create(ybus_hip_ctx **out);
+void ybus_hip_ctx_free(ybus_hip_ctx *ctx);
+/* Gpu::name (lib.rs:210-212): writes the device name like last_error_message; *needed gets
+ * the length the whole name needs, NUL included. */
+int32_t ybus_hip_ctx_name(ybus_hip_ctx *ctx, char *buf, size_t len, size_t *needed);
+/* Gpu::memory (lib.rs:215-218): free and total device memory in bytes. */
+int32_t ybus_hip_ctx_memory(ybus_hip_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* ---- factor upload: GpuFactor::from_cholesky / from_lu (factor.rs:103-178) ---- */
+
+/* GpuFactor::from_cholesky (factor.rs:103-119): P A P^T = L L^T with L in CSR form, each row
+ * ending with its diagonal (Cholesky::l(), cholesky.rs:342-344), and P as perm: row i of
+ * P A P^T is row perm[i] of A (Cholesky::permutation(), cholesky.rs:348-350). */
+int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx *ctx, uint64_t n, const uint64_t *l_ptr,
+                                        const uint64_t *l_idx, const double *l_val,
+                                        const uint64_t *perm, ybus_hip_factor **out);
+
+/* GpuFactor::from_lu (factor.rs:123-133): P A Q = L U with L (unit diagonal stored first in
+ * each column) and U (diagonal stored last) in CSC form (Lu::l(), Lu::u(), lu.rs:473-480),
+ * row_perm as P (Lu::permutation(), lu.rs:483-485) and col_perm as Q
+ * (Lu::column_permutation(), lu.rs:489-491). */
+int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx *ctx, uint64_t n, const uint64_t *l_ptr,
+                                  const uint64_t *l_idx, const double *l_val,
+                                  const uint64_t *u_ptr, const uint64_t *u_idx,
+                                  const double *u_val, const uint64_t *row_perm,
+                                  const uint64_t *col_perm, ybus_hip_factor **out);
+
+void ybus_hip_factor_free(ybus_hip_factor *f);
+
+/* GpuFactor::dim (factor.rs:181-183). */
+uint64_t ybus_hip_factor_dim(const ybus_hip_factor *f);
+/* GpuFactor::block_width / set_block_width (factor.rs:186-199): set clamps to at least 1, and
+ * at most 64 for a single-row factor. */
+uint64_t ybus_hip_factor_block_width(const ybus_hip_factor *f);
+void ybus_hip_factor_set_block_width(ybus_hip_factor *f, uint64_t width);
+/* GpuFactor::timings / reset_timings (factor.rs:201-210). */
+void ybus_hip_factor_timings(const ybus_hip_factor *f, ybus_hip_timings *out);
+void ybus_hip_factor_reset_timings(ybus_hip_factor *f);
+
+/* The matrix A itself, which FP32_REFINED needs for its FP64 residual b - A x: rows x rows in
+ * CSR (row_major = 1, CsrMatrix) or CSC (row_major = 0, CscMatrix) form, as factored. No
+ * counterpart in the reference, whose solves don't refine. */
+int32_t ybus_hip_factor_set_matrix(ybus_hip_factor *f, int32_t row_major, const uint64_t *ptr,
+                                   const uint64_t *idx, const double *val);
+
+/* ---- host-vector solves ---- */
+
+/* GpuFactor::solve_many (factor.rs:234-289): solves A x = b for `count` right-hand sides, in
+ * blocks of block_width() columns, FP64. YBUS_HIP_SHAPE_MISMATCH is not possible here (the
+ * caller's vectors are dim() long by construction of the ABI). */
+int32_t ybus_hip_factor_solve_many(ybus_hip_factor *f, const double *rhs, uint64_t count,
+                                   double *x);
+
+/* A^T x = b for each right-hand side (Lu::solve_transpose, lu.rs:502-533; for a Cholesky factor
+ * A^T = A), FP64, in blocks of block_width() columns. */
+int32_t ybus_hip_factor_solve_transpose_many(ybus_hip_factor *f, const double *rhs,
+                                             uint64_t count, double *x);
+
+/* FP32 solves refined in FP64 (new): each right-hand side follows refine_normwise
+ * (accuracy.rs:83-122): solve, then while ||b - A x||inf / ||b||inf > tolerance and at most
+ * max_steps times, solve for the residual and add the correction, stopping early when a step
+ * fails to halve the ratio or gives NaN, and keeping the best iterate. steps[k] and
+ * ratio[k] (either may be NULL) get each column's corrections and final relative residual.
+ * Needs ybus_hip_factor_set_matrix. */
+int32_t ybus_hip_factor_solve_many_refined(ybus_hip_factor *f, const double *rhs, uint64_t count,
+                                           double tolerance, uint64_t max_steps, double *x,
+                                           uint64_t *steps, double *ratio);
+
+/* ---- device-resident blocks ---- */
+
+/* Copies `count` right-hand sides to the device, into the factor's staged block (at most
+ * block_width() of them), without solving: the device-resident solves below start from it. */
+int32_t ybus_hip_factor_stage(ybus_hip_factor *f, const double *rhs, uint64_t count);
+
+/* Stages k pairs (as pair_solves takes them) instead: solve_staged in YBUS_HIP_FP64 then builds
+ * each right-hand side e_plus - e_minus on the device, as pair_solves does. */
+int32_t ybus_hip_factor_stage_pairs(ybus_hip_factor *f, const int64_t *plus, const int64_t *minus,
+                                    uint64_t k);
+
+/* Solves the staged block on the device in `mode` (YBUS_HIP_FP64, _FP32_REFINED or _TRANSPOSE;
+ * tolerance and max_steps are read by FP32_REFINED only), leaving the solutions there as a
+ * block, which borrows the factor until freed. */
+int32_t ybus_hip_factor_solve_staged(ybus_hip_factor *f, int32_t mode, double tolerance,
+                                     uint64_t max_steps, ybus_hip_block **out);
+
+/* GpuFactor::pair_solves (pairs.rs:153-202): solves A y = e_plus - e_minus for each of k pairs
+ * (k <= block_width()), the right-hand sides built on the device; -1 means no entry.
+ * INVALID_ARGUMENT for more pairs than the width, an unknown >= dim(), or plus == minus. */
+int32_t ybus_hip_factor_pair_solves(ybus_hip_factor *f, const int64_t *plus, const int64_t *minus,
+                                    uint64_t k, ybus_hip_block **out);
+
+/* GpuFactor::pair_updates (pairs.rs:209-253): pair_solves, then each column's rank-one update of
+ * base (dim() values): x = base + alpha y, w = inverse_weight - c^T y, z = c^T x / w,
+ * out = x + z y, with w == 0 flagged singular and given x. */
+int32_t ybus_hip_factor_pair_updates(ybus_hip_factor *f, const double *base, const int64_t *plus,
+                                     const int64_t *minus, const double *alpha,
+                                     const double *inverse_weight, uint64_t k,
+                                     ybus_hip_block **out);
+
+void ybus_hip_block_free(ybus_hip_block *b);
+/* GpuBlock::columns (pairs.rs:258-260). */
+uint64_t ybus_hip_block_columns(const ybus_hip_block *b);
+/* GpuBlock::singular (pairs.rs:264-266): one byte per column. */
+int32_t ybus_hip_block_singular(const ybus_hip_block *b, uint8_t *out);
+/* Each column's refinement steps and relative residual (FP32_REFINED blocks; zeros otherwise). */
+int32_t ybus_hip_block_refinement(const ybus_hip_block *b, uint64_t *steps, double *ratio);
+/* GpuBlock::solutions (pairs.rs:269-290): columns() vectors of dim() values, by unknown. */
+int32_t ybus_hip_block_solutions(ybus_hip_block *b, double *out);
+/* GpuBlock::maxima (pairs.rs:293-308): each column's largest |value|, its unknown (the
+ * smallest unknown on a tie) and whether any value isn't finite. */
+int32_t ybus_hip_block_maxima(ybus_hip_block *b, ybus_hip_column_max *out);
+/* GpuBlock::product (pairs.rs:373-392): the operator's M theta + offset for every column, with
+ * column c's entry at zero[c] (-1 for none) set to zero: columns() vectors of rows() values. */
+int32_t ybus_hip_block_product(ybus_hip_block *b, ybus_hip_operator *op, const int64_t *zero,
+                               double *out);
+/* GpuBlock::product_maxima (pairs.rs:396-413): product reduced to |value| * weight maxima. */
+int32_t ybus_hip_block_product_maxima(ybus_hip_block *b, ybus_hip_operator *op,
+                                      const int64_t *zero, ybus_hip_column_max *out);
+/* Kernel statistics of this factor's device solves (HIP events), and their reset. */
+void ybus_hip_factor_kernel_stats(const ybus_hip_factor *f, ybus_hip_kernel_stats *out);
+void ybus_hip_factor_reset_kernel_stats(ybus_hip_factor *f);
+
+/* ---- host-side symbolic analysis (no device needed; new) ----
+ * The level schedule the device sweeps follow: level[i] of each factor row is one more than the
+ * deepest row it depends on (0 for none). For a Cholesky factor (arrays as in upload_cholesky)
+ * the forward sweep is L and the backward sweep L^T; supernode_start gets the first column of
+ * each fundamental supernode of L's elimination tree (multifrontal.rs:94-138) and n, at most
+ * n + 1 values, and *supernodes their count. For an LU factor (arrays as in upload_lu) the
+ * sweeps are L, U, U^T and L^T (the transposed solve). Any output may be NULL. */
+int32_t ybus_hip_analyze_cholesky(uint64_t n, const uint64_t *l_ptr, const uint64_t *l_idx,
+                                  int32_t *forward_level, int32_t *backward_level,
+                                  uint64_t *supernode_start, uint64_t *supernodes);
+int32_t ybus_hip_analyze_lu(uint64_t n, const uint64_t *l_ptr, const uint64_t *l_idx,
+                            const uint64_t *u_ptr, const uint64_t *u_idx, int32_t *forward_level,
+                            int32_t *backward_level, int32_t *transpose_forward_level,
+                            int32_t *transpose_backward_level);
+
+/* ---- operator: GpuFactor::operator, GpuOperator (pairs.rs:66-149) ---- */
+
+/* GpuFactor::operator (pairs.rs:112-149): a rows x dim() CSR matrix applied to solutions. */
+int32_t ybus_hip_factor_operator(ybus_hip_factor *f, uint64_t rows, const uint64_t *ptr,
+                                 const uint64_t *idx, const double *val, ybus_hip_operator **out);
+void ybus_hip_operator_free(ybus_hip_operator *op);
+uint64_t ybus_hip_operator_rows(const ybus_hip_operator *op);
+/* GpuOperator::set_offset / set_weights (pairs.rs:73-85): rows() values each. */
+int32_t ybus_hip_operator_set_offset(ybus_hip_operator *op, const double *offset);
+int32_t ybus_hip_operator_set_weights(ybus_hip_operator *op, const double *weights);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YBUS_HIP_H */

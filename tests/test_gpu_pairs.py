@@ -1,0 +1,177 @@
+"""Blocks built, solved and updated on the device against the same formulas on the host from the
+oracle's CPU solves (ybus_cuda/tests/gpu_pairs_test.rs restated): pair solves and maxima,
+rank-one updates including a singular one, operator products with entries zeroed and weighted
+maxima, and the single-row width cap."""
+import numpy as np
+import pytest
+
+import oracle as O
+import ybus_amd as Y
+from tests.conftest import Lcg
+from tests.fixtures import random_spd, random_square_cuda
+from tests.gpu_common import gpu  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+BAR = 1e-12
+
+
+def pair_vector(n, pair):
+    c = np.zeros(n)
+    if pair.plus is not None:
+        c[pair.plus] = 1.0
+    if pair.minus is not None:
+        c[pair.minus] = -1.0
+    return c
+
+
+def dot(pair, v):
+    return (v[pair.plus] if pair.plus is not None else 0.0) - (v[pair.minus] if pair.minus is not None else 0.0)
+
+
+def host_update(cpu, base, update):
+    """gpu_pairs_test.rs:48-62."""
+    n = len(base)
+    y = cpu.solve(pair_vector(n, update.pair))
+    x = base + update.alpha * y
+    w = -dot(update.pair, y) + update.inverse_weight
+    if w == 0.0:
+        return x, True
+    z = dot(update.pair, x) / w
+    return x + z * y, False
+
+
+def max_difference(a, b):
+    scale = max(1.0, float(np.max(np.abs(b))))
+    return float(np.max(np.abs(a - b))) / scale
+
+
+def pairs(n, count, seed):
+    """gpu_pairs_test.rs:72-84."""
+    rng = Lcg(seed)
+    out = []
+    for k in range(count):
+        plus = rng.next() % n if k % 5 != 3 else None
+        minus = rng.next() % n if k % 7 != 2 else None
+        if minus is not None and minus == plus:
+            minus = None
+        out.append(Y.Pair(plus, minus))
+    return out
+
+
+def operator_matrix(rows, n, seed):
+    rng = Lcg(seed)
+    t = []
+    for i in range(rows):
+        for _ in range(3):
+            t.append((i, rng.next() % n, rng.value()))
+    return O.Matrix.from_triplets(rows, n, t)
+
+
+def factors(n):
+    return [("cholesky", O.Cholesky(random_spd(n, 3, 21))), ("lu", O.Lu(random_square_cuda(n, 4, 22)))]
+
+
+def upload(gpu, cpu):
+    return Y.GpuFactor.from_cholesky(gpu, cpu) if isinstance(cpu, O.Cholesky) else Y.GpuFactor.from_lu(gpu, cpu)
+
+
+def test_pair_solves_and_their_maxima_match_the_host(gpu):
+    n = 200
+    for name, cpu in factors(n):
+        factor = upload(gpu, cpu)
+        factor.set_block_width(32)
+        ps = pairs(n, 29, 5)
+        block = factor.pair_solves(ps)
+        assert block.columns() == 29
+        assert not any(block.singular())
+        solutions = block.solutions()
+        maxima = block.maxima()
+        for pair, solution, mx in zip(ps, solutions, maxima):
+            expected = cpu.solve(pair_vector(n, pair))
+            assert max_difference(solution, expected) < BAR, name
+            # the right-hand side is built as the host builds it: the same bits
+            assert solution.tobytes() == expected.tobytes(), name
+            largest = float(np.max(np.abs(expected)))
+            assert abs(mx.value - largest) <= BAR * largest
+            if mx.row is not None:
+                assert abs(abs(solution[mx.row]) - mx.value) <= BAR * largest
+                # the smallest unknown on a tie
+                assert mx.row == int(np.flatnonzero(np.abs(solution) == mx.value)[0])
+            else:
+                assert pair.plus is None and pair.minus is None
+            assert not mx.not_finite
+        block.free()
+
+
+def test_rank_one_updates_match_the_host_including_a_singular_one(gpu):
+    n = 150
+    rng = Lcg(9)
+    base = np.array([rng.value() for _ in range(n)])
+    for name, cpu in factors(n):
+        factor = upload(gpu, cpu)
+        factor.set_block_width(16)
+        updates = [Y.PairUpdate(p, rng.value(), 2.0 + rng.value()) for p in pairs(n, 15, 6)]
+        updates.append(Y.PairUpdate(Y.Pair(None, None), 0.5, 0.0))
+        block = factor.pair_updates(base, updates)
+        solutions = block.solutions()
+        singular = block.singular()
+        for k, (update, solution) in enumerate(zip(updates, solutions)):
+            expected, sing = host_update(cpu, base, update)
+            assert singular[k] == sing, (name, k)
+            assert max_difference(solution, expected) < BAR, (name, k)
+        assert singular[15]
+        block.free()
+
+
+def test_operator_products_and_maxima_match_the_host(gpu):
+    n, rows = 120, 90
+    rng = Lcg(31)
+    base = np.array([rng.value() for _ in range(n)])
+    offset = np.array([rng.value() for _ in range(rows)])
+    weights = np.array([0.0 if i % 4 == 0 else 1.0 + abs(rng.value()) for i in range(rows)])
+    m = operator_matrix(rows, n, 32)
+    for name, cpu in factors(n):
+        factor = upload(gpu, cpu)
+        factor.set_block_width(8)
+        op = factor.operator(m)
+        op.set_offset(offset)
+        op.set_weights(weights)
+        updates = [Y.PairUpdate(p, 0.25, 3.0) for p in pairs(n, 7, 33)]
+        zero = [k * 11 if k % 2 == 0 else None for k in range(7)]
+        block = factor.pair_updates(base, updates)
+        products = block.product(op, zero)
+        maxima = block.product_maxima(op, zero)
+        for k in range(7):
+            theta, _ = host_update(cpu, base, updates[k])
+            expected = m.mul_vec(theta) + offset
+            if zero[k] is not None:
+                expected[zero[k]] = 0.0
+                assert products[k][zero[k]] == 0.0
+            assert max_difference(products[k], expected) < BAR, (name, k)
+            row, value = None, 0.0
+            for i, (e, w) in enumerate(zip(products[k], weights)):
+                if abs(e) * w > value:
+                    row, value = i, abs(e) * w
+            assert maxima[k] == Y.ColumnMax(row, value, False), (name, k)
+        block.free()
+
+
+def test_a_single_row_factor_keeps_its_blocks_narrow(gpu):
+    lu = O.Lu(O.Matrix.from_triplets(1, 1, [(0, 0, 4.0)], row_major=False))
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    factor.set_block_width(1024)
+    assert factor.block_width() == 64
+    solutions = factor.pair_solves([Y.Pair(0, None)] * 64).solutions()
+    assert all(list(s) == [0.25] for s in solutions)
+
+
+def test_caller_bugs_are_invalid_arguments(gpu):
+    factor = Y.GpuFactor.from_cholesky(gpu, O.Cholesky(random_spd(20, 10, 2)))
+    factor.set_block_width(4)
+    with pytest.raises(Y.InvalidArgument):
+        factor.pair_solves([Y.Pair(0, 1)] * 5)
+    with pytest.raises(Y.InvalidArgument):
+        factor.pair_solves([Y.Pair(20, None)])
+    with pytest.raises(Y.InvalidArgument):
+        factor.pair_solves([Y.Pair(3, 3)])

@@ -1,0 +1,563 @@
+"""ybus_amd: ybus's GPU factor-solve path on MI355X, from Python.
+
+A thin ctypes face over ``libybus_hip.so`` (the C ABI in ``include/ybus_hip.h``) that mirrors
+the reference crate ybus_cuda's API, name for name (ybus_cuda/src/lib.rs, factor.rs, pairs.rs):
+``Gpu``, ``GpuFactor``, ``GpuBlock``, ``GpuOperator``, ``Pair``, ``PairUpdate``, ``ColumnMax``,
+``Timings`` and the ``GpuError`` family for ``CudaError``. All compute runs in the HIP library;
+there is no CPU path, and importing this package on a machine without the built library fails
+loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libybus_hip.so")
+
+OK, NO_DEVICE, INVALID_ARGUMENT, SHAPE_MISMATCH, TOO_LARGE, DEVICE_ERROR, OUT_OF_MEMORY, INTERNAL = range(8)
+FP64, FP32_REFINED, TRANSPOSE = 0, 1, 2
+
+# ---- errors: CudaError (ybus_cuda/src/lib.rs:88-124) ----
+
+
+class GpuError(Exception):
+    """An error from the GPU path, with the C ABI's status code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+class ShapeMismatch(GpuError):
+    """A right-hand side doesn't have one entry per row of the factorization."""
+
+    def __init__(self, expected: int, found: int):
+        super().__init__(SHAPE_MISMATCH, f"a right-hand side has {found} entries, but the factorization has "
+                                         f"{expected} rows")
+        self.expected = expected
+        self.found = found
+
+
+class TooLarge(GpuError):
+    pass
+
+
+class InvalidArgument(GpuError, ValueError):
+    """A caller bug the reference asserts on (a panic there)."""
+
+
+class DeviceError(GpuError):
+    pass
+
+
+_ERRORS = {TOO_LARGE: TooLarge, INVALID_ARGUMENT: InvalidArgument, DEVICE_ERROR: DeviceError}
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """The loaded HIP library; raises if it hasn't been built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ybus_amd.build` "
+                               "(the GPU path has no CPU fallback)")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class _Timings(C.Structure):
+    _fields_ = [("upload", C.c_double), ("solve", C.c_double), ("download", C.c_double), ("blocks", C.c_uint64)]
+
+
+class _ColumnMax(C.Structure):
+    _fields_ = [("row", C.c_int64), ("value", C.c_double), ("not_finite", C.c_int32)]
+
+
+class _KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("milliseconds", C.c_double), ("bytes", C.c_uint64)]
+
+
+P = C.c_void_p
+U64 = C.c_uint64
+
+
+def _declare(L):
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+    i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
+    f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+    u8p = np.ctypeslib.ndpointer(np.uint8, flags="C")
+    i32 = C.c_int32
+    sig = {
+        "ybus_hip_last_error_message": (C.c_size_t, [C.c_char_p, C.c_size_t]),
+        "ybus_hip_ctx_create": (i32, [C.POINTER(P)]),
+        "ybus_hip_ctx_free": (None, [P]),
+        "ybus_hip_ctx_name": (i32, [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "ybus_hip_ctx_memory": (i32, [P, C.POINTER(U64), C.POINTER(U64)]),
+        "ybus_hip_factor_upload_cholesky": (i32, [P, U64, u64p, u64p, f64p, u64p, C.POINTER(P)]),
+        "ybus_hip_factor_upload_lu": (i32, [P, U64, u64p, u64p, f64p, u64p, u64p, f64p, u64p, u64p, C.POINTER(P)]),
+        "ybus_hip_factor_free": (None, [P]),
+        "ybus_hip_factor_dim": (U64, [P]),
+        "ybus_hip_factor_block_width": (U64, [P]),
+        "ybus_hip_factor_set_block_width": (None, [P, U64]),
+        "ybus_hip_factor_timings": (None, [P, C.POINTER(_Timings)]),
+        "ybus_hip_factor_reset_timings": (None, [P]),
+        "ybus_hip_factor_set_matrix": (i32, [P, i32, u64p, u64p, f64p]),
+        "ybus_hip_factor_solve_many": (i32, [P, f64p, U64, f64p]),
+        "ybus_hip_factor_solve_transpose_many": (i32, [P, f64p, U64, f64p]),
+        "ybus_hip_factor_solve_many_refined": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
+        "ybus_hip_factor_stage": (i32, [P, f64p, U64]),
+        "ybus_hip_factor_stage_pairs": (i32, [P, i64p, i64p, U64]),
+        "ybus_hip_factor_solve_staged": (i32, [P, i32, C.c_double, U64, C.POINTER(P)]),
+        "ybus_hip_factor_pair_solves": (i32, [P, i64p, i64p, U64, C.POINTER(P)]),
+        "ybus_hip_factor_pair_updates": (i32, [P, f64p, i64p, i64p, f64p, f64p, U64, C.POINTER(P)]),
+        "ybus_hip_block_free": (None, [P]),
+        "ybus_hip_block_columns": (U64, [P]),
+        "ybus_hip_block_singular": (i32, [P, u8p]),
+        "ybus_hip_block_refinement": (i32, [P, u64p, f64p]),
+        "ybus_hip_block_solutions": (i32, [P, f64p]),
+        "ybus_hip_block_maxima": (i32, [P, C.POINTER(_ColumnMax)]),
+        "ybus_hip_block_product": (i32, [P, P, i64p, f64p]),
+        "ybus_hip_block_product_maxima": (i32, [P, P, i64p, C.POINTER(_ColumnMax)]),
+        "ybus_hip_factor_kernel_stats": (None, [P, C.POINTER(_KernelStats)]),
+        "ybus_hip_factor_reset_kernel_stats": (None, [P]),
+        "ybus_hip_factor_operator": (i32, [P, U64, u64p, u64p, f64p, C.POINTER(P)]),
+        "ybus_hip_operator_free": (None, [P]),
+        "ybus_hip_operator_rows": (U64, [P]),
+        "ybus_hip_operator_set_offset": (i32, [P, f64p]),
+        "ybus_hip_operator_set_weights": (i32, [P, f64p]),
+        "ybus_hip_analyze_cholesky": (i32, [U64, u64p, u64p, P, P, P, P]),
+        "ybus_hip_analyze_lu": (i32, [U64, u64p, u64p, u64p, u64p, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _message() -> str:
+    buf = C.create_string_buffer(1024)
+    library().ybus_hip_last_error_message(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def _check(status: int):
+    if status != OK:
+        raise _ERRORS.get(status, GpuError)(status, _message())
+
+
+def _u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---- value types (pairs.rs:16-50, factor.rs:21-31) ----
+
+
+@dataclass(frozen=True)
+class Pair:
+    """A right-hand side e_plus - e_minus over the unknowns (None for no entry)."""
+
+    plus: Optional[int] = None
+    minus: Optional[int] = None
+
+
+@dataclass(frozen=True)
+class PairUpdate:
+    """One column of GpuFactor.pair_updates (pairs.rs:24-37)."""
+
+    pair: Pair
+    alpha: float
+    inverse_weight: float
+
+
+@dataclass(frozen=True)
+class ColumnMax:
+    """A column's largest |value| * weight, its row (None if every product is zero) and
+    whether any value isn't finite (pairs.rs:39-50)."""
+
+    row: Optional[int]
+    value: float
+    not_finite: bool
+
+
+@dataclass(frozen=True)
+class Timings:
+    upload: float = 0.0
+    solve: float = 0.0
+    download: float = 0.0
+    blocks: int = 0
+
+
+@dataclass(frozen=True)
+class KernelStats:
+    launches: int
+    milliseconds: float
+    bytes: int
+
+
+# ---- host-side symbolic analysis (no device needed) ----
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+def analyze_cholesky(n, l_ptr, l_idx):
+    """The level of each row of the forward (L) and backward (L^T) sweeps, and the fundamental
+    supernodes' first columns (with n appended) of a Cholesky factor's L in CSR form."""
+    fwd, bwd = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    starts, count = np.zeros(n + 1, np.uint64), U64()
+    _check(library().ybus_hip_analyze_cholesky(int(n), _u64(l_ptr), _u64(l_idx), _ptr(fwd), _ptr(bwd), _ptr(starts),
+                                               C.addressof(count)))
+    return dict(forward=fwd, backward=bwd, supernodes=starts[: int(count.value) + 1].astype(np.int64))
+
+
+def analyze_lu(n, l_ptr, l_idx, u_ptr, u_idx):
+    """The level of each row of an LU factor's four sweeps: L, U, U^T and L^T."""
+    out = [np.zeros(n, np.int32) for _ in range(4)]
+    _check(library().ybus_hip_analyze_lu(int(n), _u64(l_ptr), _u64(l_idx), _u64(u_ptr), _u64(u_idx),
+                                         *[_ptr(o) for o in out]))
+    return dict(zip(["forward", "backward", "transpose_forward", "transpose_backward"], out))
+
+
+# ---- Gpu (lib.rs:166-219) ----
+
+
+class Gpu:
+    """The first HIP device and the one stream all of this library's work runs on in order."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def new() -> Optional["Gpu"]:
+        """The first device, or None on a machine without one (Gpu::new's Ok(None))."""
+        h = P()
+        status = library().ybus_hip_ctx_create(C.byref(h))
+        if status == NO_DEVICE:
+            return None
+        _check(status)
+        return Gpu(h)
+
+    def name(self) -> str:
+        needed = C.c_size_t()
+        buf = C.create_string_buffer(256)
+        _check(library().ybus_hip_ctx_name(self._h, buf, 256, C.byref(needed)))
+        return buf.value.decode()
+
+    def memory(self):
+        free, total = U64(), U64()
+        _check(library().ybus_hip_ctx_memory(self._h, C.byref(free), C.byref(total)))
+        return int(free.value), int(total.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            library().ybus_hip_ctx_free(self._h)
+            self._h = None
+
+
+def _rhs_array(rhs, n: int) -> np.ndarray:
+    """The right-hand sides as a (count, n) array, checked like solve_many (factor.rs:234-244)."""
+    if isinstance(rhs, np.ndarray) and rhs.ndim == 2:
+        if rhs.shape[1] != n and rhs.shape[0] > 0:
+            raise ShapeMismatch(n, rhs.shape[1])
+        return _f64(rhs)
+    rows = list(rhs)
+    for b in rows:
+        if len(b) != n:
+            raise ShapeMismatch(n, len(b))
+    if not rows:
+        return np.zeros((0, n))
+    return _f64(np.array(rows, dtype=np.float64).reshape(len(rows), n))
+
+
+def _pairs(pairs: Sequence[Pair]):
+    plus = np.array([-1 if p.plus is None else int(p.plus) for p in pairs], dtype=np.int64)
+    minus = np.array([-1 if p.minus is None else int(p.minus) for p in pairs], dtype=np.int64)
+    return plus, minus
+
+
+# ---- GpuFactor (factor.rs, pairs.rs) ----
+
+
+class GpuFactor:
+    """A factorization uploaded once, for solving many right-hand sides against it."""
+
+    def __init__(self, gpu: Gpu, handle, n: int):
+        self._gpu = gpu  # keeps the context alive
+        self._h = handle
+        self._n = n
+        self._block: Optional[GpuBlock] = None
+
+    @staticmethod
+    def from_cholesky(gpu: Gpu, cholesky) -> "GpuFactor":
+        """Uploads P A P^T = L L^T (factor.rs:103-119). `cholesky` has ``l`` (CSR: ptr, idx,
+        data, each row ending with its diagonal) and ``perm``, as ybus_sparse's Cholesky."""
+        l = cholesky.l
+        return GpuFactor.from_cholesky_arrays(gpu, len(cholesky.perm), l.ptr, l.idx, l.data, cholesky.perm)
+
+    @staticmethod
+    def from_cholesky_arrays(gpu: Gpu, n, l_ptr, l_idx, l_val, perm) -> "GpuFactor":
+        h = P()
+        _check(library().ybus_hip_factor_upload_cholesky(gpu._h, int(n), _u64(l_ptr), _u64(l_idx), _f64(l_val),
+                                                         _u64(perm), C.byref(h)))
+        return GpuFactor(gpu, h, int(n))
+
+    @staticmethod
+    def from_lu(gpu: Gpu, lu) -> "GpuFactor":
+        """Uploads P A Q = L U (factor.rs:123-133). `lu` has ``l`` and ``u`` (CSC) and
+        ``row_perm``/``col_perm``, as ybus_sparse's Lu."""
+        return GpuFactor.from_lu_arrays(gpu, len(lu.row_perm), lu.l.ptr, lu.l.idx, lu.l.data, lu.u.ptr, lu.u.idx,
+                                        lu.u.data, lu.row_perm, lu.col_perm)
+
+    @staticmethod
+    def from_lu_arrays(gpu: Gpu, n, l_ptr, l_idx, l_val, u_ptr, u_idx, u_val, row_perm, col_perm) -> "GpuFactor":
+        h = P()
+        _check(library().ybus_hip_factor_upload_lu(gpu._h, int(n), _u64(l_ptr), _u64(l_idx), _f64(l_val),
+                                                   _u64(u_ptr), _u64(u_idx), _f64(u_val), _u64(row_perm),
+                                                   _u64(col_perm), C.byref(h)))
+        return GpuFactor(gpu, h, int(n))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            if self._block is not None:
+                self._block.free()
+            library().ybus_hip_factor_free(self._h)
+            self._h = None
+
+    def dim(self) -> int:
+        return self._n
+
+    def block_width(self) -> int:
+        return int(library().ybus_hip_factor_block_width(self._h))
+
+    def set_block_width(self, width: int):
+        library().ybus_hip_factor_set_block_width(self._h, int(width))
+
+    def timings(self) -> Timings:
+        t = _Timings()
+        library().ybus_hip_factor_timings(self._h, C.byref(t))
+        return Timings(t.upload, t.solve, t.download, int(t.blocks))
+
+    def reset_timings(self):
+        library().ybus_hip_factor_reset_timings(self._h)
+
+    def kernel_stats(self) -> KernelStats:
+        s = _KernelStats()
+        library().ybus_hip_factor_kernel_stats(self._h, C.byref(s))
+        return KernelStats(int(s.launches), float(s.milliseconds), int(s.bytes))
+
+    def reset_kernel_stats(self):
+        library().ybus_hip_factor_reset_kernel_stats(self._h)
+
+    def set_matrix(self, a):
+        """The matrix itself (CSR when ``a.row_major`` else CSC), for FP32_REFINED residuals."""
+        _check(library().ybus_hip_factor_set_matrix(self._h, int(bool(a.row_major)), _u64(a.ptr), _u64(a.idx),
+                                                    _f64(a.data)))
+
+    def _release_block(self):
+        if self._block is not None:
+            self._block.free()
+
+    def solve_many(self, rhs) -> np.ndarray:
+        """A x = b for each right-hand side, FP64, as a (count, n) array (factor.rs:234-289)."""
+        b = _rhs_array(rhs, self._n)
+        self._release_block()
+        x = np.zeros_like(b)
+        if b.shape[0]:
+            _check(library().ybus_hip_factor_solve_many(self._h, b, b.shape[0], x))
+        return x
+
+    def solve_transpose_many(self, rhs) -> np.ndarray:
+        """A^T x = b for each right-hand side, FP64 (Lu::solve_transpose, lu.rs:502-533)."""
+        b = _rhs_array(rhs, self._n)
+        self._release_block()
+        x = np.zeros_like(b)
+        if b.shape[0]:
+            _check(library().ybus_hip_factor_solve_transpose_many(self._h, b, b.shape[0], x))
+        return x
+
+    def solve_many_refined(self, rhs, tolerance: float = 1e-12, max_steps: int = 10):
+        """FP32 solves refined in FP64 to `tolerance` (refine_normwise's rule,
+        accuracy.rs:83-122): (solutions, steps per column, relative residual per column)."""
+        b = _rhs_array(rhs, self._n)
+        self._release_block()
+        x = np.zeros_like(b)
+        steps = np.zeros(b.shape[0], np.uint64)
+        ratio = np.zeros(b.shape[0])
+        if b.shape[0]:
+            _check(library().ybus_hip_factor_solve_many_refined(self._h, b, b.shape[0], float(tolerance),
+                                                                int(max_steps), x, steps, ratio))
+        return x, steps.astype(np.int64), ratio
+
+    def stage(self, rhs):
+        """Copies right-hand sides (at most block_width()) to the device without solving."""
+        b = _rhs_array(rhs, self._n)
+        self._release_block()
+        _check(library().ybus_hip_factor_stage(self._h, b, b.shape[0]))
+
+    def stage_pairs(self, pairs: Sequence[Pair]):
+        """Copies pairs (at most block_width()) to the device; solve_staged(FP64) then solves
+        them as pair_solves does."""
+        self._release_block()
+        plus, minus = _pairs(pairs)
+        _check(library().ybus_hip_factor_stage_pairs(self._h, plus, minus, len(plus)))
+
+    def solve_staged(self, mode: int = FP64, tolerance: float = 1e-12, max_steps: int = 10) -> "GpuBlock":
+        """Solves the staged block on the device, keeping the solutions there."""
+        self._release_block()
+        h = P()
+        _check(library().ybus_hip_factor_solve_staged(self._h, int(mode), float(tolerance), int(max_steps),
+                                                      C.byref(h)))
+        return self._new_block(h)
+
+    def _new_block(self, h) -> "GpuBlock":
+        block = GpuBlock(self, h)
+        self._block = block
+        return block
+
+    def pair_solves(self, pairs: Sequence[Pair]) -> "GpuBlock":
+        """A y = e_plus - e_minus for each pair, kept on the device (pairs.rs:194-202)."""
+        self._release_block()
+        plus, minus = _pairs(pairs)
+        h = P()
+        _check(library().ybus_hip_factor_pair_solves(self._h, plus, minus, len(plus), C.byref(h)))
+        return self._new_block(h)
+
+    def pair_updates(self, base, updates: Sequence[PairUpdate]) -> "GpuBlock":
+        """pair_solves, then each column's rank-one update of `base` (pairs.rs:209-253)."""
+        self._release_block()
+        base = _f64(base)
+        if base.shape[0] != self._n:
+            raise InvalidArgument(INVALID_ARGUMENT, "one base value per unknown")
+        plus, minus = _pairs([u.pair for u in updates])
+        alpha = _f64([u.alpha for u in updates])
+        weight = _f64([u.inverse_weight for u in updates])
+        h = P()
+        _check(library().ybus_hip_factor_pair_updates(self._h, base, plus, minus, alpha, weight, len(plus),
+                                                      C.byref(h)))
+        return self._new_block(h)
+
+    def operator(self, matrix) -> "GpuOperator":
+        """A CSR matrix with one column per unknown, applied to blocks (pairs.rs:112-149)."""
+        if matrix.cols != self._n:
+            raise InvalidArgument(INVALID_ARGUMENT, "one column per unknown")
+        h = P()
+        _check(library().ybus_hip_factor_operator(self._h, int(matrix.rows), _u64(matrix.ptr), _u64(matrix.idx),
+                                                  _f64(matrix.data), C.byref(h)))
+        return GpuOperator(self, h, int(matrix.rows))
+
+
+class GpuBlock:
+    """Solutions on the device, one column per right-hand side; it borrows its factor until
+    freed (pairs.rs:88-96)."""
+
+    def __init__(self, factor: GpuFactor, handle):
+        self._factor = factor
+        self._h = handle
+
+    def free(self):
+        if self._h:
+            library().ybus_hip_block_free(self._h)
+            self._h = None
+            if self._factor._block is self:
+                self._factor._block = None
+
+    def __del__(self):
+        self.free()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+    def _live(self):
+        if not self._h:
+            raise InvalidArgument(INVALID_ARGUMENT, "the block was freed")
+        return self._h
+
+    def columns(self) -> int:
+        return int(library().ybus_hip_block_columns(self._live()))
+
+    def singular(self):
+        out = np.zeros(self.columns(), np.uint8)
+        _check(library().ybus_hip_block_singular(self._live(), out))
+        return [bool(v) for v in out]
+
+    def refinement(self):
+        k = self.columns()
+        steps, ratio = np.zeros(k, np.uint64), np.zeros(k)
+        _check(library().ybus_hip_block_refinement(self._live(), steps, ratio))
+        return steps.astype(np.int64), ratio
+
+    def solutions(self) -> np.ndarray:
+        out = np.zeros((self.columns(), self._factor.dim()))
+        _check(library().ybus_hip_block_solutions(self._live(), out))
+        return out
+
+    @staticmethod
+    def _maxima(arr) -> list:
+        return [ColumnMax(None if m.row < 0 else int(m.row), float(m.value), bool(m.not_finite)) for m in arr]
+
+    def maxima(self):
+        k = self.columns()
+        arr = (_ColumnMax * max(k, 1))()
+        _check(library().ybus_hip_block_maxima(self._live(), arr))
+        return self._maxima(arr[:k])
+
+    def product(self, operator: "GpuOperator", zero) -> np.ndarray:
+        z = np.array([-1 if r is None else int(r) for r in zero], dtype=np.int64)
+        if len(z) != self.columns():
+            raise InvalidArgument(INVALID_ARGUMENT, "one entry to zero (or none) per column")
+        out = np.zeros((self.columns(), operator.rows()))
+        _check(library().ybus_hip_block_product(self._live(), operator._h, z, out))
+        return out
+
+    def product_maxima(self, operator: "GpuOperator", zero):
+        z = np.array([-1 if r is None else int(r) for r in zero], dtype=np.int64)
+        if len(z) != self.columns():
+            raise InvalidArgument(INVALID_ARGUMENT, "one entry to zero (or none) per column")
+        k = self.columns()
+        arr = (_ColumnMax * max(k, 1))()
+        _check(library().ybus_hip_block_product_maxima(self._live(), operator._h, z, arr))
+        return self._maxima(arr[:k])
+
+
+class GpuOperator:
+    """M theta + offset applied to a factor's blocks (pairs.rs:52-86)."""
+
+    def __init__(self, factor: GpuFactor, handle, rows: int):
+        self._factor = factor
+        self._h = handle
+        self._rows = rows
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            library().ybus_hip_operator_free(self._h)
+            self._h = None
+
+    def rows(self) -> int:
+        return self._rows
+
+    def set_offset(self, offset):
+        offset = _f64(offset)
+        if offset.shape[0] != self._rows:
+            raise InvalidArgument(INVALID_ARGUMENT, "one offset per row")
+        _check(library().ybus_hip_operator_set_offset(self._h, offset))
+
+    def set_weights(self, weights):
+        weights = _f64(weights)
+        if weights.shape[0] != self._rows:
+            raise InvalidArgument(INVALID_ARGUMENT, "one weight per row")
+        _check(library().ybus_hip_operator_set_weights(self._h, weights))

@@ -1,0 +1,82 @@
+// Host-side symbolic analysis of an uploaded factor: each triangular sweep read as rows to pull
+// (every row subtracting its dependencies in the order ybus_sparse's CPU solve does), scheduled
+// into level sets, with the solve's gather and scatter permutations folded into the rows a sweep
+// reads and writes; the fundamental supernodes of a Cholesky factor's elimination tree; and the
+// tile-width choice for a block of right-hand sides.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ybus_hip {
+
+// A triangular matrix as rows to pull: row i is (b_i - sum over its entries of val * x[dep]),
+// the entries in the order they are subtracted, then divided by diag[i] (no division when
+// `unit`). `lower` rows depend only on earlier rows, upper rows only on later ones.
+struct RowForm {
+  int64_t n = 0;
+  bool lower = true;
+  bool unit = false;
+  std::vector<int64_t> ptr;  // n + 1
+  std::vector<int32_t> dep;
+  std::vector<double> val;
+  std::vector<double> diag;  // n, unless unit
+};
+
+// A sweep scheduled for the device: rows in level order (position k), each reading its own
+// right-hand side at src[k] of the source block, writing its result at dst[k] of the
+// destination block, and reading dependencies at dep[p] of the destination block.
+struct Sweep {
+  int32_t n = 0;
+  int32_t levels = 0;
+  bool unit = false;
+  std::vector<int32_t> level_ptr;  // levels + 1
+  std::vector<int32_t> src, dst;   // n each
+  std::vector<int32_t> row_ptr;    // n + 1
+  std::vector<int32_t> dep;
+  std::vector<double> val;
+  std::vector<double> diag;  // n, unless unit
+  int32_t widest_level = 0;
+};
+
+// Schedules `form`: level[i] is one more than the deepest of row i's dependencies; rows are
+// ordered by (level, row). src_map / dst_map give each factor row's row in the source and
+// destination blocks (dependencies are mapped through dst_map).
+Sweep schedule(const RowForm& form, const std::vector<int64_t>& src_map, const std::vector<int64_t>& dst_map);
+
+// Each row's level: one more than the deepest of its dependencies (0 for none).
+std::vector<int32_t> levels_of(const RowForm& form);
+
+// The four ways a factor is read, built from its host arrays.
+// Cholesky (cholesky.rs:468-501): L in CSR (each row ends with its diagonal) forward, and L^T
+// (L in CSC) backward, both ascending.
+RowForm cholesky_forward(int64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, const double* l_val);
+RowForm cholesky_backward(int64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, const double* l_val);
+// LU (lu.rs:640-649, 770-785): the CPU sweeps L and U by columns; pulled by rows, row i of L
+// takes its entries in ascending column order and row i of U in descending column order, which
+// is the order the column sweeps update y[i] in.
+RowForm lu_forward(int64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, const double* l_val);
+RowForm lu_backward(int64_t n, const uint64_t* u_ptr, const uint64_t* u_idx, const double* u_val);
+// LU transposed (lu.rs:506-533): U^T forward, each column of U read as a row (diagonal last),
+// then unit L^T backward, each column of L below its diagonal read as a row.
+RowForm lu_transpose_forward(int64_t n, const uint64_t* u_ptr, const uint64_t* u_idx, const double* u_val);
+RowForm lu_transpose_backward(int64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, const double* l_val);
+
+// Fundamental supernodes of a Cholesky factor (multifrontal.rs:94-138): the elimination tree
+// comes from L's pattern (the parent of column j is the first row below j in column j).
+struct Supernodes {
+  std::vector<int64_t> start;  // one per supernode, plus n at the end
+  int64_t largest = 0;
+  int64_t in_chains = 0;  // columns in supernodes of two or more columns
+};
+Supernodes cholesky_supernodes(int64_t n, const uint64_t* l_ptr, const uint64_t* l_idx);
+
+// Validates a CSR/CSC pattern: pointers non-decreasing from 0, indices below `minor`.
+bool valid_compressed(int64_t major, int64_t minor, const uint64_t* ptr, const uint64_t* idx, std::string* why);
+
+// The RHS tile width (columns one workgroup owns) for `ncols` columns on `cus` compute units: the
+// widest power of two up to 64 that still gives at least two tiles per compute unit, else 1.
+int tile_width_for(int64_t ncols, int cus);
+
+}  // namespace ybus_hip

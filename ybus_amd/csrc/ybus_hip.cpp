@@ -1,0 +1,994 @@
+// The C ABI of ybus_hip (include/ybus_hip.h): contexts, factor upload and analysis, device
+// workspaces, and the solve entry points, over the kernels in kernels.hip. Every entry point
+// catches its own failures and reports them as a status and a thread-local message.
+#include "ybus_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "analysis.hpp"
+#include "kernels.hpp"
+
+using namespace ybus_hip;
+
+namespace {
+
+thread_local std::string last_error;
+
+struct Failure : std::runtime_error {
+  int32_t status;
+  Failure(int32_t s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+[[noreturn]] void fail(int32_t status, const std::string& message) { throw Failure(status, message); }
+
+void check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    fail(YBUS_HIP_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+  fail(YBUS_HIP_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int32_t guard(F f) {
+  try {
+    f();
+    return YBUS_HIP_OK;
+  } catch (const Failure& e) {
+    last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    last_error = "host allocation failed";
+    return YBUS_HIP_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    last_error = e.what();
+    return YBUS_HIP_INTERNAL;
+  }
+}
+
+// Device memory that frees itself
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t len = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    len = 0;
+  }
+  // At least `n` elements (contents not kept when it grows)
+  void reserve(size_t n) {
+    if (n <= len && p) return;
+    release();
+    check(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+    len = n;
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    reserve(v.size());
+    if (!v.empty()) check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+  }
+};
+
+// Page-locked host staging, grown as needed (factor.rs:63-89)
+struct Staging {
+  double* p = nullptr;
+  size_t len = 0;
+  ~Staging() {
+    if (p) (void)hipHostFree(p);
+  }
+  double* get(size_t n) {
+    if (n > len || !p) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      len = 0;
+      check(hipHostMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+      len = n;
+    }
+    return p;
+  }
+};
+
+struct DevSweep {
+  DevBuf<int32_t> level_ptr, src, dst, row_ptr, dep;
+  DevBuf<double> val64, diag64;
+  DevBuf<float> val32, diag32;
+  Sweep host;
+  void upload(Sweep s, hipStream_t stream) {
+    host = std::move(s);
+    level_ptr.upload(host.level_ptr, stream);
+    src.upload(host.src, stream);
+    dst.upload(host.dst, stream);
+    row_ptr.upload(host.row_ptr, stream);
+    dep.upload(host.dep, stream);
+    val64.upload(host.val, stream);
+    std::vector<float> v32(host.val.begin(), host.val.end());
+    val32.upload(v32, stream);
+    if (!host.unit) {
+      diag64.upload(host.diag, stream);
+      std::vector<float> d32(host.diag.begin(), host.diag.end());
+      diag32.upload(d32, stream);
+    }
+    check(hipStreamSynchronize(stream), "upload factor");
+  }
+  SweepDev view() const {
+    SweepDev v;
+    v.n = host.n;
+    v.levels = host.levels;
+    v.level_ptr = level_ptr.p;
+    v.src = src.p;
+    v.dst = dst.p;
+    v.row_ptr = row_ptr.p;
+    v.dep = dep.p;
+    v.val64 = val64.p;
+    v.val32 = val32.p;
+    v.diag64 = host.unit ? nullptr : diag64.p;
+    v.diag32 = host.unit ? nullptr : diag32.p;
+    return v;
+  }
+  size_t entries() const { return host.dep.size(); }
+};
+
+struct DevCsr {
+  int32_t rows = 0, cols = 0;
+  DevBuf<int32_t> ptr, col;
+  DevBuf<double> val;
+  size_t nnz = 0;
+  CsrDev view() const { return CsrDev{rows, cols, ptr.p, col.p, val.p}; }
+};
+
+constexpr uint64_t MAX_DEFAULT_WIDTH = 32768;
+constexpr uint64_t SINGLE_ROW_WIDTH = 64;
+constexpr int64_t INDEX_LIMIT = 0x7fffffff;
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+double seconds_since(std::chrono::steady_clock::time_point t) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+
+std::vector<int32_t> to_i32(const uint64_t* v, size_t n, size_t entries) {
+  std::vector<int32_t> out(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (v[i] > (uint64_t)INDEX_LIMIT)
+      fail(YBUS_HIP_TOO_LARGE, "a matrix with " + std::to_string(entries) + " entries is too large for 32-bit indices");
+    out[i] = (int32_t)v[i];
+  }
+  return out;
+}
+
+void check_permutation(const uint64_t* perm, uint64_t n, const char* what) {
+  if (!perm) fail(YBUS_HIP_INVALID_ARGUMENT, std::string(what) + " is null");
+  std::vector<bool> seen(n, false);
+  for (uint64_t i = 0; i < n; ++i) {
+    if (perm[i] >= n || seen[perm[i]]) fail(YBUS_HIP_INVALID_ARGUMENT, std::string(what) + " is not a permutation");
+    seen[perm[i]] = true;
+  }
+}
+
+}  // namespace
+
+struct ybus_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string name;
+  int cus = 256;
+  ~ybus_hip_ctx() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct ybus_hip_block;
+
+struct ybus_hip_factor {
+  ybus_hip_ctx* ctx = nullptr;
+  int64_t n = 0;
+  bool lu = false;
+  uint64_t width = 1;
+  int64_t nnz_factor = 0;  // S: entries swept by one forward and backward solve
+  DevSweep fwd, bwd, tfwd, tbwd;
+  bool has_transpose = false;
+  DevCsr a;  // the matrix itself, by user rows, for the FP64 residual
+  // Workspace, `cols` columns of ld
+  int64_t cols = 0, ld = 0;
+  DevBuf<double> b, t, x, x1, r, vec;
+  DevBuf<float> t32, d32;
+  DevBuf<int32_t> plus, minus, isteps, singular;
+  DevBuf<double> dratio;
+  int64_t staged = 0;
+  bool staged_pairs = false;  // the staged block is pairs (plus, minus), not vectors (b)
+  ybus_hip_block* borrowed = nullptr;
+  Staging staging;
+  ybus_hip_timings timings{};
+  ybus_hip_kernel_stats stats{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int tile_override = 0, threads = 256;
+
+  ~ybus_hip_factor() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+
+  hipStream_t stream() const { return ctx->stream; }
+
+  // Sizes the workspace for `ncols` columns; whatever was staged is dropped, since the solves
+  // that call this overwrite the staged buffers
+  void ensure_workspace(int64_t ncols, bool refine) {
+    staged = 0;
+    staged_pairs = false;
+    if (ncols > cols) {
+      cols = ncols;
+      ld = round_up(std::max<int64_t>(ncols, 1), 16);
+      b.release();
+      t.release();
+      x.release();
+      x1.release();
+      r.release();
+      t32.release();
+      d32.release();
+      vec.release();
+    }
+    const size_t elems = (size_t)std::max<int64_t>(n, 1) * (size_t)ld;
+    b.reserve(elems);
+    t.reserve(elems);
+    x.reserve(elems);
+    vec.reserve(elems);
+    plus.reserve(ld);
+    minus.reserve(ld);
+    singular.reserve(ld);
+    isteps.reserve(ld);
+    dratio.reserve(ld);
+    if (refine) {
+      x1.reserve(elems);
+      r.reserve(elems);
+      t32.reserve(elems);
+      d32.reserve(elems);
+    }
+  }
+
+  int tile_for(int64_t ncols) const {
+    if (tile_override) return tile_override;
+    return tile_width_for(ncols, ctx->cus);
+  }
+
+  void begin_kernel() { check(hipEventRecord(ev0, stream()), "hipEventRecord"); }
+  void end_kernel(uint64_t bytes) {
+    check(hipEventRecord(ev1, stream()), "hipEventRecord");
+    check(hipEventSynchronize(ev1), "hipEventSynchronize");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+    stats.launches += 1;
+    stats.milliseconds += ms;
+    stats.bytes += bytes;
+  }
+
+  // Algorithmic bytes of one FP64 block solve of `ncols` columns: the factor's values and
+  // indices swept once, its row pointers, and each column's right-hand side in and solution out
+  uint64_t bytes64(int64_t ncols, bool pairs) const {
+    const uint64_t factor = (uint64_t)nnz_factor * 12 + 8 * (uint64_t)(n + 1);
+    const uint64_t columns = pairs ? (uint64_t)ncols * ((uint64_t)n * 8 + 8) : (uint64_t)ncols * (uint64_t)n * 16;
+    return factor + columns;
+  }
+
+  void solve64(const SweepDev& f, const SweepDev& bw, const RhsDev& rhs, int64_t ncols, bool pairs) {
+    begin_kernel();
+    check(launch_solve64(stream(), tile_for(ncols), threads, f, bw, rhs, t.p, x.p, ld, (int32_t)ncols),
+          "solve kernel");
+    end_kernel(bytes64(ncols, pairs));
+  }
+
+  void refine(int64_t ncols, double tol, int32_t max_steps) {
+    if (!a.ptr.p) fail(YBUS_HIP_INVALID_ARGUMENT, "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix)");
+    begin_kernel();
+    check(launch_refine(stream(), tile_for(ncols), threads, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p, x.p,
+                        x1.p, r.p, ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
+          "refine kernel");
+    // bytes: per column, its own corrections k_c; the factor and matrix once per sweep and
+    // residual of the longest column
+    std::vector<int32_t> steps(ncols);
+    check(hipEventRecord(ev1, stream()), "hipEventRecord");
+    check(hipMemcpyAsync(steps.data(), isteps.p, ncols * sizeof(int32_t), hipMemcpyDeviceToHost, stream()), "steps");
+    check(hipStreamSynchronize(stream()), "refine");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+    uint64_t kmax = 0, ksum = 0;
+    for (int32_t s : steps) {
+      kmax = std::max<uint64_t>(kmax, (uint64_t)s);
+      ksum += (uint64_t)s;
+    }
+    const uint64_t nn = (uint64_t)n;
+    const uint64_t sweep = (uint64_t)nnz_factor * 8 + 8 * (nn + 1);
+    const uint64_t resid = (uint64_t)a.nnz * 12 + 4 * (nn + 1);
+    stats.launches += 1;
+    stats.milliseconds += ms;
+    stats.bytes += (kmax + 1) * sweep + (kmax + 1) * resid + (uint64_t)ncols * nn * 16 + ksum * nn * 24;
+  }
+};
+
+struct ybus_hip_block {
+  ybus_hip_factor* f = nullptr;
+  int64_t columns = 0;
+  bool updated = false;  // results in t (pair updates) rather than x
+  std::vector<uint8_t> singular;
+  std::vector<uint64_t> steps;
+  std::vector<double> ratio;
+  const double* data() const { return updated ? f->t.p : f->x.p; }
+};
+
+struct ybus_hip_operator {
+  ybus_hip_factor* f = nullptr;
+  DevCsr m;
+  DevBuf<double> offset, weight, out, best;
+  DevBuf<int32_t> zero, best_row, not_finite;
+  int64_t out_cols = 0;
+};
+
+// ---- C ABI ----
+
+extern "C" {
+
+size_t ybus_hip_last_error_message(char* buf, size_t len) {
+  if (buf && len) {
+    size_t n = std::min(len - 1, last_error.size());
+    std::memcpy(buf, last_error.data(), n);
+    buf[n] = 0;
+  }
+  return last_error.size() + 1;
+}
+
+int32_t ybus_hip_ctx_create(ybus_hip_ctx** out) {
+  return guard([&] {
+    if (!out) fail(YBUS_HIP_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) fail(YBUS_HIP_NO_DEVICE, "no HIP device");
+    auto ctx = std::make_unique<ybus_hip_ctx>();
+    ctx->device = 0;
+    check(hipSetDevice(0), "hipSetDevice");
+    hipDeviceProp_t prop;
+    check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+    ctx->name = prop.name;
+    ctx->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    *out = ctx.release();
+  });
+}
+
+void ybus_hip_ctx_free(ybus_hip_ctx* ctx) { delete ctx; }
+
+int32_t ybus_hip_ctx_name(ybus_hip_ctx* ctx, char* buf, size_t len, size_t* needed) {
+  return guard([&] {
+    if (!ctx) fail(YBUS_HIP_INVALID_ARGUMENT, "ctx is null");
+    if (buf && len) {
+      size_t n = std::min(len - 1, ctx->name.size());
+      std::memcpy(buf, ctx->name.data(), n);
+      buf[n] = 0;
+    }
+    if (needed) *needed = ctx->name.size() + 1;
+  });
+}
+
+int32_t ybus_hip_ctx_memory(ybus_hip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+  return guard([&] {
+    if (!ctx) fail(YBUS_HIP_INVALID_ARGUMENT, "ctx is null");
+    size_t f = 0, t = 0;
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+  });
+}
+
+static void finish_factor(ybus_hip_factor* f) {
+  check(hipEventCreate(&f->ev0), "hipEventCreate");
+  check(hipEventCreate(&f->ev1), "hipEventCreate");
+  // factor.rs:158-160: two blocks of n doubles per column in a quarter of the free memory;
+  // here clamped at 32768 columns, enough tiles to fill every compute unit many times over
+  size_t free_b = 0, total_b = 0;
+  check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+  uint64_t w = free_b / 4 / (2 * 8 * (uint64_t)std::max<int64_t>(f->n, 1));
+  f->width = std::clamp<uint64_t>(w, 1, MAX_DEFAULT_WIDTH);
+  ybus_hip_factor_set_block_width(f, f->width);
+  const char* env = std::getenv("YBUS_HIP_TILE");
+  if (env) f->tile_override = std::atoi(env);
+  const char* thr = std::getenv("YBUS_HIP_THREADS");
+  if (thr) f->threads = std::atoi(thr);
+}
+
+int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx,
+                                        const double* l_val, const uint64_t* perm, ybus_hip_factor** out) {
+  return guard([&] {
+    if (!ctx || !out || !l_ptr || (!l_idx && l_ptr[n]) || (!l_val && l_ptr[n]))
+      fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    check_permutation(perm, n, "perm");
+    std::string why;
+    if (!valid_compressed((int64_t)n, (int64_t)n, l_ptr, l_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "L: " + why);
+    const uint64_t entries = l_ptr[n];
+    if (entries > (uint64_t)INDEX_LIMIT || n > (uint64_t)INDEX_LIMIT)
+      fail(YBUS_HIP_TOO_LARGE, "a factor with " + std::to_string(entries) + " entries is too large for 32-bit indices");
+    for (uint64_t i = 0; i < n; ++i) {
+      if (l_ptr[i + 1] == l_ptr[i] || l_idx[l_ptr[i + 1] - 1] != i)
+        fail(YBUS_HIP_INVALID_ARGUMENT, "each row of L must end with its diagonal");
+    }
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto f = std::make_unique<ybus_hip_factor>();
+    f->ctx = ctx;
+    f->n = (int64_t)n;
+    f->lu = false;
+    f->nnz_factor = 2 * (int64_t)entries;
+    std::vector<int64_t> gather(n), ident(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      gather[i] = (int64_t)perm[i];
+      ident[i] = (int64_t)i;
+    }
+    f->fwd.upload(schedule(cholesky_forward(n, l_ptr, l_idx, l_val), gather, ident), ctx->stream);
+    f->bwd.upload(schedule(cholesky_backward(n, l_ptr, l_idx, l_val), ident, gather), ctx->stream);
+    finish_factor(f.get());
+    *out = f.release();
+  });
+}
+
+int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx,
+                                  const double* l_val, const uint64_t* u_ptr, const uint64_t* u_idx, const double* u_val,
+                                  const uint64_t* row_perm, const uint64_t* col_perm, ybus_hip_factor** out) {
+  return guard([&] {
+    if (!ctx || !out || !l_ptr || !u_ptr) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if ((l_ptr[n] && (!l_idx || !l_val)) || (u_ptr[n] && (!u_idx || !u_val)))
+      fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    check_permutation(row_perm, n, "row_perm");
+    check_permutation(col_perm, n, "col_perm");
+    std::string why;
+    if (!valid_compressed((int64_t)n, (int64_t)n, l_ptr, l_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "L: " + why);
+    if (!valid_compressed((int64_t)n, (int64_t)n, u_ptr, u_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "U: " + why);
+    const uint64_t entries = l_ptr[n] + u_ptr[n];
+    if (entries > (uint64_t)INDEX_LIMIT || n > (uint64_t)INDEX_LIMIT)
+      fail(YBUS_HIP_TOO_LARGE, "a factor with " + std::to_string(entries) + " entries is too large for 32-bit indices");
+    for (uint64_t j = 0; j < n; ++j) {
+      if (l_ptr[j + 1] == l_ptr[j] || l_idx[l_ptr[j]] != j)
+        fail(YBUS_HIP_INVALID_ARGUMENT, "each column of L must start with its unit diagonal");
+      if (u_ptr[j + 1] == u_ptr[j] || u_idx[u_ptr[j + 1] - 1] != j)
+        fail(YBUS_HIP_INVALID_ARGUMENT, "each column of U must end with its diagonal");
+    }
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto f = std::make_unique<ybus_hip_factor>();
+    f->ctx = ctx;
+    f->n = (int64_t)n;
+    f->lu = true;
+    f->nnz_factor = (int64_t)entries;
+    std::vector<int64_t> rp(n), cp(n), ident(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      rp[i] = (int64_t)row_perm[i];
+      cp[i] = (int64_t)col_perm[i];
+      ident[i] = (int64_t)i;
+    }
+    f->fwd.upload(schedule(lu_forward(n, l_ptr, l_idx, l_val), rp, ident), ctx->stream);
+    f->bwd.upload(schedule(lu_backward(n, u_ptr, u_idx, u_val), ident, cp), ctx->stream);
+    f->tfwd.upload(schedule(lu_transpose_forward(n, u_ptr, u_idx, u_val), cp, ident), ctx->stream);
+    f->tbwd.upload(schedule(lu_transpose_backward(n, l_ptr, l_idx, l_val), ident, rp), ctx->stream);
+    f->has_transpose = true;
+    finish_factor(f.get());
+    *out = f.release();
+  });
+}
+
+void ybus_hip_factor_free(ybus_hip_factor* f) { delete f; }
+
+uint64_t ybus_hip_factor_dim(const ybus_hip_factor* f) { return f ? (uint64_t)f->n : 0; }
+
+uint64_t ybus_hip_factor_block_width(const ybus_hip_factor* f) { return f ? f->width : 0; }
+
+void ybus_hip_factor_set_block_width(ybus_hip_factor* f, uint64_t width) {
+  if (!f) return;
+  const uint64_t most = f->n == 1 ? SINGLE_ROW_WIDTH : UINT64_MAX;
+  f->width = std::clamp<uint64_t>(width, 1, most);
+}
+
+void ybus_hip_factor_timings(const ybus_hip_factor* f, ybus_hip_timings* out) {
+  if (f && out) *out = f->timings;
+}
+
+void ybus_hip_factor_reset_timings(ybus_hip_factor* f) {
+  if (f) f->timings = ybus_hip_timings{};
+}
+
+void ybus_hip_factor_kernel_stats(const ybus_hip_factor* f, ybus_hip_kernel_stats* out) {
+  if (f && out) *out = f->stats;
+}
+
+void ybus_hip_factor_reset_kernel_stats(ybus_hip_factor* f) {
+  if (f) f->stats = ybus_hip_kernel_stats{};
+}
+
+int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const uint64_t* ptr, const uint64_t* idx,
+                                   const double* val) {
+  return guard([&] {
+    if (!f || !ptr) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    const int64_t n = f->n;
+    std::string why;
+    if (!valid_compressed(n, n, ptr, idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "A: " + why);
+    const uint64_t nnz = ptr[n];
+    if (nnz > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the matrix is too large for 32-bit indices");
+    std::vector<int32_t> rptr(n + 1), cols(nnz);
+    std::vector<double> vals(nnz);
+    if (row_major) {
+      for (int64_t i = 0; i <= n; ++i) rptr[i] = (int32_t)ptr[i];
+      for (uint64_t p = 0; p < nnz; ++p) {
+        cols[p] = (int32_t)idx[p];
+        vals[p] = val[p];
+      }
+    } else {
+      // CSC to CSR by a counting sort by row: each row's columns come out ascending, the order
+      // CsrMatrix::mul_vec sums them in (product.rs:17-27)
+      std::vector<int32_t> cnt(n + 1, 0);
+      for (uint64_t p = 0; p < nnz; ++p) cnt[idx[p] + 1] += 1;
+      for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+      rptr = cnt;
+      std::vector<int32_t> cursor(cnt.begin(), cnt.end() - 1);
+      for (int64_t j = 0; j < n; ++j)
+        for (uint64_t p = ptr[j]; p < ptr[j + 1]; ++p) {
+          int32_t q = cursor[idx[p]]++;
+          cols[q] = (int32_t)j;
+          vals[q] = val[p];
+        }
+    }
+    f->a.rows = f->a.cols = (int32_t)n;
+    f->a.nnz = nnz;
+    f->a.ptr.upload(rptr, f->stream());
+    f->a.col.upload(cols, f->stream());
+    f->a.val.upload(vals, f->stream());
+    check(hipStreamSynchronize(f->stream()), "upload matrix");
+  });
+}
+
+// Copies count host vectors into the device block b (count <= f->cols)
+static void upload_vectors(ybus_hip_factor* f, const double* rhs, int64_t count) {
+  const size_t len = (size_t)count * (size_t)f->n;
+  double* host = f->staging.get(len);
+  std::memcpy(host, rhs, len * sizeof(double));
+  check(hipMemcpyAsync(f->vec.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+  check(launch_to_block(f->stream(), f->vec.p, f->b.p, f->n, count, f->ld), "to_block");
+  check(hipStreamSynchronize(f->stream()), "upload");
+}
+
+static void download_vectors(ybus_hip_factor* f, const double* block, int64_t count, double* out) {
+  const size_t len = (size_t)count * (size_t)f->n;
+  double* host = f->staging.get(len);
+  check(launch_from_block(f->stream(), block, f->vec.p, f->n, count, f->ld), "from_block");
+  check(hipMemcpyAsync(host, f->vec.p, len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  check(hipStreamSynchronize(f->stream()), "download");
+  std::memcpy(out, host, len * sizeof(double));
+}
+
+static void require_unborrowed(const ybus_hip_factor* f) {
+  if (f->borrowed) fail(YBUS_HIP_INVALID_ARGUMENT, "a block borrows this factor: free it first");
+}
+
+static int32_t solve_many_mode(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x, int mode,
+                               double tol, uint64_t max_steps, uint64_t* steps, double* ratio) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    require_unborrowed(f);
+    if (count == 0 || f->n == 0) return;
+    if (!rhs || !x) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
+    const bool refine = mode == YBUS_HIP_FP32_REFINED;
+    f->ensure_workspace(width, refine);
+    for (uint64_t start = 0; start < count; start += (uint64_t)width) {
+      const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
+      const double* in = rhs + start * (uint64_t)f->n;
+      double* outp = x + start * (uint64_t)f->n;
+      auto t0 = std::chrono::steady_clock::now();
+      upload_vectors(f, in, chunk);
+      auto t1 = std::chrono::steady_clock::now();
+      if (mode == YBUS_HIP_FP64) {
+        f->solve64(f->fwd.view(), f->bwd.view(), RhsDev{f->b.p, nullptr, nullptr}, chunk, false);
+      } else if (mode == YBUS_HIP_TRANSPOSE) {
+        const DevSweep& tf = f->has_transpose ? f->tfwd : f->fwd;
+        const DevSweep& tb = f->has_transpose ? f->tbwd : f->bwd;
+        f->solve64(tf.view(), tb.view(), RhsDev{f->b.p, nullptr, nullptr}, chunk, false);
+      } else {
+        f->refine(chunk, tol, (int32_t)max_steps);
+        std::vector<int32_t> st(chunk);
+        std::vector<double> ra(chunk);
+        check(hipMemcpy(st.data(), f->isteps.p, chunk * sizeof(int32_t), hipMemcpyDeviceToHost), "steps");
+        check(hipMemcpy(ra.data(), f->dratio.p, chunk * sizeof(double), hipMemcpyDeviceToHost), "ratio");
+        for (int64_t c = 0; c < chunk; ++c) {
+          if (steps) steps[start + c] = (uint64_t)st[c];
+          if (ratio) ratio[start + c] = ra[c];
+        }
+      }
+      auto t2 = std::chrono::steady_clock::now();
+      download_vectors(f, f->x.p, chunk, outp);
+      f->timings.upload += std::chrono::duration<double>(t1 - t0).count();
+      f->timings.solve += std::chrono::duration<double>(t2 - t1).count();
+      f->timings.download += seconds_since(t2);
+      f->timings.blocks += 1;
+    }
+  });
+}
+
+int32_t ybus_hip_factor_solve_many(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x) {
+  return solve_many_mode(f, rhs, count, x, YBUS_HIP_FP64, 0.0, 0, nullptr, nullptr);
+}
+
+int32_t ybus_hip_factor_solve_transpose_many(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x) {
+  return solve_many_mode(f, rhs, count, x, YBUS_HIP_TRANSPOSE, 0.0, 0, nullptr, nullptr);
+}
+
+int32_t ybus_hip_factor_solve_many_refined(ybus_hip_factor* f, const double* rhs, uint64_t count, double tolerance,
+                                           uint64_t max_steps, double* x, uint64_t* steps, double* ratio) {
+  if (f && !f->a.ptr.p) {
+    last_error = "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix)";
+    return YBUS_HIP_INVALID_ARGUMENT;
+  }
+  return solve_many_mode(f, rhs, count, x, YBUS_HIP_FP32_REFINED, tolerance, max_steps, steps, ratio);
+}
+
+int32_t ybus_hip_factor_stage(ybus_hip_factor* f, const double* rhs, uint64_t count) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    require_unborrowed(f);
+    if (count > f->width) fail(YBUS_HIP_INVALID_ARGUMENT, "more right-hand sides than block_width()");
+    if (count && !rhs) fail(YBUS_HIP_INVALID_ARGUMENT, "rhs is null");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    f->ensure_workspace((int64_t)count, true);
+    if (count && f->n) upload_vectors(f, rhs, (int64_t)count);
+    f->staged = (int64_t)count;
+    f->staged_pairs = false;
+  });
+}
+
+int32_t ybus_hip_factor_solve_staged(ybus_hip_factor* f, int32_t mode, double tolerance, uint64_t max_steps,
+                                     ybus_hip_block** out) {
+  return guard([&] {
+    if (!f || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    require_unborrowed(f);
+    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_FP32_REFINED && mode != YBUS_HIP_TRANSPOSE)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "unknown mode");
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    auto block = std::make_unique<ybus_hip_block>();
+    block->f = f;
+    block->columns = f->staged;
+    block->singular.assign(f->staged, 0);
+    block->steps.assign(f->staged, 0);
+    block->ratio.assign(f->staged, 0.0);
+    if (f->staged_pairs && mode != YBUS_HIP_FP64)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "staged pairs are solved in YBUS_HIP_FP64 only");
+    if (f->staged && f->n) {
+      check(hipSetDevice(f->ctx->device), "hipSetDevice");
+      if (f->staged_pairs) {
+        f->solve64(f->fwd.view(), f->bwd.view(), RhsDev{nullptr, f->plus.p, f->minus.p}, f->staged, true);
+      } else if (mode == YBUS_HIP_FP64) {
+        f->solve64(f->fwd.view(), f->bwd.view(), RhsDev{f->b.p, nullptr, nullptr}, f->staged, false);
+      } else if (mode == YBUS_HIP_TRANSPOSE) {
+        const DevSweep& tf = f->has_transpose ? f->tfwd : f->fwd;
+        const DevSweep& tb = f->has_transpose ? f->tbwd : f->bwd;
+        f->solve64(tf.view(), tb.view(), RhsDev{f->b.p, nullptr, nullptr}, f->staged, false);
+      } else {
+        f->refine(f->staged, tolerance, (int32_t)max_steps);
+        std::vector<int32_t> st(f->staged);
+        check(hipMemcpy(st.data(), f->isteps.p, f->staged * sizeof(int32_t), hipMemcpyDeviceToHost), "steps");
+        check(hipMemcpy(block->ratio.data(), f->dratio.p, f->staged * sizeof(double), hipMemcpyDeviceToHost), "ratio");
+        for (int64_t c = 0; c < f->staged; ++c) block->steps[c] = (uint64_t)st[c];
+      }
+    }
+    f->borrowed = block.get();
+    *out = block.release();
+  });
+}
+
+// pairs.rs:153-187: checks the pairs and copies them to the device (f->plus, f->minus)
+static void stage_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* minus, uint64_t k) {
+  require_unborrowed(f);
+  if (k > f->width)
+    fail(YBUS_HIP_INVALID_ARGUMENT,
+         "at most block_width() (" + std::to_string(f->width) + ") pairs, not " + std::to_string(k));
+  if (k && (!plus || !minus)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+  std::vector<int32_t> p(k), m(k);
+  for (uint64_t c = 0; c < k; ++c) {
+    if (plus[c] < -1 || plus[c] >= f->n || minus[c] < -1 || minus[c] >= f->n)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "unknowns below dim() (" + std::to_string(f->n) + ")");
+    if (plus[c] >= 0 && plus[c] == minus[c]) fail(YBUS_HIP_INVALID_ARGUMENT, "a pair's two unknowns must differ");
+    p[c] = (int32_t)plus[c];
+    m[c] = (int32_t)minus[c];
+  }
+  check(hipSetDevice(f->ctx->device), "hipSetDevice");
+  f->ensure_workspace((int64_t)k, false);
+  if (k) {
+    check(hipMemcpyAsync(f->plus.p, p.data(), k * sizeof(int32_t), hipMemcpyHostToDevice, f->stream()), "H2D");
+    check(hipMemcpyAsync(f->minus.p, m.data(), k * sizeof(int32_t), hipMemcpyHostToDevice, f->stream()), "H2D");
+    check(hipStreamSynchronize(f->stream()), "stage pairs");
+  }
+}
+
+// Checks the pairs and solves them into the block's first buffer (x)
+static void solve_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* minus, uint64_t k) {
+  stage_pairs(f, plus, minus, k);
+  if (!k || !f->n) return;
+  f->solve64(f->fwd.view(), f->bwd.view(), RhsDev{nullptr, f->plus.p, f->minus.p}, (int64_t)k, true);
+}
+
+int32_t ybus_hip_factor_stage_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* minus, uint64_t k) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    stage_pairs(f, plus, minus, k);
+    f->staged = (int64_t)k;
+    f->staged_pairs = true;
+  });
+}
+
+int32_t ybus_hip_factor_pair_solves(ybus_hip_factor* f, const int64_t* plus, const int64_t* minus, uint64_t k,
+                                    ybus_hip_block** out) {
+  return guard([&] {
+    if (!f || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    solve_pairs(f, plus, minus, k);
+    auto block = std::make_unique<ybus_hip_block>();
+    block->f = f;
+    block->columns = (int64_t)k;
+    block->singular.assign(k, 0);
+    block->steps.assign(k, 0);
+    block->ratio.assign(k, 0.0);
+    f->borrowed = block.get();
+    *out = block.release();
+  });
+}
+
+int32_t ybus_hip_factor_pair_updates(ybus_hip_factor* f, const double* base, const int64_t* plus,
+                                     const int64_t* minus, const double* alpha, const double* inverse_weight,
+                                     uint64_t k, ybus_hip_block** out) {
+  return guard([&] {
+    if (!f || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (!base && f->n) fail(YBUS_HIP_INVALID_ARGUMENT, "one base value per unknown");
+    if (k && (!alpha || !inverse_weight)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    solve_pairs(f, plus, minus, k);
+    auto block = std::make_unique<ybus_hip_block>();
+    block->f = f;
+    block->columns = (int64_t)k;
+    block->updated = true;
+    block->singular.assign(k, 0);
+    block->steps.assign(k, 0);
+    block->ratio.assign(k, 0.0);
+    if (k && f->n) {
+      DevBuf<double> dbase, dalpha, dweight;
+      dbase.upload(std::vector<double>(base, base + f->n), f->stream());
+      dalpha.upload(std::vector<double>(alpha, alpha + k), f->stream());
+      dweight.upload(std::vector<double>(inverse_weight, inverse_weight + k), f->stream());
+      check(hipMemsetAsync(f->singular.p, 0, k * sizeof(int32_t), f->stream()), "memset");
+      check(launch_pair_update(f->stream(), f->x.p, f->t.p, (int32_t)f->n, (int32_t)k, f->ld, dbase.p, f->plus.p,
+                               f->minus.p, dalpha.p, dweight.p, f->singular.p),
+            "pair_update");
+      std::vector<int32_t> s(k);
+      check(hipMemcpyAsync(s.data(), f->singular.p, k * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
+      check(hipStreamSynchronize(f->stream()), "pair_update");
+      for (uint64_t c = 0; c < k; ++c) block->singular[c] = s[c] != 0;
+    }
+    f->borrowed = block.get();
+    *out = block.release();
+  });
+}
+
+void ybus_hip_block_free(ybus_hip_block* b) {
+  if (!b) return;
+  if (b->f && b->f->borrowed == b) b->f->borrowed = nullptr;
+  delete b;
+}
+
+uint64_t ybus_hip_block_columns(const ybus_hip_block* b) { return b ? (uint64_t)b->columns : 0; }
+
+int32_t ybus_hip_block_singular(const ybus_hip_block* b, uint8_t* out) {
+  return guard([&] {
+    if (!b || (!out && b->columns)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    std::copy(b->singular.begin(), b->singular.end(), out);
+  });
+}
+
+int32_t ybus_hip_block_refinement(const ybus_hip_block* b, uint64_t* steps, double* ratio) {
+  return guard([&] {
+    if (!b) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (steps) std::copy(b->steps.begin(), b->steps.end(), steps);
+    if (ratio) std::copy(b->ratio.begin(), b->ratio.end(), ratio);
+  });
+}
+
+int32_t ybus_hip_block_solutions(ybus_hip_block* b, double* out) {
+  return guard([&] {
+    if (!b || (!out && b->columns)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (!b->columns || !b->f->n) return;
+    check(hipSetDevice(b->f->ctx->device), "hipSetDevice");
+    download_vectors(b->f, b->data(), b->columns, out);
+  });
+}
+
+static void maxima_of(ybus_hip_factor* f, const double* block, int32_t rows, int64_t ld, const double* weight,
+                      int64_t columns, ybus_hip_column_max* out) {
+  DevBuf<double> best;
+  DevBuf<int32_t> row, bad;
+  best.reserve(columns);
+  row.reserve(columns);
+  bad.reserve(columns);
+  check(launch_column_max(f->stream(), block, rows, (int32_t)columns, ld, weight, best.p, row.p, bad.p), "column_max");
+  std::vector<double> hb(columns);
+  std::vector<int32_t> hr(columns), hn(columns);
+  check(hipMemcpyAsync(hb.data(), best.p, columns * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  check(hipMemcpyAsync(hr.data(), row.p, columns * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  check(hipMemcpyAsync(hn.data(), bad.p, columns * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  check(hipStreamSynchronize(f->stream()), "column_max");
+  for (int64_t c = 0; c < columns; ++c) out[c] = ybus_hip_column_max{hr[c], hb[c], hn[c]};
+}
+
+int32_t ybus_hip_block_maxima(ybus_hip_block* b, ybus_hip_column_max* out) {
+  return guard([&] {
+    if (!b || (!out && b->columns)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (!b->columns) return;
+    ybus_hip_factor* f = b->f;
+    if (!f->n) {
+      for (int64_t c = 0; c < b->columns; ++c) out[c] = ybus_hip_column_max{-1, 0.0, 0};
+      return;
+    }
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    DevBuf<double> ones;
+    ones.upload(std::vector<double>(f->n, 1.0), f->stream());
+    maxima_of(f, b->data(), (int32_t)f->n, f->ld, ones.p, b->columns, out);
+  });
+}
+
+// pairs.rs:312-366: M theta + offset per column, then the zeroed entries, left in op->out
+static void apply(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero) {
+  ybus_hip_factor* f = b->f;
+  if (op->f != f) fail(YBUS_HIP_INVALID_ARGUMENT, "an operator of this factorization");
+  if (b->columns && !zero) fail(YBUS_HIP_INVALID_ARGUMENT, "one entry to zero (or none) per column");
+  std::vector<int32_t> z(b->columns);
+  for (int64_t c = 0; c < b->columns; ++c) {
+    if (zero[c] < -1 || zero[c] >= op->m.rows)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "rows to zero below the operator's rows");
+    z[c] = (int32_t)zero[c];
+  }
+  check(hipSetDevice(f->ctx->device), "hipSetDevice");
+  op->zero.upload(z, f->stream());
+  op->out.reserve((size_t)std::max<int32_t>(op->m.rows, 1) * (size_t)std::max<int64_t>(b->columns, 1));
+  op->out_cols = b->columns;
+  if (b->columns && op->m.rows)
+    check(launch_operator(f->stream(), op->m.view(), b->data(), f->ld, op->offset.p, op->zero.p, op->out.p,
+                          b->columns, (int32_t)b->columns),
+          "operator");
+}
+
+int32_t ybus_hip_block_product(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero, double* out) {
+  return guard([&] {
+    if (!b || !op) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    apply(b, op, zero);
+    const int64_t rows = op->m.rows, cols = b->columns;
+    if (!rows || !cols) return;
+    if (!out) fail(YBUS_HIP_INVALID_ARGUMENT, "out is null");
+    // out is row-interleaved (rows x cols); the caller gets one vector of rows per column
+    std::vector<double> host((size_t)rows * cols);
+    check(hipMemcpyAsync(host.data(), op->out.p, host.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         b->f->stream()),
+          "D2H");
+    check(hipStreamSynchronize(b->f->stream()), "product");
+    for (int64_t c = 0; c < cols; ++c)
+      for (int64_t r = 0; r < rows; ++r) out[c * rows + r] = host[r * cols + c];
+  });
+}
+
+int32_t ybus_hip_block_product_maxima(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero,
+                                      ybus_hip_column_max* out) {
+  return guard([&] {
+    if (!b || !op) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    apply(b, op, zero);
+    if (!b->columns) return;
+    if (!out) fail(YBUS_HIP_INVALID_ARGUMENT, "out is null");
+    if (!op->m.rows) {
+      for (int64_t c = 0; c < b->columns; ++c) out[c] = ybus_hip_column_max{-1, 0.0, 0};
+      return;
+    }
+    maxima_of(b->f, op->out.p, op->m.rows, b->columns, op->weight.p, b->columns, out);
+  });
+}
+
+static void copy_levels(const RowForm& form, int32_t* out) {
+  if (!out) return;
+  std::vector<int32_t> level = levels_of(form);
+  std::copy(level.begin(), level.end(), out);
+}
+
+int32_t ybus_hip_analyze_cholesky(uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, int32_t* forward_level,
+                                  int32_t* backward_level, uint64_t* supernode_start, uint64_t* supernodes) {
+  return guard([&] {
+    if (!l_ptr || (!l_idx && l_ptr[n])) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    std::string why;
+    if (!valid_compressed((int64_t)n, (int64_t)n, l_ptr, l_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "L: " + why);
+    for (uint64_t i = 0; i < n; ++i)
+      if (l_ptr[i + 1] == l_ptr[i] || l_idx[l_ptr[i + 1] - 1] != i)
+        fail(YBUS_HIP_INVALID_ARGUMENT, "each row of L must end with its diagonal");
+    std::vector<double> zeros(l_ptr[n], 0.0);
+    copy_levels(cholesky_forward((int64_t)n, l_ptr, l_idx, zeros.data()), forward_level);
+    copy_levels(cholesky_backward((int64_t)n, l_ptr, l_idx, zeros.data()), backward_level);
+    Supernodes s = cholesky_supernodes((int64_t)n, l_ptr, l_idx);
+    if (supernode_start) std::copy(s.start.begin(), s.start.end(), supernode_start);
+    if (supernodes) *supernodes = s.start.size() - 1;
+  });
+}
+
+int32_t ybus_hip_analyze_lu(uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, const uint64_t* u_ptr,
+                            const uint64_t* u_idx, int32_t* forward_level, int32_t* backward_level,
+                            int32_t* transpose_forward_level, int32_t* transpose_backward_level) {
+  return guard([&] {
+    if (!l_ptr || !u_ptr || (!l_idx && l_ptr[n]) || (!u_idx && u_ptr[n]))
+      fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    std::string why;
+    if (!valid_compressed((int64_t)n, (int64_t)n, l_ptr, l_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "L: " + why);
+    if (!valid_compressed((int64_t)n, (int64_t)n, u_ptr, u_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "U: " + why);
+    for (uint64_t j = 0; j < n; ++j)
+      if (l_ptr[j + 1] == l_ptr[j] || l_idx[l_ptr[j]] != j || u_ptr[j + 1] == u_ptr[j] || u_idx[u_ptr[j + 1] - 1] != j)
+        fail(YBUS_HIP_INVALID_ARGUMENT, "L must start and U end each column with its diagonal");
+    std::vector<double> lz(l_ptr[n], 0.0), uz(u_ptr[n], 0.0);
+    copy_levels(lu_forward((int64_t)n, l_ptr, l_idx, lz.data()), forward_level);
+    copy_levels(lu_backward((int64_t)n, u_ptr, u_idx, uz.data()), backward_level);
+    copy_levels(lu_transpose_forward((int64_t)n, u_ptr, u_idx, uz.data()), transpose_forward_level);
+    copy_levels(lu_transpose_backward((int64_t)n, l_ptr, l_idx, lz.data()), transpose_backward_level);
+  });
+}
+
+int32_t ybus_hip_factor_operator(ybus_hip_factor* f, uint64_t rows, const uint64_t* ptr, const uint64_t* idx,
+                                 const double* val, ybus_hip_operator** out) {
+  return guard([&] {
+    if (!f || !ptr || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    std::string why;
+    if (!valid_compressed((int64_t)rows, f->n, ptr, idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "operator: " + why);
+    const uint64_t nnz = ptr[rows];
+    if (nnz > (uint64_t)INDEX_LIMIT || rows > (uint64_t)INDEX_LIMIT)
+      fail(YBUS_HIP_TOO_LARGE, "an operator with " + std::to_string(nnz) + " entries is too large for 32-bit indices");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    auto op = std::make_unique<ybus_hip_operator>();
+    op->f = f;
+    op->m.rows = (int32_t)rows;
+    op->m.cols = (int32_t)f->n;
+    op->m.nnz = nnz;
+    op->m.ptr.upload(to_i32(ptr, rows + 1, nnz), f->stream());
+    op->m.col.upload(to_i32(idx, nnz, nnz), f->stream());
+    op->m.val.upload(std::vector<double>(val, val + nnz), f->stream());
+    op->offset.upload(std::vector<double>(std::max<uint64_t>(rows, 1), 0.0), f->stream());
+    op->weight.upload(std::vector<double>(std::max<uint64_t>(rows, 1), 1.0), f->stream());
+    check(hipStreamSynchronize(f->stream()), "operator");
+    *out = op.release();
+  });
+}
+
+void ybus_hip_operator_free(ybus_hip_operator* op) { delete op; }
+
+uint64_t ybus_hip_operator_rows(const ybus_hip_operator* op) { return op ? (uint64_t)op->m.rows : 0; }
+
+int32_t ybus_hip_operator_set_offset(ybus_hip_operator* op, const double* offset) {
+  return guard([&] {
+    if (!op || (!offset && op->m.rows)) fail(YBUS_HIP_INVALID_ARGUMENT, "one offset per row");
+    if (!op->m.rows) return;
+    check(hipMemcpy(op->offset.p, offset, op->m.rows * sizeof(double), hipMemcpyHostToDevice), "H2D");
+  });
+}
+
+int32_t ybus_hip_operator_set_weights(ybus_hip_operator* op, const double* weights) {
+  return guard([&] {
+    if (!op || (!weights && op->m.rows)) fail(YBUS_HIP_INVALID_ARGUMENT, "one weight per row");
+    if (!op->m.rows) return;
+    check(hipMemcpy(op->weight.p, weights, op->m.rows * sizeof(double), hipMemcpyHostToDevice), "H2D");
+  });
+}
+
+}  // extern "C"
