@@ -205,8 +205,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the factor-solve path is measured per GPU "
-              "(see DESIGN.md for how it shards); run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the factor-solve path is measured on one GPU's share "
+              "of the work (right-hand-side blocks shard across GPUs with no exchange); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import ybus_amd as Y

@@ -78,7 +78,10 @@ typedef struct {
 typedef struct {
   uint64_t launches;
   double milliseconds;
-  uint64_t bytes; /* algorithmic bytes of those launches (see DESIGN.md's traffic model) */
+  /* algorithmic bytes of those launches: per FP64 block of c columns, the factor's entries once
+   * (8-byte value and 4-byte index each, S = entries of both sweeps) and row pointers, plus 16
+   * bytes per column entry (right-hand side in, solution out): S * 12 + 8 (n + 1) + 16 n c */
+  uint64_t bytes;
 } ybus_hip_kernel_stats;
 
 /* The message of the calling thread's last failure (ybus_ffi/src/lib.rs:54-80): copies at most
@@ -207,6 +210,14 @@ int32_t ybus_hip_block_product(ybus_hip_block *b, ybus_hip_operator *op, const i
 /* GpuBlock::product_maxima (pairs.rs:396-413): product reduced to |value| * weight maxima. */
 int32_t ybus_hip_block_product_maxima(ybus_hip_block *b, ybus_hip_operator *op,
                                       const int64_t *zero, ybus_hip_column_max *out);
+/* Which kernels solve this factor's blocks (chosen at upload from dim() and the device's LDS):
+ * YBUS_HIP_PATH_COLUMNS, one workgroup per right-hand side with its solution in LDS and blocks
+ * stored a column (a right-hand side) at a time; or YBUS_HIP_PATH_WIDE, tiles of columns in
+ * global memory with row-interleaved blocks. Setting the environment variable YBUS_HIP_PATH to
+ * "wide" before an upload forces the wide path. */
+enum { YBUS_HIP_PATH_COLUMNS = 0, YBUS_HIP_PATH_WIDE = 1 };
+int32_t ybus_hip_factor_path(const ybus_hip_factor *f);
+
 /* Kernel statistics of this factor's device solves (HIP events), and their reset. */
 void ybus_hip_factor_kernel_stats(const ybus_hip_factor *f, ybus_hip_kernel_stats *out);
 void ybus_hip_factor_reset_kernel_stats(ybus_hip_factor *f);

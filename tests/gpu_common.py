@@ -15,3 +15,14 @@ def gpu():
             pytest.skip("no HIP device")
         raise RuntimeError("no HIP device: the gpu tests need an MI355X")
     return g
+
+
+@pytest.fixture(params=["columns", "wide"])
+def path(request, monkeypatch):
+    """Runs a test on both kernel paths: factors uploaded while YBUS_HIP_PATH=wide take the wide
+    (global-memory tile) path, the others the LDS column path where the factor fits."""
+    if request.param == "wide":
+        monkeypatch.setenv("YBUS_HIP_PATH", "wide")
+    else:
+        monkeypatch.delenv("YBUS_HIP_PATH", raising=False)
+    return request.param

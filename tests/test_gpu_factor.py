@@ -10,9 +10,9 @@ import oracle as O
 import ybus_amd as Y
 from tests.fixtures import (random_spd, random_square_cuda, right_hand_sides, two_by_two_blocks,
                             worst_relative_difference)
-from tests.gpu_common import gpu  # noqa: F401
+from tests.gpu_common import gpu, path  # noqa: F401
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("path")]
 
 BAR = 1e-12
 
@@ -113,13 +113,16 @@ def test_fp32_refined_solves_reach_the_tolerance(gpu, kind):
     factor.set_matrix(a)
     rhs = right_hand_sides(a.rows, 70, 12)
     x, steps, ratio = factor.solve_many_refined(rhs, 1e-12, 10)
+    cpu = f.solve_many(rhs)
     for k in range(rhs.shape[0]):
         # the reported ratio is the relative residual of the returned x, by the CPU's own
         # residual (the device sums A x in the same order, so the bits agree)
         assert ratio[k] == a.relative_residual(x[k], rhs[k])
-        assert ratio[k] <= 1e-12, (k, ratio[k], steps[k])
+        # 1e-12, unless the column's FP64 floor is above it: ||b - fl(A x)|| / ||b|| cannot
+        # drop below the rounding of A x itself, which the CPU's FP64 solve also meets
+        floor = a.relative_residual(cpu[k], rhs[k])
+        assert ratio[k] <= max(1e-12, 2.0 * floor), (k, ratio[k], floor, steps[k])
         assert 1 <= steps[k] <= 10
-    cpu = f.solve_many(rhs)
     assert worst_relative_difference(x, cpu) < 1e-9
 
 
@@ -141,9 +144,7 @@ def test_staged_blocks_solve_on_the_device(gpu):
     # a block borrows its factor until freed
     block = factor.solve_staged(Y.FP64)
     with pytest.raises(Y.InvalidArgument):
-        Y.library()  # the borrow is enforced by the C ABI:
-        st = Y.library().ybus_hip_factor_stage(factor._h, rhs, 1)
-        Y._check(st)
+        Y._check(Y.library().ybus_hip_factor_stage(factor._h, rhs, 1))
     block.free()
 
 
@@ -158,7 +159,16 @@ def test_synthetic_dc_grids_match_the_cpu_bit_for_bit(gpu, buses):
     cpu = chol.solve_many(rhs, threads=4)
     assert got.tobytes() == cpu.tobytes()
     for k in range(0, 96, 17):
-        assert bp.relative_residual(got[k], rhs[k]) <= 1e-12
+        # backward stable: the residual is within a small multiple of the rounding of A x, the
+        # floor ||b - fl(A x)|| / ||b|| of any FP64 solution (above 1e-12 for these
+        # injections, where |A||x| is far larger than |b|)
+        assert bp.relative_residual(got[k], rhs[k]) <= max(1e-12, fp64_floor(bp, got[k], rhs[k]))
+
+
+def fp64_floor(a, x, b):
+    """32 unit roundoffs of || |A| |x| ||inf / ||b||inf."""
+    abs_ax = O.Matrix(a.rows, a.cols, a.row_major, a.ptr, a.idx, np.abs(a.data)).mul_vec(np.abs(x))
+    return 32 * 2.0**-53 * float(np.max(abs_ax)) / float(np.max(np.abs(b)))
 
 
 def test_synthetic_jacobian_lu_matches_the_cpu(gpu):
@@ -171,4 +181,6 @@ def test_synthetic_jacobian_lu_matches_the_cpu(gpu):
     assert factor.solve_transpose_many(rhs).tobytes() == lu.solve_transpose_many(rhs, threads=4).tobytes()
     factor.set_matrix(jac)
     x, steps, ratio = factor.solve_many_refined(rhs, 1e-12, 10)
-    assert np.all(ratio <= 1e-12)
+    cpu = lu.solve_many(rhs, threads=4)
+    floors = np.array([jac.relative_residual(cpu[k], rhs[k]) for k in range(64)])
+    assert np.all(ratio <= np.maximum(1e-12, 2.0 * floors)), (ratio.max(), floors.max())

@@ -9,9 +9,9 @@ import oracle as O
 import ybus_amd as Y
 from tests.conftest import Lcg
 from tests.fixtures import random_spd, random_square_cuda
-from tests.gpu_common import gpu  # noqa: F401
+from tests.gpu_common import gpu, path  # noqa: F401
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("path")]
 
 BAR = 1e-12
 

@@ -38,6 +38,32 @@ struct RhsDev {
   const int32_t* minus = nullptr;
 };
 
+// Right-hand sides of a column-major block (column c at c * ld, by unknown), or pairs built on
+// the fly when `block` is null.
+struct ColumnsDev {
+  const double* block = nullptr;
+  int64_t ld = 0;
+  const int32_t* plus = nullptr;
+  const int32_t* minus = nullptr;
+};
+
+// LDS bytes the column solves need for n rows (and `threads` per workgroup for refinement).
+size_t solve_lds_bytes(int32_t n, bool fp32);
+size_t refine_lds_bytes(int32_t n, int threads);
+
+// One workgroup per column, the solution in LDS in factor order: rhs_row[u] is the factor row
+// holding right-hand side entry u, sol_of[u] the factor row holding solution entry u; the
+// sweeps are in factor order. x is column-major (ldx). fp32 sweeps in FP32 (to FP64 output).
+hipError_t launch_solve_lds(hipStream_t stream, int threads, bool fp32, const SweepDev& fwd, const SweepDev& bwd,
+                            int32_t n, const int32_t* rhs_row, const int32_t* sol_of, const ColumnsDev& rhs,
+                            double* x, int64_t ldx, int32_t ncols);
+
+// FP32 sweeps in LDS refined in FP64, one workgroup per column, blocks column-major (ld).
+hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fwd, const SweepDev& bwd,
+                             const CsrDev& a, const int32_t* rhs_row, const int32_t* sol_of, const double* b,
+                             double* x0, double* x1, double* r, int64_t ld, int32_t ncols, double tol,
+                             int32_t max_steps, int32_t* steps, double* ratio);
+
 // FP64 forward then backward sweep over column tiles of width `tile` (1..64, a power of two),
 // `threads` per workgroup (64..1024): t (packed) is the forward result, x the solution.
 hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
@@ -56,18 +82,23 @@ hipError_t launch_to_block(hipStream_t stream, const double* vectors, double* bl
 hipError_t launch_from_block(hipStream_t stream, const double* block, double* vectors, int64_t n, int64_t count,
                              int64_t ld);
 
-// pair_update (ybus_cuda/src/kernels.cu:24-58) on a solved block y (n x ld, by unknown).
-hipError_t launch_pair_update(hipStream_t stream, const double* y, double* out, int32_t n, int32_t ncols, int64_t ld,
-                              const double* base, const int32_t* plus, const int32_t* minus, const double* alpha,
-                              const double* inverse_weight, int32_t* singular);
+// Blocks below are addressed with strides: entry (i, c) at i * rs + c * cs (row-interleaved:
+// rs = ld, cs = 1; column-major: rs = 1, cs = ld).
+
+// pair_update (ybus_cuda/src/kernels.cu:24-58) on a solved block y (by unknown).
+hipError_t launch_pair_update(hipStream_t stream, const double* y, double* out, int32_t n, int32_t ncols, int64_t rs,
+                              int64_t cs, const double* base, const int32_t* plus, const int32_t* minus,
+                              const double* alpha, const double* inverse_weight, int32_t* singular);
 
 // column_max (kernels.cu:78-128): per column, the largest |v| * weight[row] (smallest row on a
 // tie, -1 if every product is zero) and whether any value isn't finite.
-hipError_t launch_column_max(hipStream_t stream, const double* block, int32_t rows, int32_t ncols, int64_t ld,
-                             const double* weight, double* best, int32_t* best_row, int32_t* not_finite);
+hipError_t launch_column_max(hipStream_t stream, const double* block, int32_t rows, int32_t ncols, int64_t rs,
+                             int64_t cs, const double* weight, double* best, int32_t* best_row, int32_t* not_finite);
 
-// out = M x + offset for each column, then out[zero[c]][c] = 0 where zero[c] >= 0.
-hipError_t launch_operator(hipStream_t stream, const CsrDev& m, const double* x, int64_t ldx, const double* offset,
-                           const int32_t* zero, double* out, int64_t ldo, int32_t ncols);
+// out = M x + offset for each column, then out(zero[c], c) = 0 where zero[c] >= 0; out's entry
+// (r, c) at r * ors + c * ocs.
+hipError_t launch_operator(hipStream_t stream, const CsrDev& m, const double* x, int64_t rs, int64_t cs,
+                           const double* offset, const int32_t* zero, double* out, int64_t ors, int64_t ocs,
+                           int32_t ncols);
 
 }  // namespace ybus_hip

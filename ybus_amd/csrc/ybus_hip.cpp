@@ -191,18 +191,37 @@ struct ybus_hip_ctx {
 
 struct ybus_hip_block;
 
+// How a factor's blocks are laid out and solved. Columns: each right-hand side contiguous (the
+// host's own layout, ld = n), solved one workgroup per column with the vector in LDS; for any n
+// whose solution fits in LDS. Wide: row-interleaved (n rows of ld columns), solved by column
+// tiles in global memory; for larger n.
+struct Layout {
+  bool columns = true;
+  int64_t ld = 0;
+  int64_t rs() const { return columns ? 1 : ld; }
+  int64_t cs() const { return columns ? ld : 1; }
+};
+
 struct ybus_hip_factor {
   ybus_hip_ctx* ctx = nullptr;
   int64_t n = 0;
   bool lu = false;
   uint64_t width = 1;
   int64_t nnz_factor = 0;  // S: entries swept by one forward and backward solve
+  bool columns = true;     // the LDS column path (else the wide path)
+  size_t lds_cap = 0;
+  // wide path: schedules reading and writing permuted blocks
   DevSweep fwd, bwd, tfwd, tbwd;
+  // column path: schedules in factor order, and each unknown's factor row for the right-hand
+  // side (rhs_row) and the solution (sol_of), for the solve and the transposed solve
+  DevSweep cfwd, cbwd, ctfwd, ctbwd;
+  DevBuf<int32_t> rhs_row, sol_of, t_rhs_row, t_sol_of;
   bool has_transpose = false;
   DevCsr a;  // the matrix itself, by user rows, for the FP64 residual
-  // Workspace, `cols` columns of ld
-  int64_t cols = 0, ld = 0;
-  DevBuf<double> b, t, x, x1, r, vec;
+  // workspace for `cols` columns
+  int64_t cols = 0;
+  Layout layout;
+  DevBuf<double> b, t, x, x1, r, y2, vec;
   DevBuf<float> t32, d32;
   DevBuf<int32_t> plus, minus, isteps, singular;
   DevBuf<double> dratio;
@@ -227,33 +246,36 @@ struct ybus_hip_factor {
   void ensure_workspace(int64_t ncols, bool refine) {
     staged = 0;
     staged_pairs = false;
+    const int64_t nn = std::max<int64_t>(n, 1);
     if (ncols > cols) {
       cols = ncols;
-      ld = round_up(std::max<int64_t>(ncols, 1), 16);
-      b.release();
-      t.release();
-      x.release();
-      x1.release();
-      r.release();
+      for (DevBuf<double>* d : {&b, &t, &x, &x1, &r, &y2, &vec}) d->release();
       t32.release();
       d32.release();
-      vec.release();
     }
-    const size_t elems = (size_t)std::max<int64_t>(n, 1) * (size_t)ld;
+    layout.columns = columns;
+    layout.ld = columns ? nn : round_up(std::max<int64_t>(cols, 1), 16);
+    const size_t elems = (size_t)nn * (size_t)(columns ? std::max<int64_t>(cols, 1) : layout.ld);
     b.reserve(elems);
-    t.reserve(elems);
     x.reserve(elems);
-    vec.reserve(elems);
-    plus.reserve(ld);
-    minus.reserve(ld);
-    singular.reserve(ld);
-    isteps.reserve(ld);
-    dratio.reserve(ld);
+    y2.reserve(elems);
+    const size_t per = (size_t)std::max<int64_t>(cols, 16);
+    plus.reserve(per);
+    minus.reserve(per);
+    singular.reserve(per);
+    isteps.reserve(per);
+    dratio.reserve(per);
+    if (!columns) {
+      t.reserve(elems);
+      vec.reserve(elems);
+    }
     if (refine) {
       x1.reserve(elems);
       r.reserve(elems);
-      t32.reserve(elems);
-      d32.reserve(elems);
+      if (!columns) {
+        t32.reserve(elems);
+        d32.reserve(elems);
+      }
     }
   }
 
@@ -275,26 +297,53 @@ struct ybus_hip_factor {
 
   // Algorithmic bytes of one FP64 block solve of `ncols` columns: the factor's values and
   // indices swept once, its row pointers, and each column's right-hand side in and solution out
+  // (for pairs, two indices in and the solution out)
   uint64_t bytes64(int64_t ncols, bool pairs) const {
     const uint64_t factor = (uint64_t)nnz_factor * 12 + 8 * (uint64_t)(n + 1);
-    const uint64_t columns = pairs ? (uint64_t)ncols * ((uint64_t)n * 8 + 8) : (uint64_t)ncols * (uint64_t)n * 16;
-    return factor + columns;
+    const uint64_t columns_bytes =
+        pairs ? (uint64_t)ncols * ((uint64_t)n * 8 + 8) : (uint64_t)ncols * (uint64_t)n * 16;
+    return factor + columns_bytes;
   }
 
-  void solve64(const SweepDev& f, const SweepDev& bw, const RhsDev& rhs, int64_t ncols, bool pairs) {
+  // FP64 solve (or transposed solve) of the block: from b, or from pairs, into x
+  void solve64(bool transpose, bool pairs, int64_t ncols) {
+    if (transpose && !has_transpose) transpose = false;  // A^T = A for a Cholesky factor
     begin_kernel();
-    check(launch_solve64(stream(), tile_for(ncols), threads, f, bw, rhs, t.p, x.p, ld, (int32_t)ncols),
-          "solve kernel");
+    if (columns) {
+      const DevSweep& f = transpose ? ctfwd : cfwd;
+      const DevSweep& bw = transpose ? ctbwd : cbwd;
+      const DevBuf<int32_t>& rr = transpose ? t_rhs_row : rhs_row;
+      const DevBuf<int32_t>& so = transpose ? t_sol_of : sol_of;
+      ColumnsDev rhs{pairs ? nullptr : b.p, layout.ld, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
+      check(launch_solve_lds(stream(), threads, false, f.view(), bw.view(), (int32_t)n, rr.p, so.p, rhs, x.p,
+                             layout.ld, (int32_t)ncols),
+            "solve kernel");
+    } else {
+      const DevSweep& f = transpose ? tfwd : fwd;
+      const DevSweep& bw = transpose ? tbwd : bwd;
+      RhsDev rhs{pairs ? nullptr : b.p, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
+      check(launch_solve64(stream(), tile_for(ncols), threads, f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
+                           (int32_t)ncols),
+            "solve kernel");
+    }
     end_kernel(bytes64(ncols, pairs));
   }
 
   void refine(int64_t ncols, double tol, int32_t max_steps) {
     if (!a.ptr.p) fail(YBUS_HIP_INVALID_ARGUMENT, "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix)");
     begin_kernel();
-    check(launch_refine(stream(), tile_for(ncols), threads, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p, x.p,
-                        x1.p, r.p, ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
-          "refine kernel");
-    // bytes: per column, its own corrections k_c; the factor and matrix once per sweep and
+    if (columns) {
+      int th = 1;
+      while (th * 2 <= threads) th *= 2;
+      check(launch_refine_lds(stream(), th, cfwd.view(), cbwd.view(), a.view(), rhs_row.p, sol_of.p, b.p, x.p, x1.p,
+                              r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
+            "refine kernel");
+    } else {
+      check(launch_refine(stream(), tile_for(ncols), threads, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p,
+                          x.p, x1.p, r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
+            "refine kernel");
+    }
+    // bytes: per column, its own corrections k_c; the factor and the matrix once per sweep and
     // residual of the longest column
     std::vector<int32_t> steps(ncols);
     check(hipEventRecord(ev1, stream()), "hipEventRecord");
@@ -319,11 +368,11 @@ struct ybus_hip_factor {
 struct ybus_hip_block {
   ybus_hip_factor* f = nullptr;
   int64_t columns = 0;
-  bool updated = false;  // results in t (pair updates) rather than x
+  const double* data = nullptr;
+  Layout layout;
   std::vector<uint8_t> singular;
   std::vector<uint64_t> steps;
   std::vector<double> ratio;
-  const double* data() const { return updated ? f->t.p : f->x.p; }
 };
 
 struct ybus_hip_operator {
@@ -333,6 +382,18 @@ struct ybus_hip_operator {
   DevBuf<int32_t> zero, best_row, not_finite;
   int64_t out_cols = 0;
 };
+
+static std::unique_ptr<ybus_hip_block> new_block(ybus_hip_factor* f, int64_t columns, const double* data) {
+  auto block = std::make_unique<ybus_hip_block>();
+  block->f = f;
+  block->columns = columns;
+  block->data = data;
+  block->layout = f->layout;
+  block->singular.assign(columns, 0);
+  block->steps.assign(columns, 0);
+  block->ratio.assign(columns, 0.0);
+  return block;
+}
 
 // ---- C ABI ----
 
@@ -391,20 +452,46 @@ int32_t ybus_hip_ctx_memory(ybus_hip_ctx* ctx, uint64_t* free_bytes, uint64_t* t
   });
 }
 
+// The largest dynamic LDS a workgroup may use on the context's device
+static size_t lds_capacity(const ybus_hip_ctx* ctx) {
+  int bytes = 0;
+  if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess || bytes <= 0)
+    bytes = 64 * 1024;
+  return (size_t)bytes;
+}
+
+// Chooses the path (the solution of one column in LDS, or wide tiles) and sets the defaults
+static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
+  f->ctx = ctx;
+  f->n = n;
+  const char* thr = std::getenv("YBUS_HIP_THREADS");
+  if (thr) f->threads = std::clamp(std::atoi(thr), 64, 1024) / 64 * 64;
+  const char* tile = std::getenv("YBUS_HIP_TILE");
+  if (tile) f->tile_override = std::atoi(tile);
+  f->lds_cap = lds_capacity(ctx);
+  const char* path = std::getenv("YBUS_HIP_PATH");
+  f->columns = solve_lds_bytes((int32_t)std::max<int64_t>(n, 1), false) <= f->lds_cap &&
+               refine_lds_bytes((int32_t)std::max<int64_t>(n, 1), f->threads) <= f->lds_cap;
+  if (path && std::string(path) == "wide") f->columns = false;
+}
+
 static void finish_factor(ybus_hip_factor* f) {
   check(hipEventCreate(&f->ev0), "hipEventCreate");
   check(hipEventCreate(&f->ev1), "hipEventCreate");
   // factor.rs:158-160: two blocks of n doubles per column in a quarter of the free memory;
-  // here clamped at 32768 columns, enough tiles to fill every compute unit many times over
+  // here clamped at 32768 columns, enough workgroups to fill every compute unit many times over
   size_t free_b = 0, total_b = 0;
   check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
   uint64_t w = free_b / 4 / (2 * 8 * (uint64_t)std::max<int64_t>(f->n, 1));
   f->width = std::clamp<uint64_t>(w, 1, MAX_DEFAULT_WIDTH);
   ybus_hip_factor_set_block_width(f, f->width);
-  const char* env = std::getenv("YBUS_HIP_TILE");
-  if (env) f->tile_override = std::atoi(env);
-  const char* thr = std::getenv("YBUS_HIP_THREADS");
-  if (thr) f->threads = std::atoi(thr);
+}
+
+// Each unknown's factor row: the inverse of a gather (or scatter) permutation
+static std::vector<int32_t> inverse_i32(const std::vector<int64_t>& perm) {
+  std::vector<int32_t> inv(perm.size());
+  for (size_t i = 0; i < perm.size(); ++i) inv[perm[i]] = (int32_t)i;
+  return inv;
 }
 
 int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx,
@@ -425,8 +512,7 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
     }
     check(hipSetDevice(ctx->device), "hipSetDevice");
     auto f = std::make_unique<ybus_hip_factor>();
-    f->ctx = ctx;
-    f->n = (int64_t)n;
+    begin_factor(f.get(), ctx, (int64_t)n);
     f->lu = false;
     f->nnz_factor = 2 * (int64_t)entries;
     std::vector<int64_t> gather(n), ident(n);
@@ -434,8 +520,19 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
       gather[i] = (int64_t)perm[i];
       ident[i] = (int64_t)i;
     }
-    f->fwd.upload(schedule(cholesky_forward(n, l_ptr, l_idx, l_val), gather, ident), ctx->stream);
-    f->bwd.upload(schedule(cholesky_backward(n, l_ptr, l_idx, l_val), ident, gather), ctx->stream);
+    const RowForm lower = cholesky_forward(n, l_ptr, l_idx, l_val);
+    const RowForm upper = cholesky_backward(n, l_ptr, l_idx, l_val);
+    if (f->columns) {
+      // cholesky.rs:570-579: y[i] = b[perm[i]], and x[perm[i]] = y[i]
+      f->cfwd.upload(schedule(lower, ident, ident), ctx->stream);
+      f->cbwd.upload(schedule(upper, ident, ident), ctx->stream);
+      f->rhs_row.upload(inverse_i32(gather), ctx->stream);
+      f->sol_of.upload(inverse_i32(gather), ctx->stream);
+    } else {
+      f->fwd.upload(schedule(lower, gather, ident), ctx->stream);
+      f->bwd.upload(schedule(upper, ident, gather), ctx->stream);
+    }
+    check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
     *out = f.release();
   });
@@ -465,8 +562,7 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
     }
     check(hipSetDevice(ctx->device), "hipSetDevice");
     auto f = std::make_unique<ybus_hip_factor>();
-    f->ctx = ctx;
-    f->n = (int64_t)n;
+    begin_factor(f.get(), ctx, (int64_t)n);
     f->lu = true;
     f->nnz_factor = (int64_t)entries;
     std::vector<int64_t> rp(n), cp(n), ident(n);
@@ -475,11 +571,26 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
       cp[i] = (int64_t)col_perm[i];
       ident[i] = (int64_t)i;
     }
-    f->fwd.upload(schedule(lu_forward(n, l_ptr, l_idx, l_val), rp, ident), ctx->stream);
-    f->bwd.upload(schedule(lu_backward(n, u_ptr, u_idx, u_val), ident, cp), ctx->stream);
-    f->tfwd.upload(schedule(lu_transpose_forward(n, u_ptr, u_idx, u_val), cp, ident), ctx->stream);
-    f->tbwd.upload(schedule(lu_transpose_backward(n, l_ptr, l_idx, l_val), ident, rp), ctx->stream);
+    const RowForm l = lu_forward(n, l_ptr, l_idx, l_val), u = lu_backward(n, u_ptr, u_idx, u_val);
+    const RowForm ut = lu_transpose_forward(n, u_ptr, u_idx, u_val), lt = lu_transpose_backward(n, l_ptr, l_idx, l_val);
+    if (f->columns) {
+      // lu.rs:770-785: y[i] = b[P[i]], x[Q[i]] = y[i]; transposed (lu.rs:506-533): through Q, then P
+      f->cfwd.upload(schedule(l, ident, ident), ctx->stream);
+      f->cbwd.upload(schedule(u, ident, ident), ctx->stream);
+      f->ctfwd.upload(schedule(ut, ident, ident), ctx->stream);
+      f->ctbwd.upload(schedule(lt, ident, ident), ctx->stream);
+      f->rhs_row.upload(inverse_i32(rp), ctx->stream);
+      f->sol_of.upload(inverse_i32(cp), ctx->stream);
+      f->t_rhs_row.upload(inverse_i32(cp), ctx->stream);
+      f->t_sol_of.upload(inverse_i32(rp), ctx->stream);
+    } else {
+      f->fwd.upload(schedule(l, rp, ident), ctx->stream);
+      f->bwd.upload(schedule(u, ident, cp), ctx->stream);
+      f->tfwd.upload(schedule(ut, cp, ident), ctx->stream);
+      f->tbwd.upload(schedule(lt, ident, rp), ctx->stream);
+    }
     f->has_transpose = true;
+    check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
     *out = f.release();
   });
@@ -505,6 +616,10 @@ void ybus_hip_factor_reset_timings(ybus_hip_factor* f) {
   if (f) f->timings = ybus_hip_timings{};
 }
 
+int32_t ybus_hip_factor_path(const ybus_hip_factor* f) {
+  return f && !f->columns ? YBUS_HIP_PATH_WIDE : YBUS_HIP_PATH_COLUMNS;
+}
+
 void ybus_hip_factor_kernel_stats(const ybus_hip_factor* f, ybus_hip_kernel_stats* out) {
   if (f && out) *out = f->stats;
 }
@@ -521,6 +636,7 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
     std::string why;
     if (!valid_compressed(n, n, ptr, idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "A: " + why);
     const uint64_t nnz = ptr[n];
+    if (nnz && (!idx || !val)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     if (nnz > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the matrix is too large for 32-bit indices");
     std::vector<int32_t> rptr(n + 1), cols(nnz);
     std::vector<double> vals(nnz);
@@ -554,27 +670,47 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
   });
 }
 
-// Copies count host vectors into the device block b (count <= f->cols)
+// Copies count host vectors into the device block b, in the factor's layout
 static void upload_vectors(ybus_hip_factor* f, const double* rhs, int64_t count) {
   const size_t len = (size_t)count * (size_t)f->n;
   double* host = f->staging.get(len);
   std::memcpy(host, rhs, len * sizeof(double));
-  check(hipMemcpyAsync(f->vec.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
-  check(launch_to_block(f->stream(), f->vec.p, f->b.p, f->n, count, f->ld), "to_block");
+  if (f->layout.columns) {
+    check(hipMemcpyAsync(f->b.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+  } else {
+    check(hipMemcpyAsync(f->vec.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+    check(launch_to_block(f->stream(), f->vec.p, f->b.p, f->n, count, f->layout.ld), "to_block");
+  }
   check(hipStreamSynchronize(f->stream()), "upload");
 }
 
-static void download_vectors(ybus_hip_factor* f, const double* block, int64_t count, double* out) {
+static void download_vectors(ybus_hip_factor* f, const double* block, const Layout& layout, int64_t count,
+                             double* out) {
   const size_t len = (size_t)count * (size_t)f->n;
   double* host = f->staging.get(len);
-  check(launch_from_block(f->stream(), block, f->vec.p, f->n, count, f->ld), "from_block");
-  check(hipMemcpyAsync(host, f->vec.p, len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  if (layout.columns) {
+    check(hipMemcpyAsync(host, block, len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  } else {
+    check(launch_from_block(f->stream(), block, f->vec.p, f->n, count, layout.ld), "from_block");
+    check(hipMemcpyAsync(host, f->vec.p, len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  }
   check(hipStreamSynchronize(f->stream()), "download");
   std::memcpy(out, host, len * sizeof(double));
 }
 
 static void require_unborrowed(const ybus_hip_factor* f) {
   if (f->borrowed) fail(YBUS_HIP_INVALID_ARGUMENT, "a block borrows this factor: free it first");
+}
+
+static void read_refinement(ybus_hip_factor* f, int64_t count, uint64_t* steps, double* ratio) {
+  std::vector<int32_t> st(count);
+  std::vector<double> ra(count);
+  check(hipMemcpy(st.data(), f->isteps.p, count * sizeof(int32_t), hipMemcpyDeviceToHost), "steps");
+  check(hipMemcpy(ra.data(), f->dratio.p, count * sizeof(double), hipMemcpyDeviceToHost), "ratio");
+  for (int64_t c = 0; c < count; ++c) {
+    if (steps) steps[c] = (uint64_t)st[c];
+    if (ratio) ratio[c] = ra[c];
+  }
 }
 
 static int32_t solve_many_mode(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x, int mode,
@@ -585,36 +721,24 @@ static int32_t solve_many_mode(ybus_hip_factor* f, const double* rhs, uint64_t c
     if (count == 0 || f->n == 0) return;
     if (!rhs || !x) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    const bool refine = mode == YBUS_HIP_FP32_REFINED;
+    if (refine && !f->a.ptr.p) fail(YBUS_HIP_INVALID_ARGUMENT, "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix)");
     check(hipSetDevice(f->ctx->device), "hipSetDevice");
     const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
-    const bool refine = mode == YBUS_HIP_FP32_REFINED;
     f->ensure_workspace(width, refine);
     for (uint64_t start = 0; start < count; start += (uint64_t)width) {
       const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
-      const double* in = rhs + start * (uint64_t)f->n;
-      double* outp = x + start * (uint64_t)f->n;
       auto t0 = std::chrono::steady_clock::now();
-      upload_vectors(f, in, chunk);
+      upload_vectors(f, rhs + start * (uint64_t)f->n, chunk);
       auto t1 = std::chrono::steady_clock::now();
-      if (mode == YBUS_HIP_FP64) {
-        f->solve64(f->fwd.view(), f->bwd.view(), RhsDev{f->b.p, nullptr, nullptr}, chunk, false);
-      } else if (mode == YBUS_HIP_TRANSPOSE) {
-        const DevSweep& tf = f->has_transpose ? f->tfwd : f->fwd;
-        const DevSweep& tb = f->has_transpose ? f->tbwd : f->bwd;
-        f->solve64(tf.view(), tb.view(), RhsDev{f->b.p, nullptr, nullptr}, chunk, false);
-      } else {
+      if (refine) {
         f->refine(chunk, tol, (int32_t)max_steps);
-        std::vector<int32_t> st(chunk);
-        std::vector<double> ra(chunk);
-        check(hipMemcpy(st.data(), f->isteps.p, chunk * sizeof(int32_t), hipMemcpyDeviceToHost), "steps");
-        check(hipMemcpy(ra.data(), f->dratio.p, chunk * sizeof(double), hipMemcpyDeviceToHost), "ratio");
-        for (int64_t c = 0; c < chunk; ++c) {
-          if (steps) steps[start + c] = (uint64_t)st[c];
-          if (ratio) ratio[start + c] = ra[c];
-        }
+        read_refinement(f, chunk, steps ? steps + start : nullptr, ratio ? ratio + start : nullptr);
+      } else {
+        f->solve64(mode == YBUS_HIP_TRANSPOSE, false, chunk);
       }
       auto t2 = std::chrono::steady_clock::now();
-      download_vectors(f, f->x.p, chunk, outp);
+      download_vectors(f, f->x.p, f->layout, chunk, x + start * (uint64_t)f->n);
       f->timings.upload += std::chrono::duration<double>(t1 - t0).count();
       f->timings.solve += std::chrono::duration<double>(t2 - t1).count();
       f->timings.download += seconds_since(t2);
@@ -633,10 +757,6 @@ int32_t ybus_hip_factor_solve_transpose_many(ybus_hip_factor* f, const double* r
 
 int32_t ybus_hip_factor_solve_many_refined(ybus_hip_factor* f, const double* rhs, uint64_t count, double tolerance,
                                            uint64_t max_steps, double* x, uint64_t* steps, double* ratio) {
-  if (f && !f->a.ptr.p) {
-    last_error = "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix)";
-    return YBUS_HIP_INVALID_ARGUMENT;
-  }
   return solve_many_mode(f, rhs, count, x, YBUS_HIP_FP32_REFINED, tolerance, max_steps, steps, ratio);
 }
 
@@ -651,45 +771,6 @@ int32_t ybus_hip_factor_stage(ybus_hip_factor* f, const double* rhs, uint64_t co
     if (count && f->n) upload_vectors(f, rhs, (int64_t)count);
     f->staged = (int64_t)count;
     f->staged_pairs = false;
-  });
-}
-
-int32_t ybus_hip_factor_solve_staged(ybus_hip_factor* f, int32_t mode, double tolerance, uint64_t max_steps,
-                                     ybus_hip_block** out) {
-  return guard([&] {
-    if (!f || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
-    require_unborrowed(f);
-    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_FP32_REFINED && mode != YBUS_HIP_TRANSPOSE)
-      fail(YBUS_HIP_INVALID_ARGUMENT, "unknown mode");
-    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
-    auto block = std::make_unique<ybus_hip_block>();
-    block->f = f;
-    block->columns = f->staged;
-    block->singular.assign(f->staged, 0);
-    block->steps.assign(f->staged, 0);
-    block->ratio.assign(f->staged, 0.0);
-    if (f->staged_pairs && mode != YBUS_HIP_FP64)
-      fail(YBUS_HIP_INVALID_ARGUMENT, "staged pairs are solved in YBUS_HIP_FP64 only");
-    if (f->staged && f->n) {
-      check(hipSetDevice(f->ctx->device), "hipSetDevice");
-      if (f->staged_pairs) {
-        f->solve64(f->fwd.view(), f->bwd.view(), RhsDev{nullptr, f->plus.p, f->minus.p}, f->staged, true);
-      } else if (mode == YBUS_HIP_FP64) {
-        f->solve64(f->fwd.view(), f->bwd.view(), RhsDev{f->b.p, nullptr, nullptr}, f->staged, false);
-      } else if (mode == YBUS_HIP_TRANSPOSE) {
-        const DevSweep& tf = f->has_transpose ? f->tfwd : f->fwd;
-        const DevSweep& tb = f->has_transpose ? f->tbwd : f->bwd;
-        f->solve64(tf.view(), tb.view(), RhsDev{f->b.p, nullptr, nullptr}, f->staged, false);
-      } else {
-        f->refine(f->staged, tolerance, (int32_t)max_steps);
-        std::vector<int32_t> st(f->staged);
-        check(hipMemcpy(st.data(), f->isteps.p, f->staged * sizeof(int32_t), hipMemcpyDeviceToHost), "steps");
-        check(hipMemcpy(block->ratio.data(), f->dratio.p, f->staged * sizeof(double), hipMemcpyDeviceToHost), "ratio");
-        for (int64_t c = 0; c < f->staged; ++c) block->steps[c] = (uint64_t)st[c];
-      }
-    }
-    f->borrowed = block.get();
-    *out = block.release();
   });
 }
 
@@ -717,13 +798,6 @@ static void stage_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* 
   }
 }
 
-// Checks the pairs and solves them into the block's first buffer (x)
-static void solve_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* minus, uint64_t k) {
-  stage_pairs(f, plus, minus, k);
-  if (!k || !f->n) return;
-  f->solve64(f->fwd.view(), f->bwd.view(), RhsDev{nullptr, f->plus.p, f->minus.p}, (int64_t)k, true);
-}
-
 int32_t ybus_hip_factor_stage_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* minus, uint64_t k) {
   return guard([&] {
     if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
@@ -733,17 +807,38 @@ int32_t ybus_hip_factor_stage_pairs(ybus_hip_factor* f, const int64_t* plus, con
   });
 }
 
+int32_t ybus_hip_factor_solve_staged(ybus_hip_factor* f, int32_t mode, double tolerance, uint64_t max_steps,
+                                     ybus_hip_block** out) {
+  return guard([&] {
+    if (!f || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    require_unborrowed(f);
+    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_FP32_REFINED && mode != YBUS_HIP_TRANSPOSE)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "unknown mode");
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    if (f->staged_pairs && mode != YBUS_HIP_FP64)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "staged pairs are solved in YBUS_HIP_FP64 only");
+    auto block = new_block(f, f->staged, f->x.p);
+    if (f->staged && f->n) {
+      check(hipSetDevice(f->ctx->device), "hipSetDevice");
+      if (mode == YBUS_HIP_FP32_REFINED) {
+        f->refine(f->staged, tolerance, (int32_t)max_steps);
+        read_refinement(f, f->staged, block->steps.data(), block->ratio.data());
+      } else {
+        f->solve64(mode == YBUS_HIP_TRANSPOSE, f->staged_pairs, f->staged);
+      }
+    }
+    f->borrowed = block.get();
+    *out = block.release();
+  });
+}
+
 int32_t ybus_hip_factor_pair_solves(ybus_hip_factor* f, const int64_t* plus, const int64_t* minus, uint64_t k,
                                     ybus_hip_block** out) {
   return guard([&] {
     if (!f || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
-    solve_pairs(f, plus, minus, k);
-    auto block = std::make_unique<ybus_hip_block>();
-    block->f = f;
-    block->columns = (int64_t)k;
-    block->singular.assign(k, 0);
-    block->steps.assign(k, 0);
-    block->ratio.assign(k, 0.0);
+    stage_pairs(f, plus, minus, k);
+    if (k && f->n) f->solve64(false, true, (int64_t)k);
+    auto block = new_block(f, (int64_t)k, f->x.p);
     f->borrowed = block.get();
     *out = block.release();
   });
@@ -756,22 +851,17 @@ int32_t ybus_hip_factor_pair_updates(ybus_hip_factor* f, const double* base, con
     if (!f || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     if (!base && f->n) fail(YBUS_HIP_INVALID_ARGUMENT, "one base value per unknown");
     if (k && (!alpha || !inverse_weight)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
-    solve_pairs(f, plus, minus, k);
-    auto block = std::make_unique<ybus_hip_block>();
-    block->f = f;
-    block->columns = (int64_t)k;
-    block->updated = true;
-    block->singular.assign(k, 0);
-    block->steps.assign(k, 0);
-    block->ratio.assign(k, 0.0);
+    stage_pairs(f, plus, minus, k);
+    if (k && f->n) f->solve64(false, true, (int64_t)k);
+    auto block = new_block(f, (int64_t)k, f->y2.p);
     if (k && f->n) {
       DevBuf<double> dbase, dalpha, dweight;
       dbase.upload(std::vector<double>(base, base + f->n), f->stream());
       dalpha.upload(std::vector<double>(alpha, alpha + k), f->stream());
       dweight.upload(std::vector<double>(inverse_weight, inverse_weight + k), f->stream());
       check(hipMemsetAsync(f->singular.p, 0, k * sizeof(int32_t), f->stream()), "memset");
-      check(launch_pair_update(f->stream(), f->x.p, f->t.p, (int32_t)f->n, (int32_t)k, f->ld, dbase.p, f->plus.p,
-                               f->minus.p, dalpha.p, dweight.p, f->singular.p),
+      check(launch_pair_update(f->stream(), f->x.p, f->y2.p, (int32_t)f->n, (int32_t)k, f->layout.rs(),
+                               f->layout.cs(), dbase.p, f->plus.p, f->minus.p, dalpha.p, dweight.p, f->singular.p),
             "pair_update");
       std::vector<int32_t> s(k);
       check(hipMemcpyAsync(s.data(), f->singular.p, k * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
@@ -811,18 +901,19 @@ int32_t ybus_hip_block_solutions(ybus_hip_block* b, double* out) {
     if (!b || (!out && b->columns)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     if (!b->columns || !b->f->n) return;
     check(hipSetDevice(b->f->ctx->device), "hipSetDevice");
-    download_vectors(b->f, b->data(), b->columns, out);
+    download_vectors(b->f, b->data, b->layout, b->columns, out);
   });
 }
 
-static void maxima_of(ybus_hip_factor* f, const double* block, int32_t rows, int64_t ld, const double* weight,
-                      int64_t columns, ybus_hip_column_max* out) {
+static void maxima_of(ybus_hip_factor* f, const double* block, int32_t rows, int64_t rs, int64_t cs,
+                      const double* weight, int64_t columns, ybus_hip_column_max* out) {
   DevBuf<double> best;
   DevBuf<int32_t> row, bad;
   best.reserve(columns);
   row.reserve(columns);
   bad.reserve(columns);
-  check(launch_column_max(f->stream(), block, rows, (int32_t)columns, ld, weight, best.p, row.p, bad.p), "column_max");
+  check(launch_column_max(f->stream(), block, rows, (int32_t)columns, rs, cs, weight, best.p, row.p, bad.p),
+        "column_max");
   std::vector<double> hb(columns);
   std::vector<int32_t> hr(columns), hn(columns);
   check(hipMemcpyAsync(hb.data(), best.p, columns * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
@@ -844,11 +935,12 @@ int32_t ybus_hip_block_maxima(ybus_hip_block* b, ybus_hip_column_max* out) {
     check(hipSetDevice(f->ctx->device), "hipSetDevice");
     DevBuf<double> ones;
     ones.upload(std::vector<double>(f->n, 1.0), f->stream());
-    maxima_of(f, b->data(), (int32_t)f->n, f->ld, ones.p, b->columns, out);
+    maxima_of(f, b->data, (int32_t)f->n, b->layout.rs(), b->layout.cs(), ones.p, b->columns, out);
   });
 }
 
-// pairs.rs:312-366: M theta + offset per column, then the zeroed entries, left in op->out
+// pairs.rs:312-366: M theta + offset per column, then the zeroed entries, left in op->out in the
+// block's layout (column-major blocks: one vector of rows per column; wide: rows x columns)
 static void apply(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero) {
   ybus_hip_factor* f = b->f;
   if (op->f != f) fail(YBUS_HIP_INVALID_ARGUMENT, "an operator of this factorization");
@@ -863,10 +955,12 @@ static void apply(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero)
   op->zero.upload(z, f->stream());
   op->out.reserve((size_t)std::max<int32_t>(op->m.rows, 1) * (size_t)std::max<int64_t>(b->columns, 1));
   op->out_cols = b->columns;
-  if (b->columns && op->m.rows)
-    check(launch_operator(f->stream(), op->m.view(), b->data(), f->ld, op->offset.p, op->zero.p, op->out.p,
-                          b->columns, (int32_t)b->columns),
+  if (b->columns && op->m.rows) {
+    const int64_t ors = b->layout.columns ? 1 : b->columns, ocs = b->layout.columns ? op->m.rows : 1;
+    check(launch_operator(f->stream(), op->m.view(), b->data, b->layout.rs(), b->layout.cs(), op->offset.p,
+                          op->zero.p, op->out.p, ors, ocs, (int32_t)b->columns),
           "operator");
+  }
 }
 
 int32_t ybus_hip_block_product(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero, double* out) {
@@ -876,7 +970,14 @@ int32_t ybus_hip_block_product(ybus_hip_block* b, ybus_hip_operator* op, const i
     const int64_t rows = op->m.rows, cols = b->columns;
     if (!rows || !cols) return;
     if (!out) fail(YBUS_HIP_INVALID_ARGUMENT, "out is null");
-    // out is row-interleaved (rows x cols); the caller gets one vector of rows per column
+    // the caller gets one vector of rows per column
+    if (b->layout.columns) {
+      check(hipMemcpyAsync(out, op->out.p, (size_t)rows * cols * sizeof(double), hipMemcpyDeviceToHost,
+                           b->f->stream()),
+            "D2H");
+      check(hipStreamSynchronize(b->f->stream()), "product");
+      return;
+    }
     std::vector<double> host((size_t)rows * cols);
     check(hipMemcpyAsync(host.data(), op->out.p, host.size() * sizeof(double), hipMemcpyDeviceToHost,
                          b->f->stream()),
@@ -898,7 +999,10 @@ int32_t ybus_hip_block_product_maxima(ybus_hip_block* b, ybus_hip_operator* op, 
       for (int64_t c = 0; c < b->columns; ++c) out[c] = ybus_hip_column_max{-1, 0.0, 0};
       return;
     }
-    maxima_of(b->f, op->out.p, op->m.rows, b->columns, op->weight.p, b->columns, out);
+    if (b->layout.columns)
+      maxima_of(b->f, op->out.p, op->m.rows, 1, op->m.rows, op->weight.p, b->columns, out);
+    else
+      maxima_of(b->f, op->out.p, op->m.rows, b->columns, 1, op->weight.p, b->columns, out);
   });
 }
 
