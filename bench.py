@@ -68,9 +68,13 @@ def cpu_threads() -> int:
     return max(1, min(16, os.cpu_count() or 1))
 
 
+CPU_SECONDS = 10.0
+
+
 def cpu_pair_baseline(chol, plus, minus, sample: int):
     """par_solve_many on `sample` of the pair right-hand sides (dense vectors, as the reference's
-    CPU path takes them), on this host's cores; returns (RHS per second, threads, the block)."""
+    CPU path takes them), repeated for CPU_SECONDS on this host's cores; returns (RHS per second,
+    threads, the block)."""
     n = chol.n
     threads = cpu_threads()
     dense = np.zeros((sample, n))
@@ -85,7 +89,7 @@ def cpu_pair_baseline(chol, plus, minus, sample: int):
     while True:
         out = chol.solve_many(dense, threads)
         reps += 1
-        if time.perf_counter() - t0 >= 1.0:
+        if time.perf_counter() - t0 >= CPU_SECONDS:
             break
     elapsed = time.perf_counter() - t0
     return reps * sample / elapsed, threads, out
@@ -125,6 +129,7 @@ def run_ptdf(gpu, args):
 
     factor = Y.GpuFactor.from_cholesky(gpu, chol)
     factor.set_block_width(rhs_count)
+    kernel = "solve_lds_kernel" if factor.path(rhs_count) == "columns" else "solve64_kernel"
     pairs = [Y.Pair(None if p < 0 else int(p), None if m < 0 else int(m)) for p, m in zip(plus, minus)]
     factor.stage_pairs(pairs)
     for _ in range(args.warmup):
@@ -141,7 +146,7 @@ def run_ptdf(gpu, args):
         f"avg over {stats.launches} launches")
     solutions = block.solutions()
     block.free()
-    return dict(grid=grid, bprime=bprime, chol=chol, plus=plus, minus=minus, nnz_l=nnz_l, rhs=rhs_count,
+    return dict(grid=grid, bprime=bprime, chol=chol, plus=plus, minus=minus, nnz_l=nnz_l, rhs=rhs_count, kernel=kernel,
                 elapsed=elapsed, stats=stats, solutions=solutions, factor=factor)
 
 
@@ -196,6 +201,20 @@ def run_precision(gpu, args):
     return out
 
 
+def copy_calibration(mib: int):
+    """A plain device-to-device copy of `mib` MiB, five times: the known-size kernel that scales
+    the HBM counters (FETCH_SIZE, WRITE_SIZE) read for the solve kernels."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[copy] {mib} MiB copied 5 times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,12 +222,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB on the device (to scale HBM counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the factor-solve path is measured on one GPU's share "
               "of the work (right-hand-side blocks shard across GPUs with no exchange); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import ybus_amd as Y
 
@@ -244,7 +269,7 @@ def main():
         rate, threads, _ = cpu_pair_baseline(chol, ptdf["plus"], ptdf["minus"], min(2000, rhs_count))
         cpu_baseline = {"value": rate * nnz_l, "unit": "RHS·nnz(L)/s", "cores": threads, "kind": "port",
                         "sample": "par_solve_many (oracle, one RHS per thread) on the first 2,000 of the 13,500 "
-                                  "pair RHS as dense vectors, repeated for >= 1 s"}
+                                  "pair RHS as dense vectors, repeated for >= %g s" % CPU_SECONDS}
 
     line = {
         "metric": METRIC,
@@ -264,7 +289,7 @@ def main():
                    "nnz_l": nnz_l, "rhs": rhs_count, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "solve64_kernel", "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch},
+                     "kernel": ptdf["kernel"], "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu_baseline,
         "parity": {"sampled_columns": int(len(sample)), "bit_exact_columns": bit_exact,
                    "max_rel_diff_vs_cpu": max_rel},

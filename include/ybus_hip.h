@@ -210,13 +210,15 @@ int32_t ybus_hip_block_product(ybus_hip_block *b, ybus_hip_operator *op, const i
 /* GpuBlock::product_maxima (pairs.rs:396-413): product reduced to |value| * weight maxima. */
 int32_t ybus_hip_block_product_maxima(ybus_hip_block *b, ybus_hip_operator *op,
                                       const int64_t *zero, ybus_hip_column_max *out);
-/* Which kernels solve this factor's blocks (chosen at upload from dim() and the device's LDS):
- * YBUS_HIP_PATH_COLUMNS, one workgroup per right-hand side with its solution in LDS and blocks
- * stored a column (a right-hand side) at a time; or YBUS_HIP_PATH_WIDE, tiles of columns in
- * global memory with row-interleaved blocks. Setting the environment variable YBUS_HIP_PATH to
- * "wide" before an upload forces the wide path. */
+/* Which kernels solve a block of `columns` right-hand sides with this factor:
+ * YBUS_HIP_PATH_COLUMNS, one workgroup per right-hand side with its solution in LDS and the block
+ * stored a column (a right-hand side) at a time, for factors whose solution fits in LDS and
+ * blocks of at most twice as many columns as the device has compute units; or
+ * YBUS_HIP_PATH_WIDE, tiles of columns in global memory with a row-interleaved block. Setting the
+ * environment variable YBUS_HIP_PATH to "wide" or "columns" before an upload makes that factor
+ * always take the wide path, or the column path wherever the factor fits. */
 enum { YBUS_HIP_PATH_COLUMNS = 0, YBUS_HIP_PATH_WIDE = 1 };
-int32_t ybus_hip_factor_path(const ybus_hip_factor *f);
+int32_t ybus_hip_factor_path(const ybus_hip_factor *f, uint64_t columns);
 
 /* Kernel statistics of this factor's device solves (HIP events), and their reset. */
 void ybus_hip_factor_kernel_stats(const ybus_hip_factor *f, ybus_hip_kernel_stats *out);

@@ -20,9 +20,7 @@ def gpu():
 @pytest.fixture(params=["columns", "wide"])
 def path(request, monkeypatch):
     """Runs a test on both kernel paths: factors uploaded while YBUS_HIP_PATH=wide take the wide
-    (global-memory tile) path, the others the LDS column path where the factor fits."""
-    if request.param == "wide":
-        monkeypatch.setenv("YBUS_HIP_PATH", "wide")
-    else:
-        monkeypatch.delenv("YBUS_HIP_PATH", raising=False)
+    (global-memory tile) path for every block, those uploaded while YBUS_HIP_PATH=columns the LDS
+    column path for every block wherever the factor fits."""
+    monkeypatch.setenv("YBUS_HIP_PATH", request.param)
     return request.param

@@ -126,7 +126,7 @@ def _declare(L):
         "ybus_hip_block_product": (i32, [P, P, i64p, f64p]),
         "ybus_hip_block_product_maxima": (i32, [P, P, i64p, C.POINTER(_ColumnMax)]),
         "ybus_hip_factor_kernel_stats": (None, [P, C.POINTER(_KernelStats)]),
-        "ybus_hip_factor_path": (i32, [P]),
+        "ybus_hip_factor_path": (i32, [P, U64]),
         "ybus_hip_factor_reset_kernel_stats": (None, [P]),
         "ybus_hip_factor_operator": (i32, [P, U64, u64p, u64p, f64p, C.POINTER(P)]),
         "ybus_hip_operator_free": (None, [P]),
@@ -353,9 +353,10 @@ class GpuFactor:
     def reset_timings(self):
         library().ybus_hip_factor_reset_timings(self._h)
 
-    def path(self) -> str:
-        """'columns' (one workgroup per right-hand side, its solution in LDS) or 'wide'."""
-        return "wide" if library().ybus_hip_factor_path(self._h) == 1 else "columns"
+    def path(self, columns: int) -> str:
+        """The kernels a block of `columns` right-hand sides takes: 'columns' (one workgroup per
+        right-hand side, its solution in LDS) or 'wide' (column tiles in global memory)."""
+        return "wide" if library().ybus_hip_factor_path(self._h, columns) == 1 else "columns"
 
     def kernel_stats(self) -> KernelStats:
         s = _KernelStats()

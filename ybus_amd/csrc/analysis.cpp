@@ -259,7 +259,7 @@ bool valid_compressed(int64_t major, int64_t minor, const uint64_t* ptr, const u
 
 int tile_width_for(int64_t ncols, int cus) {
   for (int tw = 64; tw > 1; tw /= 2)
-    if ((ncols + tw - 1) / tw >= 2 * (int64_t)cus) return tw;
+    if ((ncols + tw - 1) / tw * 10 >= 9 * (int64_t)cus) return tw;
   return 1;
 }
 

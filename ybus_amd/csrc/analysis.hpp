@@ -76,7 +76,9 @@ Supernodes cholesky_supernodes(int64_t n, const uint64_t* l_ptr, const uint64_t*
 bool valid_compressed(int64_t major, int64_t minor, const uint64_t* ptr, const uint64_t* idx, std::string* why);
 
 // The RHS tile width (columns one workgroup owns) for `ncols` columns on `cus` compute units: the
-// widest power of two up to 64 that still gives at least two tiles per compute unit, else 1.
+// widest power of two up to 64 that still gives nine tiles for every ten compute units, else 1.
+// (A tile's cost is set by its levels, not its width, so one round of wide tiles beats two
+// rounds of narrow ones.)
 int tile_width_for(int64_t ncols, int cus);
 
 }  // namespace ybus_hip

@@ -35,6 +35,11 @@ struct Tile {
   }
 };
 
+// Bytes of an n-entry vector in LDS, rounded up to 16 so what follows it stays aligned
+__host__ __device__ __forceinline__ size_t lds_vector_bytes(int32_t n, size_t elem) {
+  return ((size_t)n * elem + 15) / 16 * 16;
+}
+
 // A max that ignores NaN, as the reference's infinity-norm fold does (accuracy.rs:66-69)
 __device__ __forceinline__ double fold_max(double m, double v) { return v > m ? v : m; }
 
@@ -51,86 +56,211 @@ __device__ void reduce_max(double* scratch, double mine, double* out, const Tile
   __syncthreads();
 }
 
+// ---- wide tiles ----
+
+// One row's record in registers
+template <typename T>
+struct Record {
+  int32_t dst, src, p0, p1;
+  int32_t dep[RECORD_ENTRIES];
+  T val[RECORD_ENTRIES];
+  T diag;
+};
+
+template <typename T>
+__device__ __forceinline__ void fetch_values(const SweepDev& s, int32_t k, Record<T>& r);
+template <>
+__device__ __forceinline__ void fetch_values<double>(const SweepDev& s, int32_t k, Record<double>& r) {
+  const double2 a = s.rec.val01[k], b = s.rec.val23[k];
+  r.val[0] = a.x;
+  r.val[1] = a.y;
+  r.val[2] = b.x;
+  r.val[3] = b.y;
+  r.diag = s.unit ? 1.0 : s.diag64[k];
+}
+template <>
+__device__ __forceinline__ void fetch_values<float>(const SweepDev& s, int32_t k, Record<float>& r) {
+  const float4 a = s.rec.val32[k];
+  r.val[0] = a.x;
+  r.val[1] = a.y;
+  r.val[2] = a.z;
+  r.val[3] = a.w;
+  r.diag = s.unit ? 1.0f : s.diag32[k];
+}
+
+// Issues the (independent, coalesced) loads of position k's record
+template <typename T>
+__device__ __forceinline__ void fetch(const SweepDev& s, int32_t k, Record<T>& r) {
+  const int4 h = s.rec.head[k], d = s.rec.dep[k];
+  r.dst = h.x;
+  r.src = h.y;
+  r.p0 = h.z;
+  r.p1 = h.w;
+  r.dep[0] = d.x;
+  r.dep[1] = d.y;
+  r.dep[2] = d.z;
+  r.dep[3] = d.w;
+  fetch_values<T>(s, k, r);
+}
+
+template <typename T>
+__device__ __forceinline__ const T* values_of(const SweepDev& s);
+template <>
+__device__ __forceinline__ const double* values_of<double>(const SweepDev& s) {
+  return s.val64;
+}
+template <>
+__device__ __forceinline__ const float* values_of<float>(const SweepDev& s) {
+  return s.val32;
+}
+
+// Copies a sweep's level pointers into LDS (every thread must call; ends with a barrier)
+__device__ void stage_levels(const SweepDev& s, int32_t* lp) {
+  for (int32_t l = threadIdx.x; l <= s.levels; l += blockDim.x) lp[l] = s.level_ptr[l];
+  __syncthreads();
+}
+
+// R rows' records of one thread: positions k, k + slots, ..., k + (R - 1) slots of one level.
+// Positions past the level's end are clamped to its last row, so every fetch issues the same
+// loads (the wait counts the compiler places stay exact) and only real rows are stored.
+template <typename T, int R>
+struct Batch {
+  Record<T> r[R];
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void fetch_batch(const SweepDev& s, int32_t k, int32_t slots, int32_t end, Batch<T, R>& b) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int32_t kk = k + q * slots;
+    fetch<T>(s, kk < end ? kk : end - 1, b.r[q]);
+  }
+}
+
+// The batch a thread starts level m with (a harmless refetch of the last level past the end)
+template <typename T, int R>
+__device__ __forceinline__ void fetch_first(const SweepDev& s, const int32_t* lp, int32_t m, int slot, int slots,
+                                            Batch<T, R>& b) {
+  if (m >= s.levels) m = s.levels - 1;
+  fetch_batch<T, R>(s, lp[m] + slot, slots, lp[m + 1], b);
+}
+
+// The rows of one batch: every gather of the R rows is issued before the first subtraction, so
+// a thread waits for one round trip per batch rather than per row. A row's terms are subtracted
+// in its packed order, the inline four first.
+template <typename T, int TW, int R, class Rhs, class Store>
+__device__ __forceinline__ void wide_rows(const SweepDev& s, const T* __restrict__ val, int32_t k, int32_t end,
+                                          const T* __restrict__ dep_buf, int64_t ld, const Tile<TW>& g,
+                                          const Batch<T, R>& b, Rhs& rhs, Store& store) {
+  T acc[R], x[R][RECORD_ENTRIES];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    acc[q] = rhs(b.r[q].src);
+#pragma unroll
+    for (int i = 0; i < RECORD_ENTRIES; ++i) x[q][i] = dep_buf[b.r[q].dep[i] * ld + g.col];
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (q > 0 && k + q * g.slots >= end) break;
+    const Record<T>& r = b.r[q];
+    const int32_t len = r.p1 - r.p0;
+    T a = acc[q];
+#pragma unroll
+    for (int i = 0; i < RECORD_ENTRIES; ++i)
+      if (len > i) a = a - r.val[i] * x[q][i];
+    for (int32_t p = r.p0 + RECORD_ENTRIES; p < r.p1; ++p) a = a - val[p] * dep_buf[s.dep[p] * ld + g.col];
+    store(r.dst, s.unit ? a : a / r.diag);
+  }
+}
+
+// One level of a tile, starting from the thread's prefetched first batch
+template <typename T, int TW, int R, class Rhs, class Store>
+__device__ __forceinline__ void wide_level(const SweepDev& s, const T* __restrict__ val, int32_t begin, int32_t end,
+                                           const T* __restrict__ dep_buf, int64_t ld, const Tile<TW>& g,
+                                           const Batch<T, R>& first, Rhs& rhs, Store& store) {
+  int32_t k = begin + g.slot;
+  if (k >= end) return;
+  wide_rows<T, TW, R>(s, val, k, end, dep_buf, ld, g, first, rhs, store);
+  for (k += R * g.slots; k < end; k += R * g.slots) {
+    Batch<T, R> b;
+    fetch_batch<T, R>(s, k, g.slots, end, b);
+    wide_rows<T, TW, R>(s, val, k, end, dep_buf, ld, g, b, rhs, store);
+  }
+}
+
+// One sweep of a tile: every row (as its record gives it) is rhs(src) minus its terms against
+// dep_buf (the destination block), divided by its diagonal, and handed to store(dst, value).
+// Lane (slot, c) takes rows lp[l] + slot + j slots of level l, R at a time. The first batch of
+// level l + 1 is fetched while level l runs (two batches alternate, so no register copies wait
+// on loads in flight). A barrier ends each level: it orders the global stores of a workgroup's
+// waves, which share one CU's L1. Every thread takes every barrier; `act` only masks the rows.
+template <typename T, int TW, int R, class Rhs, class Store>
+__device__ void wide_sweep(const SweepDev& s, const int32_t* lp, const T* __restrict__ dep_buf, int64_t ld,
+                           const Tile<TW>& g, bool act, Rhs rhs, Store store) {
+  const T* __restrict__ val = values_of<T>(s);
+  const int32_t levels = s.levels;
+  if (levels == 0) return;
+  Batch<T, R> a, b;
+  fetch_first<T, R>(s, lp, 0, g.slot, g.slots, a);
+  for (int32_t l = 0; l < levels; l += 2) {
+    fetch_first<T, R>(s, lp, l + 1, g.slot, g.slots, b);
+    if (act) wide_level<T, TW, R>(s, val, lp[l], lp[l + 1], dep_buf, ld, g, a, rhs, store);
+    __syncthreads();
+    if (l + 1 >= levels) break;
+    fetch_first<T, R>(s, lp, l + 2, g.slot, g.slots, a);
+    if (act) wide_level<T, TW, R>(s, val, lp[l + 1], lp[l + 2], dep_buf, ld, g, b, rhs, store);
+    __syncthreads();
+  }
+}
+
 // ---- FP64 solve: forward into t, barrier, backward into x ----
 
-template <int TW>
+template <int TW, int R>
 __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, double* __restrict__ t,
                                                        double* __restrict__ x, int64_t ld, int32_t ncols) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem);
+  int32_t* lpb = lpf + fwd.levels + 1;
+  stage_levels(fwd, lpf);
+  stage_levels(bwd, lpb);
   const Tile<TW> g(ncols);
-  for (int32_t l = 0; l < fwd.levels; ++l) {
-    const int32_t end = fwd.level_ptr[l + 1];
-    if (g.on) {
-      for (int32_t k = fwd.level_ptr[l] + g.slot; k < end; k += g.slots) {
-        const int32_t s = fwd.src[k];
-        double acc;
-        if (rhs.block) {
-          acc = rhs.block[s * ld + g.col];
-        } else {
-          // e_plus - e_minus as the host builds it: 1.0, -1.0 or 0.0
-          acc = (s == rhs.plus[g.col] ? 1.0 : 0.0) - (s == rhs.minus[g.col] ? 1.0 : 0.0);
-        }
-        const int32_t pe = fwd.row_ptr[k + 1];
-        for (int32_t p = fwd.row_ptr[k]; p < pe; ++p) acc = acc - fwd.val64[p] * t[fwd.dep[p] * ld + g.col];
-        t[fwd.dst[k] * ld + g.col] = fwd.diag64 ? acc / fwd.diag64[k] : acc;
-      }
-    }
-    __syncthreads();
-  }
-  for (int32_t l = 0; l < bwd.levels; ++l) {
-    const int32_t end = bwd.level_ptr[l + 1];
-    if (g.on) {
-      for (int32_t k = bwd.level_ptr[l] + g.slot; k < end; k += g.slots) {
-        double acc = t[bwd.src[k] * ld + g.col];
-        const int32_t pe = bwd.row_ptr[k + 1];
-        for (int32_t p = bwd.row_ptr[k]; p < pe; ++p) acc = acc - bwd.val64[p] * x[bwd.dep[p] * ld + g.col];
-        x[bwd.dst[k] * ld + g.col] = bwd.diag64 ? acc / bwd.diag64[k] : acc;
-      }
-    }
-    __syncthreads();
-  }
+  const int32_t col = g.col;
+  const int32_t plus = !rhs.block && g.on ? rhs.plus[col] : -1;
+  const int32_t minus = !rhs.block && g.on ? rhs.minus[col] : -1;
+  auto from_rhs = [&](int32_t src) -> double {
+    if (rhs.block) return rhs.block[src * ld + col];
+    // e_plus - e_minus as the host builds it: 1.0, -1.0 or 0.0
+    return (src == plus ? 1.0 : 0.0) - (src == minus ? 1.0 : 0.0);
+  };
+  wide_sweep<double, TW, R>(fwd, lpf, t, ld, g, g.on, from_rhs, [&](int32_t dst, double v) { t[dst * ld + col] = v; });
+  wide_sweep<double, TW, R>(bwd, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; },
+                            [&](int32_t dst, double v) { x[dst * ld + col] = v; });
 }
 
 // ---- FP32 sweeps refined in FP64 ----
 
 // Forward FP32 sweep of one tile: t[dst] = (float(src[src_row]) - sum val * t[dep]) / diag
-template <int TW>
-__device__ void forward32(const SweepDev& s, const double* __restrict__ src, float* __restrict__ t, int64_t ld,
-                          const Tile<TW>& g, bool act) {
-  for (int32_t l = 0; l < s.levels; ++l) {
-    const int32_t end = s.level_ptr[l + 1];
-    if (act) {
-      for (int32_t k = s.level_ptr[l] + g.slot; k < end; k += g.slots) {
-        float acc = (float)src[s.src[k] * ld + g.col];
-        const int32_t pe = s.row_ptr[k + 1];
-        for (int32_t p = s.row_ptr[k]; p < pe; ++p) acc = acc - s.val32[p] * t[s.dep[p] * ld + g.col];
-        t[s.dst[k] * ld + g.col] = s.diag32 ? acc / s.diag32[k] : acc;
-      }
-    }
-    __syncthreads();
-  }
+template <int TW, int R>
+__device__ void forward32(const SweepDev& s, const int32_t* lp, const double* __restrict__ src, float* __restrict__ t,
+                          int64_t ld, const Tile<TW>& g, bool act) {
+  const int32_t col = g.col;
+  wide_sweep<float, TW, R>(s, lp, t, ld, g, act, [&](int32_t r) { return (float)src[r * ld + col]; },
+                        [&](int32_t dst, float v) { t[dst * ld + col] = v; });
 }
 
 // Backward FP32 sweep of one tile into d (its own dependencies), updating the FP64 iterate:
 // xn[dst] = xc[dst] + double(d) (or double(d) alone when xc is null)
-template <int TW>
-__device__ void backward32(const SweepDev& s, const float* __restrict__ t, float* __restrict__ d,
+template <int TW, int R>
+__device__ void backward32(const SweepDev& s, const int32_t* lp, const float* __restrict__ t, float* __restrict__ d,
                            const double* __restrict__ xc, double* __restrict__ xn, int64_t ld, const Tile<TW>& g,
                            bool act) {
-  for (int32_t l = 0; l < s.levels; ++l) {
-    const int32_t end = s.level_ptr[l + 1];
-    if (act) {
-      for (int32_t k = s.level_ptr[l] + g.slot; k < end; k += g.slots) {
-        float acc = t[s.src[k] * ld + g.col];
-        const int32_t pe = s.row_ptr[k + 1];
-        for (int32_t p = s.row_ptr[k]; p < pe; ++p) acc = acc - s.val32[p] * d[s.dep[p] * ld + g.col];
-        const float v = s.diag32 ? acc / s.diag32[k] : acc;
-        const int64_t at = s.dst[k] * ld + g.col;
-        d[at] = v;
-        xn[at] = xc ? xc[at] + (double)v : (double)v;
-      }
-    }
-    __syncthreads();
-  }
+  const int32_t col = g.col;
+  wide_sweep<float, TW, R>(s, lp, d, ld, g, act, [&](int32_t r) { return t[r * ld + col]; },
+                        [&](int32_t dst, float v) {
+                          const int64_t at = dst * ld + col;
+                          d[at] = v;
+                          xn[at] = xc ? xc[at] + (double)v : (double)v;
+                        });
 }
 
 // r = b - A x for the tile's columns (each row summed from 0 in index order, as mul_vec's
@@ -155,7 +285,7 @@ __device__ __forceinline__ double ratio_of(double r_norm, double b_norm) {
   return b_norm > 0.0 ? r_norm / b_norm : (r_norm > 0.0 ? __builtin_inf() : 0.0);
 }
 
-template <int TW>
+template <int TW, int R>
 __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd, CsrDev a, const double* __restrict__ b,
                                                       float* __restrict__ t, float* __restrict__ d,
                                                       double* __restrict__ x0, double* __restrict__ x1,
@@ -165,6 +295,11 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
   __shared__ double scratch[1024];
   __shared__ double s_bnorm[TW], s_rnorm[TW], s_ratio[TW], s_best[TW];
   __shared__ int32_t s_best_buf[TW], s_steps[TW], s_active[TW];
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem);
+  int32_t* lpb = lpf + fwd.levels + 1;
+  stage_levels(fwd, lpf);
+  stage_levels(bwd, lpb);
   const Tile<TW> g(ncols);
 
   // ||b||inf per column
@@ -174,8 +309,8 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
   reduce_max<TW>(scratch, m, s_bnorm, g);
 
   // x0 = solve(b), its residual and ratio
-  forward32<TW>(fwd, b, t, ld, g, g.on);
-  backward32<TW>(bwd, t, d, nullptr, x0, ld, g, g.on);
+  forward32<TW, R>(fwd, lpf, b, t, ld, g, g.on);
+  backward32<TW, R>(bwd, lpb, t, d, nullptr, x0, ld, g, g.on);
   m = residual<TW>(a, b, x0, r, ld, g, g.on);
   reduce_max<TW>(scratch, m, s_rnorm, g);
   if (g.slot == 0) {
@@ -198,8 +333,8 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
     for (int c = 0; c < TW; ++c) any = any || s_active[c];
     if (!any) break;
     const bool act = g.on && s_active[g.lane];
-    forward32<TW>(fwd, r, t, ld, g, act);
-    backward32<TW>(bwd, t, d, cur, nxt, ld, g, act);
+    forward32<TW, R>(fwd, lpf, r, t, ld, g, act);
+    backward32<TW, R>(bwd, lpb, t, d, cur, nxt, ld, g, act);
     m = residual<TW>(a, b, nxt, r, ld, g, act);
     reduce_max<TW>(scratch, m, s_rnorm, g);
     if (g.slot == 0 && act) {
@@ -237,53 +372,64 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
 // global memory sees only the column's right-hand side read once and its solution written once,
 // both contiguous. The sweep schedules used here are in factor order (src = dst = row).
 
-template <typename T>
-__device__ __forceinline__ const T* values_of(const SweepDev& s);
-template <>
-__device__ __forceinline__ const double* values_of<double>(const SweepDev& s) {
-  return s.val64;
-}
-template <>
-__device__ __forceinline__ const float* values_of<float>(const SweepDev& s) {
-  return s.val32;
-}
-template <typename T>
-__device__ __forceinline__ const T* diagonal_of(const SweepDev& s);
-template <>
-__device__ __forceinline__ const double* diagonal_of<double>(const SweepDev& s) {
-  return s.diag64;
-}
-template <>
-__device__ __forceinline__ const float* diagonal_of<float>(const SweepDev& s) {
-  return s.diag32;
+// A barrier that orders LDS only: the record prefetches in flight stay in flight across it
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// One in-place sweep over xs, a barrier after each level. Each row's terms are fetched four at
-// a time before any is subtracted (so their latencies overlap) and subtracted in order.
+// One in-place sweep over xs (factor order), a barrier after each level. The level bounds are
+// the same for every lane (scalar loads); each thread takes rows lp[l] + slot, + slots, ... of
+// level l. The record of its first row in each of the next two levels is fetched two levels
+// ahead, so a narrow level (the common case) waits only on its LDS reads and arithmetic; the
+// further rows of a wide level are fetched as they come. A row's terms are subtracted in its
+// packed order: the inline four first, then the rest from the sweep's arrays.
+template <typename T>
+__device__ __forceinline__ void lds_row(const SweepDev& s, const T* __restrict__ val, T* xs, const Record<T>& r) {
+  const int32_t len = r.p1 - r.p0;
+  T acc = xs[r.dst];
+  const T x0 = xs[r.dep[0]], x1 = xs[r.dep[1]], x2 = xs[r.dep[2]], x3 = xs[r.dep[3]];
+  if (len > 0) acc = acc - r.val[0] * x0;
+  if (len > 1) acc = acc - r.val[1] * x1;
+  if (len > 2) acc = acc - r.val[2] * x2;
+  if (len > 3) acc = acc - r.val[3] * x3;
+  for (int32_t p = r.p0 + RECORD_ENTRIES; p < r.p1; ++p) acc = acc - val[p] * xs[s.dep[p]];
+  xs[r.dst] = s.unit ? acc : acc / r.diag;
+}
+
 template <typename T>
 __device__ void lds_sweep(const SweepDev& s, T* xs) {
   const T* __restrict__ val = values_of<T>(s);
-  const T* __restrict__ diag = diagonal_of<T>(s);
-  for (int32_t l = 0; l < s.levels; ++l) {
-    const int32_t end = s.level_ptr[l + 1];
-    for (int32_t k = s.level_ptr[l] + (int32_t)threadIdx.x; k < end; k += (int32_t)blockDim.x) {
-      const int32_t i = s.dst[k];
-      int32_t p = s.row_ptr[k];
-      const int32_t pe = s.row_ptr[k + 1];
-      T acc = xs[i];
-      for (; p + 4 <= pe; p += 4) {
-        const int32_t d0 = s.dep[p], d1 = s.dep[p + 1], d2 = s.dep[p + 2], d3 = s.dep[p + 3];
-        const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
-        const T x0 = xs[d0], x1 = xs[d1], x2 = xs[d2], x3 = xs[d3];
-        acc = acc - v0 * x0;
-        acc = acc - v1 * x1;
-        acc = acc - v2 * x2;
-        acc = acc - v3 * x3;
+  const int32_t* __restrict__ lp = s.level_ptr;
+  const int32_t slot = threadIdx.x, slots = blockDim.x;
+  const int32_t levels = s.levels;
+  // this thread's first row of level l, if it has one
+  auto first = [&](int32_t l, Record<T>& r) -> bool {
+    if (l >= levels) return false;
+    const int32_t k = lp[l] + slot;
+    if (k >= lp[l + 1]) return false;
+    fetch<T>(s, k, r);
+    return true;
+  };
+  Record<T> ra, rb, rc;
+  bool ha = first(0, ra), hb = first(1, rb);
+  for (int32_t l = 0; l < levels; ++l) {
+    const bool hc = first(l + 2, rc);
+    if (ha) {
+      lds_row<T>(s, val, xs, ra);
+      const int32_t end = lp[l + 1];
+      for (int32_t k = lp[l] + slot + slots; k < end; k += slots) {
+        Record<T> r;
+        fetch<T>(s, k, r);
+        lds_row<T>(s, val, xs, r);
       }
-      for (; p < pe; ++p) acc = acc - val[p] * xs[s.dep[p]];
-      xs[i] = diag ? acc / diag[k] : acc;
     }
-    __syncthreads();
+    ra = rb;
+    ha = hb;
+    rb = rc;
+    hb = hc;
+    lds_barrier();
   }
 }
 
@@ -358,8 +504,8 @@ __global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev
                                                           double* __restrict__ ratio_out) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* scratch = reinterpret_cast<double*>(smem);
-  float* xs = reinterpret_cast<float*>(smem + blockDim.x * sizeof(double));
   const int32_t n = a.rows;
+  float* xs = reinterpret_cast<float*>(smem + blockDim.x * sizeof(double));
   const int64_t col = blockIdx.x;
   const double* b = bblock + col * ld;
   double* xa = x0b + col * ld;
@@ -620,26 +766,53 @@ __global__ void operator_kernel(CsrDev m, const double* __restrict__ x, int64_t 
     default: return hipErrorInvalidValue; \
   }
 
-hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
+// Rows per thread batch in the wide sweeps: 1, 2 or 4
+#define DISPATCH_ROWS(rows, TWV, F) \
+  switch (rows) {                   \
+    case 1: F(TWV, 1); break;       \
+    case 2: F(TWV, 2); break;       \
+    case 4: F(TWV, 4); break;       \
+    default: return hipErrorInvalidValue; \
+  }
+
+hipError_t launch_solve64(hipStream_t stream, int tile, int rows, int threads, const SweepDev& fwd, const SweepDev& bwd,
                           const RhsDev& rhs, double* t, double* x, int64_t ld, int32_t ncols) {
   if (threads < tile || threads > 1024 || threads % 64) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
-#define L(TWV) hipLaunchKernelGGL(solve64_kernel<TWV>, dim3(grid), dim3(threads), 0, stream, fwd, bwd, rhs, t, x, ld, ncols)
+  const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
+#define L2(TWV, RV)                                                                                                 \
+  {                                                                                                              \
+    hipError_t e = hipFuncSetAttribute((const void*)solve64_kernel<TWV, RV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                       (int)bytes);                                                              \
+    if (e != hipSuccess) return e;                                                                               \
+    hipLaunchKernelGGL((solve64_kernel<TWV, RV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, t, x, ld,    \
+                       ncols);                                                                                   \
+  }
+#define L(TWV) DISPATCH_ROWS(rows, TWV, L2)
   DISPATCH_TILE(tile, L)
 #undef L
+#undef L2
   return hipGetLastError();
 }
 
-hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
+hipError_t launch_refine(hipStream_t stream, int tile, int rows, int threads, const SweepDev& fwd, const SweepDev& bwd,
                          const CsrDev& a, const double* b, float* t, float* d, double* x0, double* x1, double* r,
                          int64_t ld, int32_t ncols, double tol, int32_t max_steps, int32_t* steps, double* ratio) {
   if (threads < tile || threads > 1024 || threads % 64 || (threads & (threads - 1))) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
-#define L(TWV)                                                                                                  \
-  hipLaunchKernelGGL(refine_kernel<TWV>, dim3(grid), dim3(threads), 0, stream, fwd, bwd, a, b, t, d, x0, x1, r, ld, \
-                     ncols, tol, max_steps, steps, ratio)
+  const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
+#define L2(TWV, RV)                                                                                                 \
+  {                                                                                                              \
+    hipError_t e = hipFuncSetAttribute((const void*)refine_kernel<TWV, RV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                       (int)bytes);                                                              \
+    if (e != hipSuccess) return e;                                                                               \
+    hipLaunchKernelGGL((refine_kernel<TWV, RV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, d, x0, x1, \
+                       r, ld, ncols, tol, max_steps, steps, ratio);                                              \
+  }
+#define L(TWV) DISPATCH_ROWS(rows, TWV, L2)
   DISPATCH_TILE(tile, L)
 #undef L
+#undef L2
   return hipGetLastError();
 }
 
@@ -690,15 +863,19 @@ hipError_t launch_operator(hipStream_t stream, const CsrDev& m, const double* x,
   return hipGetLastError();
 }
 
-size_t solve_lds_bytes(int32_t n, bool fp32) { return (size_t)n * (fp32 ? sizeof(float) : sizeof(double)); }
+size_t solve_lds_bytes(int32_t n, bool fp32, int32_t) {
+  return lds_vector_bytes(n, fp32 ? sizeof(float) : sizeof(double));
+}
 
-size_t refine_lds_bytes(int32_t n, int threads) { return (size_t)threads * sizeof(double) + (size_t)n * sizeof(float); }
+size_t refine_lds_bytes(int32_t n, int threads, int32_t) {
+  return (size_t)threads * sizeof(double) + lds_vector_bytes(n, sizeof(float));
+}
 
 hipError_t launch_solve_lds(hipStream_t stream, int threads, bool fp32, const SweepDev& fwd, const SweepDev& bwd,
                             int32_t n, const int32_t* rhs_row, const int32_t* sol_of, const ColumnsDev& rhs,
                             double* x, int64_t ldx, int32_t ncols) {
   if (threads < 64 || threads > 1024 || threads % 64) return hipErrorInvalidValue;
-  const size_t bytes = solve_lds_bytes(n, fp32);
+  const size_t bytes = solve_lds_bytes(n, fp32, fwd.levels + bwd.levels);
   if (fp32) {
     hipError_t e = hipFuncSetAttribute((const void*)solve_lds_kernel<float>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -720,7 +897,7 @@ hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fw
                              double* x0, double* x1, double* r, int64_t ld, int32_t ncols, double tol,
                              int32_t max_steps, int32_t* steps, double* ratio) {
   if (threads < 64 || threads > 1024 || (threads & (threads - 1))) return hipErrorInvalidValue;
-  const size_t bytes = refine_lds_bytes(a.rows, threads);
+  const size_t bytes = refine_lds_bytes(a.rows, threads, fwd.levels + bwd.levels);
   hipError_t e = hipFuncSetAttribute((const void*)refine_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)bytes);
   if (e != hipSuccess) return e;

@@ -100,21 +100,51 @@ struct Staging {
   }
 };
 
+// A sweep on the device: its level pointers, its rows' records (analysis order, field by
+// field), and the CSR arrays holding every entry (the records inline the first few of each row)
 struct DevSweep {
   DevBuf<int32_t> level_ptr, src, dst, row_ptr, dep;
   DevBuf<double> val64, diag64;
   DevBuf<float> val32, diag32;
+  DevBuf<int4> head, dep4;
+  DevBuf<double2> val01, val23;
+  DevBuf<float4> v32;
   Sweep host;
   void upload(Sweep s, hipStream_t stream) {
     host = std::move(s);
+    const int32_t n = host.n;
+    std::vector<int4> hd(n), dp(n);
+    std::vector<double2> a(n), b(n);
+    std::vector<float4> c(n);
+    for (int32_t k = 0; k < n; ++k) {
+      const int32_t p0 = host.row_ptr[k], p1 = host.row_ptr[k + 1];
+      hd[k] = make_int4(host.dst[k], host.src[k], p0, p1);
+      int32_t d[RECORD_ENTRIES];
+      double v[RECORD_ENTRIES];
+      for (int q = 0; q < RECORD_ENTRIES; ++q) {
+        // unused slots point at the row's own destination (a valid index) with value 0
+        const bool inside = p0 + q < p1;
+        d[q] = inside ? host.dep[p0 + q] : host.dst[k];
+        v[q] = inside ? host.val[p0 + q] : 0.0;
+      }
+      dp[k] = make_int4(d[0], d[1], d[2], d[3]);
+      a[k] = make_double2(v[0], v[1]);
+      b[k] = make_double2(v[2], v[3]);
+      c[k] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+    }
+    head.upload(hd, stream);
+    dep4.upload(dp, stream);
+    val01.upload(a, stream);
+    val23.upload(b, stream);
+    v32.upload(c, stream);
     level_ptr.upload(host.level_ptr, stream);
     src.upload(host.src, stream);
     dst.upload(host.dst, stream);
     row_ptr.upload(host.row_ptr, stream);
     dep.upload(host.dep, stream);
     val64.upload(host.val, stream);
-    std::vector<float> v32(host.val.begin(), host.val.end());
-    val32.upload(v32, stream);
+    std::vector<float> v32v(host.val.begin(), host.val.end());
+    val32.upload(v32v, stream);
     if (!host.unit) {
       diag64.upload(host.diag, stream);
       std::vector<float> d32(host.diag.begin(), host.diag.end());
@@ -135,6 +165,12 @@ struct DevSweep {
     v.val32 = val32.p;
     v.diag64 = host.unit ? nullptr : diag64.p;
     v.diag32 = host.unit ? nullptr : diag32.p;
+    v.rec.head = head.p;
+    v.rec.dep = dep4.p;
+    v.rec.val01 = val01.p;
+    v.rec.val23 = val23.p;
+    v.rec.val32 = v32.p;
+    v.unit = host.unit;
     return v;
   }
   size_t entries() const { return host.dep.size(); }
@@ -191,10 +227,11 @@ struct ybus_hip_ctx {
 
 struct ybus_hip_block;
 
-// How a factor's blocks are laid out and solved. Columns: each right-hand side contiguous (the
-// host's own layout, ld = n), solved one workgroup per column with the vector in LDS; for any n
-// whose solution fits in LDS. Wide: row-interleaved (n rows of ld columns), solved by column
-// tiles in global memory; for larger n.
+// How a block is laid out and solved. Columns: each right-hand side contiguous (the host's own
+// layout, ld = n), solved one workgroup per column with the vector in LDS; for a factor whose
+// solution fits in LDS and a block of few enough columns that one or two rounds of workgroups
+// take them all. Wide: row-interleaved (n rows of ld columns), solved by column tiles in global
+// memory; for everything else.
 struct Layout {
   bool columns = true;
   int64_t ld = 0;
@@ -208,7 +245,9 @@ struct ybus_hip_factor {
   bool lu = false;
   uint64_t width = 1;
   int64_t nnz_factor = 0;  // S: entries swept by one forward and backward solve
-  bool columns = true;     // the LDS column path (else the wide path)
+  bool fits = false;       // one column's solution fits in LDS (the column path is built)
+  int force = 0;           // 0: choose the path per block, 1: always wide, 2: columns where it fits
+  int64_t column_limit = 0;  // the widest block the column path takes by default
   size_t lds_cap = 0;
   // wide path: schedules reading and writing permuted blocks
   DevSweep fwd, bwd, tfwd, tbwd;
@@ -232,7 +271,7 @@ struct ybus_hip_factor {
   ybus_hip_timings timings{};
   ybus_hip_kernel_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int tile_override = 0, threads = 256;
+  int tile_override = 0, threads_wide = 1024, threads_columns = 512, rows = 1;
 
   ~ybus_hip_factor() {
     if (ev0) (void)hipEventDestroy(ev0);
@@ -253,7 +292,8 @@ struct ybus_hip_factor {
       t32.release();
       d32.release();
     }
-    layout.columns = columns;
+    layout.columns = use_columns(ncols);
+    const bool columns = layout.columns;
     layout.ld = columns ? nn : round_up(std::max<int64_t>(cols, 1), 16);
     const size_t elems = (size_t)nn * (size_t)(columns ? std::max<int64_t>(cols, 1) : layout.ld);
     b.reserve(elems);
@@ -277,6 +317,12 @@ struct ybus_hip_factor {
         d32.reserve(elems);
       }
     }
+  }
+
+  // Whether a block of `ncols` columns takes the column path
+  bool use_columns(int64_t ncols) const {
+    if (!fits || force == 1) return false;
+    return force == 2 || ncols <= column_limit;
   }
 
   int tile_for(int64_t ncols) const {
@@ -309,20 +355,20 @@ struct ybus_hip_factor {
   void solve64(bool transpose, bool pairs, int64_t ncols) {
     if (transpose && !has_transpose) transpose = false;  // A^T = A for a Cholesky factor
     begin_kernel();
-    if (columns) {
+    if (layout.columns) {
       const DevSweep& f = transpose ? ctfwd : cfwd;
       const DevSweep& bw = transpose ? ctbwd : cbwd;
       const DevBuf<int32_t>& rr = transpose ? t_rhs_row : rhs_row;
       const DevBuf<int32_t>& so = transpose ? t_sol_of : sol_of;
       ColumnsDev rhs{pairs ? nullptr : b.p, layout.ld, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
-      check(launch_solve_lds(stream(), threads, false, f.view(), bw.view(), (int32_t)n, rr.p, so.p, rhs, x.p,
+      check(launch_solve_lds(stream(), threads_columns, false, f.view(), bw.view(), (int32_t)n, rr.p, so.p, rhs, x.p,
                              layout.ld, (int32_t)ncols),
             "solve kernel");
     } else {
       const DevSweep& f = transpose ? tfwd : fwd;
       const DevSweep& bw = transpose ? tbwd : bwd;
       RhsDev rhs{pairs ? nullptr : b.p, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
-      check(launch_solve64(stream(), tile_for(ncols), threads, f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
+      check(launch_solve64(stream(), tile_for(ncols), rows, threads_wide, f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
                            (int32_t)ncols),
             "solve kernel");
     }
@@ -332,14 +378,14 @@ struct ybus_hip_factor {
   void refine(int64_t ncols, double tol, int32_t max_steps) {
     if (!a.ptr.p) fail(YBUS_HIP_INVALID_ARGUMENT, "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix)");
     begin_kernel();
-    if (columns) {
+    if (layout.columns) {
       int th = 1;
-      while (th * 2 <= threads) th *= 2;
+      while (th * 2 <= threads_columns) th *= 2;
       check(launch_refine_lds(stream(), th, cfwd.view(), cbwd.view(), a.view(), rhs_row.p, sol_of.p, b.p, x.p, x1.p,
                               r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     } else {
-      check(launch_refine(stream(), tile_for(ncols), threads, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p,
+      check(launch_refine(stream(), tile_for(ncols), rows, threads_wide, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p,
                           x.p, x1.p, r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     }
@@ -465,14 +511,25 @@ static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   f->ctx = ctx;
   f->n = n;
   const char* thr = std::getenv("YBUS_HIP_THREADS");
-  if (thr) f->threads = std::clamp(std::atoi(thr), 64, 1024) / 64 * 64;
+  if (thr) f->threads_wide = f->threads_columns = std::clamp(std::atoi(thr), 64, 1024) / 64 * 64;
   const char* tile = std::getenv("YBUS_HIP_TILE");
   if (tile) f->tile_override = std::atoi(tile);
-  f->lds_cap = lds_capacity(ctx);
+  const char* rows = std::getenv("YBUS_HIP_ROWS");
+  if (rows) f->rows = std::atoi(rows);
   const char* path = std::getenv("YBUS_HIP_PATH");
-  f->columns = solve_lds_bytes((int32_t)std::max<int64_t>(n, 1), false) <= f->lds_cap &&
-               refine_lds_bytes((int32_t)std::max<int64_t>(n, 1), f->threads) <= f->lds_cap;
-  if (path && std::string(path) == "wide") f->columns = false;
+  if (path) f->force = std::string(path) == "wide" ? 1 : std::string(path) == "columns" ? 2 : 0;
+  f->column_limit = 2 * (int64_t)ctx->cus;
+  f->lds_cap = lds_capacity(ctx);
+}
+
+// Whether one column's solution fits in LDS, for FP64 solves and for FP32 refinement; the column
+// path is built only then (YBUS_HIP_PATH=wide skips it)
+static bool fits_columns(const ybus_hip_factor* f, int32_t levels) {
+  if (f->force == 1) return false;
+  const int32_t n = (int32_t)std::max<int64_t>(f->n, 1);
+  int th = 1;
+  while (th * 2 <= f->threads_columns) th *= 2;
+  return solve_lds_bytes(n, false, levels) <= f->lds_cap && refine_lds_bytes(n, th, levels) <= f->lds_cap;
 }
 
 static void finish_factor(ybus_hip_factor* f) {
@@ -522,16 +579,17 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
     }
     const RowForm lower = cholesky_forward(n, l_ptr, l_idx, l_val);
     const RowForm upper = cholesky_backward(n, l_ptr, l_idx, l_val);
-    if (f->columns) {
+    Sweep sf = schedule(lower, ident, ident), sb = schedule(upper, ident, ident);
+    f->fits = fits_columns(f.get(), sf.levels + sb.levels);
+    if (f->fits) {
       // cholesky.rs:570-579: y[i] = b[perm[i]], and x[perm[i]] = y[i]
-      f->cfwd.upload(schedule(lower, ident, ident), ctx->stream);
-      f->cbwd.upload(schedule(upper, ident, ident), ctx->stream);
+      f->cfwd.upload(std::move(sf), ctx->stream);
+      f->cbwd.upload(std::move(sb), ctx->stream);
       f->rhs_row.upload(inverse_i32(gather), ctx->stream);
       f->sol_of.upload(inverse_i32(gather), ctx->stream);
-    } else {
-      f->fwd.upload(schedule(lower, gather, ident), ctx->stream);
-      f->bwd.upload(schedule(upper, ident, gather), ctx->stream);
     }
+    f->fwd.upload(schedule(lower, gather, ident), ctx->stream);
+    f->bwd.upload(schedule(upper, ident, gather), ctx->stream);
     check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
     *out = f.release();
@@ -573,22 +631,24 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
     }
     const RowForm l = lu_forward(n, l_ptr, l_idx, l_val), u = lu_backward(n, u_ptr, u_idx, u_val);
     const RowForm ut = lu_transpose_forward(n, u_ptr, u_idx, u_val), lt = lu_transpose_backward(n, l_ptr, l_idx, l_val);
-    if (f->columns) {
+    Sweep sl = schedule(l, ident, ident), su = schedule(u, ident, ident);
+    Sweep stu = schedule(ut, ident, ident), stl = schedule(lt, ident, ident);
+    f->fits = fits_columns(f.get(), std::max(sl.levels + su.levels, stu.levels + stl.levels));
+    if (f->fits) {
       // lu.rs:770-785: y[i] = b[P[i]], x[Q[i]] = y[i]; transposed (lu.rs:506-533): through Q, then P
-      f->cfwd.upload(schedule(l, ident, ident), ctx->stream);
-      f->cbwd.upload(schedule(u, ident, ident), ctx->stream);
-      f->ctfwd.upload(schedule(ut, ident, ident), ctx->stream);
-      f->ctbwd.upload(schedule(lt, ident, ident), ctx->stream);
+      f->cfwd.upload(std::move(sl), ctx->stream);
+      f->cbwd.upload(std::move(su), ctx->stream);
+      f->ctfwd.upload(std::move(stu), ctx->stream);
+      f->ctbwd.upload(std::move(stl), ctx->stream);
       f->rhs_row.upload(inverse_i32(rp), ctx->stream);
       f->sol_of.upload(inverse_i32(cp), ctx->stream);
       f->t_rhs_row.upload(inverse_i32(cp), ctx->stream);
       f->t_sol_of.upload(inverse_i32(rp), ctx->stream);
-    } else {
-      f->fwd.upload(schedule(l, rp, ident), ctx->stream);
-      f->bwd.upload(schedule(u, ident, cp), ctx->stream);
-      f->tfwd.upload(schedule(ut, cp, ident), ctx->stream);
-      f->tbwd.upload(schedule(lt, ident, rp), ctx->stream);
     }
+    f->fwd.upload(schedule(l, rp, ident), ctx->stream);
+    f->bwd.upload(schedule(u, ident, cp), ctx->stream);
+    f->tfwd.upload(schedule(ut, cp, ident), ctx->stream);
+    f->tbwd.upload(schedule(lt, ident, rp), ctx->stream);
     f->has_transpose = true;
     check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
@@ -616,8 +676,9 @@ void ybus_hip_factor_reset_timings(ybus_hip_factor* f) {
   if (f) f->timings = ybus_hip_timings{};
 }
 
-int32_t ybus_hip_factor_path(const ybus_hip_factor* f) {
-  return f && !f->columns ? YBUS_HIP_PATH_WIDE : YBUS_HIP_PATH_COLUMNS;
+int32_t ybus_hip_factor_path(const ybus_hip_factor* f, uint64_t columns) {
+  return f && f->use_columns((int64_t)std::min<uint64_t>(columns, INT64_MAX)) ? YBUS_HIP_PATH_COLUMNS
+                                                                                : YBUS_HIP_PATH_WIDE;
 }
 
 void ybus_hip_factor_kernel_stats(const ybus_hip_factor* f, ybus_hip_kernel_stats* out) {
