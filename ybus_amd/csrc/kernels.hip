@@ -76,7 +76,7 @@ __device__ __forceinline__ void fetch_values<double>(const SweepDev& s, int32_t 
   r.val[1] = a.y;
   r.val[2] = b.x;
   r.val[3] = b.y;
-  r.diag = s.unit ? 1.0 : s.diag64[k];
+  r.diag = s.rec.diag64[k];
 }
 template <>
 __device__ __forceinline__ void fetch_values<float>(const SweepDev& s, int32_t k, Record<float>& r) {
@@ -85,7 +85,7 @@ __device__ __forceinline__ void fetch_values<float>(const SweepDev& s, int32_t k
   r.val[1] = a.y;
   r.val[2] = a.z;
   r.val[3] = a.w;
-  r.diag = s.unit ? 1.0f : s.diag32[k];
+  r.diag = s.rec.diag32[k];
 }
 
 // Issues the (independent, coalesced) loads of position k's record
@@ -101,6 +101,16 @@ __device__ __forceinline__ void fetch(const SweepDev& s, int32_t k, Record<T>& r
   r.dep[2] = d.z;
   r.dep[3] = d.w;
   fetch_values<T>(s, k, r);
+}
+
+// Waits until every field of r has arrived. A record slot is refilled while the loads of its
+// previous fetch may still be in flight (a thread with no row in that level never reads it); the
+// compiler orders such a refill behind every load then in flight, so waiting on the slot's own
+// loads first keeps the other slots' prefetches in flight.
+template <typename T>
+__device__ __forceinline__ void await_record(const Record<T>& r) {
+  __asm__ volatile("" ::"v"(r.dst), "v"(r.src), "v"(r.p0), "v"(r.p1), "v"(r.dep[0]), "v"(r.dep[1]), "v"(r.dep[2]),
+                   "v"(r.dep[3]), "v"(r.val[0]), "v"(r.val[1]), "v"(r.val[2]), "v"(r.val[3]), "v"(r.diag));
 }
 
 template <typename T>
@@ -120,102 +130,147 @@ __device__ void stage_levels(const SweepDev& s, int32_t* lp) {
   __syncthreads();
 }
 
-// R rows' records of one thread: positions k, k + slots, ..., k + (R - 1) slots of one level.
-// Positions past the level's end are clamped to its last row, so every fetch issues the same
-// loads (the wait counts the compiler places stay exact) and only real rows are stored.
-template <typename T, int R>
-struct Batch {
-  Record<T> r[R];
+// Terms of a row past its inline four are taken this many at a time: the chunk's dependency
+// indices and values are loaded together, then its dependencies' values together, so a long row
+// (a separator's, near the top of the schedule, where levels are one or two rows wide) waits
+// for two round trips per chunk rather than per term.
+constexpr int OVERFLOW_CHUNK = 8;
+// The wide sweeps keep more records in flight, so they take their chunks half as long
+constexpr int WIDE_CHUNK = 4;
+
+// A thread's record of position k of a level ending at `end`, clamped to the level's last row
+// (so every fetch issues the same loads, and the wait counts the compiler places stay exact)
+template <typename T>
+__device__ __forceinline__ void fetch_clamped(const SweepDev& s, int32_t k, int32_t end, Record<T>& r) {
+  fetch<T>(s, k < end ? k : end - 1, r);
+}
+
+// The record a thread starts level m with (a harmless refetch of the last level past the end)
+template <typename T>
+__device__ __forceinline__ void fetch_first(const SweepDev& s, const int32_t* lp, int32_t m, int slot, Record<T>& r) {
+  if (m >= s.levels) m = s.levels - 1;
+  fetch_clamped<T>(s, lp[m] + slot, lp[m + 1], r);
+}
+
+// A row's right-hand side and inline dependencies, loaded (or in flight)
+template <typename T>
+struct Gathered {
+  T rhs;
+  T x[RECORD_ENTRIES];
 };
 
-template <typename T, int R>
-__device__ __forceinline__ void fetch_batch(const SweepDev& s, int32_t k, int32_t slots, int32_t end, Batch<T, R>& b) {
+template <typename T, int TW, class Rhs>
+__device__ __forceinline__ void gather(const Record<T>& r, const T* __restrict__ dep_buf, int64_t ld,
+                                       const Tile<TW>& g, Rhs& rhs, Gathered<T>& o) {
+  o.rhs = rhs(r.src);
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int32_t kk = k + q * slots;
-    fetch<T>(s, kk < end ? kk : end - 1, b.r[q]);
-  }
+  for (int i = 0; i < RECORD_ENTRIES; ++i) o.x[i] = dep_buf[r.dep[i] * ld + g.col];
 }
 
-// The batch a thread starts level m with (a harmless refetch of the last level past the end)
-template <typename T, int R>
-__device__ __forceinline__ void fetch_first(const SweepDev& s, const int32_t* lp, int32_t m, int slot, int slots,
-                                            Batch<T, R>& b) {
-  if (m >= s.levels) m = s.levels - 1;
-  fetch_batch<T, R>(s, lp[m] + slot, slots, lp[m + 1], b);
+// A row's value: its right-hand side minus its terms in packed order (the inline four from o,
+// the rest in chunks), over its diagonal
+template <typename T, int TW>
+__device__ __forceinline__ T finish(const SweepDev& s, const T* __restrict__ val, const Record<T>& r,
+                                    const Gathered<T>& o, const T* __restrict__ dep_buf, int64_t ld,
+                                    const Tile<TW>& g) {
+  const int32_t len = r.p1 - r.p0;
+  T a = o.rhs;
+#pragma unroll
+  for (int i = 0; i < RECORD_ENTRIES; ++i)
+    if (len > i) a = a - r.val[i] * o.x[i];
+  for (int32_t p = r.p0 + RECORD_ENTRIES; p < r.p1; p += WIDE_CHUNK) {
+    int32_t dd[WIDE_CHUNK];
+    T vv[WIDE_CHUNK], xx[WIDE_CHUNK];
+#pragma unroll
+    for (int i = 0; i < WIDE_CHUNK; ++i) {
+      const int32_t pi = p + i < r.p1 ? p + i : r.p1 - 1;
+      dd[i] = s.dep[pi];
+      vv[i] = val[pi];
+    }
+#pragma unroll
+    for (int i = 0; i < WIDE_CHUNK; ++i) xx[i] = dep_buf[dd[i] * ld + g.col];
+#pragma unroll
+    for (int i = 0; i < WIDE_CHUNK; ++i)
+      if (p + i < r.p1) a = a - vv[i] * xx[i];
+  }
+  return s.unit ? a : a / r.diag;
 }
 
-// The rows of one batch: every gather of the R rows is issued before the first subtraction, so
-// a thread waits for one round trip per batch rather than per row. A row's terms are subtracted
-// in its packed order, the inline four first.
-template <typename T, int TW, int R, class Rhs, class Store>
-__device__ __forceinline__ void wide_rows(const SweepDev& s, const T* __restrict__ val, int32_t k, int32_t end,
-                                          const T* __restrict__ dep_buf, int64_t ld, const Tile<TW>& g,
-                                          const Batch<T, R>& b, Rhs& rhs, Store& store) {
-  T acc[R], x[R][RECORD_ENTRIES];
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    acc[q] = rhs(b.r[q].src);
-#pragma unroll
-    for (int i = 0; i < RECORD_ENTRIES; ++i) x[q][i] = dep_buf[b.r[q].dep[i] * ld + g.col];
-  }
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    if (q > 0 && k + q * g.slots >= end) break;
-    const Record<T>& r = b.r[q];
-    const int32_t len = r.p1 - r.p0;
-    T a = acc[q];
-#pragma unroll
-    for (int i = 0; i < RECORD_ENTRIES; ++i)
-      if (len > i) a = a - r.val[i] * x[q][i];
-    for (int32_t p = r.p0 + RECORD_ENTRIES; p < r.p1; ++p) a = a - val[p] * dep_buf[s.dep[p] * ld + g.col];
-    store(r.dst, s.unit ? a : a / r.diag);
-  }
-}
-
-// One level of a tile, starting from the thread's prefetched first batch
-template <typename T, int TW, int R, class Rhs, class Store>
+// One level of a tile: rows begin + slot, + slots, ... from the thread's prefetched first
+// record. The rounds are software-pipelined: a row's gathers are issued, then the previous row's
+// value is stored, then the next row's record is fetched, and only then does the thread wait,
+// for the gathers. A wait for a load waits for every memory operation issued before it, stores
+// included, so storing after the next gathers keeps the store's latency off the wait; the
+// records alternate between two registers sets, so no register copy waits on loads in flight.
+template <typename T, int TW, class Rhs, class Store>
 __device__ __forceinline__ void wide_level(const SweepDev& s, const T* __restrict__ val, int32_t begin, int32_t end,
                                            const T* __restrict__ dep_buf, int64_t ld, const Tile<TW>& g,
-                                           const Batch<T, R>& first, Rhs& rhs, Store& store) {
+                                           const Record<T>& first, Rhs& rhs, Store& store) {
   int32_t k = begin + g.slot;
   if (k >= end) return;
-  wide_rows<T, TW, R>(s, val, k, end, dep_buf, ld, g, first, rhs, store);
-  for (k += R * g.slots; k < end; k += R * g.slots) {
-    Batch<T, R> b;
-    fetch_batch<T, R>(s, k, g.slots, end, b);
-    wide_rows<T, TW, R>(s, val, k, end, dep_buf, ld, g, b, rhs, store);
+  const int32_t step = g.slots;
+  Gathered<T> o;
+  gather<T, TW>(first, dep_buf, ld, g, rhs, o);
+  if (k + step >= end) {
+    store(first.dst, finish<T, TW>(s, val, first, o, dep_buf, ld, g));
+    return;
   }
+  Record<T> b, c;
+  fetch_clamped<T>(s, k + step, end, b);
+  T v = finish<T, TW>(s, val, first, o, dep_buf, ld, g);
+  int32_t at = first.dst;
+  for (k += step;; k += 2 * step) {
+    // b holds row k
+    gather<T, TW>(b, dep_buf, ld, g, rhs, o);
+    store(at, v);
+    fetch_clamped<T>(s, k + step, end, c);
+    v = finish<T, TW>(s, val, b, o, dep_buf, ld, g);
+    at = b.dst;
+    if (k + step >= end) break;
+    // c holds row k + step
+    gather<T, TW>(c, dep_buf, ld, g, rhs, o);
+    store(at, v);
+    await_record<T>(b);
+    fetch_clamped<T>(s, k + 2 * step, end, b);
+    v = finish<T, TW>(s, val, c, o, dep_buf, ld, g);
+    at = c.dst;
+    if (k + 2 * step >= end) break;
+    await_record<T>(c);
+  }
+  store(at, v);
 }
 
 // One sweep of a tile: every row (as its record gives it) is rhs(src) minus its terms against
 // dep_buf (the destination block), divided by its diagonal, and handed to store(dst, value).
-// Lane (slot, c) takes rows lp[l] + slot + j slots of level l, R at a time. The first batch of
-// level l + 1 is fetched while level l runs (two batches alternate, so no register copies wait
-// on loads in flight). A barrier ends each level: it orders the global stores of a workgroup's
-// waves, which share one CU's L1. Every thread takes every barrier; `act` only masks the rows.
-template <typename T, int TW, int R, class Rhs, class Store>
+// Lane (slot, c) takes rows lp[l] + slot + j slots of level l. The first record of each level
+// is fetched two levels ahead (two records alternate, so no register copies wait on loads in
+// flight). A barrier ends each level: it orders the global stores of a workgroup's waves, which
+// share one CU's L1. Every thread takes every barrier; `act` only masks the rows.
+template <typename T, int TW, class Rhs, class Store>
 __device__ void wide_sweep(const SweepDev& s, const int32_t* lp, const T* __restrict__ dep_buf, int64_t ld,
                            const Tile<TW>& g, bool act, Rhs rhs, Store store) {
   const T* __restrict__ val = values_of<T>(s);
   const int32_t levels = s.levels;
   if (levels == 0) return;
-  Batch<T, R> a, b;
-  fetch_first<T, R>(s, lp, 0, g.slot, g.slots, a);
+  Record<T> a, b;
+  fetch_first<T>(s, lp, 0, g.slot, a);
+  fetch_first<T>(s, lp, 1, g.slot, b);
   for (int32_t l = 0; l < levels; l += 2) {
-    fetch_first<T, R>(s, lp, l + 1, g.slot, g.slots, b);
-    if (act) wide_level<T, TW, R>(s, val, lp[l], lp[l + 1], dep_buf, ld, g, a, rhs, store);
+    if (act) wide_level<T, TW>(s, val, lp[l], lp[l + 1], dep_buf, ld, g, a, rhs, store);
     __syncthreads();
+    await_record<T>(a);
+    fetch_first<T>(s, lp, l + 2, g.slot, a);
     if (l + 1 >= levels) break;
-    fetch_first<T, R>(s, lp, l + 2, g.slot, g.slots, a);
-    if (act) wide_level<T, TW, R>(s, val, lp[l + 1], lp[l + 2], dep_buf, ld, g, b, rhs, store);
+    if (act) wide_level<T, TW>(s, val, lp[l + 1], lp[l + 2], dep_buf, ld, g, b, rhs, store);
     __syncthreads();
+    await_record<T>(b);
+    fetch_first<T>(s, lp, l + 3, g.slot, b);
   }
 }
 
 // ---- FP64 solve: forward into t, barrier, backward into x ----
 
-template <int TW, int R>
+template <int TW>
 __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, double* __restrict__ t,
                                                        double* __restrict__ x, int64_t ld, int32_t ncols) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -232,30 +287,30 @@ __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bw
     // e_plus - e_minus as the host builds it: 1.0, -1.0 or 0.0
     return (src == plus ? 1.0 : 0.0) - (src == minus ? 1.0 : 0.0);
   };
-  wide_sweep<double, TW, R>(fwd, lpf, t, ld, g, g.on, from_rhs, [&](int32_t dst, double v) { t[dst * ld + col] = v; });
-  wide_sweep<double, TW, R>(bwd, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; },
+  wide_sweep<double, TW>(fwd, lpf, t, ld, g, g.on, from_rhs, [&](int32_t dst, double v) { t[dst * ld + col] = v; });
+  wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; },
                             [&](int32_t dst, double v) { x[dst * ld + col] = v; });
 }
 
 // ---- FP32 sweeps refined in FP64 ----
 
 // Forward FP32 sweep of one tile: t[dst] = (float(src[src_row]) - sum val * t[dep]) / diag
-template <int TW, int R>
+template <int TW>
 __device__ void forward32(const SweepDev& s, const int32_t* lp, const double* __restrict__ src, float* __restrict__ t,
                           int64_t ld, const Tile<TW>& g, bool act) {
   const int32_t col = g.col;
-  wide_sweep<float, TW, R>(s, lp, t, ld, g, act, [&](int32_t r) { return (float)src[r * ld + col]; },
+  wide_sweep<float, TW>(s, lp, t, ld, g, act, [&](int32_t r) { return (float)src[r * ld + col]; },
                         [&](int32_t dst, float v) { t[dst * ld + col] = v; });
 }
 
 // Backward FP32 sweep of one tile into d (its own dependencies), updating the FP64 iterate:
 // xn[dst] = xc[dst] + double(d) (or double(d) alone when xc is null)
-template <int TW, int R>
+template <int TW>
 __device__ void backward32(const SweepDev& s, const int32_t* lp, const float* __restrict__ t, float* __restrict__ d,
                            const double* __restrict__ xc, double* __restrict__ xn, int64_t ld, const Tile<TW>& g,
                            bool act) {
   const int32_t col = g.col;
-  wide_sweep<float, TW, R>(s, lp, d, ld, g, act, [&](int32_t r) { return t[r * ld + col]; },
+  wide_sweep<float, TW>(s, lp, d, ld, g, act, [&](int32_t r) { return t[r * ld + col]; },
                         [&](int32_t dst, float v) {
                           const int64_t at = dst * ld + col;
                           d[at] = v;
@@ -285,7 +340,7 @@ __device__ __forceinline__ double ratio_of(double r_norm, double b_norm) {
   return b_norm > 0.0 ? r_norm / b_norm : (r_norm > 0.0 ? __builtin_inf() : 0.0);
 }
 
-template <int TW, int R>
+template <int TW>
 __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd, CsrDev a, const double* __restrict__ b,
                                                       float* __restrict__ t, float* __restrict__ d,
                                                       double* __restrict__ x0, double* __restrict__ x1,
@@ -309,8 +364,8 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
   reduce_max<TW>(scratch, m, s_bnorm, g);
 
   // x0 = solve(b), its residual and ratio
-  forward32<TW, R>(fwd, lpf, b, t, ld, g, g.on);
-  backward32<TW, R>(bwd, lpb, t, d, nullptr, x0, ld, g, g.on);
+  forward32<TW>(fwd, lpf, b, t, ld, g, g.on);
+  backward32<TW>(bwd, lpb, t, d, nullptr, x0, ld, g, g.on);
   m = residual<TW>(a, b, x0, r, ld, g, g.on);
   reduce_max<TW>(scratch, m, s_rnorm, g);
   if (g.slot == 0) {
@@ -333,8 +388,8 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
     for (int c = 0; c < TW; ++c) any = any || s_active[c];
     if (!any) break;
     const bool act = g.on && s_active[g.lane];
-    forward32<TW, R>(fwd, lpf, r, t, ld, g, act);
-    backward32<TW, R>(bwd, lpb, t, d, cur, nxt, ld, g, act);
+    forward32<TW>(fwd, lpf, r, t, ld, g, act);
+    backward32<TW>(bwd, lpb, t, d, cur, nxt, ld, g, act);
     m = residual<TW>(a, b, nxt, r, ld, g, act);
     reduce_max<TW>(scratch, m, s_rnorm, g);
     if (g.slot == 0 && act) {
@@ -379,12 +434,8 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// One in-place sweep over xs (factor order), a barrier after each level. The level bounds are
-// the same for every lane (scalar loads); each thread takes rows lp[l] + slot, + slots, ... of
-// level l. The record of its first row in each of the next two levels is fetched two levels
-// ahead, so a narrow level (the common case) waits only on its LDS reads and arithmetic; the
-// further rows of a wide level are fetched as they come. A row's terms are subtracted in its
-// packed order: the inline four first, then the rest from the sweep's arrays.
+// One row in place: xs[dst] = (xs[dst] - its terms) / diag, the terms subtracted in the row's
+// packed order, the inline four first, then the rest from the sweep's arrays.
 template <typename T>
 __device__ __forceinline__ void lds_row(const SweepDev& s, const T* __restrict__ val, T* xs, const Record<T>& r) {
   const int32_t len = r.p1 - r.p0;
@@ -394,43 +445,83 @@ __device__ __forceinline__ void lds_row(const SweepDev& s, const T* __restrict__
   if (len > 1) acc = acc - r.val[1] * x1;
   if (len > 2) acc = acc - r.val[2] * x2;
   if (len > 3) acc = acc - r.val[3] * x3;
-  for (int32_t p = r.p0 + RECORD_ENTRIES; p < r.p1; ++p) acc = acc - val[p] * xs[s.dep[p]];
+  // the rest in chunks: a chunk's loads are all issued before its first term is subtracted
+  for (int32_t p = r.p0 + RECORD_ENTRIES; p < r.p1; p += OVERFLOW_CHUNK) {
+    int32_t dd[OVERFLOW_CHUNK];
+    T vv[OVERFLOW_CHUNK], xx[OVERFLOW_CHUNK];
+#pragma unroll
+    for (int q = 0; q < OVERFLOW_CHUNK; ++q) {
+      const int32_t pq = p + q < r.p1 ? p + q : r.p1 - 1;
+      dd[q] = s.dep[pq];
+      vv[q] = val[pq];
+    }
+#pragma unroll
+    for (int q = 0; q < OVERFLOW_CHUNK; ++q) xx[q] = xs[dd[q]];
+#pragma unroll
+    for (int q = 0; q < OVERFLOW_CHUNK; ++q)
+      if (p + q < r.p1) acc = acc - vv[q] * xx[q];
+  }
   xs[r.dst] = s.unit ? acc : acc / r.diag;
 }
 
+// Levels a column sweep fetches its records ahead of use
+constexpr int LDS_AHEAD = 4;
+
+// One in-place sweep over xs (factor order), a barrier after each level; lp is the sweep's level
+// pointers, staged in LDS. Thread `slot` takes rows lp[l] + slot, + slots, ... of level l. The
+// record of its first row of each level is fetched LDS_AHEAD levels before it is used, into one
+// of LDS_AHEAD register slots that the loop (unrolled by LDS_AHEAD) never copies; the fetch is
+// issued whether or not the thread has a row there (the position is clamped to the sweep's
+// last), so the number of loads in flight, and the waits the compiler places, are the same on
+// every path. A narrow level (the common case) then waits only on LDS and its arithmetic; the
+// further rows of a wide level are fetched as they come.
 template <typename T>
-__device__ void lds_sweep(const SweepDev& s, T* xs) {
+__device__ void lds_sweep(const SweepDev& s, const int32_t* lp, T* xs) {
   const T* __restrict__ val = values_of<T>(s);
-  const int32_t* __restrict__ lp = s.level_ptr;
   const int32_t slot = threadIdx.x, slots = blockDim.x;
   const int32_t levels = s.levels;
-  // this thread's first row of level l, if it has one
-  auto first = [&](int32_t l, Record<T>& r) -> bool {
-    if (l >= levels) return false;
-    const int32_t k = lp[l] + slot;
-    if (k >= lp[l + 1]) return false;
-    fetch<T>(s, k, r);
-    return true;
+  if (levels == 0) return;
+  const int32_t last = lp[levels] - 1;
+  auto ahead = [&](int32_t m, Record<T>& r) {
+    const int32_t k = m < levels ? lp[m] + slot : last;
+    fetch<T>(s, k < last ? k : last, r);
   };
-  Record<T> ra, rb, rc;
-  bool ha = first(0, ra), hb = first(1, rb);
-  for (int32_t l = 0; l < levels; ++l) {
-    const bool hc = first(l + 2, rc);
-    if (ha) {
-      lds_row<T>(s, val, xs, ra);
-      const int32_t end = lp[l + 1];
-      for (int32_t k = lp[l] + slot + slots; k < end; k += slots) {
-        Record<T> r;
-        fetch<T>(s, k, r);
-        lds_row<T>(s, val, xs, r);
+  // level l from the record slot holding it; `refill` then fetches level l + LDS_AHEAD into it
+  auto level = [&](int32_t l, Record<T>& r, bool refill) {
+    const int32_t end = lp[l + 1];
+    int32_t k = lp[l] + slot;
+    if (k < end) {
+      lds_row<T>(s, val, xs, r);
+      for (k += slots; k < end; k += slots) {
+        Record<T> extra;
+        fetch<T>(s, k, extra);
+        lds_row<T>(s, val, xs, extra);
       }
     }
-    ra = rb;
-    ha = hb;
-    rb = rc;
-    hb = hc;
+    if (refill) {
+      await_record<T>(r);
+      ahead(l + LDS_AHEAD, r);
+    }
     lds_barrier();
+  };
+  Record<T> r[LDS_AHEAD];
+#pragma unroll
+  for (int d = 0; d < LDS_AHEAD; ++d) ahead(d, r[d]);
+  // whole rounds of LDS_AHEAD levels, with no exit but the loop's own test (so the loop's head is
+  // reached only from a full round, and its waits stay LDS_AHEAD - 1 fetches deep), then the rest
+  int32_t l0 = 0;
+  for (; l0 + LDS_AHEAD <= levels; l0 += LDS_AHEAD) {
+#pragma unroll
+    for (int d = 0; d < LDS_AHEAD; ++d) level(l0 + d, r[d], true);
   }
+#pragma unroll
+  for (int d = 0; d < LDS_AHEAD; ++d)
+    if (l0 + d < levels) level(l0 + d, r[d], false);
+}
+
+// Copies a sweep's level pointers into LDS (every thread must call; the caller's barrier ends it)
+__device__ __forceinline__ void stage_level_ptr(const SweepDev& s, int32_t* lp) {
+  for (int32_t l = threadIdx.x; l <= s.levels; l += blockDim.x) lp[l] = s.level_ptr[l];
 }
 
 template <typename T>
@@ -440,6 +531,10 @@ __global__ void __launch_bounds__(1024) solve_lds_kernel(SweepDev fwd, SweepDev 
                                                          double* __restrict__ x, int64_t ldx) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* xs = reinterpret_cast<T*>(smem);
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem + lds_vector_bytes(n, sizeof(T)));
+  int32_t* lpb = lpf + fwd.levels + 1;
+  stage_level_ptr(fwd, lpf);
+  stage_level_ptr(bwd, lpb);
   const int64_t col = blockIdx.x;
   if (rhs.block) {
     const double* b = rhs.block + col * rhs.ld;
@@ -455,8 +550,8 @@ __global__ void __launch_bounds__(1024) solve_lds_kernel(SweepDev fwd, SweepDev 
     }
   }
   __syncthreads();
-  lds_sweep<T>(fwd, xs);
-  lds_sweep<T>(bwd, xs);
+  lds_sweep<T>(fwd, lpf, xs);
+  lds_sweep<T>(bwd, lpb, xs);
   double* out = x + col * ldx;
   for (int32_t u = threadIdx.x; u < n; u += blockDim.x) out[u] = (double)xs[sol_of[u]];
 }
@@ -506,6 +601,10 @@ __global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev
   double* scratch = reinterpret_cast<double*>(smem);
   const int32_t n = a.rows;
   float* xs = reinterpret_cast<float*>(smem + blockDim.x * sizeof(double));
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem + blockDim.x * sizeof(double) + lds_vector_bytes(n, sizeof(float)));
+  int32_t* lpb = lpf + fwd.levels + 1;
+  stage_level_ptr(fwd, lpf);
+  stage_level_ptr(bwd, lpb);
   const int64_t col = blockIdx.x;
   const double* b = bblock + col * ld;
   double* xa = x0b + col * ld;
@@ -518,8 +617,8 @@ __global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev
 
   for (int32_t u = threadIdx.x; u < n; u += blockDim.x) xs[rhs_row[u]] = (float)b[u];
   __syncthreads();
-  lds_sweep<float>(fwd, xs);
-  lds_sweep<float>(bwd, xs);
+  lds_sweep<float>(fwd, lpf, xs);
+  lds_sweep<float>(bwd, lpb, xs);
   for (int32_t u = threadIdx.x; u < n; u += blockDim.x) xa[u] = (double)xs[sol_of[u]];
   __syncthreads();
   double ratio = ratio_of(block_max(scratch, column_residual(a, b, xa, r)), bnorm);
@@ -532,8 +631,8 @@ __global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev
   for (int32_t step = 1; active; ++step) {
     for (int32_t u = threadIdx.x; u < n; u += blockDim.x) xs[rhs_row[u]] = (float)r[u];
     __syncthreads();
-    lds_sweep<float>(fwd, xs);
-    lds_sweep<float>(bwd, xs);
+    lds_sweep<float>(fwd, lpf, xs);
+    lds_sweep<float>(bwd, lpb, xs);
     for (int32_t u = threadIdx.x; u < n; u += blockDim.x) nxt[u] = cur[u] + (double)xs[sol_of[u]];
     __syncthreads();
     const double previous = ratio;
@@ -766,53 +865,41 @@ __global__ void operator_kernel(CsrDev m, const double* __restrict__ x, int64_t 
     default: return hipErrorInvalidValue; \
   }
 
-// Rows per thread batch in the wide sweeps: 1, 2 or 4
-#define DISPATCH_ROWS(rows, TWV, F) \
-  switch (rows) {                   \
-    case 1: F(TWV, 1); break;       \
-    case 2: F(TWV, 2); break;       \
-    case 4: F(TWV, 4); break;       \
-    default: return hipErrorInvalidValue; \
-  }
 
-hipError_t launch_solve64(hipStream_t stream, int tile, int rows, int threads, const SweepDev& fwd, const SweepDev& bwd,
+hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                           const RhsDev& rhs, double* t, double* x, int64_t ld, int32_t ncols) {
   if (threads < tile || threads > 1024 || threads % 64) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
   const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
-#define L2(TWV, RV)                                                                                                 \
+#define L(TWV)                                                                                                 \
   {                                                                                                              \
-    hipError_t e = hipFuncSetAttribute((const void*)solve64_kernel<TWV, RV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+    hipError_t e = hipFuncSetAttribute((const void*)solve64_kernel<TWV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                        (int)bytes);                                                              \
     if (e != hipSuccess) return e;                                                                               \
-    hipLaunchKernelGGL((solve64_kernel<TWV, RV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, t, x, ld,    \
+    hipLaunchKernelGGL((solve64_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, t, x, ld,    \
                        ncols);                                                                                   \
   }
-#define L(TWV) DISPATCH_ROWS(rows, TWV, L2)
   DISPATCH_TILE(tile, L)
 #undef L
-#undef L2
   return hipGetLastError();
 }
 
-hipError_t launch_refine(hipStream_t stream, int tile, int rows, int threads, const SweepDev& fwd, const SweepDev& bwd,
+hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                          const CsrDev& a, const double* b, float* t, float* d, double* x0, double* x1, double* r,
                          int64_t ld, int32_t ncols, double tol, int32_t max_steps, int32_t* steps, double* ratio) {
   if (threads < tile || threads > 1024 || threads % 64 || (threads & (threads - 1))) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
   const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
-#define L2(TWV, RV)                                                                                                 \
+#define L(TWV)                                                                                                 \
   {                                                                                                              \
-    hipError_t e = hipFuncSetAttribute((const void*)refine_kernel<TWV, RV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+    hipError_t e = hipFuncSetAttribute((const void*)refine_kernel<TWV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                        (int)bytes);                                                              \
     if (e != hipSuccess) return e;                                                                               \
-    hipLaunchKernelGGL((refine_kernel<TWV, RV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, d, x0, x1, \
+    hipLaunchKernelGGL((refine_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, d, x0, x1, \
                        r, ld, ncols, tol, max_steps, steps, ratio);                                              \
   }
-#define L(TWV) DISPATCH_ROWS(rows, TWV, L2)
   DISPATCH_TILE(tile, L)
 #undef L
-#undef L2
   return hipGetLastError();
 }
 
@@ -863,12 +950,12 @@ hipError_t launch_operator(hipStream_t stream, const CsrDev& m, const double* x,
   return hipGetLastError();
 }
 
-size_t solve_lds_bytes(int32_t n, bool fp32, int32_t) {
-  return lds_vector_bytes(n, fp32 ? sizeof(float) : sizeof(double));
+size_t solve_lds_bytes(int32_t n, bool fp32, int32_t levels) {
+  return lds_vector_bytes(n, fp32 ? sizeof(float) : sizeof(double)) + (size_t)(levels + 2) * sizeof(int32_t);
 }
 
-size_t refine_lds_bytes(int32_t n, int threads, int32_t) {
-  return (size_t)threads * sizeof(double) + lds_vector_bytes(n, sizeof(float));
+size_t refine_lds_bytes(int32_t n, int threads, int32_t levels) {
+  return (size_t)threads * sizeof(double) + lds_vector_bytes(n, sizeof(float)) + (size_t)(levels + 2) * sizeof(int32_t);
 }
 
 hipError_t launch_solve_lds(hipStream_t stream, int threads, bool fp32, const SweepDev& fwd, const SweepDev& bwd,

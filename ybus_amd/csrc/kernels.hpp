@@ -10,7 +10,8 @@ namespace ybus_hip {
 
 // A sweep row's work as a record: the row it writes (dst) and reads its own right-hand side from
 // (src), its entries p0..p1 in the sweep's arrays, the first RECORD_ENTRIES of them inline
-// (dependency and value), and its diagonal. The records are stored field by field (one array
+// (dependency and value), and its diagonal (stored as 1 for a unit-diagonal sweep, so a fetch is
+// the same loads for every sweep). The records are stored field by field (one array
 // of 16-byte items per field, indexed by position), so the lanes of a wavefront that fetch
 // consecutive rows' records read consecutive 16-byte items: every fetch is coalesced, and a
 // thread can fetch its next rows' records ahead with independent loads.
@@ -21,6 +22,8 @@ struct RecordsDev {
   const double2* val01 = nullptr;  // their FP64 values, two at a time (unused ones: 0)
   const double2* val23 = nullptr;
   const float4* val32 = nullptr;   // their FP32 values
+  const double* diag64 = nullptr;  // the row's diagonal (1 for a unit diagonal)
+  const float* diag32 = nullptr;
 };
 
 // One scheduled triangular sweep on the device (see analysis.hpp's Sweep).
@@ -83,14 +86,14 @@ hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fw
                              int32_t max_steps, int32_t* steps, double* ratio);
 
 // FP64 forward then backward sweep over column tiles of width `tile` (1..64, a power of two),
-// `threads` per workgroup (64..1024), each thread taking `rows` (1, 2 or 4) rows of a level at a
-// time: t (packed) is the forward result, x the solution.
-hipError_t launch_solve64(hipStream_t stream, int tile, int rows, int threads, const SweepDev& fwd, const SweepDev& bwd,
+// `threads` per workgroup (64..1024): t (packed) is the forward result, x the solution. t and x
+// hold a pad row after their n rows.
+hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                           const RhsDev& rhs, double* t, double* x, int64_t ld, int32_t ncols);
 
 // FP32 sweeps refined in FP64 to `tol` (refine_normwise's rule), one workgroup per column tile;
 // the best iterate ends in x0. steps/ratio get each column's corrections and relative residual.
-hipError_t launch_refine(hipStream_t stream, int tile, int rows, int threads, const SweepDev& fwd, const SweepDev& bwd,
+hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                          const CsrDev& a, const double* b, float* t, float* d, double* x0, double* x1, double* r,
                          int64_t ld, int32_t ncols, double tol, int32_t max_steps, int32_t* steps, double* ratio);
 

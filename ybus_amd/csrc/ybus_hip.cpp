@@ -109,22 +109,34 @@ struct DevSweep {
   DevBuf<int4> head, dep4;
   DevBuf<double2> val01, val23;
   DevBuf<float4> v32;
+  DevBuf<double> rdiag64;
+  DevBuf<float> rdiag32;
   Sweep host;
-  void upload(Sweep s, hipStream_t stream) {
+  // `wide`: the sweep reads its dependencies from a row-interleaved block with a pad row after
+  // its n rows; a record's unused inline slots point there (one line every workgroup keeps in
+  // cache) rather than at a row of the block, so a row with fewer than four terms issues no load
+  // that misses. A column sweep's unused slots point at the row's own slot in LDS.
+  void upload(Sweep s, hipStream_t stream, bool wide) {
     host = std::move(s);
     const int32_t n = host.n;
     std::vector<int4> hd(n), dp(n);
     std::vector<double2> a(n), b(n);
     std::vector<float4> c(n);
+    std::vector<double> rd(n, 1.0);
+    std::vector<float> rd32(n, 1.0f);
     for (int32_t k = 0; k < n; ++k) {
+      if (!host.unit) {
+        rd[k] = host.diag[k];
+        rd32[k] = (float)host.diag[k];
+      }
       const int32_t p0 = host.row_ptr[k], p1 = host.row_ptr[k + 1];
       hd[k] = make_int4(host.dst[k], host.src[k], p0, p1);
       int32_t d[RECORD_ENTRIES];
       double v[RECORD_ENTRIES];
       for (int q = 0; q < RECORD_ENTRIES; ++q) {
-        // unused slots point at the row's own destination (a valid index) with value 0
+        // unused slots: value 0, and an index that is valid and cheap to read
         const bool inside = p0 + q < p1;
-        d[q] = inside ? host.dep[p0 + q] : host.dst[k];
+        d[q] = inside ? host.dep[p0 + q] : (wide ? n : host.dst[k]);
         v[q] = inside ? host.val[p0 + q] : 0.0;
       }
       dp[k] = make_int4(d[0], d[1], d[2], d[3]);
@@ -137,6 +149,8 @@ struct DevSweep {
     val01.upload(a, stream);
     val23.upload(b, stream);
     v32.upload(c, stream);
+    rdiag64.upload(rd, stream);
+    rdiag32.upload(rd32, stream);
     level_ptr.upload(host.level_ptr, stream);
     src.upload(host.src, stream);
     dst.upload(host.dst, stream);
@@ -170,6 +184,8 @@ struct DevSweep {
     v.rec.val01 = val01.p;
     v.rec.val23 = val23.p;
     v.rec.val32 = v32.p;
+    v.rec.diag64 = rdiag64.p;
+    v.rec.diag32 = rdiag32.p;
     v.unit = host.unit;
     return v;
   }
@@ -271,7 +287,7 @@ struct ybus_hip_factor {
   ybus_hip_timings timings{};
   ybus_hip_kernel_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int tile_override = 0, threads_wide = 1024, threads_columns = 512, rows = 1;
+  int tile_override = 0, threads_wide = 512, threads_columns = 512;
 
   ~ybus_hip_factor() {
     if (ev0) (void)hipEventDestroy(ev0);
@@ -296,8 +312,10 @@ struct ybus_hip_factor {
     const bool columns = layout.columns;
     layout.ld = columns ? nn : round_up(std::max<int64_t>(cols, 1), 16);
     const size_t elems = (size_t)nn * (size_t)(columns ? std::max<int64_t>(cols, 1) : layout.ld);
+    // the blocks the wide sweeps read dependencies from carry a pad row (see DevSweep::upload)
+    const size_t padded = columns ? elems : elems + (size_t)layout.ld;
     b.reserve(elems);
-    x.reserve(elems);
+    x.reserve(padded);
     y2.reserve(elems);
     const size_t per = (size_t)std::max<int64_t>(cols, 16);
     plus.reserve(per);
@@ -306,15 +324,15 @@ struct ybus_hip_factor {
     isteps.reserve(per);
     dratio.reserve(per);
     if (!columns) {
-      t.reserve(elems);
+      t.reserve(padded);
       vec.reserve(elems);
     }
     if (refine) {
       x1.reserve(elems);
       r.reserve(elems);
       if (!columns) {
-        t32.reserve(elems);
-        d32.reserve(elems);
+        t32.reserve(padded);
+        d32.reserve(padded);
       }
     }
   }
@@ -368,7 +386,7 @@ struct ybus_hip_factor {
       const DevSweep& f = transpose ? tfwd : fwd;
       const DevSweep& bw = transpose ? tbwd : bwd;
       RhsDev rhs{pairs ? nullptr : b.p, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
-      check(launch_solve64(stream(), tile_for(ncols), rows, threads_wide, f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
+      check(launch_solve64(stream(), tile_for(ncols), threads_wide, f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
                            (int32_t)ncols),
             "solve kernel");
     }
@@ -385,7 +403,7 @@ struct ybus_hip_factor {
                               r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     } else {
-      check(launch_refine(stream(), tile_for(ncols), rows, threads_wide, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p,
+      check(launch_refine(stream(), tile_for(ncols), threads_wide, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p,
                           x.p, x1.p, r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     }
@@ -514,8 +532,6 @@ static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   if (thr) f->threads_wide = f->threads_columns = std::clamp(std::atoi(thr), 64, 1024) / 64 * 64;
   const char* tile = std::getenv("YBUS_HIP_TILE");
   if (tile) f->tile_override = std::atoi(tile);
-  const char* rows = std::getenv("YBUS_HIP_ROWS");
-  if (rows) f->rows = std::atoi(rows);
   const char* path = std::getenv("YBUS_HIP_PATH");
   if (path) f->force = std::string(path) == "wide" ? 1 : std::string(path) == "columns" ? 2 : 0;
   f->column_limit = 2 * (int64_t)ctx->cus;
@@ -583,13 +599,13 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
     f->fits = fits_columns(f.get(), sf.levels + sb.levels);
     if (f->fits) {
       // cholesky.rs:570-579: y[i] = b[perm[i]], and x[perm[i]] = y[i]
-      f->cfwd.upload(std::move(sf), ctx->stream);
-      f->cbwd.upload(std::move(sb), ctx->stream);
+      f->cfwd.upload(std::move(sf), ctx->stream, false);
+      f->cbwd.upload(std::move(sb), ctx->stream, false);
       f->rhs_row.upload(inverse_i32(gather), ctx->stream);
       f->sol_of.upload(inverse_i32(gather), ctx->stream);
     }
-    f->fwd.upload(schedule(lower, gather, ident), ctx->stream);
-    f->bwd.upload(schedule(upper, ident, gather), ctx->stream);
+    f->fwd.upload(schedule(lower, gather, ident), ctx->stream, true);
+    f->bwd.upload(schedule(upper, ident, gather), ctx->stream, true);
     check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
     *out = f.release();
@@ -636,19 +652,19 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
     f->fits = fits_columns(f.get(), std::max(sl.levels + su.levels, stu.levels + stl.levels));
     if (f->fits) {
       // lu.rs:770-785: y[i] = b[P[i]], x[Q[i]] = y[i]; transposed (lu.rs:506-533): through Q, then P
-      f->cfwd.upload(std::move(sl), ctx->stream);
-      f->cbwd.upload(std::move(su), ctx->stream);
-      f->ctfwd.upload(std::move(stu), ctx->stream);
-      f->ctbwd.upload(std::move(stl), ctx->stream);
+      f->cfwd.upload(std::move(sl), ctx->stream, false);
+      f->cbwd.upload(std::move(su), ctx->stream, false);
+      f->ctfwd.upload(std::move(stu), ctx->stream, false);
+      f->ctbwd.upload(std::move(stl), ctx->stream, false);
       f->rhs_row.upload(inverse_i32(rp), ctx->stream);
       f->sol_of.upload(inverse_i32(cp), ctx->stream);
       f->t_rhs_row.upload(inverse_i32(cp), ctx->stream);
       f->t_sol_of.upload(inverse_i32(rp), ctx->stream);
     }
-    f->fwd.upload(schedule(l, rp, ident), ctx->stream);
-    f->bwd.upload(schedule(u, ident, cp), ctx->stream);
-    f->tfwd.upload(schedule(ut, cp, ident), ctx->stream);
-    f->tbwd.upload(schedule(lt, ident, rp), ctx->stream);
+    f->fwd.upload(schedule(l, rp, ident), ctx->stream, true);
+    f->bwd.upload(schedule(u, ident, cp), ctx->stream, true);
+    f->tfwd.upload(schedule(ut, cp, ident), ctx->stream, true);
+    f->tbwd.upload(schedule(lt, ident, rp), ctx->stream, true);
     f->has_transpose = true;
     check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
