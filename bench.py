@@ -108,6 +108,20 @@ def cpu_dense_baseline(factor, rhs):
     return reps * rhs.shape[0] / (time.perf_counter() - t0), threads, out
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on the PTDF block, from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by what the counter reads for a plain
+    1 GiB device copy); None when there is no summary for this kernel."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_ptdf_10k_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 # ---- GPU legs ----
 
 def pair_block(grid):
@@ -249,6 +263,7 @@ def main():
     kernel_ms = stats.milliseconds / stats.launches
     bytes_per_launch = stats.bytes / stats.launches
     achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
+    traffic, traffic_source = profiled_traffic(ptdf["kernel"])
 
     # parity: sampled columns against the CPU (bit for bit)
     sample = np.arange(0, rhs_count, max(1, rhs_count // 256))
@@ -288,7 +303,7 @@ def main():
                                "on the device, FP64 forward+backward", "buses": 10000, "n": int(chol.n),
                    "nnz_l": nnz_l, "rhs": rhs_count, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": ptdf["kernel"], "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu_baseline,
         "parity": {"sampled_columns": int(len(sample)), "bit_exact_columns": bit_exact,
