@@ -160,6 +160,17 @@ int32_t ybus_hip_factor_solve_many_refined(ybus_hip_factor *f, const double *rhs
                                            double tolerance, uint64_t max_steps, double *x,
                                            uint64_t *steps, double *ratio);
 
+/* The adjoint of A x = b (new: the reference has no GPU gradient; its CPU counterpart is the
+ * composition of Lu::solve_transpose, lu.rs:502-533, with an outer product). For `count` pairs
+ * (x_bar_s, x_s), x_bar_s being the gradient of a scalar loss with respect to the solution x_s of
+ * A x_s = b_s: b_bar_s = A^-T x_bar_s (the loss's gradient with respect to b_s; FP64, the CPU's
+ * bits), and a_bar = -(sum over s of b_bar_s x_s^T) on A's stored pattern (its gradient with
+ * respect to A's stored values), each entry summed over s in ascending order, so it is the same
+ * on every run. Needs ybus_hip_factor_set_matrix; a_bar has one value per stored entry of that
+ * matrix, in the order the caller stored them. */
+int32_t ybus_hip_factor_adjoint(ybus_hip_factor *f, const double *x_bar, const double *x,
+                                uint64_t count, double *b_bar, double *a_bar);
+
 /* ---- device-resident blocks ---- */
 
 /* Copies `count` right-hand sides to the device, into the factor's staged block (at most

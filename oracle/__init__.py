@@ -464,3 +464,26 @@ def injections(seed: int, stream: int, buses: int, count: int) -> np.ndarray:
     out = np.zeros(count * (buses - 1))
     lib().yo_injections(seed, stream, buses, count, out)
     return out.reshape(count, buses - 1)
+
+
+# ---- the adjoint (no reference counterpart: the composition the GPU adjoint is checked against) ----
+
+def adjoint(factor, a: Matrix, x_bar, x):
+    """CPU composition of the adjoint of A x = b: b_bar_s = A^-T x_bar_s (Lu.solve_transpose_many,
+    lu.rs:502-533; a Cholesky factor's A^T = A, so its solve_many), and a_bar = -(sum over s of
+    b_bar_s x_s^T) on A's stored pattern, each stored entry summed over s in ascending order, one
+    product and one addition rounded at a time. Returns (b_bar, a_bar), a_bar in A's storage
+    order."""
+    x_bar = np.asarray(x_bar, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    if isinstance(factor, Lu):
+        b_bar = factor.solve_transpose_many(x_bar) if len(x_bar) else np.zeros_like(x_bar)
+    else:
+        b_bar = factor.solve_many(x_bar) if len(x_bar) else np.zeros_like(x_bar)
+    major = np.repeat(np.arange(len(a.ptr) - 1), np.diff(np.asarray(a.ptr, dtype=np.int64)))
+    minor = np.asarray(a.idx, dtype=np.int64)
+    rows, cols = (major, minor) if a.row_major else (minor, major)
+    acc = np.zeros(a.nnz)
+    for s in range(len(x)):
+        acc = acc + b_bar[s, rows] * x[s, cols]
+    return b_bar, -acc

@@ -112,6 +112,7 @@ def _declare(L):
         "ybus_hip_factor_solve_many": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_transpose_many": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_many_refined": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
+        "ybus_hip_factor_adjoint": (i32, [P, f64p, f64p, U64, f64p, f64p]),
         "ybus_hip_factor_stage": (i32, [P, f64p, U64]),
         "ybus_hip_factor_stage_pairs": (i32, [P, i64p, i64p, U64]),
         "ybus_hip_factor_solve_staged": (i32, [P, i32, C.c_double, U64, C.POINTER(P)]),
@@ -298,6 +299,7 @@ class GpuFactor:
         self._gpu = gpu  # keeps the context alive
         self._h = handle
         self._n = n
+        self._nnz_a = -1
         self._block: Optional[GpuBlock] = None
 
     @staticmethod
@@ -370,6 +372,7 @@ class GpuFactor:
         """The matrix itself (CSR when ``a.row_major`` else CSC), for FP32_REFINED residuals."""
         _check(library().ybus_hip_factor_set_matrix(self._h, int(bool(a.row_major)), _u64(a.ptr), _u64(a.idx),
                                                     _f64(a.data)))
+        self._nnz_a = int(np.asarray(a.ptr)[-1])
 
     def _release_block(self):
         if self._block is not None:
@@ -405,6 +408,20 @@ class GpuFactor:
             _check(library().ybus_hip_factor_solve_many_refined(self._h, b, b.shape[0], float(tolerance),
                                                                 int(max_steps), x, steps, ratio))
         return x, steps.astype(np.int64), ratio
+
+    def adjoint(self, x_bar, x):
+        """The adjoint of A x = b (needs set_matrix): for vectors x_bar_s (a loss's gradient with
+        respect to the solutions x_s), returns (b_bar, a_bar): b_bar_s = A^-T x_bar_s, and a_bar =
+        -(sum over s of b_bar_s x_s^T) on A's stored pattern, in the matrix's storage order."""
+        xb = _rhs_array(x_bar, self._n)
+        xs = _rhs_array(x, self._n)
+        if xb.shape != xs.shape:
+            raise ShapeMismatch(xb.shape[0], xs.shape[0])
+        self._release_block()
+        b_bar = np.zeros_like(xb)
+        a_bar = np.zeros(max(self._nnz_a, 0))
+        _check(library().ybus_hip_factor_adjoint(self._h, xb, xs, xb.shape[0], b_bar, a_bar))
+        return b_bar, a_bar
 
     def stage(self, rhs):
         """Copies right-hand sides (at most block_width()) to the device without solving."""

@@ -849,6 +849,24 @@ __global__ void operator_kernel(CsrDev m, const double* __restrict__ x, int64_t 
   out[r * ors + c * ocs] = zero[c] == r ? 0.0 : sum + offset[r];
 }
 
+// ---- adjoint: the gradient with respect to A's stored values (SDDMM on A's pattern) ----
+
+// For each stored entry q = (row[q], col[q]) of A: acc[q] = (first ? 0 : acc[q]) + the sum over
+// the block's columns c, in ascending order, of bbar(row, c) * x(col, c). Blocks of columns taken
+// one after another therefore give each entry's sum over all columns in one fixed order, with no
+// atomics: the result is the same on every run.
+__global__ void sddmm_kernel(const int32_t* __restrict__ row, const int32_t* __restrict__ col, int64_t nnz,
+                             const double* __restrict__ bbar, const double* __restrict__ x, int64_t rs, int64_t cs,
+                             int32_t ncols, double* __restrict__ acc, int32_t first) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  const double* __restrict__ bi = bbar + (int64_t)row[q] * rs;
+  const double* __restrict__ xj = x + (int64_t)col[q] * rs;
+  double sum = first ? 0.0 : acc[q];
+  for (int32_t c = 0; c < ncols; ++c) sum = sum + bi[c * cs] * xj[c * cs];
+  acc[q] = sum;
+}
+
 }  // namespace
 
 // ---- launchers ----
@@ -937,6 +955,15 @@ hipError_t launch_column_max(hipStream_t stream, const double* block, int32_t ro
   const unsigned grid = (unsigned)((ncols + 63) / 64);
   hipLaunchKernelGGL(column_max_kernel, dim3(grid), dim3(256), 0, stream, block, rows, ncols, rs, cs, weight, best,
                      best_row, not_finite);
+  return hipGetLastError();
+}
+
+hipError_t launch_sddmm(hipStream_t stream, const int32_t* row, const int32_t* col, int64_t nnz, const double* bbar,
+                        const double* x, int64_t rs, int64_t cs, int32_t ncols, double* acc, bool first) {
+  if (nnz <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((nnz + 255) / 256);
+  hipLaunchKernelGGL(sddmm_kernel, dim3(grid), dim3(256), 0, stream, row, col, nnz, bbar, x, rs, cs, ncols, acc,
+                     first ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -123,4 +123,10 @@ hipError_t launch_operator(hipStream_t stream, const CsrDev& m, const double* x,
                            const double* offset, const int32_t* zero, double* out, int64_t ors, int64_t ocs,
                            int32_t ncols);
 
+// The adjoint's matrix gradient on A's pattern: for each stored entry q = (row[q], col[q]),
+// acc[q] = (first ? 0 : acc[q]) + sum over columns c ascending of bbar(row, c) * x(col, c); both
+// blocks addressed with strides (rs, cs) as above.
+hipError_t launch_sddmm(hipStream_t stream, const int32_t* row, const int32_t* col, int64_t nnz, const double* bbar,
+                        const double* x, int64_t rs, int64_t cs, int32_t ncols, double* acc, bool first);
+
 }  // namespace ybus_hip

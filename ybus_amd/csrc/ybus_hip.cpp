@@ -197,6 +197,8 @@ struct DevCsr {
   DevBuf<int32_t> ptr, col;
   DevBuf<double> val;
   size_t nnz = 0;
+  DevBuf<int32_t> row;         // each entry's row (the adjoint's SDDMM takes entries one by one)
+  std::vector<int64_t> order;  // each entry's position in the matrix as the caller stored it
   CsrDev view() const { return CsrDev{rows, cols, ptr.p, col.p, val.p}; }
 };
 
@@ -280,6 +282,7 @@ struct ybus_hip_factor {
   DevBuf<float> t32, d32;
   DevBuf<int32_t> plus, minus, isteps, singular;
   DevBuf<double> dratio;
+  DevBuf<double> abar;  // the adjoint's running sums, one per stored entry of the matrix
   int64_t staged = 0;
   bool staged_pairs = false;  // the staged block is pairs (plus, minus), not vectors (b)
   ybus_hip_block* borrowed = nullptr;
@@ -717,11 +720,13 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
     if (nnz > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the matrix is too large for 32-bit indices");
     std::vector<int32_t> rptr(n + 1), cols(nnz);
     std::vector<double> vals(nnz);
+    std::vector<int64_t> order(nnz);
     if (row_major) {
       for (int64_t i = 0; i <= n; ++i) rptr[i] = (int32_t)ptr[i];
       for (uint64_t p = 0; p < nnz; ++p) {
         cols[p] = (int32_t)idx[p];
         vals[p] = val[p];
+        order[p] = (int64_t)p;
       }
     } else {
       // CSC to CSR by a counting sort by row: each row's columns come out ascending, the order
@@ -736,27 +741,35 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
           int32_t q = cursor[idx[p]]++;
           cols[q] = (int32_t)j;
           vals[q] = val[p];
+          order[q] = (int64_t)p;
         }
     }
+    std::vector<int32_t> rows(nnz);
+    for (int64_t i = 0; i < n; ++i)
+      for (int32_t q = rptr[i]; q < rptr[i + 1]; ++q) rows[q] = (int32_t)i;
     f->a.rows = f->a.cols = (int32_t)n;
     f->a.nnz = nnz;
     f->a.ptr.upload(rptr, f->stream());
     f->a.col.upload(cols, f->stream());
     f->a.val.upload(vals, f->stream());
+    f->a.row.upload(rows, f->stream());
+    f->a.order = std::move(order);
     check(hipStreamSynchronize(f->stream()), "upload matrix");
   });
 }
 
-// Copies count host vectors into the device block b, in the factor's layout
-static void upload_vectors(ybus_hip_factor* f, const double* rhs, int64_t count) {
+// Copies count host vectors into a device block of the workspace (the right-hand sides b unless
+// another is named), in the workspace's layout
+static void upload_vectors(ybus_hip_factor* f, const double* rhs, int64_t count, double* block = nullptr) {
+  if (!block) block = f->b.p;
   const size_t len = (size_t)count * (size_t)f->n;
   double* host = f->staging.get(len);
   std::memcpy(host, rhs, len * sizeof(double));
   if (f->layout.columns) {
-    check(hipMemcpyAsync(f->b.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+    check(hipMemcpyAsync(block, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
   } else {
     check(hipMemcpyAsync(f->vec.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
-    check(launch_to_block(f->stream(), f->vec.p, f->b.p, f->n, count, f->layout.ld), "to_block");
+    check(launch_to_block(f->stream(), f->vec.p, block, f->n, count, f->layout.ld), "to_block");
   }
   check(hipStreamSynchronize(f->stream()), "upload");
 }
@@ -835,6 +848,46 @@ int32_t ybus_hip_factor_solve_transpose_many(ybus_hip_factor* f, const double* r
 int32_t ybus_hip_factor_solve_many_refined(ybus_hip_factor* f, const double* rhs, uint64_t count, double tolerance,
                                            uint64_t max_steps, double* x, uint64_t* steps, double* ratio) {
   return solve_many_mode(f, rhs, count, x, YBUS_HIP_FP32_REFINED, tolerance, max_steps, steps, ratio);
+}
+
+int32_t ybus_hip_factor_adjoint(ybus_hip_factor* f, const double* x_bar, const double* x, uint64_t count,
+                                double* b_bar, double* a_bar) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    require_unborrowed(f);
+    if (!f->a.ptr.p) fail(YBUS_HIP_INVALID_ARGUMENT, "the adjoint needs the matrix (ybus_hip_factor_set_matrix)");
+    const int64_t nnz = (int64_t)f->a.nnz;
+    if (count && (!x_bar || !x || !b_bar)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (nnz && !a_bar) fail(YBUS_HIP_INVALID_ARGUMENT, "a_bar is null");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    std::vector<double> acc((size_t)nnz, 0.0);
+    if (count && f->n) {
+      const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
+      f->ensure_workspace(width, false);
+      f->abar.reserve((size_t)std::max<int64_t>(nnz, 1));
+      const uint64_t n = (uint64_t)f->n;
+      for (uint64_t start = 0; start < count; start += (uint64_t)width) {
+        const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
+        // b_bar = A^-T x_bar (the transposed solve, lu.rs:502-533; A^T = A for a Cholesky factor)
+        upload_vectors(f, x_bar + start * n, chunk);
+        f->solve64(true, false, chunk);
+        download_vectors(f, f->x.p, f->layout, chunk, b_bar + start * n);
+        // the solutions x beside it, then the running sums on A's pattern
+        upload_vectors(f, x + start * n, chunk, f->y2.p);
+        check(launch_sddmm(f->stream(), f->a.row.p, f->a.col.p, nnz, f->x.p, f->y2.p, f->layout.rs(),
+                           f->layout.cs(), (int32_t)chunk, f->abar.p, start == 0),
+              "adjoint kernel");
+      }
+      if (nnz) {
+        check(hipMemcpyAsync(acc.data(), f->abar.p, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost,
+                             f->stream()),
+              "D2H");
+        check(hipStreamSynchronize(f->stream()), "adjoint");
+      }
+    }
+    // A_bar = -(sum over the vectors of b_bar_s x_s^T), in the caller's storage order
+    for (int64_t q = 0; q < nnz; ++q) a_bar[f->a.order[q]] = -acc[q];
+  });
 }
 
 int32_t ybus_hip_factor_stage(ybus_hip_factor* f, const double* rhs, uint64_t count) {
