@@ -229,6 +229,55 @@ def copy_calibration(mib: int):
     log(f"[copy] {mib} MiB copied 5 times")
 
 
+def run_adjoint(gpu, args):
+    """The adjoint on the 70,000-bus grid: 64 zero-sum N(0,1) injection scenarios, branch limits
+    U[0.5, 2.0] pu, loss = sum relu(|f| - limit)^2 with f = Bf theta. The device solves theta,
+    the host forms x_bar = Bf^T (2 relu(|f| - limit) sign f), and the device computes
+    b_bar = B'^-T x_bar and A_bar = -(sum b_bar theta^T) on B''s pattern."""
+    import scipy.sparse as sp
+
+    import oracle as O
+    import ybus_amd as Y
+
+    buses, seed = 70000, SEED + 4
+    grid = O.make_grid(buses, seed)
+    bprime, bf, _ = grid.dc()
+    chol = O.Cholesky(bprime)
+    p = O.injections(seed, 4, buses, 64)
+    limit = 0.5 + 1.5 * O.uniforms(seed, 5, bf.rows)
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    factor.set_matrix(bprime)
+    factor.set_block_width(64)
+    theta = factor.solve_many(p)
+    m = sp.csr_matrix((bf.data, bf.idx, bf.ptr), shape=(bf.rows, bf.cols))
+    f = (m @ theta.T).T
+    over = np.maximum(np.abs(f) - limit, 0.0)
+    x_bar = np.ascontiguousarray((m.T @ (2.0 * over * np.sign(f)).T).T)
+    loss = float(np.sum(over * over))
+    for _ in range(max(1, args.warmup)):
+        factor.adjoint(x_bar, theta)
+    factor.reset_kernel_stats()
+    reps = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b_bar, a_bar = factor.adjoint(x_bar, theta)
+    seconds = (time.perf_counter() - t0) / reps
+    stats = factor.kernel_stats()
+    want_b, want_a = O.adjoint(chol, bprime, x_bar, theta)
+    out = dict(workload="70,000-bus grid (n=%d), B' AMD Cholesky (nnz(L)=%d, nnz(B')=%d), 64 scenarios, "
+                        "relu^2 flow-limit loss" % (chol.n, int(chol.l.nnz), bprime.nnz),
+               loss=loss, seconds_per_adjoint=seconds,
+               transposed_solve_kernel_ms=stats.milliseconds / stats.launches,
+               b_bar_bit_exact=bool(b_bar.tobytes() == want_b.tobytes()),
+               a_bar_bit_exact=bool(a_bar.tobytes() == want_a.tobytes()),
+               note="seconds_per_adjoint includes the host-vector copies (2 x 64 x n doubles in, "
+                    "64 x n + nnz(B') out)")
+    log(f"[adjoint] {seconds * 1e3:.2f} ms per adjoint (host vectors), transposed solve "
+        f"{out['transposed_solve_kernel_ms']:.3f} ms; bit-exact b_bar {out['b_bar_bit_exact']}, "
+        f"a_bar {out['a_bar_bit_exact']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +285,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB on the device (to scale HBM counters) and exit")
     args = ap.parse_args()
@@ -311,6 +361,8 @@ def main():
     }
     if not args.no_precision:
         line["precision"] = run_precision(gpu, args)
+    if not args.no_adjoint:
+        line["adjoint"] = run_adjoint(gpu, args)
     print(json.dumps(line), flush=True)
 
 

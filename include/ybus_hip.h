@@ -235,6 +235,33 @@ int32_t ybus_hip_factor_path(const ybus_hip_factor *f, uint64_t columns);
 void ybus_hip_factor_kernel_stats(const ybus_hip_factor *f, ybus_hip_kernel_stats *out);
 void ybus_hip_factor_reset_kernel_stats(ybus_hip_factor *f);
 
+/* ---- batches of factors (new: the reference solves one factor per call, factor.rs:234-289) ----
+ *
+ * Many uploaded factors, each with its own right-hand sides, solved in one persistent launch per
+ * class of factors (factors grouped by how many of their solution vectors fit a compute unit's
+ * LDS). Workgroups take (factor, column) items from a device queue, longest first, so small and
+ * large systems share the GPU without a launch per factor. The launches are captured once per
+ * staging in a HIP graph and replayed by each solve. FP64, the CPU's bits. A batch borrows its
+ * factors: they must outlive it, and each must take the column path (dim() small enough for one
+ * solution vector to fit in LDS; larger factors are rejected with YBUS_HIP_INVALID_ARGUMENT). */
+int32_t ybus_hip_batch_create(ybus_hip_ctx *ctx, ybus_hip_batch **out);
+void ybus_hip_batch_free(ybus_hip_batch *b);
+int32_t ybus_hip_batch_add(ybus_hip_batch *b, ybus_hip_factor *f);
+uint64_t ybus_hip_batch_factors(const ybus_hip_batch *b);
+/* Copies the right-hand sides to the device: counts[i] vectors of factor i's dim(), for each
+ * factor in the order added, all one after another. */
+int32_t ybus_hip_batch_stage(ybus_hip_batch *b, const double *rhs, const uint64_t *counts);
+/* Solves what is staged; the solutions stay on the device until ybus_hip_batch_solutions. */
+int32_t ybus_hip_batch_solve_staged(ybus_hip_batch *b);
+/* The staged right-hand sides' solutions, in the same arrangement as the right-hand sides. */
+int32_t ybus_hip_batch_solutions(ybus_hip_batch *b, double *x);
+/* stage + solve_staged + solutions. */
+int32_t ybus_hip_batch_solve(ybus_hip_batch *b, const double *rhs, const uint64_t *counts,
+                             double *x);
+/* HIP-event timings of the batch's solves (one launch = one replay of its graph). */
+void ybus_hip_batch_kernel_stats(const ybus_hip_batch *b, ybus_hip_kernel_stats *out);
+void ybus_hip_batch_reset_kernel_stats(ybus_hip_batch *b);
+
 /* ---- host-side symbolic analysis (no device needed; new) ----
  * The level schedule the device sweeps follow: level[i] of each factor row is one more than the
  * deepest row it depends on (0 for none). For a Cholesky factor (arrays as in upload_cholesky)

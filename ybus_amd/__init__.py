@@ -134,6 +134,16 @@ def _declare(L):
         "ybus_hip_operator_rows": (U64, [P]),
         "ybus_hip_operator_set_offset": (i32, [P, f64p]),
         "ybus_hip_operator_set_weights": (i32, [P, f64p]),
+        "ybus_hip_batch_create": (i32, [P, C.POINTER(P)]),
+        "ybus_hip_batch_free": (None, [P]),
+        "ybus_hip_batch_add": (i32, [P, P]),
+        "ybus_hip_batch_factors": (U64, [P]),
+        "ybus_hip_batch_stage": (i32, [P, C.c_void_p, u64p]),
+        "ybus_hip_batch_solve_staged": (i32, [P]),
+        "ybus_hip_batch_solutions": (i32, [P, C.c_void_p]),
+        "ybus_hip_batch_solve": (i32, [P, C.c_void_p, u64p, C.c_void_p]),
+        "ybus_hip_batch_kernel_stats": (None, [P, C.POINTER(_KernelStats)]),
+        "ybus_hip_batch_reset_kernel_stats": (None, [P]),
         "ybus_hip_analyze_cholesky": (i32, [U64, u64p, u64p, P, P, P, P]),
         "ybus_hip_analyze_lu": (i32, [U64, u64p, u64p, u64p, u64p, P, P, P, P]),
     }
@@ -584,3 +594,74 @@ class GpuOperator:
         if weights.shape[0] != self._rows:
             raise InvalidArgument(INVALID_ARGUMENT, "one weight per row")
         _check(library().ybus_hip_operator_set_weights(self._h, weights))
+
+
+# ---- batches of factors (new) ----
+
+
+class GpuBatch:
+    """Many factors solved together: each factor's right-hand sides, all in one persistent launch
+    per LDS size class, replayed from a captured HIP graph. The batch keeps its factors alive."""
+
+    def __init__(self, gpu: Gpu, handle):
+        self._gpu = gpu
+        self._h = handle
+        self._factors = []
+        self._staged = None
+
+    @staticmethod
+    def new(gpu: Gpu) -> "GpuBatch":
+        h = P()
+        _check(library().ybus_hip_batch_create(gpu._h, C.byref(h)))
+        return GpuBatch(gpu, h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            library().ybus_hip_batch_free(self._h)
+            self._h = None
+
+    def add(self, factor: GpuFactor):
+        _check(library().ybus_hip_batch_add(self._h, factor._h))
+        self._factors.append(factor)
+        self._staged = None
+
+    def factors(self) -> int:
+        return int(library().ybus_hip_batch_factors(self._h))
+
+    def stage(self, rhs):
+        """rhs: one (count_i, dim_i) array per factor, in the order added."""
+        if len(rhs) != len(self._factors):
+            raise InvalidArgument(INVALID_ARGUMENT, "one block of right-hand sides per factor")
+        blocks = [_rhs_array(r, f.dim()) for r, f in zip(rhs, self._factors)]
+        counts = np.array([b.shape[0] for b in blocks], dtype=np.uint64)
+        flat = np.ascontiguousarray(np.concatenate([b.ravel() for b in blocks])) if blocks else np.zeros(0)
+        _check(library().ybus_hip_batch_stage(self._h, flat.ctypes.data if flat.size else None, counts))
+        self._staged = [(int(c), f.dim()) for c, f in zip(counts, self._factors)]
+
+    def solve_staged(self):
+        _check(library().ybus_hip_batch_solve_staged(self._h))
+
+    def solutions(self):
+        if self._staged is None:
+            raise InvalidArgument(INVALID_ARGUMENT, "nothing staged")
+        total = sum(c * n for c, n in self._staged)
+        flat = np.zeros(total)
+        _check(library().ybus_hip_batch_solutions(self._h, flat.ctypes.data if total else None))
+        out, at = [], 0
+        for c, n in self._staged:
+            out.append(flat[at: at + c * n].reshape(c, n))
+            at += c * n
+        return out
+
+    def solve(self, rhs):
+        self.stage(rhs)
+        self.solve_staged()
+        return self.solutions()
+
+    def kernel_stats(self) -> KernelStats:
+        s = _KernelStats()
+        library().ybus_hip_batch_kernel_stats(self._h, C.byref(s))
+        return KernelStats(int(s.launches), float(s.milliseconds), int(s.bytes))
+
+    def reset_kernel_stats(self):
+        library().ybus_hip_batch_reset_kernel_stats(self._h)

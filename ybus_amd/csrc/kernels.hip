@@ -849,6 +849,54 @@ __global__ void operator_kernel(CsrDev m, const double* __restrict__ x, int64_t 
   out[r * ors + c * ocs] = zero[c] == r ? 0.0 : sum + offset[r];
 }
 
+// ---- batches of factors: persistent column solves fed by a device queue ----
+
+// One right-hand side b (contiguous, by unknown) of one factor solved with its solution in LDS:
+// smem holds the vector in factor order, then both sweeps' level pointers; out gets x by unknown.
+// Every thread must call; the caller's barrier must separate it from the next use of smem.
+__device__ void column_solve(const SweepDev& fwd, const SweepDev& bwd, int32_t n, const int32_t* __restrict__ rhs_row,
+                             const int32_t* __restrict__ sol_of, const double* __restrict__ b,
+                             double* __restrict__ out, unsigned char* smem) {
+  double* xs = reinterpret_cast<double*>(smem);
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem + lds_vector_bytes(n, sizeof(double)));
+  int32_t* lpb = lpf + fwd.levels + 1;
+  stage_level_ptr(fwd, lpf);
+  stage_level_ptr(bwd, lpb);
+  for (int32_t u = threadIdx.x; u < n; u += blockDim.x) xs[rhs_row[u]] = b[u];
+  __syncthreads();
+  lds_sweep<double>(fwd, lpf, xs);
+  lds_sweep<double>(bwd, lpb, xs);
+  for (int32_t u = threadIdx.x; u < n; u += blockDim.x) out[u] = xs[sol_of[u]];
+}
+
+// A persistent kernel over a queue of (factor, column) items: each workgroup takes the next item
+// (one atomic add on the queue's counter by one thread), solves it, and takes another, until the
+// queue is empty; every workgroup then sees an item past the end and returns. The items are
+// ordered longest first, so the last ones to start are the shortest.
+__global__ void __launch_bounds__(1024) batch_kernel(const BatchEntryDev* __restrict__ entries,
+                                                     const int2* __restrict__ items, int32_t total,
+                                                     int32_t* __restrict__ queue, const double* __restrict__ rhs,
+                                                     double* __restrict__ x) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_item;
+  for (;;) {
+    __syncthreads();  // the previous item is finished with smem and s_item
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int32_t it = s_item;
+    if (it >= total) return;
+    const int2 w = items[it];
+    const BatchEntryDev e = entries[w.x];
+    const int64_t at = e.offset + (int64_t)w.y * e.n;
+    column_solve(e.fwd, e.bwd, e.n, e.rhs_row, e.sol_of, rhs + at, x + at, smem);
+  }
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int32_t value, int32_t count) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < count) p[i] = value;
+}
+
 // ---- adjoint: the gradient with respect to A's stored values (SDDMM on A's pattern) ----
 
 // For each stored entry q = (row[q], col[q]) of A: acc[q] = (first ? 0 : acc[q]) + the sum over
@@ -955,6 +1003,23 @@ hipError_t launch_column_max(hipStream_t stream, const double* block, int32_t ro
   const unsigned grid = (unsigned)((ncols + 63) / 64);
   hipLaunchKernelGGL(column_max_kernel, dim3(grid), dim3(256), 0, stream, block, rows, ncols, rs, cs, weight, best,
                      best_row, not_finite);
+  return hipGetLastError();
+}
+
+hipError_t prepare_batch(size_t lds_bytes) {
+  return hipFuncSetAttribute((const void*)batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+hipError_t launch_batch(hipStream_t stream, int grid, int threads, size_t lds_bytes, const BatchEntryDev* entries,
+                        const int2* items, int32_t total, int32_t* queue, const double* rhs, double* x) {
+  if (grid <= 0 || threads < 64 || threads > 1024 || threads % 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(threads), lds_bytes, stream, entries, items, total, queue, rhs, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t stream, int32_t* p, int32_t value, int32_t count) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, p, value, count);
   return hipGetLastError();
 }
 

@@ -67,6 +67,17 @@ struct ColumnsDev {
   const int32_t* minus = nullptr;
 };
 
+// One factor of a batch: its column-path sweeps and maps, its order, and where its first column
+// starts in the batch's right-hand-side and solution arrays (columns of n values, one after
+// another).
+struct BatchEntryDev {
+  SweepDev fwd, bwd;
+  const int32_t* rhs_row = nullptr;
+  const int32_t* sol_of = nullptr;
+  int32_t n = 0;
+  int64_t offset = 0;
+};
+
 // LDS bytes the column solves need for n rows and the given total of both sweeps' levels (and
 // `threads` per workgroup for refinement).
 size_t solve_lds_bytes(int32_t n, bool fp32, int32_t levels);
@@ -122,6 +133,15 @@ hipError_t launch_column_max(hipStream_t stream, const double* block, int32_t ro
 hipError_t launch_operator(hipStream_t stream, const CsrDev& m, const double* x, int64_t rs, int64_t cs,
                            const double* offset, const int32_t* zero, double* out, int64_t ors, int64_t ocs,
                            int32_t ncols);
+
+// Batches: a persistent launch of `grid` workgroups that take (factor, column) items from the
+// queue counter (which must hold 0 at the launch) until `total` are taken. prepare_batch allows
+// the kernel lds_bytes of LDS (call it before a launch or a graph capture).
+hipError_t prepare_batch(size_t lds_bytes);
+hipError_t launch_batch(hipStream_t stream, int grid, int threads, size_t lds_bytes, const BatchEntryDev* entries,
+                        const int2* items, int32_t total, int32_t* queue, const double* rhs, double* x);
+// p[0..count) = value, as a kernel (so it can sit in a captured graph)
+hipError_t launch_fill(hipStream_t stream, int32_t* p, int32_t value, int32_t count);
 
 // The adjoint's matrix gradient on A's pattern: for each stored entry q = (row[q], col[q]),
 // acc[q] = (first ? 0 : acc[q]) + sum over columns c ascending of bbar(row, c) * x(col, c); both

@@ -1226,3 +1226,235 @@ int32_t ybus_hip_operator_set_weights(ybus_hip_operator* op, const double* weigh
 }
 
 }  // extern "C"
+
+// ---- batches: many factors' columns in one persistent launch per LDS size class ----
+
+struct ybus_hip_batch {
+  ybus_hip_ctx* ctx = nullptr;
+  std::vector<ybus_hip_factor*> factors;
+  // what is staged: counts per factor, where each factor's columns start, the device arrays
+  std::vector<uint64_t> counts;
+  std::vector<int64_t> offsets;
+  int64_t values = 0;
+  bool staged = false;
+  DevBuf<double> rhs, x;
+  DevBuf<BatchEntryDev> entries;
+  DevBuf<int2> items;
+  DevBuf<int32_t> queues;
+  // one persistent launch per class of factors that fit `per_cu` workgroups to a compute unit
+  struct Class {
+    int per_cu = 1, threads = 1024;
+    size_t lds = 0;
+    int32_t first = 0, count = 0;
+  };
+  std::vector<Class> classes;
+  Staging staging;
+  // the launches, captured once per staging and replayed
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ybus_hip_kernel_stats stats{};
+
+  hipStream_t stream() const { return ctx->stream; }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~ybus_hip_batch() {
+    drop_graph();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+  // the queue counters to 0, then each class's launch; on the batch's stream
+  void enqueue() {
+    check(launch_fill(stream(), queues.p, 0, (int32_t)classes.size()), "batch queue reset");
+    for (size_t c = 0; c < classes.size(); ++c) {
+      const Class& k = classes[c];
+      const int grid = std::max(1, std::min<int>(k.count, k.per_cu * ctx->cus));
+      check(launch_batch(stream(), grid, k.threads, k.lds, entries.p, items.p + k.first, k.count, queues.p + c, rhs.p,
+                         x.p),
+            "batch kernel");
+    }
+  }
+};
+
+extern "C" {
+
+int32_t ybus_hip_batch_create(ybus_hip_ctx* ctx, ybus_hip_batch** out) {
+  return guard([&] {
+    if (!ctx || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto b = std::make_unique<ybus_hip_batch>();
+    b->ctx = ctx;
+    check(hipEventCreate(&b->ev0), "hipEventCreate");
+    check(hipEventCreate(&b->ev1), "hipEventCreate");
+    *out = b.release();
+  });
+}
+
+void ybus_hip_batch_free(ybus_hip_batch* b) { delete b; }
+
+uint64_t ybus_hip_batch_factors(const ybus_hip_batch* b) { return b ? (uint64_t)b->factors.size() : 0; }
+
+int32_t ybus_hip_batch_add(ybus_hip_batch* b, ybus_hip_factor* f) {
+  return guard([&] {
+    if (!b || !f) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (f->ctx != b->ctx) fail(YBUS_HIP_INVALID_ARGUMENT, "the factor belongs to another context");
+    if (!f->fits)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "a batch takes factors whose solution fits in LDS (dim " + std::to_string(f->n) +
+                                          " does not)");
+    b->factors.push_back(f);
+    b->staged = false;
+    b->drop_graph();
+  });
+}
+
+int32_t ybus_hip_batch_stage(ybus_hip_batch* b, const double* rhs, const uint64_t* counts) {
+  return guard([&] {
+    if (!b) fail(YBUS_HIP_INVALID_ARGUMENT, "batch is null");
+    const size_t nf = b->factors.size();
+    if (nf && !counts) fail(YBUS_HIP_INVALID_ARGUMENT, "one count per factor");
+    check(hipSetDevice(b->ctx->device), "hipSetDevice");
+    b->staged = false;
+    b->drop_graph();
+    b->counts.assign(counts, counts + nf);
+    b->offsets.assign(nf, 0);
+    int64_t values = 0, columns = 0;
+    for (size_t i = 0; i < nf; ++i) {
+      if (b->counts[i] > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many right-hand sides");
+      b->offsets[i] = values;
+      values += (int64_t)b->counts[i] * b->factors[i]->n;
+      columns += (int64_t)b->counts[i];
+    }
+    if (columns > INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many right-hand sides");
+    if (values && !rhs) fail(YBUS_HIP_INVALID_ARGUMENT, "rhs is null");
+    b->values = values;
+    // the factors' entries, and their items in classes by how many fit a compute unit's LDS
+    std::vector<BatchEntryDev> entries(std::max<size_t>(nf, 1));
+    struct Item {
+      int per_cu;
+      int64_t cost;
+      int2 w;
+      size_t lds;
+    };
+    std::vector<Item> all;
+    all.reserve((size_t)columns);
+    for (size_t i = 0; i < nf; ++i) {
+      ybus_hip_factor* f = b->factors[i];
+      BatchEntryDev& e = entries[i];
+      e.fwd = f->cfwd.view();
+      e.bwd = f->cbwd.view();
+      e.rhs_row = f->rhs_row.p;
+      e.sol_of = f->sol_of.p;
+      e.n = (int32_t)f->n;
+      e.offset = b->offsets[i];
+      const size_t lds = solve_lds_bytes((int32_t)std::max<int64_t>(f->n, 1), false, e.fwd.levels + e.bwd.levels);
+      const int per_cu = (int)std::clamp<size_t>(f->lds_cap / std::max<size_t>(lds, 1), 1, 4);
+      // a column's time is set by its levels, then by its rows
+      const int64_t cost = (int64_t)(e.fwd.levels + e.bwd.levels) * 4096 + f->n;
+      for (uint64_t c = 0; c < b->counts[i]; ++c) all.push_back(Item{per_cu, cost, make_int2((int)i, (int)c), lds});
+    }
+    std::stable_sort(all.begin(), all.end(), [](const Item& p, const Item& q) {
+      return p.per_cu != q.per_cu ? p.per_cu < q.per_cu : p.cost > q.cost;
+    });
+    std::vector<int2> items(std::max<size_t>(all.size(), 1));
+    b->classes.clear();
+    for (size_t k = 0; k < all.size(); ++k) {
+      items[k] = all[k].w;
+      if (b->classes.empty() || b->classes.back().per_cu != all[k].per_cu) {
+        ybus_hip_batch::Class c;
+        c.per_cu = all[k].per_cu;
+        c.threads = c.per_cu == 1 ? 1024 : c.per_cu == 2 ? 512 : 256;
+        c.first = (int32_t)k;
+        b->classes.push_back(c);
+      }
+      ybus_hip_batch::Class& c = b->classes.back();
+      c.count += 1;
+      c.lds = std::max(c.lds, all[k].lds);
+    }
+    b->entries.upload(entries, b->stream());
+    b->items.upload(items, b->stream());
+    b->queues.reserve(std::max<size_t>(b->classes.size(), 1));
+    b->rhs.reserve((size_t)std::max<int64_t>(values, 1));
+    b->x.reserve((size_t)std::max<int64_t>(values, 1));
+    if (values) {
+      double* host = b->staging.get((size_t)values);
+      std::memcpy(host, rhs, (size_t)values * sizeof(double));
+      check(hipMemcpyAsync(b->rhs.p, host, (size_t)values * sizeof(double), hipMemcpyHostToDevice, b->stream()),
+            "H2D");
+    }
+    check(hipStreamSynchronize(b->stream()), "batch stage");
+    b->staged = true;
+  });
+}
+
+int32_t ybus_hip_batch_solve_staged(ybus_hip_batch* b) {
+  return guard([&] {
+    if (!b) fail(YBUS_HIP_INVALID_ARGUMENT, "batch is null");
+    if (!b->staged) fail(YBUS_HIP_INVALID_ARGUMENT, "nothing staged (ybus_hip_batch_stage)");
+    if (b->classes.empty()) return;
+    check(hipSetDevice(b->ctx->device), "hipSetDevice");
+    if (!b->exec) {
+      for (const ybus_hip_batch::Class& c : b->classes) check(prepare_batch(c.lds), "batch kernel attribute");
+      check(hipStreamBeginCapture(b->stream(), hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        b->enqueue();
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(b->stream(), &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      check(hipStreamEndCapture(b->stream(), &b->graph), "hipStreamEndCapture");
+      check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    check(hipEventRecord(b->ev0, b->stream()), "hipEventRecord");
+    check(hipGraphLaunch(b->exec, b->stream()), "hipGraphLaunch");
+    check(hipEventRecord(b->ev1, b->stream()), "hipEventRecord");
+    check(hipEventSynchronize(b->ev1), "batch solve");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, b->ev0, b->ev1), "hipEventElapsedTime");
+    b->stats.launches += 1;
+    b->stats.milliseconds += ms;
+    uint64_t bytes = 0;
+    for (size_t i = 0; i < b->factors.size(); ++i)
+      if (b->counts[i])
+        bytes += (uint64_t)b->factors[i]->nnz_factor * 12 + 8 * (uint64_t)(b->factors[i]->n + 1) +
+                 b->counts[i] * (uint64_t)b->factors[i]->n * 16;
+    b->stats.bytes += bytes;
+  });
+}
+
+int32_t ybus_hip_batch_solutions(ybus_hip_batch* b, double* x) {
+  return guard([&] {
+    if (!b) fail(YBUS_HIP_INVALID_ARGUMENT, "batch is null");
+    if (!b->staged) fail(YBUS_HIP_INVALID_ARGUMENT, "nothing staged (ybus_hip_batch_stage)");
+    if (!b->values) return;
+    if (!x) fail(YBUS_HIP_INVALID_ARGUMENT, "x is null");
+    double* host = b->staging.get((size_t)b->values);
+    check(hipMemcpyAsync(host, b->x.p, (size_t)b->values * sizeof(double), hipMemcpyDeviceToHost, b->stream()), "D2H");
+    check(hipStreamSynchronize(b->stream()), "batch download");
+    std::memcpy(x, host, (size_t)b->values * sizeof(double));
+  });
+}
+
+int32_t ybus_hip_batch_solve(ybus_hip_batch* b, const double* rhs, const uint64_t* counts, double* x) {
+  int32_t status = ybus_hip_batch_stage(b, rhs, counts);
+  if (status != YBUS_HIP_OK) return status;
+  status = ybus_hip_batch_solve_staged(b);
+  if (status != YBUS_HIP_OK) return status;
+  return ybus_hip_batch_solutions(b, x);
+}
+
+void ybus_hip_batch_kernel_stats(const ybus_hip_batch* b, ybus_hip_kernel_stats* out) {
+  if (b && out) *out = b->stats;
+}
+
+void ybus_hip_batch_reset_kernel_stats(ybus_hip_batch* b) {
+  if (b) b->stats = ybus_hip_kernel_stats{};
+}
+
+}  // extern "C"
