@@ -175,3 +175,38 @@ def test_caller_bugs_are_invalid_arguments(gpu):
         factor.pair_solves([Y.Pair(20, None)])
     with pytest.raises(Y.InvalidArgument):
         factor.pair_solves([Y.Pair(3, 3)])
+
+
+@pytest.mark.parametrize("kind", ["cholesky", "lu"])
+def test_pair_blocks_give_the_cpu_bits_with_and_without_the_reach(gpu, monkeypatch, kind):
+    """A pair block on the wide path sweeps only each tile's reach forward; the rows it skips are
+    the CPU's exact +0.0, so the bits are the CPU's either way (YBUS_HIP_REACH=0 sweeps every
+    row)."""
+    g = O.make_grid(1500, 4242)
+    if kind == "cholesky":
+        bp, _, _ = g.dc()
+        cpu = O.Cholesky(bp)
+        n = bp.rows
+    else:
+        jac, _ = g.jacobian(4242)
+        cpu = O.Lu(jac, O.COLAMD)
+        n = jac.rows
+    rng = np.random.default_rng(3)
+    k = 900
+    plus = rng.integers(-1, n, size=k)
+    minus = rng.integers(-1, n, size=k)
+    minus[minus == plus] = -1
+    dense = np.zeros((k, n))
+    for c in range(k):
+        if plus[c] >= 0:
+            dense[c, plus[c]] = 1.0
+        if minus[c] >= 0:
+            dense[c, minus[c]] = -1.0
+    want = cpu.solve_many(dense, threads=4)
+    pairs = [Y.Pair(None if p < 0 else int(p), None if m < 0 else int(m)) for p, m in zip(plus, minus)]
+    for reach in ["1", "0"]:
+        monkeypatch.setenv("YBUS_HIP_REACH", reach)
+        factor = Y.GpuFactor.from_cholesky(gpu, cpu) if kind == "cholesky" else Y.GpuFactor.from_lu(gpu, cpu)
+        factor.set_block_width(k)
+        with factor.pair_solves(pairs) as block:
+            assert block.solutions().tobytes() == want.tobytes(), reach

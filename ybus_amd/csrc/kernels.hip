@@ -140,17 +140,31 @@ constexpr int WIDE_CHUNK = 4;
 
 // A thread's record of position k of a level ending at `end`, clamped to the level's last row
 // (so every fetch issues the same loads, and the wait counts the compiler places stay exact)
-template <typename T>
-__device__ __forceinline__ void fetch_clamped(const SweepDev& s, int32_t k, int32_t end, Record<T>& r) {
-  fetch<T>(s, k < end ? k : end - 1, r);
+// `at` maps a level-list index to a record position (the identity when a tile sweeps every row;
+// a tile's own list when it sweeps only some).
+template <typename T, class At>
+__device__ __forceinline__ void fetch_clamped(const SweepDev& s, int32_t k, int32_t end, Record<T>& r, At& at) {
+  fetch<T>(s, at(k < end ? k : end - 1), r);
 }
 
-// The record a thread starts level m with (a harmless refetch of the last level past the end)
-template <typename T>
-__device__ __forceinline__ void fetch_first(const SweepDev& s, const int32_t* lp, int32_t m, int slot, Record<T>& r) {
+// The record a thread starts level m with (a harmless refetch of the last level past the end;
+// a level may be empty in a tile's own list, and then the fetch takes the list's first entry)
+template <typename T, class At>
+__device__ __forceinline__ void fetch_first(const SweepDev& s, const int32_t* lp, int32_t m, int slot, Record<T>& r,
+                                            At& at) {
   if (m >= s.levels) m = s.levels - 1;
-  fetch_clamped<T>(s, lp[m] + slot, lp[m + 1], r);
+  const int32_t begin = lp[m], end = lp[m + 1];
+  const int32_t k = begin + slot;
+  fetch<T>(s, at(k < end ? k : (end > begin ? end - 1 : lp[0])), r);
 }
+
+// Every dependency is live (a tile that sweeps every row)
+struct AllLive {
+  __device__ __forceinline__ bool operator()(int32_t) const { return true; }
+};
+struct Identity {
+  __device__ __forceinline__ int32_t operator()(int32_t k) const { return k; }
+};
 
 // A row's right-hand side and inline dependencies, loaded (or in flight)
 template <typename T>
@@ -159,20 +173,26 @@ struct Gathered {
   T x[RECORD_ENTRIES];
 };
 
-template <typename T, int TW, class Rhs>
-__device__ __forceinline__ void gather(const Record<T>& r, const T* __restrict__ dep_buf, int64_t ld,
-                                       const Tile<TW>& g, Rhs& rhs, Gathered<T>& o) {
+// `live(d)` tells whether row d of dep_buf holds a value; a dead row reads as 0 (its load goes to
+// the pad row, which every tile keeps in cache).
+template <typename T, int TW, class Rhs, class Live>
+__device__ __forceinline__ void gather(const Record<T>& r, const T* __restrict__ dep_buf, int64_t ld, int32_t pad,
+                                       const Tile<TW>& g, Rhs& rhs, Live& live, Gathered<T>& o) {
   o.rhs = rhs(r.src);
 #pragma unroll
-  for (int i = 0; i < RECORD_ENTRIES; ++i) o.x[i] = dep_buf[r.dep[i] * ld + g.col];
+  for (int i = 0; i < RECORD_ENTRIES; ++i) {
+    const bool on = live(r.dep[i]);
+    const T v = dep_buf[(on ? r.dep[i] : pad) * ld + g.col];
+    o.x[i] = on ? v : (T)0;
+  }
 }
 
 // A row's value: its right-hand side minus its terms in packed order (the inline four from o,
 // the rest in chunks), over its diagonal
-template <typename T, int TW>
+template <typename T, int TW, class Live>
 __device__ __forceinline__ T finish(const SweepDev& s, const T* __restrict__ val, const Record<T>& r,
-                                    const Gathered<T>& o, const T* __restrict__ dep_buf, int64_t ld,
-                                    const Tile<TW>& g) {
+                                    const Gathered<T>& o, const T* __restrict__ dep_buf, int64_t ld, int32_t pad,
+                                    const Tile<TW>& g, Live& live) {
   const int32_t len = r.p1 - r.p0;
   T a = o.rhs;
 #pragma unroll
@@ -188,7 +208,11 @@ __device__ __forceinline__ T finish(const SweepDev& s, const T* __restrict__ val
       vv[i] = val[pi];
     }
 #pragma unroll
-    for (int i = 0; i < WIDE_CHUNK; ++i) xx[i] = dep_buf[dd[i] * ld + g.col];
+    for (int i = 0; i < WIDE_CHUNK; ++i) {
+      const bool on = live(dd[i]);
+      const T v = dep_buf[(on ? dd[i] : pad) * ld + g.col];
+      xx[i] = on ? v : (T)0;
+    }
 #pragma unroll
     for (int i = 0; i < WIDE_CHUNK; ++i)
       if (p + i < r.p1) a = a - vv[i] * xx[i];
@@ -202,42 +226,43 @@ __device__ __forceinline__ T finish(const SweepDev& s, const T* __restrict__ val
 // for the gathers. A wait for a load waits for every memory operation issued before it, stores
 // included, so storing after the next gathers keeps the store's latency off the wait; the
 // records alternate between two registers sets, so no register copy waits on loads in flight.
-template <typename T, int TW, class Rhs, class Store>
+template <typename T, int TW, class Rhs, class Store, class At, class Live>
 __device__ __forceinline__ void wide_level(const SweepDev& s, const T* __restrict__ val, int32_t begin, int32_t end,
                                            const T* __restrict__ dep_buf, int64_t ld, const Tile<TW>& g,
-                                           const Record<T>& first, Rhs& rhs, Store& store) {
+                                           const Record<T>& first, Rhs& rhs, Store& store, At& at, Live& live) {
+  const int32_t pad = s.n;
   int32_t k = begin + g.slot;
   if (k >= end) return;
   const int32_t step = g.slots;
   Gathered<T> o;
-  gather<T, TW>(first, dep_buf, ld, g, rhs, o);
+  gather<T, TW>(first, dep_buf, ld, pad, g, rhs, live, o);
   if (k + step >= end) {
-    store(first.dst, finish<T, TW>(s, val, first, o, dep_buf, ld, g));
+    store(first.dst, finish<T, TW>(s, val, first, o, dep_buf, ld, pad, g, live));
     return;
   }
   Record<T> b, c;
-  fetch_clamped<T>(s, k + step, end, b);
-  T v = finish<T, TW>(s, val, first, o, dep_buf, ld, g);
-  int32_t at = first.dst;
+  fetch_clamped<T>(s, k + step, end, b, at);
+  T v = finish<T, TW>(s, val, first, o, dep_buf, ld, pad, g, live);
+  int32_t dst = first.dst;
   for (k += step;; k += 2 * step) {
     // b holds row k
-    gather<T, TW>(b, dep_buf, ld, g, rhs, o);
-    store(at, v);
-    fetch_clamped<T>(s, k + step, end, c);
-    v = finish<T, TW>(s, val, b, o, dep_buf, ld, g);
-    at = b.dst;
+    gather<T, TW>(b, dep_buf, ld, pad, g, rhs, live, o);
+    store(dst, v);
+    fetch_clamped<T>(s, k + step, end, c, at);
+    v = finish<T, TW>(s, val, b, o, dep_buf, ld, pad, g, live);
+    dst = b.dst;
     if (k + step >= end) break;
     // c holds row k + step
-    gather<T, TW>(c, dep_buf, ld, g, rhs, o);
-    store(at, v);
+    gather<T, TW>(c, dep_buf, ld, pad, g, rhs, live, o);
+    store(dst, v);
     await_record<T>(b);
-    fetch_clamped<T>(s, k + 2 * step, end, b);
-    v = finish<T, TW>(s, val, c, o, dep_buf, ld, g);
-    at = c.dst;
+    fetch_clamped<T>(s, k + 2 * step, end, b, at);
+    v = finish<T, TW>(s, val, c, o, dep_buf, ld, pad, g, live);
+    dst = c.dst;
     if (k + 2 * step >= end) break;
     await_record<T>(c);
   }
-  store(at, v);
+  store(dst, v);
 }
 
 // One sweep of a tile: every row (as its record gives it) is rhs(src) minus its terms against
@@ -246,25 +271,25 @@ __device__ __forceinline__ void wide_level(const SweepDev& s, const T* __restric
 // is fetched two levels ahead (two records alternate, so no register copies wait on loads in
 // flight). A barrier ends each level: it orders the global stores of a workgroup's waves, which
 // share one CU's L1. Every thread takes every barrier; `act` only masks the rows.
-template <typename T, int TW, class Rhs, class Store>
+template <typename T, int TW, class Rhs, class Store, class At = Identity, class Live = AllLive>
 __device__ void wide_sweep(const SweepDev& s, const int32_t* lp, const T* __restrict__ dep_buf, int64_t ld,
-                           const Tile<TW>& g, bool act, Rhs rhs, Store store) {
+                           const Tile<TW>& g, bool act, Rhs rhs, Store store, At at = At(), Live live = Live()) {
   const T* __restrict__ val = values_of<T>(s);
   const int32_t levels = s.levels;
-  if (levels == 0) return;
+  if (levels == 0 || lp[levels] == lp[0]) return;
   Record<T> a, b;
-  fetch_first<T>(s, lp, 0, g.slot, a);
-  fetch_first<T>(s, lp, 1, g.slot, b);
+  fetch_first<T>(s, lp, 0, g.slot, a, at);
+  fetch_first<T>(s, lp, 1, g.slot, b, at);
   for (int32_t l = 0; l < levels; l += 2) {
-    if (act) wide_level<T, TW>(s, val, lp[l], lp[l + 1], dep_buf, ld, g, a, rhs, store);
+    if (act) wide_level<T, TW>(s, val, lp[l], lp[l + 1], dep_buf, ld, g, a, rhs, store, at, live);
     __syncthreads();
     await_record<T>(a);
-    fetch_first<T>(s, lp, l + 2, g.slot, a);
+    fetch_first<T>(s, lp, l + 2, g.slot, a, at);
     if (l + 1 >= levels) break;
-    if (act) wide_level<T, TW>(s, val, lp[l + 1], lp[l + 2], dep_buf, ld, g, b, rhs, store);
+    if (act) wide_level<T, TW>(s, val, lp[l + 1], lp[l + 2], dep_buf, ld, g, b, rhs, store, at, live);
     __syncthreads();
     await_record<T>(b);
-    fetch_first<T>(s, lp, l + 3, g.slot, b);
+    fetch_first<T>(s, lp, l + 3, g.slot, b, at);
   }
 }
 
@@ -289,7 +314,53 @@ __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bw
   };
   wide_sweep<double, TW>(fwd, lpf, t, ld, g, g.on, from_rhs, [&](int32_t dst, double v) { t[dst * ld + col] = v; });
   wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; },
-                            [&](int32_t dst, double v) { x[dst * ld + col] = v; });
+                         [&](int32_t dst, double v) { x[dst * ld + col] = v; });
+}
+
+// A pair block whose forward sweep visits only each tile's reach: the rows of the forward
+// sweep that depend, directly or through other rows, on one of the tile's plus or minus
+// unknowns. Every other forward row of the tile is exactly +0.0 in the CPU's solve (its
+// right-hand side is +0.0, its terms are products with +0.0, and its diagonal is positive or
+// unit), so it is neither computed nor stored: a bitmap in LDS marks the reach, a dependency
+// outside it reads as 0, and the backward sweep reads its right-hand side only inside it.
+template <int TW>
+__global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, ReachDev reach,
+                                                             double* __restrict__ t, double* __restrict__ x, int64_t ld,
+                                                             int32_t ncols) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem);
+  int32_t* lpb = lpf + fwd.levels + 1;
+  uint32_t* live = reinterpret_cast<uint32_t*>(lpb + bwd.levels + 1);
+  const int32_t words = fwd.n / 32 + 1;  // a bit for every row and for the pad row (always clear)
+  const int32_t* __restrict__ tile_lp = reach.lp + (int64_t)blockIdx.x * (fwd.levels + 1);
+  for (int32_t l = threadIdx.x; l <= fwd.levels; l += blockDim.x) lpf[l] = tile_lp[l];
+  for (int32_t l = threadIdx.x; l <= bwd.levels; l += blockDim.x) lpb[l] = bwd.level_ptr[l];
+  for (int32_t w = threadIdx.x; w < words; w += blockDim.x) live[w] = 0u;
+  __syncthreads();
+  for (int32_t k = tile_lp[0] + threadIdx.x; k < tile_lp[fwd.levels]; k += blockDim.x) {
+    const int32_t row = fwd.dst[reach.pos[k]];
+    atomicOr(&live[row >> 5], 1u << (row & 31));
+  }
+  __syncthreads();
+  const Tile<TW> g(ncols);
+  const int32_t col = g.col;
+  const int32_t plus = g.on ? rhs.plus[col] : -1;
+  const int32_t minus = g.on ? rhs.minus[col] : -1;
+  const int32_t* __restrict__ pos = reach.pos;
+  auto at = [&](int32_t k) { return pos[k]; };
+  auto in_reach = [&](int32_t row) { return (live[row >> 5] >> (row & 31)) & 1u; };
+  auto from_pair = [&](int32_t src) -> double {
+    return (src == plus ? 1.0 : 0.0) - (src == minus ? 1.0 : 0.0);
+  };
+  wide_sweep<double, TW>(fwd, lpf, t, ld, g, g.on, from_pair, [&](int32_t dst, double v) { t[dst * ld + col] = v; },
+                         at, in_reach);
+  const int32_t pad = fwd.n;
+  auto from_t = [&](int32_t src) -> double {
+    const bool on = in_reach(src);
+    const double v = t[(on ? src : pad) * ld + col];
+    return on ? v : 0.0;
+  };
+  wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on, from_t, [&](int32_t dst, double v) { x[dst * ld + col] = v; });
 }
 
 // ---- FP32 sweeps refined in FP64 ----
@@ -944,6 +1015,29 @@ hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const Sweep
     if (e != hipSuccess) return e;                                                                               \
     hipLaunchKernelGGL((solve64_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, t, x, ld,    \
                        ncols);                                                                                   \
+  }
+  DISPATCH_TILE(tile, L)
+#undef L
+  return hipGetLastError();
+}
+
+size_t reach_lds_bytes(const SweepDev& fwd, const SweepDev& bwd) {
+  return (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t) + (size_t)(fwd.n / 32 + 1) * sizeof(uint32_t);
+}
+
+hipError_t launch_solve64_reach(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
+                                const RhsDev& rhs, const ReachDev& reach, double* t, double* x, int64_t ld,
+                                int32_t ncols) {
+  if (threads < tile || threads > 1024 || threads % 64) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
+  const size_t bytes = reach_lds_bytes(fwd, bwd);
+#define L(TWV)                                                                                                  \
+  {                                                                                                             \
+    hipError_t e = hipFuncSetAttribute((const void*)solve64_reach_kernel<TWV>,                                  \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
+    if (e != hipSuccess) return e;                                                                              \
+    hipLaunchKernelGGL((solve64_reach_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, reach, \
+                       t, x, ld, ncols);                                                                        \
   }
   DISPATCH_TILE(tile, L)
 #undef L

@@ -102,6 +102,20 @@ hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fw
 hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                           const RhsDev& rhs, double* t, double* x, int64_t ld, int32_t ncols);
 
+// The reach of a pair block's tiles: tile b's forward rows are the positions pos[lp[b (levels +
+// 1) + l] ..) of each level l, in level order (levels = the forward sweep's).
+struct ReachDev {
+  const int32_t* pos = nullptr;
+  const int32_t* lp = nullptr;
+};
+
+// LDS a reach solve needs (both sweeps' level pointers and a bit per forward row), and the solve
+// itself: a pair block (rhs.plus/minus) whose forward sweep visits only each tile's reach.
+size_t reach_lds_bytes(const SweepDev& fwd, const SweepDev& bwd);
+hipError_t launch_solve64_reach(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
+                                const RhsDev& rhs, const ReachDev& reach, double* t, double* x, int64_t ld,
+                                int32_t ncols);
+
 // FP32 sweeps refined in FP64 to `tol` (refine_normwise's rule), one workgroup per column tile;
 // the best iterate ends in x0. steps/ratio get each column's corrections and relative residual.
 hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,

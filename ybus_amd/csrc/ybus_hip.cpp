@@ -283,6 +283,12 @@ struct ybus_hip_factor {
   DevBuf<int32_t> plus, minus, isteps, singular;
   DevBuf<double> dratio;
   DevBuf<double> abar;  // the adjoint's running sums, one per stored entry of the matrix
+  // the staged pair block's reach, per tile of `reach_tile` columns (0: none staged), and the
+  // host index it is computed from (built on first use)
+  bool use_reach = true;
+  int reach_tile = 0;
+  DevBuf<int32_t> reach_pos, reach_lp;
+  std::vector<int32_t> dep_ptr, dependents, pos_of_row, row_of_src;
   int64_t staged = 0;
   bool staged_pairs = false;  // the staged block is pairs (plus, minus), not vectors (b)
   ybus_hip_block* borrowed = nullptr;
@@ -304,6 +310,7 @@ struct ybus_hip_factor {
   void ensure_workspace(int64_t ncols, bool refine) {
     staged = 0;
     staged_pairs = false;
+    reach_tile = 0;
     const int64_t nn = std::max<int64_t>(n, 1);
     if (ncols > cols) {
       cols = ncols;
@@ -351,6 +358,62 @@ struct ybus_hip_factor {
     return tile_width_for(ncols, ctx->cus);
   }
 
+  // The reach of each tile of a pair block (plus/minus: the block's unknowns, -1 for none) in the
+  // wide forward sweep: the rows that depend, directly or through others, on a row whose
+  // right-hand side is one of the tile's unknowns, as positions in level order, with each level's
+  // start per tile (a breadth-first walk over the sweep's dependents from the tile's rows).
+  void stage_reach(const std::vector<int32_t>& p, const std::vector<int32_t>& m, int tile) {
+    const Sweep& sw = fwd.host;
+    const int32_t rows = sw.n, levels = sw.levels;
+    if (dep_ptr.empty()) {
+      dep_ptr.assign(rows + 1, 0);
+      pos_of_row.assign(rows, 0);
+      row_of_src.assign(rows, 0);
+      for (int32_t k = 0; k < rows; ++k) {
+        pos_of_row[sw.dst[k]] = k;
+        row_of_src[sw.src[k]] = sw.dst[k];
+        for (int32_t q = sw.row_ptr[k]; q < sw.row_ptr[k + 1]; ++q) dep_ptr[sw.dep[q] + 1] += 1;
+      }
+      for (int32_t r = 0; r < rows; ++r) dep_ptr[r + 1] += dep_ptr[r];
+      dependents.assign(dep_ptr[rows], 0);
+      std::vector<int32_t> cursor(dep_ptr.begin(), dep_ptr.end() - 1);
+      for (int32_t k = 0; k < rows; ++k)
+        for (int32_t q = sw.row_ptr[k]; q < sw.row_ptr[k + 1]; ++q) dependents[cursor[sw.dep[q]]++] = sw.dst[k];
+    }
+    const int64_t k = (int64_t)p.size();
+    const int64_t tiles = (k + tile - 1) / tile;
+    std::vector<int32_t> stamp(rows, -1), queue, pos;
+    std::vector<int32_t> lp((size_t)tiles * (levels + 1));
+    queue.reserve(rows);
+    for (int64_t t = 0; t < tiles; ++t) {
+      queue.clear();
+      auto visit = [&](int32_t r) {
+        if (stamp[r] != (int32_t)t) {
+          stamp[r] = (int32_t)t;
+          queue.push_back(r);
+        }
+      };
+      for (int64_t c = t * tile; c < std::min<int64_t>(k, (t + 1) * tile); ++c) {
+        if (p[c] >= 0) visit(row_of_src[p[c]]);
+        if (m[c] >= 0) visit(row_of_src[m[c]]);
+      }
+      for (size_t qi = 0; qi < queue.size(); ++qi)
+        for (int32_t e = dep_ptr[queue[qi]]; e < dep_ptr[queue[qi] + 1]; ++e) visit(dependents[e]);
+      const size_t base = pos.size();
+      for (int32_t r : queue) pos.push_back(pos_of_row[r]);
+      std::sort(pos.begin() + base, pos.end());
+      if (pos.size() > (size_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the pair block's reach is too large");
+      for (int32_t l = 0; l <= levels; ++l)
+        lp[(size_t)t * (levels + 1) + l] =
+            (int32_t)(std::lower_bound(pos.begin() + base, pos.end(), sw.level_ptr[l]) - pos.begin());
+    }
+    if (pos.empty()) pos.push_back(0);
+    reach_pos.upload(pos, stream());
+    reach_lp.upload(lp, stream());
+    check(hipStreamSynchronize(stream()), "stage reach");
+    reach_tile = tile;
+  }
+
   void begin_kernel() { check(hipEventRecord(ev0, stream()), "hipEventRecord"); }
   void end_kernel(uint64_t bytes) {
     check(hipEventRecord(ev1, stream()), "hipEventRecord");
@@ -389,9 +452,16 @@ struct ybus_hip_factor {
       const DevSweep& f = transpose ? tfwd : fwd;
       const DevSweep& bw = transpose ? tbwd : bwd;
       RhsDev rhs{pairs ? nullptr : b.p, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
-      check(launch_solve64(stream(), tile_for(ncols), threads_wide, f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
-                           (int32_t)ncols),
-            "solve kernel");
+      const int tile = tile_for(ncols);
+      if (pairs && !transpose && reach_tile == tile) {
+        check(launch_solve64_reach(stream(), tile, threads_wide, f.view(), bw.view(), rhs,
+                                   ReachDev{reach_pos.p, reach_lp.p}, t.p, x.p, layout.ld, (int32_t)ncols),
+              "solve kernel");
+      } else {
+        check(launch_solve64(stream(), tile, threads_wide, f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
+                             (int32_t)ncols),
+              "solve kernel");
+      }
     }
     end_kernel(bytes64(ncols, pairs));
   }
@@ -535,6 +605,8 @@ static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   if (thr) f->threads_wide = f->threads_columns = std::clamp(std::atoi(thr), 64, 1024) / 64 * 64;
   const char* tile = std::getenv("YBUS_HIP_TILE");
   if (tile) f->tile_override = std::atoi(tile);
+  const char* reach = std::getenv("YBUS_HIP_REACH");
+  if (reach && std::string(reach) == "0") f->use_reach = false;
   const char* path = std::getenv("YBUS_HIP_PATH");
   if (path) f->force = std::string(path) == "wide" ? 1 : std::string(path) == "columns" ? 2 : 0;
   f->column_limit = 2 * (int64_t)ctx->cus;
@@ -925,6 +997,10 @@ static void stage_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* 
     check(hipMemcpyAsync(f->plus.p, p.data(), k * sizeof(int32_t), hipMemcpyHostToDevice, f->stream()), "H2D");
     check(hipMemcpyAsync(f->minus.p, m.data(), k * sizeof(int32_t), hipMemcpyHostToDevice, f->stream()), "H2D");
     check(hipStreamSynchronize(f->stream()), "stage pairs");
+    // the wide path sweeps only each tile's reach forward, where its LDS bitmap fits
+    if (!f->layout.columns && f->use_reach && f->n &&
+        reach_lds_bytes(f->fwd.view(), f->bwd.view()) <= f->lds_cap)
+      f->stage_reach(p, m, f->tile_for((int64_t)k));
   }
 }
 
