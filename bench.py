@@ -143,9 +143,11 @@ def run_ptdf(gpu, args):
 
     factor = Y.GpuFactor.from_cholesky(gpu, chol)
     factor.set_block_width(rhs_count)
+    tuning = factor.autotune(rhs_count, pairs=True)  # the library's autotuner: path, threads, tile width
+    log(f"[ptdf] autotuned: {tuning}")
     # the kernel a pair block takes: the column path's, or on the wide path the one that sweeps
     # each tile's reach forward (unless YBUS_HIP_REACH=0 turns that off)
-    if factor.path(rhs_count) == "columns":
+    if factor.path(rhs_count, pairs=True) == "columns":
         kernel = "solve_lds_kernel"
     else:
         kernel = "solve64_kernel" if os.environ.get("YBUS_HIP_REACH") == "0" else "solve64_reach_kernel"
@@ -166,6 +168,7 @@ def run_ptdf(gpu, args):
     solutions = block.solutions()
     block.free()
     return dict(grid=grid, bprime=bprime, chol=chol, plus=plus, minus=minus, nnz_l=nnz_l, rhs=rhs_count, kernel=kernel,
+                tuning=tuning,
                 elapsed=elapsed, stats=stats, solutions=solutions, factor=factor)
 
 
@@ -180,6 +183,8 @@ def run_precision(gpu, args):
     factor = Y.GpuFactor.from_lu(gpu, lu)
     factor.set_matrix(jac)
     factor.set_block_width(512)
+    tuning = factor.autotune(512)
+    log(f"[precision] autotuned: {tuning}")
     factor.stage(rhs)
     out = {}
     for name, mode in [("fp64", Y.FP64), ("fp32_refined", Y.FP32_REFINED)]:
@@ -216,6 +221,7 @@ def run_precision(gpu, args):
     out["fp64_vs_cpu_max_rel_diff"] = err
     out["speedup_fp32_refined_over_fp64"] = out["fp32_refined"]["value"] / out["fp64"]["value"]
     out["workload"] = "25,000-bus AC Newton Jacobian (n=%d), COLAMD LU, 512 N(0,1) RHS" % jac.rows
+    out["tuning"] = tuning
     out["nnz_factor"] = nnz_lu
     return out
 
@@ -425,7 +431,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "PTDF block: 10,000-bus synthetic grid, B' AMD Cholesky, 13,500 pair RHS built "
                                "on the device, FP64 forward+backward", "buses": 10000, "n": int(chol.n),
-                   "nnz_l": nnz_l, "rhs": rhs_count, "parallelism": "single GPU"},
+                   "nnz_l": nnz_l, "rhs": rhs_count, "parallelism": "single GPU", "tuning": ptdf["tuning"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": ptdf["kernel"], "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch},

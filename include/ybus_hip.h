@@ -221,7 +221,8 @@ int32_t ybus_hip_block_product(ybus_hip_block *b, ybus_hip_operator *op, const i
 /* GpuBlock::product_maxima (pairs.rs:396-413): product reduced to |value| * weight maxima. */
 int32_t ybus_hip_block_product_maxima(ybus_hip_block *b, ybus_hip_operator *op,
                                       const int64_t *zero, ybus_hip_column_max *out);
-/* Which kernels solve a block of `columns` right-hand sides with this factor:
+/* Which kernels solve a block of `columns` right-hand sides (pairs != 0: a pair block) with this
+ * factor:
  * YBUS_HIP_PATH_COLUMNS, one workgroup per right-hand side with its solution in LDS and the block
  * stored a column (a right-hand side) at a time, for factors whose solution fits in LDS and
  * blocks of at most twice as many columns as the device has compute units; or
@@ -229,7 +230,27 @@ int32_t ybus_hip_block_product_maxima(ybus_hip_block *b, ybus_hip_operator *op,
  * environment variable YBUS_HIP_PATH to "wide" or "columns" before an upload makes that factor
  * always take the wide path, or the column path wherever the factor fits. */
 enum { YBUS_HIP_PATH_COLUMNS = 0, YBUS_HIP_PATH_WIDE = 1 };
-int32_t ybus_hip_factor_path(const ybus_hip_factor *f, uint64_t columns);
+int32_t ybus_hip_factor_path(const ybus_hip_factor *f, uint64_t columns, int32_t pairs);
+
+/* The kernel configuration a block width takes: path (YBUS_HIP_PATH_*), workgroup size, column
+ * tile width (1 on the column path), and the time of the synthetic block it was chosen on. */
+typedef struct {
+  int32_t path;
+  int32_t threads;
+  int32_t tile;
+  double milliseconds;
+} ybus_hip_tuning;
+
+/* Autotuner (new): times every kernel configuration a block of `columns` right-hand sides can
+ * take with this factor (the column path at 256 and 512 threads where the factor fits; the wide
+ * path at 256, 512 and 1024 threads with column tiles of 1 to 64), each on a synthetic FP64 block
+ * of that width, and keeps the fastest for this factor's later blocks of that width (up to the
+ * next power of two) and kind: pair blocks (pairs != 0; pair_solves, pair_updates, staged pairs)
+ * or blocks of vectors (solve_many, staged vectors, refinement). `out` (may be NULL) gets the
+ * choice. YBUS_HIP_PATH, YBUS_HIP_THREADS and YBUS_HIP_TILE still override it. Whatever was
+ * staged is dropped. */
+int32_t ybus_hip_factor_autotune(ybus_hip_factor *f, uint64_t columns, int32_t pairs,
+                                 ybus_hip_tuning *out);
 
 /* Kernel statistics of this factor's device solves (HIP events), and their reset. */
 void ybus_hip_factor_kernel_stats(const ybus_hip_factor *f, ybus_hip_kernel_stats *out);

@@ -36,3 +36,32 @@ def test_a_block_wide_enough_for_both_paths_gives_the_same_bits(gpu, monkeypatch
     cpu = chol.solve_many(narrow)
     assert factor.solve_many(narrow).tobytes() == cpu.tobytes()
     assert factor.solve_many(wide).tobytes() == np.concatenate([cpu] * 40).tobytes()
+
+
+def test_the_autotuner_keeps_a_configuration_that_gives_the_cpu_bits(gpu, monkeypatch):
+    monkeypatch.delenv("YBUS_HIP_PATH", raising=False)
+    g = O.make_grid(2000, 11)
+    bp, _, _ = g.dc()
+    chol = O.Cholesky(bp)
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    rng = np.random.default_rng(1)
+    for columns in [40, 700]:
+        for pairs in [True, False]:
+            choice = factor.autotune(columns, pairs)
+            assert choice["path"] in ("columns", "wide") and choice["milliseconds"] > 0.0
+            assert factor.path(columns, pairs) == choice["path"]
+        plus = rng.integers(0, bp.rows, size=columns)
+        minus = (plus + 1 + rng.integers(0, bp.rows - 1, size=columns)) % bp.rows
+        dense = np.zeros((columns, bp.rows))
+        dense[np.arange(columns), plus] = 1.0
+        dense[np.arange(columns), minus] = -1.0
+        pairs = [Y.Pair(int(p), int(m)) for p, m in zip(plus, minus)]
+        with factor.pair_solves(pairs) as block:
+            assert block.solutions().tobytes() == chol.solve_many(dense).tobytes(), columns
+        assert factor.solve_many(dense).tobytes() == chol.solve_many(dense).tobytes()
+    with pytest.raises(Y.InvalidArgument):
+        factor.autotune(0)
+    # an environment override still decides the path
+    monkeypatch.setenv("YBUS_HIP_PATH", "wide")
+    forced = Y.GpuFactor.from_cholesky(gpu, chol)
+    assert forced.autotune(40, pairs=True)["path"] == "wide"

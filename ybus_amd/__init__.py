@@ -80,6 +80,10 @@ class _ColumnMax(C.Structure):
     _fields_ = [("row", C.c_int64), ("value", C.c_double), ("not_finite", C.c_int32)]
 
 
+class _Tuning(C.Structure):
+    _fields_ = [("path", C.c_int32), ("threads", C.c_int32), ("tile", C.c_int32), ("milliseconds", C.c_double)]
+
+
 class _KernelStats(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("milliseconds", C.c_double), ("bytes", C.c_uint64)]
 
@@ -113,6 +117,7 @@ def _declare(L):
         "ybus_hip_factor_solve_transpose_many": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_many_refined": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
         "ybus_hip_factor_adjoint": (i32, [P, f64p, f64p, U64, f64p, f64p]),
+        "ybus_hip_factor_autotune": (i32, [P, U64, i32, C.POINTER(_Tuning)]),
         "ybus_hip_factor_stage": (i32, [P, f64p, U64]),
         "ybus_hip_factor_stage_pairs": (i32, [P, i64p, i64p, U64]),
         "ybus_hip_factor_solve_staged": (i32, [P, i32, C.c_double, U64, C.POINTER(P)]),
@@ -127,7 +132,7 @@ def _declare(L):
         "ybus_hip_block_product": (i32, [P, P, i64p, f64p]),
         "ybus_hip_block_product_maxima": (i32, [P, P, i64p, C.POINTER(_ColumnMax)]),
         "ybus_hip_factor_kernel_stats": (None, [P, C.POINTER(_KernelStats)]),
-        "ybus_hip_factor_path": (i32, [P, U64]),
+        "ybus_hip_factor_path": (i32, [P, U64, i32]),
         "ybus_hip_factor_reset_kernel_stats": (None, [P]),
         "ybus_hip_factor_operator": (i32, [P, U64, u64p, u64p, f64p, C.POINTER(P)]),
         "ybus_hip_operator_free": (None, [P]),
@@ -365,10 +370,21 @@ class GpuFactor:
     def reset_timings(self):
         library().ybus_hip_factor_reset_timings(self._h)
 
-    def path(self, columns: int) -> str:
-        """The kernels a block of `columns` right-hand sides takes: 'columns' (one workgroup per
-        right-hand side, its solution in LDS) or 'wide' (column tiles in global memory)."""
-        return "wide" if library().ybus_hip_factor_path(self._h, columns) == 1 else "columns"
+    def path(self, columns: int, pairs: bool = False) -> str:
+        """The kernels a block of `columns` right-hand sides (a pair block when `pairs`) takes:
+        'columns' (one workgroup per right-hand side, its solution in LDS) or 'wide' (column
+        tiles in global memory)."""
+        return "wide" if library().ybus_hip_factor_path(self._h, columns, int(bool(pairs))) == 1 else "columns"
+
+    def autotune(self, columns: int, pairs: bool = False) -> dict:
+        """Times the kernel configurations for blocks of `columns` right-hand sides (pair blocks
+        when `pairs`, else blocks of vectors) and keeps the fastest for this factor's blocks of
+        that width and kind: {'path', 'threads', 'tile', 'milliseconds'}."""
+        self._release_block()
+        t = _Tuning()
+        _check(library().ybus_hip_factor_autotune(self._h, int(columns), int(bool(pairs)), C.byref(t)))
+        return dict(path="wide" if t.path == 1 else "columns", threads=int(t.threads), tile=int(t.tile),
+                    milliseconds=float(t.milliseconds))
 
     def kernel_stats(self) -> KernelStats:
         s = _KernelStats()

@@ -132,11 +132,9 @@ __device__ void stage_levels(const SweepDev& s, int32_t* lp) {
 
 // Terms of a row past its inline four are taken this many at a time: the chunk's dependency
 // indices and values are loaded together, then its dependencies' values together, so a long row
-// (a separator's, near the top of the schedule, where levels are one or two rows wide) waits
-// for two round trips per chunk rather than per term.
+// (a separator's near the top of a Cholesky schedule, most rows of an LU) waits for two round
+// trips per chunk rather than per term.
 constexpr int OVERFLOW_CHUNK = 8;
-// The wide sweeps keep more records in flight, so they take their chunks half as long
-constexpr int WIDE_CHUNK = 4;
 
 // A thread's record of position k of a level ending at `end`, clamped to the level's last row
 // (so every fetch issues the same loads, and the wait counts the compiler places stay exact)
@@ -198,23 +196,23 @@ __device__ __forceinline__ T finish(const SweepDev& s, const T* __restrict__ val
 #pragma unroll
   for (int i = 0; i < RECORD_ENTRIES; ++i)
     if (len > i) a = a - r.val[i] * o.x[i];
-  for (int32_t p = r.p0 + RECORD_ENTRIES; p < r.p1; p += WIDE_CHUNK) {
-    int32_t dd[WIDE_CHUNK];
-    T vv[WIDE_CHUNK], xx[WIDE_CHUNK];
+  for (int32_t p = r.p0 + RECORD_ENTRIES; p < r.p1; p += OVERFLOW_CHUNK) {
+    int32_t dd[OVERFLOW_CHUNK];
+    T vv[OVERFLOW_CHUNK], xx[OVERFLOW_CHUNK];
 #pragma unroll
-    for (int i = 0; i < WIDE_CHUNK; ++i) {
+    for (int i = 0; i < OVERFLOW_CHUNK; ++i) {
       const int32_t pi = p + i < r.p1 ? p + i : r.p1 - 1;
       dd[i] = s.dep[pi];
       vv[i] = val[pi];
     }
 #pragma unroll
-    for (int i = 0; i < WIDE_CHUNK; ++i) {
+    for (int i = 0; i < OVERFLOW_CHUNK; ++i) {
       const bool on = live(dd[i]);
       const T v = dep_buf[(on ? dd[i] : pad) * ld + g.col];
       xx[i] = on ? v : (T)0;
     }
 #pragma unroll
-    for (int i = 0; i < WIDE_CHUNK; ++i)
+    for (int i = 0; i < OVERFLOW_CHUNK; ++i)
       if (p + i < r.p1) a = a - vv[i] * xx[i];
   }
   return s.unit ? a : a / r.diag;

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -297,6 +298,35 @@ struct ybus_hip_factor {
   ybus_hip_kernel_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tile_override = 0, threads_wide = 512, threads_columns = 512;
+  bool env_threads = false;  // YBUS_HIP_THREADS set: it overrides the tuned workgroup size
+  // kernel configurations chosen by ybus_hip_factor_autotune, per block width (keyed by the
+  // power of two at or above it)
+  struct Tuned {
+    bool columns = false;
+    int threads = 512, tile = 0;
+  };
+  std::map<int, Tuned> tuned;
+  const Tuned* trial = nullptr;  // a configuration being timed by the autotuner
+
+  static int bucket(int64_t ncols) {
+    int b = 0;
+    while ((int64_t(1) << b) < ncols) ++b;
+    return b;
+  }
+  // pairs: a pair block (whose forward sweep takes each tile's reach), else a block of vectors
+  const Tuned* tuned_for(int64_t ncols, bool pairs) const {
+    if (trial) return trial;
+    auto it = tuned.find(2 * bucket(ncols) + (pairs ? 1 : 0));
+    return it == tuned.end() ? nullptr : &it->second;
+  }
+  int wide_threads(int64_t ncols, bool pairs) const {
+    const Tuned* t = tuned_for(ncols, pairs);
+    return t && !t->columns && !env_threads ? t->threads : threads_wide;
+  }
+  int column_threads(int64_t ncols, bool pairs) const {
+    const Tuned* t = tuned_for(ncols, pairs);
+    return t && t->columns && !env_threads ? t->threads : threads_columns;
+  }
 
   ~ybus_hip_factor() {
     if (ev0) (void)hipEventDestroy(ev0);
@@ -307,7 +337,7 @@ struct ybus_hip_factor {
 
   // Sizes the workspace for `ncols` columns; whatever was staged is dropped, since the solves
   // that call this overwrite the staged buffers
-  void ensure_workspace(int64_t ncols, bool refine) {
+  void ensure_workspace(int64_t ncols, bool refine, bool pairs = false) {
     staged = 0;
     staged_pairs = false;
     reach_tile = 0;
@@ -318,7 +348,7 @@ struct ybus_hip_factor {
       t32.release();
       d32.release();
     }
-    layout.columns = use_columns(ncols);
+    layout.columns = use_columns(ncols, pairs);
     const bool columns = layout.columns;
     layout.ld = columns ? nn : round_up(std::max<int64_t>(cols, 1), 16);
     const size_t elems = (size_t)nn * (size_t)(columns ? std::max<int64_t>(cols, 1) : layout.ld);
@@ -348,13 +378,17 @@ struct ybus_hip_factor {
   }
 
   // Whether a block of `ncols` columns takes the column path
-  bool use_columns(int64_t ncols) const {
+  bool use_columns(int64_t ncols, bool pairs = false) const {
     if (!fits || force == 1) return false;
-    return force == 2 || ncols <= column_limit;
+    if (force == 2) return true;
+    if (const Tuned* t = tuned_for(ncols, pairs)) return t->columns;
+    return ncols <= column_limit;
   }
 
-  int tile_for(int64_t ncols) const {
+  int tile_for(int64_t ncols, bool pairs) const {
     if (tile_override) return tile_override;
+    if (const Tuned* t = tuned_for(ncols, pairs))
+      if (!t->columns && t->tile) return t->tile;
     return tile_width_for(ncols, ctx->cus);
   }
 
@@ -445,20 +479,20 @@ struct ybus_hip_factor {
       const DevBuf<int32_t>& rr = transpose ? t_rhs_row : rhs_row;
       const DevBuf<int32_t>& so = transpose ? t_sol_of : sol_of;
       ColumnsDev rhs{pairs ? nullptr : b.p, layout.ld, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
-      check(launch_solve_lds(stream(), threads_columns, false, f.view(), bw.view(), (int32_t)n, rr.p, so.p, rhs, x.p,
+      check(launch_solve_lds(stream(), column_threads(ncols, pairs), false, f.view(), bw.view(), (int32_t)n, rr.p, so.p, rhs, x.p,
                              layout.ld, (int32_t)ncols),
             "solve kernel");
     } else {
       const DevSweep& f = transpose ? tfwd : fwd;
       const DevSweep& bw = transpose ? tbwd : bwd;
       RhsDev rhs{pairs ? nullptr : b.p, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
-      const int tile = tile_for(ncols);
+      const int tile = tile_for(ncols, pairs);
       if (pairs && !transpose && reach_tile == tile) {
-        check(launch_solve64_reach(stream(), tile, threads_wide, f.view(), bw.view(), rhs,
+        check(launch_solve64_reach(stream(), tile, wide_threads(ncols, pairs), f.view(), bw.view(), rhs,
                                    ReachDev{reach_pos.p, reach_lp.p}, t.p, x.p, layout.ld, (int32_t)ncols),
               "solve kernel");
       } else {
-        check(launch_solve64(stream(), tile, threads_wide, f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
+        check(launch_solve64(stream(), tile, wide_threads(ncols, pairs), f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
                              (int32_t)ncols),
               "solve kernel");
       }
@@ -471,12 +505,14 @@ struct ybus_hip_factor {
     begin_kernel();
     if (layout.columns) {
       int th = 1;
-      while (th * 2 <= threads_columns) th *= 2;
+      while (th * 2 <= column_threads(ncols, false)) th *= 2;
       check(launch_refine_lds(stream(), th, cfwd.view(), cbwd.view(), a.view(), rhs_row.p, sol_of.p, b.p, x.p, x1.p,
                               r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     } else {
-      check(launch_refine(stream(), tile_for(ncols), threads_wide, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p,
+      int th = 64;
+      while (th * 2 <= wide_threads(ncols, false)) th *= 2;
+      check(launch_refine(stream(), tile_for(ncols, false), th, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p,
                           x.p, x1.p, r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     }
@@ -602,7 +638,10 @@ static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   f->ctx = ctx;
   f->n = n;
   const char* thr = std::getenv("YBUS_HIP_THREADS");
-  if (thr) f->threads_wide = f->threads_columns = std::clamp(std::atoi(thr), 64, 1024) / 64 * 64;
+  if (thr) {
+    f->threads_wide = f->threads_columns = std::clamp(std::atoi(thr), 64, 1024) / 64 * 64;
+    f->env_threads = true;
+  }
   const char* tile = std::getenv("YBUS_HIP_TILE");
   if (tile) f->tile_override = std::atoi(tile);
   const char* reach = std::getenv("YBUS_HIP_REACH");
@@ -767,8 +806,8 @@ void ybus_hip_factor_reset_timings(ybus_hip_factor* f) {
   if (f) f->timings = ybus_hip_timings{};
 }
 
-int32_t ybus_hip_factor_path(const ybus_hip_factor* f, uint64_t columns) {
-  return f && f->use_columns((int64_t)std::min<uint64_t>(columns, INT64_MAX)) ? YBUS_HIP_PATH_COLUMNS
+int32_t ybus_hip_factor_path(const ybus_hip_factor* f, uint64_t columns, int32_t pairs) {
+  return f && f->use_columns((int64_t)std::min<uint64_t>(columns, INT64_MAX), pairs != 0) ? YBUS_HIP_PATH_COLUMNS
                                                                                 : YBUS_HIP_PATH_WIDE;
 }
 
@@ -992,7 +1031,7 @@ static void stage_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* 
     m[c] = (int32_t)minus[c];
   }
   check(hipSetDevice(f->ctx->device), "hipSetDevice");
-  f->ensure_workspace((int64_t)k, false);
+  f->ensure_workspace((int64_t)k, false, true);
   if (k) {
     check(hipMemcpyAsync(f->plus.p, p.data(), k * sizeof(int32_t), hipMemcpyHostToDevice, f->stream()), "H2D");
     check(hipMemcpyAsync(f->minus.p, m.data(), k * sizeof(int32_t), hipMemcpyHostToDevice, f->stream()), "H2D");
@@ -1000,7 +1039,7 @@ static void stage_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* 
     // the wide path sweeps only each tile's reach forward, where its LDS bitmap fits
     if (!f->layout.columns && f->use_reach && f->n &&
         reach_lds_bytes(f->fwd.view(), f->bwd.view()) <= f->lds_cap)
-      f->stage_reach(p, m, f->tile_for((int64_t)k));
+      f->stage_reach(p, m, f->tile_for((int64_t)k, true));
   }
 }
 
@@ -1010,6 +1049,92 @@ int32_t ybus_hip_factor_stage_pairs(ybus_hip_factor* f, const int64_t* plus, con
     stage_pairs(f, plus, minus, k);
     f->staged = (int64_t)k;
     f->staged_pairs = true;
+  });
+}
+
+int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t pairs, ybus_hip_tuning* out) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    require_unborrowed(f);
+    if (columns == 0 || columns > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_INVALID_ARGUMENT, "columns out of range");
+    if (f->n < 2) fail(YBUS_HIP_INVALID_ARGUMENT, "a factor of dim() >= 2 (pairs need two unknowns)");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    const int64_t k = (int64_t)std::min<uint64_t>(columns, f->width);
+    // a synthetic block of that width: pairs with +1 at c mod n and -1 at (7c + 1) mod n in
+    // column c, or vectors of values in [-1, 1) from a fixed linear congruential sequence
+    std::vector<int64_t> plus, minus;
+    std::vector<double> vectors;
+    if (pairs) {
+      plus.resize(k);
+      minus.resize(k);
+      for (int64_t c = 0; c < k; ++c) {
+        plus[c] = c % f->n;
+        minus[c] = (7 * c + 1) % f->n;
+        if (minus[c] == plus[c]) minus[c] = (plus[c] + 1) % f->n;
+      }
+    } else {
+      vectors.resize((size_t)k * (size_t)f->n);
+      uint64_t state = 0x9e3779b97f4a7c15ull;
+      for (double& v : vectors) {
+        state = state * 6364136223846793005ull + 1442695040888963407ull;
+        v = (double)(state >> 11) * 0x1.0p-52 - 1.0;
+      }
+    }
+    std::vector<ybus_hip_factor::Tuned> candidates;
+    if (f->fits && f->force != 1)
+      for (int th : {256, 512}) candidates.push_back({true, th, 0});
+    if (f->force != 2)
+      for (int th : {256, 512, 1024})
+        for (int tile : {1, 2, 4, 8, 16, 32, 64})
+          if (tile == 1 || tile / 2 < k) candidates.push_back({false, th, tile});
+    if (candidates.empty()) fail(YBUS_HIP_INVALID_ARGUMENT, "no kernel configuration to try");
+    const int key = 2 * ybus_hip_factor::bucket(k) + (pairs ? 1 : 0);
+    f->tuned.erase(key);
+    ybus_hip_factor::Tuned best{};
+    double best_ms = 0.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    const ybus_hip_kernel_stats saved = f->stats;
+    try {
+      for (const ybus_hip_factor::Tuned& c : candidates) {
+        f->trial = &c;
+        if (pairs) {
+          stage_pairs(f, plus.data(), minus.data(), (uint64_t)k);
+        } else {
+          f->ensure_workspace(k, false);
+          upload_vectors(f, vectors.data(), k);
+        }
+        f->solve64(false, pairs != 0, k);  // warm
+        double ms = 0.0;
+        for (int rep = 0; rep < 3; ++rep) {
+          check(hipEventRecord(e0, f->stream()), "hipEventRecord");
+          f->solve64(false, pairs != 0, k);
+          check(hipEventRecord(e1, f->stream()), "hipEventRecord");
+          check(hipEventSynchronize(e1), "autotune");
+          float t = 0.0f;
+          check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+          ms = rep == 0 ? t : std::min(ms, (double)t);
+        }
+        if (best_ms == 0.0 || ms < best_ms) {
+          best = c;
+          best_ms = ms;
+        }
+      }
+    } catch (...) {
+      f->trial = nullptr;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    f->trial = nullptr;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f->stats = saved;
+    f->tuned[key] = best;
+    f->ensure_workspace(k, false);  // drops the synthetic staging
+    if (out) *out = ybus_hip_tuning{best.columns ? YBUS_HIP_PATH_COLUMNS : YBUS_HIP_PATH_WIDE, best.threads,
+                                    best.columns ? 1 : best.tile, best_ms};
   });
 }
 
