@@ -143,7 +143,8 @@ def run_ptdf(gpu, args):
 
     factor = Y.GpuFactor.from_cholesky(gpu, chol)
     factor.set_block_width(rhs_count)
-    tuning = factor.autotune(rhs_count, pairs=True)  # the library's autotuner: path, threads, tile width
+    pairs = [Y.Pair(None if p < 0 else int(p), None if m < 0 else int(m)) for p, m in zip(plus, minus)]
+    tuning = factor.autotune(rhs_count, pairs)  # the library's autotuner: path, threads, tile width
     log(f"[ptdf] autotuned: {tuning}")
     # the kernel a pair block takes: the column path's, or on the wide path the one that sweeps
     # each tile's reach forward (unless YBUS_HIP_REACH=0 turns that off)
@@ -151,7 +152,6 @@ def run_ptdf(gpu, args):
         kernel = "solve_lds_kernel"
     else:
         kernel = "solve64_kernel" if os.environ.get("YBUS_HIP_REACH") == "0" else "solve64_reach_kernel"
-    pairs = [Y.Pair(None if p < 0 else int(p), None if m < 0 else int(m)) for p, m in zip(plus, minus)]
     factor.stage_pairs(pairs)
     for _ in range(args.warmup):
         factor.solve_staged(Y.FP64).free()

@@ -246,11 +246,12 @@ typedef struct {
  * path at 256, 512 and 1024 threads with column tiles of 1 to 64), each on a synthetic FP64 block
  * of that width, and keeps the fastest for this factor's later blocks of that width (up to the
  * next power of two) and kind: pair blocks (pairs != 0; pair_solves, pair_updates, staged pairs)
- * or blocks of vectors (solve_many, staged vectors, refinement). `out` (may be NULL) gets the
- * choice. YBUS_HIP_PATH, YBUS_HIP_THREADS and YBUS_HIP_TILE still override it. Whatever was
- * staged is dropped. */
+ * or blocks of vectors (solve_many, staged vectors, refinement). A pair block is timed on the
+ * caller's own `columns` pairs when plus and minus are given (the pairs' reach sets the forward
+ * sweep's work), else on synthetic ones. `out` (may be NULL) gets the choice. YBUS_HIP_PATH,
+ * YBUS_HIP_THREADS and YBUS_HIP_TILE still override it. Whatever was staged is dropped. */
 int32_t ybus_hip_factor_autotune(ybus_hip_factor *f, uint64_t columns, int32_t pairs,
-                                 ybus_hip_tuning *out);
+                                 const int64_t *plus, const int64_t *minus, ybus_hip_tuning *out);
 
 /* Kernel statistics of this factor's device solves (HIP events), and their reset. */
 void ybus_hip_factor_kernel_stats(const ybus_hip_factor *f, ybus_hip_kernel_stats *out);

@@ -117,7 +117,7 @@ def _declare(L):
         "ybus_hip_factor_solve_transpose_many": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_many_refined": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
         "ybus_hip_factor_adjoint": (i32, [P, f64p, f64p, U64, f64p, f64p]),
-        "ybus_hip_factor_autotune": (i32, [P, U64, i32, C.POINTER(_Tuning)]),
+        "ybus_hip_factor_autotune": (i32, [P, U64, i32, C.c_void_p, C.c_void_p, C.POINTER(_Tuning)]),
         "ybus_hip_factor_stage": (i32, [P, f64p, U64]),
         "ybus_hip_factor_stage_pairs": (i32, [P, i64p, i64p, U64]),
         "ybus_hip_factor_solve_staged": (i32, [P, i32, C.c_double, U64, C.POINTER(P)]),
@@ -376,13 +376,20 @@ class GpuFactor:
         tiles in global memory)."""
         return "wide" if library().ybus_hip_factor_path(self._h, columns, int(bool(pairs))) == 1 else "columns"
 
-    def autotune(self, columns: int, pairs: bool = False) -> dict:
-        """Times the kernel configurations for blocks of `columns` right-hand sides (pair blocks
-        when `pairs`, else blocks of vectors) and keeps the fastest for this factor's blocks of
-        that width and kind: {'path', 'threads', 'tile', 'milliseconds'}."""
+    def autotune(self, columns: int, pairs=False) -> dict:
+        """Times the kernel configurations for blocks of `columns` right-hand sides and keeps the
+        fastest for this factor's blocks of that width and kind: pair blocks when `pairs` (True
+        for synthetic pairs, or the caller's own list of Pair to time on), else blocks of vectors.
+        Returns {'path', 'threads', 'tile', 'milliseconds'}."""
         self._release_block()
         t = _Tuning()
-        _check(library().ybus_hip_factor_autotune(self._h, int(columns), int(bool(pairs)), C.byref(t)))
+        plus = minus = None
+        if pairs is not True and pairs:
+            plus, minus = _pairs(pairs)
+            columns = len(pairs)
+        _check(library().ybus_hip_factor_autotune(self._h, int(columns), int(bool(pairs)),
+                                                  plus.ctypes.data if plus is not None else None,
+                                                  minus.ctypes.data if minus is not None else None, C.byref(t)))
         return dict(path="wide" if t.path == 1 else "columns", threads=int(t.threads), tile=int(t.tile),
                     milliseconds=float(t.milliseconds))
 

@@ -1052,7 +1052,8 @@ int32_t ybus_hip_factor_stage_pairs(ybus_hip_factor* f, const int64_t* plus, con
   });
 }
 
-int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t pairs, ybus_hip_tuning* out) {
+int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t pairs, const int64_t* tune_plus,
+                                 const int64_t* tune_minus, ybus_hip_tuning* out) {
   return guard([&] {
     if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
     require_unborrowed(f);
@@ -1064,7 +1065,11 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
     // column c, or vectors of values in [-1, 1) from a fixed linear congruential sequence
     std::vector<int64_t> plus, minus;
     std::vector<double> vectors;
-    if (pairs) {
+    if (pairs && tune_plus && tune_minus) {
+      // the caller's own pairs (checked by stage_pairs like any other)
+      plus.assign(tune_plus, tune_plus + k);
+      minus.assign(tune_minus, tune_minus + k);
+    } else if (pairs) {
       plus.resize(k);
       minus.resize(k);
       for (int64_t c = 0; c < k; ++c) {
