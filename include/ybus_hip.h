@@ -74,7 +74,8 @@ typedef struct {
 } ybus_hip_column_max;
 
 /* Device kernel statistics of a factor since its last reset, from HIP events on the factor's
- * stream: launches and summed milliseconds of the solve kernels (one launch solves a block). */
+ * stream: launches and summed milliseconds of the solve kernels (one launch solves a block) and
+ * of the adjoint's SDDMM kernel. */
 typedef struct {
   uint64_t launches;
   double milliseconds;

@@ -985,9 +985,12 @@ int32_t ybus_hip_factor_adjoint(ybus_hip_factor* f, const double* x_bar, const d
         download_vectors(f, f->x.p, f->layout, chunk, b_bar + start * n);
         // the solutions x beside it, then the running sums on A's pattern
         upload_vectors(f, x + start * n, chunk, f->y2.p);
+        f->begin_kernel();
         check(launch_sddmm(f->stream(), f->a.row.p, f->a.col.p, nnz, f->x.p, f->y2.p, f->layout.rs(),
                            f->layout.cs(), (int32_t)chunk, f->abar.p, start == 0),
               "adjoint kernel");
+        // both blocks read once and the running sums read and written: 16 n c + 16 nnz(A)
+        f->end_kernel(16 * (uint64_t)f->n * (uint64_t)chunk + 16 * (uint64_t)nnz);
       }
       if (nnz) {
         check(hipMemcpyAsync(acc.data(), f->abar.p, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost,
