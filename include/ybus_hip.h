@@ -172,6 +172,23 @@ int32_t ybus_hip_factor_solve_many_refined(ybus_hip_factor *f, const double *rhs
 int32_t ybus_hip_factor_adjoint(ybus_hip_factor *f, const double *x_bar, const double *x,
                                 uint64_t count, double *b_bar, double *a_bar);
 
+/* ---- vectors already in device memory (new) ----
+ *
+ * For callers that keep their vectors on the GPU (a PyTorch tensor, a previous kernel's output):
+ * b, x, x_bar, b_bar hold `count` vectors of dim() doubles in device memory, vector c at
+ * pointer + c * ld (ld >= dim()); a_bar is device memory for one value per stored entry of the
+ * matrix given to ybus_hip_factor_set_matrix, in the caller's storage order. The caller's writes
+ * to the inputs must be finished before the call (synchronize the stream that made them); the
+ * call returns with its outputs written. Results are bit for bit those of the host-vector
+ * calls. */
+/* solve_many (mode YBUS_HIP_FP64) or solve_transpose_many (YBUS_HIP_TRANSPOSE) on device vectors. */
+int32_t ybus_hip_factor_solve_device(ybus_hip_factor *f, int32_t mode, const double *b,
+                                     uint64_t ld_b, uint64_t count, double *x, uint64_t ld_x);
+/* ybus_hip_factor_adjoint on device vectors (x_bar, x and b_bar share the stride ld). */
+int32_t ybus_hip_factor_adjoint_device(ybus_hip_factor *f, const double *x_bar, const double *x,
+                                       uint64_t ld, uint64_t count, double *b_bar,
+                                       double *a_bar);
+
 /* ---- device-resident blocks ---- */
 
 /* Copies `count` right-hand sides to the device, into the factor's staged block (at most

@@ -1,0 +1,70 @@
+"""Vectors already in device memory (PyTorch tensors on the GPU): the device-pointer solve and
+adjoint give the host-vector calls' bits, for contiguous and strided vectors, and the autograd
+wrapper's gradients are the oracle's adjoint bit for bit."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import ybus_amd as Y
+from tests.fixtures import random_spd, random_square_cuda, right_hand_sides
+from tests.gpu_common import gpu, path  # noqa: F401
+
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("path")]
+
+
+def cuda(a):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def test_device_solves_match_the_host_solves(gpu):
+    a = random_square_cuda(150, 4, 12)
+    lu = O.Lu(a)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    rhs = right_hand_sides(150, 23, 4)
+    assert factor.solve_device(cuda(rhs)).cpu().numpy().tobytes() == lu.solve_many(rhs).tobytes()
+    got = factor.solve_device(cuda(rhs), transpose=True).cpu().numpy()
+    assert got.tobytes() == lu.solve_transpose_many(rhs).tobytes()
+    # vectors with a stride wider than dim(), in and out
+    wide_in = torch.zeros((23, 157), dtype=torch.float64, device="cuda")
+    wide_in[:, :150] = cuda(rhs)
+    wide_out = torch.full((23, 161), 7.0, dtype=torch.float64, device="cuda")
+    factor.solve_device(wide_in[:, :150], wide_out[:, :150])
+    assert wide_out[:, :150].cpu().numpy().tobytes() == lu.solve_many(rhs).tobytes()
+    assert torch.all(wide_out[:, 150:] == 7.0)
+    factor.set_block_width(5)  # several blocks
+    assert factor.solve_device(cuda(rhs)).cpu().numpy().tobytes() == lu.solve_many(rhs).tobytes()
+
+
+def test_device_adjoint_matches_the_host_adjoint(gpu):
+    a = random_spd(140, 4, 6)
+    chol = O.Cholesky(a)
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    factor.set_matrix(a)
+    x_bar = right_hand_sides(140, 19, 1)
+    x = right_hand_sides(140, 19, 2)
+    want_b, want_a = factor.adjoint(x_bar, x)
+    b_bar, a_bar = factor.adjoint_device(cuda(x_bar), cuda(x))
+    assert b_bar.cpu().numpy().tobytes() == want_b.tobytes()
+    assert a_bar.cpu().numpy().tobytes() == want_a.tobytes()
+
+
+def test_autograd_gives_the_oracle_adjoint(gpu):
+    from ybus_amd.autograd import solve
+
+    g = O.make_grid(800, 31)
+    bp, _, _ = g.dc()
+    chol = O.Cholesky(bp)
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    factor.set_matrix(bp)
+    p = O.injections(31, 2, 800, 9)
+    w = right_hand_sides(bp.rows, 9, 3)
+    a_values = cuda(bp.data).requires_grad_(True)
+    b = cuda(p).requires_grad_(True)
+    x = solve(factor, a_values, b)
+    assert x.detach().cpu().numpy().tobytes() == chol.solve_many(p).tobytes()
+    loss = (x * cuda(w)).sum()  # d loss / d x = w exactly
+    loss.backward()
+    want_b, want_a = O.adjoint(chol, bp, w, chol.solve_many(p))
+    assert b.grad.cpu().numpy().tobytes() == want_b.tobytes()
+    assert a_values.grad.cpu().numpy().tobytes() == want_a.tobytes()

@@ -117,6 +117,8 @@ def _declare(L):
         "ybus_hip_factor_solve_transpose_many": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_many_refined": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
         "ybus_hip_factor_adjoint": (i32, [P, f64p, f64p, U64, f64p, f64p]),
+        "ybus_hip_factor_solve_device": (i32, [P, i32, C.c_void_p, U64, U64, C.c_void_p, U64]),
+        "ybus_hip_factor_adjoint_device": (i32, [P, C.c_void_p, C.c_void_p, U64, U64, C.c_void_p, C.c_void_p]),
         "ybus_hip_factor_autotune": (i32, [P, U64, i32, C.c_void_p, C.c_void_p, C.POINTER(_Tuning)]),
         "ybus_hip_factor_stage": (i32, [P, f64p, U64]),
         "ybus_hip_factor_stage_pairs": (i32, [P, i64p, i64p, U64]),
@@ -455,6 +457,54 @@ class GpuFactor:
         a_bar = np.zeros(max(self._nnz_a, 0))
         _check(library().ybus_hip_factor_adjoint(self._h, xb, xs, xb.shape[0], b_bar, a_bar))
         return b_bar, a_bar
+
+    # ---- vectors already in device memory: torch tensors (count, dim) of float64 on the GPU ----
+
+    def _device_vectors(self, t, count=None):
+        import torch
+
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda or t.dim() != 2:
+            raise InvalidArgument(INVALID_ARGUMENT, "a (count, dim) float64 tensor on the GPU")
+        if t.shape[1] != self._n:
+            raise ShapeMismatch(self._n, t.shape[1])
+        if count is not None and t.shape[0] != count:
+            raise ShapeMismatch(count, t.shape[0])
+        if t.stride(1) != 1:
+            raise InvalidArgument(INVALID_ARGUMENT, "each vector must be contiguous")
+        return t.data_ptr(), t.stride(0)
+
+    def solve_device(self, b, x=None, transpose: bool = False):
+        """Solves the vectors of the GPU tensor b (count, dim) into x (a new tensor unless given),
+        without leaving device memory; bit for bit solve_many (or solve_transpose_many)."""
+        import torch
+
+        pb, ldb = self._device_vectors(b)
+        if x is None:
+            x = torch.empty_like(b, memory_format=torch.contiguous_format)
+        px, ldx = self._device_vectors(x, b.shape[0])
+        self._release_block()
+        torch.cuda.current_stream().synchronize()
+        _check(library().ybus_hip_factor_solve_device(self._h, TRANSPOSE if transpose else FP64, pb, ldb,
+                                                        b.shape[0], px, ldx))
+        return x
+
+    def adjoint_device(self, x_bar, x):
+        """adjoint() on GPU tensors: returns (b_bar, a_bar) as new GPU tensors."""
+        import torch
+
+        if self._nnz_a < 0:
+            raise InvalidArgument(INVALID_ARGUMENT, "the adjoint needs the matrix (set_matrix)")
+        x_bar = x_bar.contiguous()
+        x = x.contiguous()
+        pxb, ld = self._device_vectors(x_bar)
+        px, ldx = self._device_vectors(x, x_bar.shape[0])
+        b_bar = torch.empty_like(x_bar)
+        a_bar = torch.empty(max(self._nnz_a, 1), dtype=torch.float64, device=x_bar.device)
+        self._release_block()
+        torch.cuda.current_stream().synchronize()
+        _check(library().ybus_hip_factor_adjoint_device(self._h, pxb, px, ld, x_bar.shape[0], b_bar.data_ptr(),
+                                                          a_bar.data_ptr()))
+        return b_bar, a_bar[: self._nnz_a]
 
     def stage(self, rhs):
         """Copies right-hand sides (at most block_width()) to the device without solving."""

@@ -984,6 +984,13 @@ __global__ void sddmm_kernel(const int32_t* __restrict__ row, const int32_t* __r
   acc[q] = sum;
 }
 
+__global__ void scatter_negated_kernel(const double* __restrict__ acc, const int32_t* __restrict__ order, int64_t nnz,
+                                       int32_t any, double* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nnz) out[order[q]] = -(any ? acc[q] : 0.0);
+}
+
+
 }  // namespace
 
 // ---- launchers ----
@@ -1121,6 +1128,14 @@ hipError_t launch_sddmm(hipStream_t stream, const int32_t* row, const int32_t* c
   const unsigned grid = (unsigned)((nnz + 255) / 256);
   hipLaunchKernelGGL(sddmm_kernel, dim3(grid), dim3(256), 0, stream, row, col, nnz, bbar, x, rs, cs, ncols, acc,
                      first ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_negated(hipStream_t stream, const double* acc, const int32_t* order, int64_t nnz, bool any,
+                                  double* out) {
+  if (nnz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_negated_kernel, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, stream, acc, order, nnz,
+                     any ? 1 : 0, out);
   return hipGetLastError();
 }
 

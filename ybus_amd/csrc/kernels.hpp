@@ -148,6 +148,11 @@ hipError_t launch_operator(hipStream_t stream, const CsrDev& m, const double* x,
                            const double* offset, const int32_t* zero, double* out, int64_t ors, int64_t ocs,
                            int32_t ncols);
 
+// out[order[q]] = -acc[q] for each stored entry q (-0.0 everywhere when `any` is false: the
+// adjoint of no vectors)
+hipError_t launch_scatter_negated(hipStream_t stream, const double* acc, const int32_t* order, int64_t nnz, bool any,
+                                  double* out);
+
 // Batches: a persistent launch of `grid` workgroups that take (factor, column) items from the
 // queue counter (which must hold 0 at the launch) until `total` are taken. prepare_batch allows
 // the kernel lds_bytes of LDS (call it before a launch or a graph capture).

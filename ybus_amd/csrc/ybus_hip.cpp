@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -200,6 +201,7 @@ struct DevCsr {
   size_t nnz = 0;
   DevBuf<int32_t> row;         // each entry's row (the adjoint's SDDMM takes entries one by one)
   std::vector<int64_t> order;  // each entry's position in the matrix as the caller stored it
+  DevBuf<int32_t> order_dev;   // the same on the device
   CsrDev view() const { return CsrDev{rows, cols, ptr.p, col.p, val.p}; }
 };
 
@@ -864,6 +866,8 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
     f->a.col.upload(cols, f->stream());
     f->a.val.upload(vals, f->stream());
     f->a.row.upload(rows, f->stream());
+    std::vector<int32_t> order32(order.begin(), order.end());
+    f->a.order_dev.upload(order32, f->stream());
     f->a.order = std::move(order);
     check(hipStreamSynchronize(f->stream()), "upload matrix");
   });
@@ -897,6 +901,43 @@ static void download_vectors(ybus_hip_factor* f, const double* block, const Layo
   }
   check(hipStreamSynchronize(f->stream()), "download");
   std::memcpy(out, host, len * sizeof(double));
+}
+
+// Copies count device vectors (vector c at src + c * ld) into a block of the workspace, in the
+// workspace's layout; the caller's writes to src must be finished
+static void copy_in_device(ybus_hip_factor* f, const double* src, int64_t ld, int64_t count, double* block) {
+  const size_t n = (size_t)f->n;
+  if (f->layout.columns) {
+    check(hipMemcpy2DAsync(block, (size_t)f->layout.ld * sizeof(double), src, (size_t)ld * sizeof(double),
+                           n * sizeof(double), (size_t)count, hipMemcpyDeviceToDevice, f->stream()),
+          "D2D");
+  } else {
+    const double* vectors = src;
+    if (ld != f->n) {
+      check(hipMemcpy2DAsync(f->vec.p, n * sizeof(double), src, (size_t)ld * sizeof(double), n * sizeof(double),
+                             (size_t)count, hipMemcpyDeviceToDevice, f->stream()),
+            "D2D");
+      vectors = f->vec.p;
+    }
+    check(launch_to_block(f->stream(), vectors, block, f->n, count, f->layout.ld), "to_block");
+  }
+}
+
+// Copies count vectors of a workspace block out to device memory (vector c at dst + c * ld)
+static void copy_out_device(ybus_hip_factor* f, const double* block, int64_t count, double* dst, int64_t ld) {
+  const size_t n = (size_t)f->n;
+  if (f->layout.columns) {
+    check(hipMemcpy2DAsync(dst, (size_t)ld * sizeof(double), block, (size_t)f->layout.ld * sizeof(double),
+                           n * sizeof(double), (size_t)count, hipMemcpyDeviceToDevice, f->stream()),
+          "D2D");
+  } else if (ld == f->n) {
+    check(launch_from_block(f->stream(), block, dst, f->n, count, f->layout.ld), "from_block");
+  } else {
+    check(launch_from_block(f->stream(), block, f->vec.p, f->n, count, f->layout.ld), "from_block");
+    check(hipMemcpy2DAsync(dst, (size_t)ld * sizeof(double), f->vec.p, n * sizeof(double), n * sizeof(double),
+                           (size_t)count, hipMemcpyDeviceToDevice, f->stream()),
+          "D2D");
+  }
 }
 
 static void require_unborrowed(const ybus_hip_factor* f) {
@@ -1001,6 +1042,69 @@ int32_t ybus_hip_factor_adjoint(ybus_hip_factor* f, const double* x_bar, const d
     }
     // A_bar = -(sum over the vectors of b_bar_s x_s^T), in the caller's storage order
     for (int64_t q = 0; q < nnz; ++q) a_bar[f->a.order[q]] = -acc[q];
+  });
+}
+
+static void check_device_vectors(const ybus_hip_factor* f, uint64_t count, uint64_t ld,
+                                 std::initializer_list<const void*> pointers) {
+  if (count == 0) return;
+  for (const void* p : pointers)
+    if (!p) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+  if (ld < (uint64_t)f->n) fail(YBUS_HIP_INVALID_ARGUMENT, "the vectors' stride is below dim()");
+  if (count > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many vectors");
+}
+
+int32_t ybus_hip_factor_solve_device(ybus_hip_factor* f, int32_t mode, const double* b, uint64_t ld_b, uint64_t count,
+                                     double* x, uint64_t ld_x) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    require_unborrowed(f);
+    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_TRANSPOSE) fail(YBUS_HIP_INVALID_ARGUMENT, "mode: FP64 or TRANSPOSE");
+    check_device_vectors(f, count, std::min(ld_b, ld_x), {b, x});
+    if (count == 0 || f->n == 0) return;
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
+    f->ensure_workspace(width, false);
+    for (uint64_t start = 0; start < count; start += (uint64_t)width) {
+      const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
+      copy_in_device(f, b + start * ld_b, (int64_t)ld_b, chunk, f->b.p);
+      f->solve64(mode == YBUS_HIP_TRANSPOSE, false, chunk);
+      copy_out_device(f, f->x.p, chunk, x + start * ld_x, (int64_t)ld_x);
+    }
+    check(hipStreamSynchronize(f->stream()), "solve");
+  });
+}
+
+int32_t ybus_hip_factor_adjoint_device(ybus_hip_factor* f, const double* x_bar, const double* x, uint64_t ld,
+                                       uint64_t count, double* b_bar, double* a_bar) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    require_unborrowed(f);
+    if (!f->a.ptr.p) fail(YBUS_HIP_INVALID_ARGUMENT, "the adjoint needs the matrix (ybus_hip_factor_set_matrix)");
+    const int64_t nnz = (int64_t)f->a.nnz;
+    check_device_vectors(f, count, ld, {x_bar, x, b_bar});
+    if (nnz && !a_bar) fail(YBUS_HIP_INVALID_ARGUMENT, "a_bar is null");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    f->abar.reserve((size_t)std::max<int64_t>(nnz, 1));
+    if (count && f->n) {
+      const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
+      f->ensure_workspace(width, false);
+      for (uint64_t start = 0; start < count; start += (uint64_t)width) {
+        const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
+        copy_in_device(f, x_bar + start * ld, (int64_t)ld, chunk, f->b.p);
+        f->solve64(true, false, chunk);
+        copy_out_device(f, f->x.p, chunk, b_bar + start * ld, (int64_t)ld);
+        copy_in_device(f, x + start * ld, (int64_t)ld, chunk, f->y2.p);
+        f->begin_kernel();
+        check(launch_sddmm(f->stream(), f->a.row.p, f->a.col.p, nnz, f->x.p, f->y2.p, f->layout.rs(),
+                           f->layout.cs(), (int32_t)chunk, f->abar.p, start == 0),
+              "adjoint kernel");
+        f->end_kernel(16 * (uint64_t)f->n * (uint64_t)chunk + 16 * (uint64_t)nnz);
+      }
+    }
+    check(launch_scatter_negated(f->stream(), f->abar.p, f->a.order_dev.p, nnz, count && f->n, a_bar),
+          "adjoint scatter");
+    check(hipStreamSynchronize(f->stream()), "adjoint");
   });
 }
 
