@@ -251,13 +251,15 @@ def copy_calibration(mib: int):
 
 def run_adjoint(gpu, args):
     """The adjoint on the 70,000-bus grid: 64 zero-sum N(0,1) injection scenarios, branch limits
-    U[0.5, 2.0] pu, loss = sum relu(|f| - limit)^2 with f = Bf theta. The device solves theta,
-    the host forms x_bar = Bf^T (2 relu(|f| - limit) sign f), and the device computes
-    b_bar = B'^-T x_bar and A_bar = -(sum b_bar theta^T) on B''s pattern."""
-    import scipy.sparse as sp
+    U[0.5, 2.0] pu, loss = sum relu(|f| - limit)^2 with f = Bf theta, theta = B'^-1 p. The whole
+    step runs in device memory: the solve through ybus_amd.autograd.solve, the flows and the loss
+    as PyTorch tensor ops (plumbing), and the backward pass through the device adjoint
+    (b_bar = B'^-T x_bar, A_bar = -(sum b_bar theta^T) on B''s pattern)."""
+    import torch
 
     import oracle as O
     import ybus_amd as Y
+    from ybus_amd.autograd import solve
 
     buses, seed = 70000, SEED + 4
     grid = O.make_grid(buses, seed)
@@ -268,34 +270,58 @@ def run_adjoint(gpu, args):
     factor = Y.GpuFactor.from_cholesky(gpu, chol)
     factor.set_matrix(bprime)
     factor.set_block_width(64)
-    theta = factor.solve_many(p)
-    m = sp.csr_matrix((bf.data, bf.idx, bf.ptr), shape=(bf.rows, bf.cols))
-    f = (m @ theta.T).T
-    over = np.maximum(np.abs(f) - limit, 0.0)
-    x_bar = np.ascontiguousarray((m.T @ (2.0 * over * np.sign(f)).T).T)
-    loss = float(np.sum(over * over))
+    # Bf has at most two entries per row: f = v0 * theta[c0] + v1 * theta[c1]
+    counts = np.diff(bf.ptr)
+    c0 = np.where(counts > 0, bf.idx[np.minimum(bf.ptr[:-1], len(bf.idx) - 1)], 0)
+    v0 = np.where(counts > 0, bf.data[np.minimum(bf.ptr[:-1], len(bf.idx) - 1)], 0.0)
+    c1 = np.where(counts > 1, bf.idx[np.minimum(bf.ptr[:-1] + 1, len(bf.idx) - 1)], 0)
+    v1 = np.where(counts > 1, bf.data[np.minimum(bf.ptr[:-1] + 1, len(bf.idx) - 1)], 0.0)
+    dev = torch.device("cuda")
+    t_c0, t_c1 = torch.tensor(c0, device=dev), torch.tensor(c1, device=dev)
+    t_v0, t_v1 = torch.tensor(v0, device=dev), torch.tensor(v1, device=dev)
+    t_lim = torch.tensor(limit, device=dev)
+    t_p = torch.tensor(p, device=dev)
+    a_values = torch.tensor(bprime.data, device=dev)
+    captured = {}
+
+    def step():
+        a = a_values.detach().requires_grad_(True)
+        b = t_p.detach().requires_grad_(True)
+        theta = solve(factor, a, b)
+        theta.register_hook(lambda g: captured.__setitem__("x_bar", g))
+        f = t_v0 * theta[:, t_c0] + t_v1 * theta[:, t_c1]
+        over = torch.relu(f.abs() - t_lim)
+        loss = (over * over).sum()
+        loss.backward()
+        return loss, theta, a.grad, b.grad
+
     for _ in range(max(1, args.warmup)):
-        factor.adjoint(x_bar, theta)
+        step()
+    torch.cuda.synchronize()
     factor.reset_kernel_stats()
     reps = max(3, args.steps // 4)
     t0 = time.perf_counter()
     for _ in range(reps):
-        b_bar, a_bar = factor.adjoint(x_bar, theta)
+        loss, theta, a_grad, b_grad = step()
+    torch.cuda.synchronize()
     seconds = (time.perf_counter() - t0) / reps
     stats = factor.kernel_stats()
-    want_b, want_a = O.adjoint(chol, bprime, x_bar, theta)
+    x_bar = captured["x_bar"].cpu().numpy()
+    theta_h = theta.detach().cpu().numpy()
+    want_b, want_a = O.adjoint(chol, bprime, x_bar, theta_h)
     out = dict(workload="70,000-bus grid (n=%d), B' AMD Cholesky (nnz(L)=%d, nnz(B')=%d), 64 scenarios, "
                         "relu^2 flow-limit loss" % (chol.n, int(chol.l.nnz), bprime.nnz),
-               loss=loss, seconds_per_adjoint=seconds,
-               device_kernel_ms_per_adjoint=stats.milliseconds / reps,
-               kernel_launches_per_adjoint=stats.launches // reps,
-               b_bar_bit_exact=bool(b_bar.tobytes() == want_b.tobytes()),
-               a_bar_bit_exact=bool(a_bar.tobytes() == want_a.tobytes()),
-               note="seconds_per_adjoint includes the host-vector copies (2 x 64 x n doubles in, "
-                    "64 x n + nnz(B') out)")
-    log(f"[adjoint] {seconds * 1e3:.2f} ms per adjoint (host vectors), device kernels "
-        f"{out['device_kernel_ms_per_adjoint']:.3f} ms (transposed solve + SDDMM); bit-exact b_bar "
-        f"{out['b_bar_bit_exact']}, a_bar {out['a_bar_bit_exact']}")
+               loss=float(loss.detach()), ms_per_step=seconds * 1e3,
+               library_kernel_ms_per_step=stats.milliseconds / reps,
+               library_kernel_launches_per_step=stats.launches // reps,
+               theta_bit_exact=bool(theta_h.tobytes() == chol.solve_many(p).tobytes()),
+               b_bar_bit_exact=bool(b_grad.cpu().numpy().tobytes() == want_b.tobytes()),
+               a_bar_bit_exact=bool(a_grad.cpu().numpy().tobytes() == want_a.tobytes()),
+               note="one step = forward solve, flows and loss, backward (the library's transposed solve and "
+                    "SDDMM), all in device memory; library kernels = forward solve + transposed solve + SDDMM")
+    log(f"[adjoint] {out['ms_per_step']:.2f} ms per device step (library kernels "
+        f"{out['library_kernel_ms_per_step']:.3f} ms in {out['library_kernel_launches_per_step']} launches); "
+        f"bit-exact theta {out['theta_bit_exact']}, b_bar {out['b_bar_bit_exact']}, a_bar {out['a_bar_bit_exact']}")
     return out
 
 
@@ -388,6 +414,11 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
+
+    if not args.no_adjoint:
+        # PyTorch ships its own HIP runtime: it must be loaded before libybus_hip.so, so that both
+        # use that one runtime (loaded the other way round, PyTorch finds no GPU)
+        import torch  # noqa: F401
 
     import ybus_amd as Y
 

@@ -7,6 +7,9 @@ b_bar x^T) on A's stored pattern, both computed on the GPU without leaving devic
 ``a_values`` is the tensor of A's stored values in the order given to ``factor.set_matrix``; the
 solve does not read it (the factor was made from those values on the host), it only carries A's
 gradient. Results are bit for bit the CPU's transposed solve and outer-product sums.
+
+PyTorch ships its own HIP runtime: import torch before the first call that loads
+libybus_hip.so (creating a ybus_amd.Gpu does), so that the process has one HIP runtime.
 """
 from __future__ import annotations
 
