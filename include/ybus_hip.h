@@ -164,12 +164,15 @@ int32_t ybus_hip_factor_solve_many_refined(ybus_hip_factor *f, const double *rhs
 /* The adjoint of A x = b (new: the reference has no GPU gradient; its CPU counterpart is the
  * composition of Lu::solve_transpose, lu.rs:502-533, with an outer product). For `count` pairs
  * (x_bar_s, x_s), x_bar_s being the gradient of a scalar loss with respect to the solution x_s of
- * A x_s = b_s: b_bar_s = A^-T x_bar_s (the loss's gradient with respect to b_s; FP64, the CPU's
- * bits), and a_bar = -(sum over s of b_bar_s x_s^T) on A's stored pattern (its gradient with
- * respect to A's stored values), each entry summed over s in ascending order, so it is the same
- * on every run. Needs ybus_hip_factor_set_matrix; a_bar has one value per stored entry of that
- * matrix, in the order the caller stored them. */
-int32_t ybus_hip_factor_adjoint(ybus_hip_factor *f, const double *x_bar, const double *x,
+ * A x_s = b_s: b_bar_s = A^-T x_bar_s (the loss's gradient with respect to b_s), and
+ * a_bar = -(sum over s of b_bar_s x_s^T) on A's stored pattern (its gradient with respect to A's
+ * stored values), each entry summed over s in ascending order, so it is the same on every run.
+ * mode YBUS_HIP_FP64: the transposed solve in FP64, the CPU's bits; YBUS_HIP_FP32_REFINED: FP32
+ * transposed sweeps refined in FP64 against A^T (refine_normwise's rule, to `tolerance`, at most
+ * max_steps corrections). Needs ybus_hip_factor_set_matrix; a_bar has one value per stored entry
+ * of that matrix, in the order the caller stored them. */
+int32_t ybus_hip_factor_adjoint(ybus_hip_factor *f, int32_t mode, double tolerance,
+                                uint64_t max_steps, const double *x_bar, const double *x,
                                 uint64_t count, double *b_bar, double *a_bar);
 
 /* ---- vectors already in device memory (new) ----
@@ -181,11 +184,15 @@ int32_t ybus_hip_factor_adjoint(ybus_hip_factor *f, const double *x_bar, const d
  * to the inputs must be finished before the call (synchronize the stream that made them); the
  * call returns with its outputs written. Results are bit for bit those of the host-vector
  * calls. */
-/* solve_many (mode YBUS_HIP_FP64) or solve_transpose_many (YBUS_HIP_TRANSPOSE) on device vectors. */
-int32_t ybus_hip_factor_solve_device(ybus_hip_factor *f, int32_t mode, const double *b,
-                                     uint64_t ld_b, uint64_t count, double *x, uint64_t ld_x);
+/* solve_many (mode YBUS_HIP_FP64), solve_transpose_many (YBUS_HIP_TRANSPOSE) or
+ * solve_many_refined (YBUS_HIP_FP32_REFINED, to `tolerance` in at most max_steps corrections)
+ * on device vectors. */
+int32_t ybus_hip_factor_solve_device(ybus_hip_factor *f, int32_t mode, double tolerance,
+                                     uint64_t max_steps, const double *b, uint64_t ld_b,
+                                     uint64_t count, double *x, uint64_t ld_x);
 /* ybus_hip_factor_adjoint on device vectors (x_bar, x and b_bar share the stride ld). */
-int32_t ybus_hip_factor_adjoint_device(ybus_hip_factor *f, const double *x_bar, const double *x,
+int32_t ybus_hip_factor_adjoint_device(ybus_hip_factor *f, int32_t mode, double tolerance,
+                                       uint64_t max_steps, const double *x_bar, const double *x,
                                        uint64_t ld, uint64_t count, double *b_bar,
                                        double *a_bar);
 

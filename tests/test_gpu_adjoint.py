@@ -85,7 +85,35 @@ def flow_loss(a: O.Matrix, bf: np.ndarray, p: np.ndarray, limit: np.ndarray):
     return float(np.sum(over * over)), x, x_bar
 
 
-def test_gradients_match_central_finite_differences(gpu):
+@pytest.mark.parametrize("kind", ["lu", "cholesky"])
+def test_refined_adjoint_agrees_with_the_fp64_adjoint(gpu, kind):
+    """FP32 transposed sweeps refined in FP64 against A^T: b_bar reaches the transposed system's
+    FP64 floor, and both gradients agree with the FP64 adjoint to 1e-8 relative (the bar the
+    configuration sets for the refined gradients)."""
+    if kind == "lu":
+        a, _ = O.make_grid(400, 8).jacobian(8)
+        cpu_factor = O.Lu(a, O.COLAMD)
+        factor = Y.GpuFactor.from_lu(gpu, cpu_factor)
+    else:
+        a, _, _ = O.make_grid(900, 9).dc()
+        cpu_factor = O.Cholesky(a)
+        factor = Y.GpuFactor.from_cholesky(gpu, cpu_factor)
+    factor.set_matrix(a)
+    x_bar = right_hand_sides(a.rows, 12, 3)
+    x = right_hand_sides(a.rows, 12, 4)
+    b64, a64 = factor.adjoint(x_bar, x)
+    b32, a32 = factor.adjoint(x_bar, x, Y.FP32_REFINED, 1e-12, 10)
+    assert np.max(np.abs(b32 - b64)) <= 1e-8 * np.max(np.abs(b64))
+    assert np.max(np.abs(a32 - a64)) <= 1e-8 * np.max(np.abs(a64))
+    at = a.to_dense().T
+    for k in range(12):
+        floor = np.max(np.abs(x_bar[k] - at @ b64[k])) / np.max(np.abs(x_bar[k]))
+        ratio = np.max(np.abs(x_bar[k] - at @ b32[k])) / np.max(np.abs(x_bar[k]))
+        assert ratio <= max(1e-12, 4.0 * floor), (k, ratio, floor)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32_refined"])
+def test_gradients_match_central_finite_differences(gpu, mode):
     g = O.make_grid(300, 20261207)
     bp, bf, _ = g.dc()
     bf = bf.to_dense() if hasattr(bf, "to_dense") else bf
@@ -96,7 +124,7 @@ def test_gradients_match_central_finite_differences(gpu):
     assert loss > 0.0
     factor = Y.GpuFactor.from_cholesky(gpu, O.Cholesky(bp))
     factor.set_matrix(bp)
-    b_bar, a_bar = factor.adjoint(x_bar, x)
+    b_bar, a_bar = factor.adjoint(x_bar, x, Y.FP64 if mode == "fp64" else Y.FP32_REFINED, 1e-12, 10)
     h = 1e-6
     # a central difference carries the loss's own rounding, a few units of 1e-16 * loss each, over
     # 2h (the loss is piecewise quadratic in p, so that is all of its error there)

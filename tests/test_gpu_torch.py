@@ -68,3 +68,24 @@ def test_autograd_gives_the_oracle_adjoint(gpu):
     want_b, want_a = O.adjoint(chol, bp, w, chol.solve_many(p))
     assert b.grad.cpu().numpy().tobytes() == want_b.tobytes()
     assert a_values.grad.cpu().numpy().tobytes() == want_a.tobytes()
+
+
+def test_autograd_in_fp32_refinement_agrees_with_fp64(gpu):
+    from ybus_amd.autograd import FP32_REFINED, solve
+
+    g = O.make_grid(1200, 41)
+    bp, _, _ = g.dc()
+    chol = O.Cholesky(bp)
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    factor.set_matrix(bp)
+    p = O.injections(41, 2, 1200, 7)
+    w = cuda(right_hand_sides(bp.rows, 7, 5))
+    grads = []
+    for mode in [0, FP32_REFINED]:
+        a_values = cuda(bp.data).requires_grad_(True)
+        b = cuda(p).requires_grad_(True)
+        x = solve(factor, a_values, b, mode)
+        (x * w).sum().backward()
+        grads.append((x.detach(), b.grad, a_values.grad))
+    for exact, refined in zip(grads[0], grads[1]):
+        assert torch.max(torch.abs(refined - exact)) <= 1e-8 * torch.max(torch.abs(exact))

@@ -116,9 +116,10 @@ def _declare(L):
         "ybus_hip_factor_solve_many": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_transpose_many": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_many_refined": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
-        "ybus_hip_factor_adjoint": (i32, [P, f64p, f64p, U64, f64p, f64p]),
-        "ybus_hip_factor_solve_device": (i32, [P, i32, C.c_void_p, U64, U64, C.c_void_p, U64]),
-        "ybus_hip_factor_adjoint_device": (i32, [P, C.c_void_p, C.c_void_p, U64, U64, C.c_void_p, C.c_void_p]),
+        "ybus_hip_factor_adjoint": (i32, [P, i32, C.c_double, U64, f64p, f64p, U64, f64p, f64p]),
+        "ybus_hip_factor_solve_device": (i32, [P, i32, C.c_double, U64, C.c_void_p, U64, U64, C.c_void_p, U64]),
+        "ybus_hip_factor_adjoint_device": (i32, [P, i32, C.c_double, U64, C.c_void_p, C.c_void_p, U64, U64,
+                                                 C.c_void_p, C.c_void_p]),
         "ybus_hip_factor_autotune": (i32, [P, U64, i32, C.c_void_p, C.c_void_p, C.POINTER(_Tuning)]),
         "ybus_hip_factor_stage": (i32, [P, f64p, U64]),
         "ybus_hip_factor_stage_pairs": (i32, [P, i64p, i64p, U64]),
@@ -444,10 +445,11 @@ class GpuFactor:
                                                                 int(max_steps), x, steps, ratio))
         return x, steps.astype(np.int64), ratio
 
-    def adjoint(self, x_bar, x):
+    def adjoint(self, x_bar, x, mode: int = FP64, tolerance: float = 1e-12, max_steps: int = 10):
         """The adjoint of A x = b (needs set_matrix): for vectors x_bar_s (a loss's gradient with
-        respect to the solutions x_s), returns (b_bar, a_bar): b_bar_s = A^-T x_bar_s, and a_bar =
-        -(sum over s of b_bar_s x_s^T) on A's stored pattern, in the matrix's storage order."""
+        respect to the solutions x_s), returns (b_bar, a_bar): b_bar_s = A^-T x_bar_s (FP64, or
+        FP32_REFINED: FP32 sweeps refined in FP64), and a_bar = -(sum over s of b_bar_s x_s^T) on
+        A's stored pattern, in the matrix's storage order."""
         xb = _rhs_array(x_bar, self._n)
         xs = _rhs_array(x, self._n)
         if xb.shape != xs.shape:
@@ -455,7 +457,8 @@ class GpuFactor:
         self._release_block()
         b_bar = np.zeros_like(xb)
         a_bar = np.zeros(max(self._nnz_a, 0))
-        _check(library().ybus_hip_factor_adjoint(self._h, xb, xs, xb.shape[0], b_bar, a_bar))
+        _check(library().ybus_hip_factor_adjoint(self._h, int(mode), float(tolerance), int(max_steps), xb, xs,
+                                                 xb.shape[0], b_bar, a_bar))
         return b_bar, a_bar
 
     # ---- vectors already in device memory: torch tensors (count, dim) of float64 on the GPU ----
@@ -473,9 +476,11 @@ class GpuFactor:
             raise InvalidArgument(INVALID_ARGUMENT, "each vector must be contiguous")
         return t.data_ptr(), t.stride(0)
 
-    def solve_device(self, b, x=None, transpose: bool = False):
+    def solve_device(self, b, x=None, transpose: bool = False, mode: int = None, tolerance: float = 1e-12,
+                     max_steps: int = 10):
         """Solves the vectors of the GPU tensor b (count, dim) into x (a new tensor unless given),
-        without leaving device memory; bit for bit solve_many (or solve_transpose_many)."""
+        without leaving device memory: bit for bit solve_many (or solve_transpose_many), or with
+        mode=FP32_REFINED solve_many_refined's solutions."""
         import torch
 
         pb, ldb = self._device_vectors(b)
@@ -484,11 +489,13 @@ class GpuFactor:
         px, ldx = self._device_vectors(x, b.shape[0])
         self._release_block()
         torch.cuda.current_stream().synchronize()
-        _check(library().ybus_hip_factor_solve_device(self._h, TRANSPOSE if transpose else FP64, pb, ldb,
-                                                        b.shape[0], px, ldx))
+        if mode is None:
+            mode = TRANSPOSE if transpose else FP64
+        _check(library().ybus_hip_factor_solve_device(self._h, int(mode), float(tolerance), int(max_steps), pb,
+                                                        ldb, b.shape[0], px, ldx))
         return x
 
-    def adjoint_device(self, x_bar, x):
+    def adjoint_device(self, x_bar, x, mode: int = FP64, tolerance: float = 1e-12, max_steps: int = 10):
         """adjoint() on GPU tensors: returns (b_bar, a_bar) as new GPU tensors."""
         import torch
 
@@ -502,8 +509,8 @@ class GpuFactor:
         a_bar = torch.empty(max(self._nnz_a, 1), dtype=torch.float64, device=x_bar.device)
         self._release_block()
         torch.cuda.current_stream().synchronize()
-        _check(library().ybus_hip_factor_adjoint_device(self._h, pxb, px, ld, x_bar.shape[0], b_bar.data_ptr(),
-                                                          a_bar.data_ptr()))
+        _check(library().ybus_hip_factor_adjoint_device(self._h, int(mode), float(tolerance), int(max_steps), pxb,
+                                                          px, ld, x_bar.shape[0], b_bar.data_ptr(), a_bar.data_ptr()))
         return b_bar, a_bar[: self._nnz_a]
 
     def stage(self, rhs):

@@ -16,21 +16,29 @@ from __future__ import annotations
 import torch
 
 
+FP64, FP32_REFINED = 0, 1
+
+
 class _Solve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, factor, a_values, b):
-        x = factor.solve_device(b.contiguous())
+    def forward(ctx, factor, a_values, b, mode, tolerance, max_steps):
+        x = factor.solve_device(b.contiguous(), mode=mode, tolerance=tolerance, max_steps=max_steps)
         ctx.factor = factor
+        ctx.settings = (mode, tolerance, max_steps)
         ctx.save_for_backward(x)
         return x
 
     @staticmethod
     def backward(ctx, x_bar):
         (x,) = ctx.saved_tensors
-        b_bar, a_bar = ctx.factor.adjoint_device(x_bar, x)
-        return None, a_bar, b_bar
+        mode, tolerance, max_steps = ctx.settings
+        b_bar, a_bar = ctx.factor.adjoint_device(x_bar, x, mode, tolerance, max_steps)
+        return None, a_bar, b_bar, None, None, None
 
 
-def solve(factor, a_values: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
-    """x = A^-1 b row by row, differentiable in A's stored values and in b (see the module)."""
-    return _Solve.apply(factor, a_values, b)
+def solve(factor, a_values: torch.Tensor, b: torch.Tensor, mode: int = FP64, tolerance: float = 1e-12,
+          max_steps: int = 10) -> torch.Tensor:
+    """x = A^-1 b row by row, differentiable in A's stored values and in b (see the module). With
+    mode=FP32_REFINED both the solve and the backward's transposed solve are FP32 sweeps refined in
+    FP64 to `tolerance` (refine_normwise's rule); FP64 gives the CPU's bits."""
+    return _Solve.apply(factor, a_values, b, mode, tolerance, max_steps)

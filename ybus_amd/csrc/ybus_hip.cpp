@@ -277,7 +277,8 @@ struct ybus_hip_factor {
   DevSweep cfwd, cbwd, ctfwd, ctbwd;
   DevBuf<int32_t> rhs_row, sol_of, t_rhs_row, t_sol_of;
   bool has_transpose = false;
-  DevCsr a;  // the matrix itself, by user rows, for the FP64 residual
+  DevCsr a;   // the matrix itself, by user rows, for the FP64 residual
+  DevCsr at;  // its transpose, by rows (A's columns), for the residual of a transposed refinement
   // workspace for `cols` columns
   int64_t cols = 0;
   Layout layout;
@@ -502,20 +503,25 @@ struct ybus_hip_factor {
     end_kernel(bytes64(ncols, pairs));
   }
 
-  void refine(int64_t ncols, double tol, int32_t max_steps) {
+  // FP32 sweeps refined in FP64 (transpose: A^T x = b, with A^T's sweeps and residual)
+  void refine(int64_t ncols, double tol, int32_t max_steps, bool transpose = false) {
     if (!a.ptr.p) fail(YBUS_HIP_INVALID_ARGUMENT, "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix)");
+    const bool t_sweeps = transpose && has_transpose;  // A^T = A for a Cholesky factor
+    const CsrDev m = transpose ? at.view() : a.view();
     begin_kernel();
     if (layout.columns) {
       int th = 1;
       while (th * 2 <= column_threads(ncols, false)) th *= 2;
-      check(launch_refine_lds(stream(), th, cfwd.view(), cbwd.view(), a.view(), rhs_row.p, sol_of.p, b.p, x.p, x1.p,
+      check(launch_refine_lds(stream(), th, (t_sweeps ? ctfwd : cfwd).view(), (t_sweeps ? ctbwd : cbwd).view(), m,
+                              (t_sweeps ? t_rhs_row : rhs_row).p, (t_sweeps ? t_sol_of : sol_of).p, b.p, x.p, x1.p,
                               r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     } else {
       int th = 64;
       while (th * 2 <= wide_threads(ncols, false)) th *= 2;
-      check(launch_refine(stream(), tile_for(ncols, false), th, fwd.view(), bwd.view(), a.view(), b.p, t32.p, d32.p,
-                          x.p, x1.p, r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
+      check(launch_refine(stream(), tile_for(ncols, false), th, (t_sweeps ? tfwd : fwd).view(),
+                          (t_sweeps ? tbwd : bwd).view(), m, b.p, t32.p, d32.p, x.p, x1.p, r.p, layout.ld,
+                          (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     }
     // bytes: per column, its own corrections k_c; the factor and the matrix once per sweep and
@@ -866,6 +872,25 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
     f->a.col.upload(cols, f->stream());
     f->a.val.upload(vals, f->stream());
     f->a.row.upload(rows, f->stream());
+    // A^T by rows: a counting sort of A's entries by column, rows ascending within each
+    {
+      std::vector<int32_t> tptr(n + 1, 0), tcol(nnz);
+      std::vector<double> tval(nnz);
+      for (uint64_t q = 0; q < nnz; ++q) tptr[cols[q] + 1] += 1;
+      for (int64_t j = 0; j < n; ++j) tptr[j + 1] += tptr[j];
+      std::vector<int32_t> cursor(tptr.begin(), tptr.end() - 1);
+      for (int64_t i = 0; i < n; ++i)
+        for (int32_t q = rptr[i]; q < rptr[i + 1]; ++q) {
+          const int32_t t = cursor[cols[q]]++;
+          tcol[t] = (int32_t)i;
+          tval[t] = vals[q];
+        }
+      f->at.rows = f->at.cols = (int32_t)n;
+      f->at.nnz = nnz;
+      f->at.ptr.upload(tptr, f->stream());
+      f->at.col.upload(tcol, f->stream());
+      f->at.val.upload(tval, f->stream());
+    }
     std::vector<int32_t> order32(order.begin(), order.end());
     f->a.order_dev.upload(order32, f->stream());
     f->a.order = std::move(order);
@@ -1002,8 +1027,8 @@ int32_t ybus_hip_factor_solve_many_refined(ybus_hip_factor* f, const double* rhs
   return solve_many_mode(f, rhs, count, x, YBUS_HIP_FP32_REFINED, tolerance, max_steps, steps, ratio);
 }
 
-int32_t ybus_hip_factor_adjoint(ybus_hip_factor* f, const double* x_bar, const double* x, uint64_t count,
-                                double* b_bar, double* a_bar) {
+int32_t ybus_hip_factor_adjoint(ybus_hip_factor* f, int32_t mode, double tolerance, uint64_t max_steps,
+                                const double* x_bar, const double* x, uint64_t count, double* b_bar, double* a_bar) {
   return guard([&] {
     if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
     require_unborrowed(f);
@@ -1011,18 +1036,24 @@ int32_t ybus_hip_factor_adjoint(ybus_hip_factor* f, const double* x_bar, const d
     const int64_t nnz = (int64_t)f->a.nnz;
     if (count && (!x_bar || !x || !b_bar)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     if (nnz && !a_bar) fail(YBUS_HIP_INVALID_ARGUMENT, "a_bar is null");
+    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_FP32_REFINED) fail(YBUS_HIP_INVALID_ARGUMENT, "mode: FP64 or FP32_REFINED");
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    const bool refined = mode == YBUS_HIP_FP32_REFINED;
     check(hipSetDevice(f->ctx->device), "hipSetDevice");
     std::vector<double> acc((size_t)nnz, 0.0);
     if (count && f->n) {
       const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
-      f->ensure_workspace(width, false);
+      f->ensure_workspace(width, refined);
       f->abar.reserve((size_t)std::max<int64_t>(nnz, 1));
       const uint64_t n = (uint64_t)f->n;
       for (uint64_t start = 0; start < count; start += (uint64_t)width) {
         const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
         // b_bar = A^-T x_bar (the transposed solve, lu.rs:502-533; A^T = A for a Cholesky factor)
         upload_vectors(f, x_bar + start * n, chunk);
-        f->solve64(true, false, chunk);
+        if (refined)
+          f->refine(chunk, tolerance, (int32_t)max_steps, true);
+        else
+          f->solve64(true, false, chunk);
         download_vectors(f, f->x.p, f->layout, chunk, b_bar + start * n);
         // the solutions x beside it, then the running sums on A's pattern
         upload_vectors(f, x + start * n, chunk, f->y2.p);
@@ -1054,29 +1085,38 @@ static void check_device_vectors(const ybus_hip_factor* f, uint64_t count, uint6
   if (count > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many vectors");
 }
 
-int32_t ybus_hip_factor_solve_device(ybus_hip_factor* f, int32_t mode, const double* b, uint64_t ld_b, uint64_t count,
-                                     double* x, uint64_t ld_x) {
+int32_t ybus_hip_factor_solve_device(ybus_hip_factor* f, int32_t mode, double tolerance, uint64_t max_steps,
+                                     const double* b, uint64_t ld_b, uint64_t count, double* x, uint64_t ld_x) {
   return guard([&] {
     if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
     require_unborrowed(f);
-    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_TRANSPOSE) fail(YBUS_HIP_INVALID_ARGUMENT, "mode: FP64 or TRANSPOSE");
+    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_TRANSPOSE && mode != YBUS_HIP_FP32_REFINED)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "unknown mode");
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    const bool refined = mode == YBUS_HIP_FP32_REFINED;
+    if (refined && !f->a.ptr.p)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix)");
     check_device_vectors(f, count, std::min(ld_b, ld_x), {b, x});
     if (count == 0 || f->n == 0) return;
     check(hipSetDevice(f->ctx->device), "hipSetDevice");
     const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
-    f->ensure_workspace(width, false);
+    f->ensure_workspace(width, refined);
     for (uint64_t start = 0; start < count; start += (uint64_t)width) {
       const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
       copy_in_device(f, b + start * ld_b, (int64_t)ld_b, chunk, f->b.p);
-      f->solve64(mode == YBUS_HIP_TRANSPOSE, false, chunk);
+      if (refined)
+        f->refine(chunk, tolerance, (int32_t)max_steps);
+      else
+        f->solve64(mode == YBUS_HIP_TRANSPOSE, false, chunk);
       copy_out_device(f, f->x.p, chunk, x + start * ld_x, (int64_t)ld_x);
     }
     check(hipStreamSynchronize(f->stream()), "solve");
   });
 }
 
-int32_t ybus_hip_factor_adjoint_device(ybus_hip_factor* f, const double* x_bar, const double* x, uint64_t ld,
-                                       uint64_t count, double* b_bar, double* a_bar) {
+int32_t ybus_hip_factor_adjoint_device(ybus_hip_factor* f, int32_t mode, double tolerance, uint64_t max_steps,
+                                       const double* x_bar, const double* x, uint64_t ld, uint64_t count,
+                                       double* b_bar, double* a_bar) {
   return guard([&] {
     if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
     require_unborrowed(f);
@@ -1084,15 +1124,21 @@ int32_t ybus_hip_factor_adjoint_device(ybus_hip_factor* f, const double* x_bar, 
     const int64_t nnz = (int64_t)f->a.nnz;
     check_device_vectors(f, count, ld, {x_bar, x, b_bar});
     if (nnz && !a_bar) fail(YBUS_HIP_INVALID_ARGUMENT, "a_bar is null");
+    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_FP32_REFINED) fail(YBUS_HIP_INVALID_ARGUMENT, "mode: FP64 or FP32_REFINED");
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    const bool refined = mode == YBUS_HIP_FP32_REFINED;
     check(hipSetDevice(f->ctx->device), "hipSetDevice");
     f->abar.reserve((size_t)std::max<int64_t>(nnz, 1));
     if (count && f->n) {
       const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
-      f->ensure_workspace(width, false);
+      f->ensure_workspace(width, refined);
       for (uint64_t start = 0; start < count; start += (uint64_t)width) {
         const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
         copy_in_device(f, x_bar + start * ld, (int64_t)ld, chunk, f->b.p);
-        f->solve64(true, false, chunk);
+        if (refined)
+          f->refine(chunk, tolerance, (int32_t)max_steps, true);
+        else
+          f->solve64(true, false, chunk);
         copy_out_device(f, f->x.p, chunk, b_bar + start * ld, (int64_t)ld);
         copy_in_device(f, x + start * ld, (int64_t)ld, chunk, f->y2.p);
         f->begin_kernel();
