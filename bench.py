@@ -284,10 +284,10 @@ def run_adjoint(gpu, args):
     a_values = torch.tensor(bprime.data, device=dev)
     captured = {}
 
-    def step():
+    def step(mode=0):
         a = a_values.detach().requires_grad_(True)
         b = t_p.detach().requires_grad_(True)
-        theta = solve(factor, a, b)
+        theta = solve(factor, a, b, mode, 1e-12, 10)
         theta.register_hook(lambda g: captured.__setitem__("x_bar", g))
         f = t_v0 * theta[:, t_c0] + t_v1 * theta[:, t_c1]
         over = torch.relu(f.abs() - t_lim)
@@ -309,6 +309,21 @@ def run_adjoint(gpu, args):
     x_bar = captured["x_bar"].cpu().numpy()
     theta_h = theta.detach().cpu().numpy()
     want_b, want_a = O.adjoint(chol, bprime, x_bar, theta_h)
+    # the same step with FP32 sweeps refined in FP64, forward and backward
+    from ybus_amd.autograd import FP32_REFINED
+
+    step(FP32_REFINED)
+    torch.cuda.synchronize()
+    factor.reset_kernel_stats()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, theta32, a_grad32, b_grad32 = step(FP32_REFINED)
+    torch.cuda.synchronize()
+    seconds32 = (time.perf_counter() - t0) / reps
+    stats32 = factor.kernel_stats()
+
+    def rel(got, want):
+        return float(torch.max(torch.abs(got - want)) / torch.max(torch.abs(want)))
     out = dict(workload="70,000-bus grid (n=%d), B' AMD Cholesky (nnz(L)=%d, nnz(B')=%d), 64 scenarios, "
                         "relu^2 flow-limit loss" % (chol.n, int(chol.l.nnz), bprime.nnz),
                loss=float(loss.detach()), ms_per_step=seconds * 1e3,
@@ -317,11 +332,16 @@ def run_adjoint(gpu, args):
                theta_bit_exact=bool(theta_h.tobytes() == chol.solve_many(p).tobytes()),
                b_bar_bit_exact=bool(b_grad.cpu().numpy().tobytes() == want_b.tobytes()),
                a_bar_bit_exact=bool(a_grad.cpu().numpy().tobytes() == want_a.tobytes()),
+               fp32_refined=dict(ms_per_step=seconds32 * 1e3, library_kernel_ms_per_step=stats32.milliseconds / reps,
+                                 theta_rel_err_vs_fp64=rel(theta32, theta), b_bar_rel_err_vs_fp64=rel(b_grad32, b_grad),
+                                 a_bar_rel_err_vs_fp64=rel(a_grad32, a_grad)),
                note="one step = forward solve, flows and loss, backward (the library's transposed solve and "
                     "SDDMM), all in device memory; library kernels = forward solve + transposed solve + SDDMM")
     log(f"[adjoint] {out['ms_per_step']:.2f} ms per device step (library kernels "
         f"{out['library_kernel_ms_per_step']:.3f} ms in {out['library_kernel_launches_per_step']} launches); "
-        f"bit-exact theta {out['theta_bit_exact']}, b_bar {out['b_bar_bit_exact']}, a_bar {out['a_bar_bit_exact']}")
+        f"bit-exact theta {out['theta_bit_exact']}, b_bar {out['b_bar_bit_exact']}, a_bar {out['a_bar_bit_exact']}; "
+        f"FP32-refined step {out['fp32_refined']['ms_per_step']:.2f} ms, gradient error vs FP64 "
+        f"{max(out['fp32_refined']['b_bar_rel_err_vs_fp64'], out['fp32_refined']['a_bar_rel_err_vs_fp64']):.2e}")
     return out
 
 

@@ -288,7 +288,7 @@ void ybus_hip_factor_reset_kernel_stats(ybus_hip_factor *f);
  * class of factors (factors grouped by how many of their solution vectors fit a compute unit's
  * LDS). Workgroups take (factor, column) items from a device queue, longest first, so small and
  * large systems share the GPU without a launch per factor. The launches are captured once per
- * staging in a HIP graph and replayed by each solve. FP64, the CPU's bits. A batch borrows its
+ * staging and mode in a HIP graph and replayed by each solve. A batch borrows its
  * factors: they must outlive it, and each must take the column path (dim() small enough for one
  * solution vector to fit in LDS; larger factors are rejected with YBUS_HIP_INVALID_ARGUMENT). */
 int32_t ybus_hip_batch_create(ybus_hip_ctx *ctx, ybus_hip_batch **out);
@@ -298,11 +298,18 @@ uint64_t ybus_hip_batch_factors(const ybus_hip_batch *b);
 /* Copies the right-hand sides to the device: counts[i] vectors of factor i's dim(), for each
  * factor in the order added, all one after another. */
 int32_t ybus_hip_batch_stage(ybus_hip_batch *b, const double *rhs, const uint64_t *counts);
-/* Solves what is staged; the solutions stay on the device until ybus_hip_batch_solutions. */
-int32_t ybus_hip_batch_solve_staged(ybus_hip_batch *b);
+/* Solves what is staged, in FP64 (YBUS_HIP_FP64: the CPU's bits) or as FP32 sweeps refined in
+ * FP64 (YBUS_HIP_FP32_REFINED: each column to `tolerance` by refine_normwise's rule, at most
+ * max_steps corrections; every factor needs ybus_hip_factor_set_matrix). The solutions stay on the
+ * device until ybus_hip_batch_solutions. */
+int32_t ybus_hip_batch_solve_staged(ybus_hip_batch *b, int32_t mode, double tolerance,
+                                    uint64_t max_steps);
 /* The staged right-hand sides' solutions, in the same arrangement as the right-hand sides. */
 int32_t ybus_hip_batch_solutions(ybus_hip_batch *b, double *x);
-/* stage + solve_staged + solutions. */
+/* Each staged column's corrections and final relative residual after an FP32-refined solve
+ * (either may be NULL), in the arrangement of the staged right-hand sides. */
+int32_t ybus_hip_batch_refinement(ybus_hip_batch *b, uint64_t *steps, double *ratio);
+/* stage + solve_staged (FP64) + solutions. */
 int32_t ybus_hip_batch_solve(ybus_hip_batch *b, const double *rhs, const uint64_t *counts,
                              double *x);
 /* HIP-event timings of the batch's solves (one launch = one replay of its graph). */

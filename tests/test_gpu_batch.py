@@ -93,3 +93,46 @@ def test_what_a_batch_cannot_take_is_refused(gpu):
     with pytest.raises(Y.InvalidArgument):
         batch.solve_staged()  # nothing staged
     assert [x.shape for x in batch.solve([np.zeros((0, 50))])] == [(0, 50)]
+
+
+def test_fp32_refined_batches_reach_the_fp64_floor_on_every_replay(gpu):
+    """The batch's FP32-refined mode refines each column in FP64 against its own factor's matrix:
+    the reported ratio is the CPU's relative residual of the returned solution, it reaches 1e-12
+    or the FP64 floor the CPU's own solve meets, and every replay of the graph gives the same bits,
+    also after FP64 solves of the same batch in between."""
+    mats = [random_spd(300, 3, 2)]
+    for buses in [2000, 12000]:
+        bp, _, _ = O.make_grid(buses, 555 + buses).dc()
+        mats.append(bp)
+    jac, _ = O.make_grid(1500, 77).jacobian(77)
+    batch = Y.GpuBatch.new(gpu)
+    cpu = []
+    for m in mats:
+        c = O.Cholesky(m)
+        f = Y.GpuFactor.from_cholesky(gpu, c)
+        f.set_matrix(m)
+        batch.add(f)
+        cpu.append((m, c))
+    j = O.Lu(jac, O.COLAMD)
+    fj = Y.GpuFactor.from_lu(gpu, j)
+    fj.set_matrix(jac)
+    batch.add(fj)
+    cpu.append((jac, j))
+    counts = [4, 7, 3, 9]
+    rhs = [right_hand_sides(m.rows, c, 50 + k) for k, ((m, _), c) in enumerate(zip(cpu, counts))]
+    batch.stage(rhs)
+    batch.solve_staged(Y.FP32_REFINED, 1e-12, 10)
+    first = [x.copy() for x in batch.solutions()]
+    for (m, c), b, x, (steps, ratio) in zip(cpu, rhs, first, batch.refinement()):
+        exact = c.solve_many(b)
+        for k in range(len(b)):
+            assert ratio[k] == m.relative_residual(x[k], b[k])
+            assert ratio[k] <= max(1e-12, 2.0 * m.relative_residual(exact[k], b[k])), (k, ratio[k])
+            assert 1 <= steps[k] <= 10
+    for _ in range(3):
+        batch.solve_staged(Y.FP64)
+        for (m, c), b, x in zip(cpu, rhs, batch.solutions()):
+            assert x.tobytes() == c.solve_many(b).tobytes()
+        batch.solve_staged(Y.FP32_REFINED, 1e-12, 10)
+        for x, want in zip(batch.solutions(), first):
+            assert x.tobytes() == want.tobytes()

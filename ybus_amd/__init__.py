@@ -147,7 +147,8 @@ def _declare(L):
         "ybus_hip_batch_add": (i32, [P, P]),
         "ybus_hip_batch_factors": (U64, [P]),
         "ybus_hip_batch_stage": (i32, [P, C.c_void_p, u64p]),
-        "ybus_hip_batch_solve_staged": (i32, [P]),
+        "ybus_hip_batch_solve_staged": (i32, [P, i32, C.c_double, U64]),
+        "ybus_hip_batch_refinement": (i32, [P, C.c_void_p, C.c_void_p]),
         "ybus_hip_batch_solutions": (i32, [P, C.c_void_p]),
         "ybus_hip_batch_solve": (i32, [P, C.c_void_p, u64p, C.c_void_p]),
         "ybus_hip_batch_kernel_stats": (None, [P, C.POINTER(_KernelStats)]),
@@ -718,8 +719,24 @@ class GpuBatch:
         _check(library().ybus_hip_batch_stage(self._h, flat.ctypes.data if flat.size else None, counts))
         self._staged = [(int(c), f.dim()) for c, f in zip(counts, self._factors)]
 
-    def solve_staged(self):
-        _check(library().ybus_hip_batch_solve_staged(self._h))
+    def solve_staged(self, mode: int = FP64, tolerance: float = 1e-12, max_steps: int = 10):
+        """FP64 (the CPU's bits) or FP32_REFINED (each column refined in FP64 to `tolerance`)."""
+        _check(library().ybus_hip_batch_solve_staged(self._h, int(mode), float(tolerance), int(max_steps)))
+
+    def refinement(self):
+        """After an FP32_REFINED solve: per factor, (steps, ratio) arrays of its columns."""
+        if self._staged is None:
+            raise InvalidArgument(INVALID_ARGUMENT, "nothing staged")
+        total = sum(c for c, _ in self._staged)
+        steps = np.zeros(total, np.uint64)
+        ratio = np.zeros(total)
+        _check(library().ybus_hip_batch_refinement(self._h, steps.ctypes.data if total else None,
+                                                   ratio.ctypes.data if total else None))
+        out, at = [], 0
+        for c, _ in self._staged:
+            out.append((steps[at: at + c].astype(np.int64), ratio[at: at + c]))
+            at += c
+        return out
 
     def solutions(self):
         if self._staged is None:

@@ -658,15 +658,15 @@ __device__ double column_residual(const CsrDev& a, const double* __restrict__ b,
 // column: x = solve(b); while the column is active, r = b - A x, d = solve(r), x' = x + d in
 // the other buffer. A column continues only while each step at least halves its ratio, so its
 // best iterate is the current or the previous one; it ends in x0.
-__global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev bwd, CsrDev a,
-                                                          const int32_t* __restrict__ rhs_row,
-                                                          const int32_t* __restrict__ sol_of,
-                                                          const double* __restrict__ bblock, double* __restrict__ x0b,
-                                                          double* __restrict__ x1b, double* __restrict__ rblock,
-                                                          int64_t ld, double tol, int32_t max_steps,
-                                                          int32_t* __restrict__ steps_out,
-                                                          double* __restrict__ ratio_out) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// One column (b, by unknown): the iterates in xa and xb, the residual in r (n values each); the
+// best iterate ends in xa, its corrections and relative residual in *steps_out and *ratio_out.
+// smem: blockDim.x doubles of scratch, the FP32 vector, both sweeps' level pointers. Every thread
+// must call; the caller's barrier must separate it from the next use of smem.
+__device__ void refine_column(const SweepDev& fwd, const SweepDev& bwd, const CsrDev& a,
+                              const int32_t* __restrict__ rhs_row, const int32_t* __restrict__ sol_of,
+                              const double* __restrict__ b, double* __restrict__ xa, double* __restrict__ xb,
+                              double* __restrict__ r, double tol, int32_t max_steps, int32_t* __restrict__ steps_out,
+                              double* __restrict__ ratio_out, unsigned char* smem) {
   double* scratch = reinterpret_cast<double*>(smem);
   const int32_t n = a.rows;
   float* xs = reinterpret_cast<float*>(smem + blockDim.x * sizeof(double));
@@ -674,11 +674,6 @@ __global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev
   int32_t* lpb = lpf + fwd.levels + 1;
   stage_level_ptr(fwd, lpf);
   stage_level_ptr(bwd, lpb);
-  const int64_t col = blockIdx.x;
-  const double* b = bblock + col * ld;
-  double* xa = x0b + col * ld;
-  double* xb = x1b + col * ld;
-  double* r = rblock + col * ld;
 
   double m = 0.0;
   for (int32_t u = threadIdx.x; u < n; u += blockDim.x) m = fold_max(m, fabs(b[u]));
@@ -719,9 +714,23 @@ __global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev
   if (best_in_b)
     for (int32_t u = threadIdx.x; u < n; u += blockDim.x) xa[u] = xb[u];
   if (threadIdx.x == 0) {
-    steps_out[col] = best_steps;
-    ratio_out[col] = best;
+    *steps_out = best_steps;
+    *ratio_out = best;
   }
+}
+
+__global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev bwd, CsrDev a,
+                                                          const int32_t* __restrict__ rhs_row,
+                                                          const int32_t* __restrict__ sol_of,
+                                                          const double* __restrict__ bblock, double* __restrict__ x0b,
+                                                          double* __restrict__ x1b, double* __restrict__ rblock,
+                                                          int64_t ld, double tol, int32_t max_steps,
+                                                          int32_t* __restrict__ steps_out,
+                                                          double* __restrict__ ratio_out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t col = blockIdx.x;
+  refine_column(fwd, bwd, a, rhs_row, sol_of, bblock + col * ld, x0b + col * ld, x1b + col * ld, rblock + col * ld, tol,
+                max_steps, steps_out + col, ratio_out + col, smem);
 }
 
 // ---- layout changes ----
@@ -961,6 +970,32 @@ __global__ void __launch_bounds__(1024) batch_kernel(const BatchEntryDev* __rest
   }
 }
 
+// The batch's FP32-refined solve: the same queue, each item refined like refine_lds_kernel's
+// columns (the factor's matrix for the residual, its own iterate and residual buffers, and its
+// corrections and ratio at its column's place among all the batch's columns)
+__global__ void __launch_bounds__(1024) batch_refine_kernel(const BatchEntryDev* __restrict__ entries,
+                                                            const int2* __restrict__ items, int32_t total,
+                                                            int32_t* __restrict__ queue, const double* __restrict__ rhs,
+                                                            double* __restrict__ x, double* __restrict__ x1,
+                                                            double* __restrict__ r, double tol, int32_t max_steps,
+                                                            int32_t* __restrict__ steps, double* __restrict__ ratio) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_item;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int32_t it = s_item;
+    if (it >= total) return;
+    const int2 w = items[it];
+    const BatchEntryDev e = entries[w.x];
+    const int64_t at = e.offset + (int64_t)w.y * e.n;
+    const int64_t column = e.first_column + w.y;
+    refine_column(e.fwd, e.bwd, e.a, e.rhs_row, e.sol_of, rhs + at, x + at, x1 + at, r + at, tol, max_steps,
+                  steps + column, ratio + column, smem);
+  }
+}
+
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, int32_t value, int32_t count) {
   const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i < count) p[i] = value;
@@ -1113,6 +1148,21 @@ hipError_t launch_batch(hipStream_t stream, int grid, int threads, size_t lds_by
                         const int2* items, int32_t total, int32_t* queue, const double* rhs, double* x) {
   if (grid <= 0 || threads < 64 || threads > 1024 || threads % 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(threads), lds_bytes, stream, entries, items, total, queue, rhs, x);
+  return hipGetLastError();
+}
+
+hipError_t prepare_batch_refine(size_t lds_bytes) {
+  return hipFuncSetAttribute((const void*)batch_refine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_bytes);
+}
+
+hipError_t launch_batch_refine(hipStream_t stream, int grid, int threads, size_t lds_bytes,
+                               const BatchEntryDev* entries, const int2* items, int32_t total, int32_t* queue,
+                               const double* rhs, double* x, double* x1, double* r, double tol, int32_t max_steps,
+                               int32_t* steps, double* ratio) {
+  if (grid <= 0 || threads < 64 || threads > 1024 || (threads & (threads - 1))) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_refine_kernel, dim3(grid), dim3(threads), lds_bytes, stream, entries, items, total, queue,
+                     rhs, x, x1, r, tol, max_steps, steps, ratio);
   return hipGetLastError();
 }
 

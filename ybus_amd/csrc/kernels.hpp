@@ -76,6 +76,8 @@ struct BatchEntryDev {
   const int32_t* sol_of = nullptr;
   int32_t n = 0;
   int64_t offset = 0;
+  CsrDev a;                  // the factor's matrix (FP32-refined batches)
+  int64_t first_column = 0;  // its first column's index among all the batch's columns
 };
 
 // LDS bytes the column solves need for n rows and the given total of both sweeps' levels (and
@@ -159,6 +161,13 @@ hipError_t launch_scatter_negated(hipStream_t stream, const double* acc, const i
 hipError_t prepare_batch(size_t lds_bytes);
 hipError_t launch_batch(hipStream_t stream, int grid, int threads, size_t lds_bytes, const BatchEntryDev* entries,
                         const int2* items, int32_t total, int32_t* queue, const double* rhs, double* x);
+// The FP32-refined batch: the same queue, each column refined in FP64 against its factor's matrix
+// (x1 and r: scratch laid out like x; steps and ratio: one per column of the batch)
+hipError_t prepare_batch_refine(size_t lds_bytes);
+hipError_t launch_batch_refine(hipStream_t stream, int grid, int threads, size_t lds_bytes,
+                               const BatchEntryDev* entries, const int2* items, int32_t total, int32_t* queue,
+                               const double* rhs, double* x, double* x1, double* r, double tol, int32_t max_steps,
+                               int32_t* steps, double* ratio);
 // p[0..count) = value, as a kernel (so it can sit in a captured graph)
 hipError_t launch_fill(hipStream_t stream, int32_t* p, int32_t value, int32_t count);
 

@@ -1591,50 +1591,128 @@ int32_t ybus_hip_operator_set_weights(ybus_hip_operator* op, const double* weigh
 struct ybus_hip_batch {
   ybus_hip_ctx* ctx = nullptr;
   std::vector<ybus_hip_factor*> factors;
-  // what is staged: counts per factor, where each factor's columns start, the device arrays
+  // what is staged: counts per factor, where each factor's values and columns start
   std::vector<uint64_t> counts;
   std::vector<int64_t> offsets;
-  int64_t values = 0;
+  int64_t values = 0, columns = 0;
   bool staged = false;
-  DevBuf<double> rhs, x;
+  DevBuf<double> rhs, x, x1, r;
+  DevBuf<int32_t> steps;
+  DevBuf<double> ratio;
   DevBuf<BatchEntryDev> entries;
-  DevBuf<int2> items;
   DevBuf<int32_t> queues;
-  // one persistent launch per class of factors that fit `per_cu` workgroups to a compute unit
+  // one persistent launch per class of factors that fit `per_cu` workgroups to a compute unit;
+  // a plan per mode (FP64, FP32-refined: their LDS needs differ)
   struct Class {
     int per_cu = 1, threads = 1024;
     size_t lds = 0;
     int32_t first = 0, count = 0;
   };
-  std::vector<Class> classes;
+  struct Plan {
+    bool ready = false;
+    std::vector<Class> classes;
+    DevBuf<int2> items;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double tol = 0.0;
+    int32_t max_steps = -1;
+    void drop_graph() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    }
+  };
+  Plan plans[2];
   Staging staging;
-  // the launches, captured once per staging and replayed
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ybus_hip_kernel_stats stats{};
 
   hipStream_t stream() const { return ctx->stream; }
-  void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
+  void drop_plans() {
+    for (Plan& p : plans) {
+      p.drop_graph();
+      p.ready = false;
+    }
   }
   ~ybus_hip_batch() {
-    drop_graph();
+    drop_plans();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
   }
+
+  // Items in classes for a mode: each factor's columns go to the class of the most workgroups
+  // per compute unit (up to 4, 1024 / k threads each) whose LDS they fit; longest first within
+  // a class (a column's time is set by its levels, then by its rows)
+  void make_plan(bool refined) {
+    Plan& plan = plans[refined ? 1 : 0];
+    struct Item {
+      int per_cu;
+      int64_t cost;
+      int2 w;
+      size_t lds;
+    };
+    std::vector<Item> all;
+    all.reserve((size_t)columns);
+    for (size_t i = 0; i < factors.size(); ++i) {
+      ybus_hip_factor* f = factors[i];
+      if (!counts[i]) continue;
+      const int32_t n = (int32_t)std::max<int64_t>(f->n, 1);
+      const int32_t levels = f->cfwd.host.levels + f->cbwd.host.levels;
+      int per_cu = 0;
+      size_t lds = 0;
+      for (int k = 4; k >= 1 && !per_cu; --k) {
+        const int threads = k == 1 ? 1024 : k == 2 ? 512 : 256;
+        const size_t bytes = refined ? refine_lds_bytes(n, threads, levels) : solve_lds_bytes(n, false, levels);
+        if ((size_t)k * bytes <= f->lds_cap) {
+          per_cu = k;
+          lds = bytes;
+        }
+      }
+      if (!per_cu)
+        fail(YBUS_HIP_INVALID_ARGUMENT,
+             "factor " + std::to_string(i) + " (dim " + std::to_string(f->n) + ") does not fit in LDS for this mode");
+      const int64_t cost = (int64_t)levels * 4096 + f->n;
+      for (uint64_t c = 0; c < counts[i]; ++c) all.push_back(Item{per_cu, cost, make_int2((int)i, (int)c), lds});
+    }
+    std::stable_sort(all.begin(), all.end(), [](const Item& p, const Item& q) {
+      return p.per_cu != q.per_cu ? p.per_cu < q.per_cu : p.cost > q.cost;
+    });
+    std::vector<int2> items(std::max<size_t>(all.size(), 1));
+    plan.classes.clear();
+    for (size_t k = 0; k < all.size(); ++k) {
+      items[k] = all[k].w;
+      if (plan.classes.empty() || plan.classes.back().per_cu != all[k].per_cu) {
+        Class c;
+        c.per_cu = all[k].per_cu;
+        c.threads = c.per_cu == 1 ? 1024 : c.per_cu == 2 ? 512 : 256;
+        c.first = (int32_t)k;
+        plan.classes.push_back(c);
+      }
+      Class& c = plan.classes.back();
+      c.count += 1;
+      c.lds = std::max(c.lds, all[k].lds);
+    }
+    plan.items.upload(items, stream());
+    check(hipStreamSynchronize(stream()), "batch plan");
+    plan.ready = true;
+  }
+
   // the queue counters to 0, then each class's launch; on the batch's stream
-  void enqueue() {
-    check(launch_fill(stream(), queues.p, 0, (int32_t)classes.size()), "batch queue reset");
-    for (size_t c = 0; c < classes.size(); ++c) {
-      const Class& k = classes[c];
+  void enqueue(bool refined, double tol, int32_t max_steps) {
+    const Plan& plan = plans[refined ? 1 : 0];
+    check(launch_fill(stream(), queues.p, 0, (int32_t)plan.classes.size()), "batch queue reset");
+    for (size_t c = 0; c < plan.classes.size(); ++c) {
+      const Class& k = plan.classes[c];
       const int grid = std::max(1, std::min<int>(k.count, k.per_cu * ctx->cus));
-      check(launch_batch(stream(), grid, k.threads, k.lds, entries.p, items.p + k.first, k.count, queues.p + c, rhs.p,
-                         x.p),
-            "batch kernel");
+      if (refined)
+        check(launch_batch_refine(stream(), grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count,
+                                  queues.p + c, rhs.p, x.p, x1.p, r.p, tol, max_steps, steps.p, ratio.p),
+              "batch kernel");
+      else
+        check(launch_batch(stream(), grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count, queues.p + c,
+                           rhs.p, x.p),
+              "batch kernel");
     }
   }
 };
@@ -1667,7 +1745,7 @@ int32_t ybus_hip_batch_add(ybus_hip_batch* b, ybus_hip_factor* f) {
                                           " does not)");
     b->factors.push_back(f);
     b->staged = false;
-    b->drop_graph();
+    b->drop_plans();
   });
 }
 
@@ -1678,67 +1756,37 @@ int32_t ybus_hip_batch_stage(ybus_hip_batch* b, const double* rhs, const uint64_
     if (nf && !counts) fail(YBUS_HIP_INVALID_ARGUMENT, "one count per factor");
     check(hipSetDevice(b->ctx->device), "hipSetDevice");
     b->staged = false;
-    b->drop_graph();
+    b->drop_plans();
     b->counts.assign(counts, counts + nf);
     b->offsets.assign(nf, 0);
     int64_t values = 0, columns = 0;
-    for (size_t i = 0; i < nf; ++i) {
-      if (b->counts[i] > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many right-hand sides");
-      b->offsets[i] = values;
-      values += (int64_t)b->counts[i] * b->factors[i]->n;
-      columns += (int64_t)b->counts[i];
-    }
-    if (columns > INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many right-hand sides");
-    if (values && !rhs) fail(YBUS_HIP_INVALID_ARGUMENT, "rhs is null");
-    b->values = values;
-    // the factors' entries, and their items in classes by how many fit a compute unit's LDS
     std::vector<BatchEntryDev> entries(std::max<size_t>(nf, 1));
-    struct Item {
-      int per_cu;
-      int64_t cost;
-      int2 w;
-      size_t lds;
-    };
-    std::vector<Item> all;
-    all.reserve((size_t)columns);
     for (size_t i = 0; i < nf; ++i) {
       ybus_hip_factor* f = b->factors[i];
+      if (b->counts[i] > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many right-hand sides");
       BatchEntryDev& e = entries[i];
       e.fwd = f->cfwd.view();
       e.bwd = f->cbwd.view();
       e.rhs_row = f->rhs_row.p;
       e.sol_of = f->sol_of.p;
       e.n = (int32_t)f->n;
-      e.offset = b->offsets[i];
-      const size_t lds = solve_lds_bytes((int32_t)std::max<int64_t>(f->n, 1), false, e.fwd.levels + e.bwd.levels);
-      const int per_cu = (int)std::clamp<size_t>(f->lds_cap / std::max<size_t>(lds, 1), 1, 4);
-      // a column's time is set by its levels, then by its rows
-      const int64_t cost = (int64_t)(e.fwd.levels + e.bwd.levels) * 4096 + f->n;
-      for (uint64_t c = 0; c < b->counts[i]; ++c) all.push_back(Item{per_cu, cost, make_int2((int)i, (int)c), lds});
+      e.offset = values;
+      e.a = f->a.view();
+      e.first_column = columns;
+      b->offsets[i] = values;
+      values += (int64_t)b->counts[i] * f->n;
+      columns += (int64_t)b->counts[i];
     }
-    std::stable_sort(all.begin(), all.end(), [](const Item& p, const Item& q) {
-      return p.per_cu != q.per_cu ? p.per_cu < q.per_cu : p.cost > q.cost;
-    });
-    std::vector<int2> items(std::max<size_t>(all.size(), 1));
-    b->classes.clear();
-    for (size_t k = 0; k < all.size(); ++k) {
-      items[k] = all[k].w;
-      if (b->classes.empty() || b->classes.back().per_cu != all[k].per_cu) {
-        ybus_hip_batch::Class c;
-        c.per_cu = all[k].per_cu;
-        c.threads = c.per_cu == 1 ? 1024 : c.per_cu == 2 ? 512 : 256;
-        c.first = (int32_t)k;
-        b->classes.push_back(c);
-      }
-      ybus_hip_batch::Class& c = b->classes.back();
-      c.count += 1;
-      c.lds = std::max(c.lds, all[k].lds);
-    }
+    if (columns > INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many right-hand sides");
+    if (values && !rhs) fail(YBUS_HIP_INVALID_ARGUMENT, "rhs is null");
+    b->values = values;
+    b->columns = columns;
     b->entries.upload(entries, b->stream());
-    b->items.upload(items, b->stream());
-    b->queues.reserve(std::max<size_t>(b->classes.size(), 1));
-    b->rhs.reserve((size_t)std::max<int64_t>(values, 1));
-    b->x.reserve((size_t)std::max<int64_t>(values, 1));
+    b->queues.reserve(8);
+    const size_t vals = (size_t)std::max<int64_t>(values, 1), cols = (size_t)std::max<int64_t>(columns, 1);
+    for (DevBuf<double>* d : {&b->rhs, &b->x, &b->x1, &b->r}) d->reserve(vals);
+    b->steps.reserve(cols);
+    b->ratio.reserve(cols);
     if (values) {
       double* host = b->staging.get((size_t)values);
       std::memcpy(host, rhs, (size_t)values * sizeof(double));
@@ -1750,28 +1798,41 @@ int32_t ybus_hip_batch_stage(ybus_hip_batch* b, const double* rhs, const uint64_
   });
 }
 
-int32_t ybus_hip_batch_solve_staged(ybus_hip_batch* b) {
+int32_t ybus_hip_batch_solve_staged(ybus_hip_batch* b, int32_t mode, double tolerance, uint64_t max_steps) {
   return guard([&] {
     if (!b) fail(YBUS_HIP_INVALID_ARGUMENT, "batch is null");
     if (!b->staged) fail(YBUS_HIP_INVALID_ARGUMENT, "nothing staged (ybus_hip_batch_stage)");
-    if (b->classes.empty()) return;
+    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_FP32_REFINED) fail(YBUS_HIP_INVALID_ARGUMENT, "mode: FP64 or FP32_REFINED");
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    const bool refined = mode == YBUS_HIP_FP32_REFINED;
+    if (refined)
+      for (size_t i = 0; i < b->factors.size(); ++i)
+        if (b->counts[i] && !b->factors[i]->a.ptr.p)
+          fail(YBUS_HIP_INVALID_ARGUMENT, "FP32 refinement needs each factor's matrix (ybus_hip_factor_set_matrix)");
+    if (!b->columns) return;
     check(hipSetDevice(b->ctx->device), "hipSetDevice");
-    if (!b->exec) {
-      for (const ybus_hip_batch::Class& c : b->classes) check(prepare_batch(c.lds), "batch kernel attribute");
+    ybus_hip_batch::Plan& plan = b->plans[refined ? 1 : 0];
+    if (!plan.ready) b->make_plan(refined);
+    if (plan.exec && refined && (plan.tol != tolerance || plan.max_steps != (int32_t)max_steps)) plan.drop_graph();
+    if (!plan.exec) {
+      for (const ybus_hip_batch::Class& c : plan.classes)
+        check(refined ? prepare_batch_refine(c.lds) : prepare_batch(c.lds), "batch kernel attribute");
       check(hipStreamBeginCapture(b->stream(), hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       try {
-        b->enqueue();
+        b->enqueue(refined, tolerance, (int32_t)max_steps);
       } catch (...) {
         hipGraph_t dead = nullptr;
         (void)hipStreamEndCapture(b->stream(), &dead);
         if (dead) (void)hipGraphDestroy(dead);
         throw;
       }
-      check(hipStreamEndCapture(b->stream(), &b->graph), "hipStreamEndCapture");
-      check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      check(hipStreamEndCapture(b->stream(), &plan.graph), "hipStreamEndCapture");
+      check(hipGraphInstantiate(&plan.exec, plan.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      plan.tol = tolerance;
+      plan.max_steps = (int32_t)max_steps;
     }
     check(hipEventRecord(b->ev0, b->stream()), "hipEventRecord");
-    check(hipGraphLaunch(b->exec, b->stream()), "hipGraphLaunch");
+    check(hipGraphLaunch(plan.exec, b->stream()), "hipGraphLaunch");
     check(hipEventRecord(b->ev1, b->stream()), "hipEventRecord");
     check(hipEventSynchronize(b->ev1), "batch solve");
     float ms = 0.0f;
@@ -1800,10 +1861,26 @@ int32_t ybus_hip_batch_solutions(ybus_hip_batch* b, double* x) {
   });
 }
 
+int32_t ybus_hip_batch_refinement(ybus_hip_batch* b, uint64_t* steps, double* ratio) {
+  return guard([&] {
+    if (!b) fail(YBUS_HIP_INVALID_ARGUMENT, "batch is null");
+    if (!b->staged) fail(YBUS_HIP_INVALID_ARGUMENT, "nothing staged (ybus_hip_batch_stage)");
+    if (!b->columns) return;
+    std::vector<int32_t> st((size_t)b->columns);
+    std::vector<double> ra((size_t)b->columns);
+    check(hipMemcpy(st.data(), b->steps.p, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost), "steps");
+    check(hipMemcpy(ra.data(), b->ratio.p, ra.size() * sizeof(double), hipMemcpyDeviceToHost), "ratio");
+    for (int64_t c = 0; c < b->columns; ++c) {
+      if (steps) steps[c] = (uint64_t)st[c];
+      if (ratio) ratio[c] = ra[c];
+    }
+  });
+}
+
 int32_t ybus_hip_batch_solve(ybus_hip_batch* b, const double* rhs, const uint64_t* counts, double* x) {
   int32_t status = ybus_hip_batch_stage(b, rhs, counts);
   if (status != YBUS_HIP_OK) return status;
-  status = ybus_hip_batch_solve_staged(b);
+  status = ybus_hip_batch_solve_staged(b, YBUS_HIP_FP64, 0.0, 0);
   if (status != YBUS_HIP_OK) return status;
   return ybus_hip_batch_solutions(b, x);
 }
