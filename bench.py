@@ -323,7 +323,7 @@ def run_adjoint(gpu, args):
     stats32 = factor.kernel_stats()
 
     def rel(got, want):
-        return float(torch.max(torch.abs(got - want)) / torch.max(torch.abs(want)))
+        return float((torch.max(torch.abs(got - want)) / torch.max(torch.abs(want))).detach())
     out = dict(workload="70,000-bus grid (n=%d), B' AMD Cholesky (nnz(L)=%d, nnz(B')=%d), 64 scenarios, "
                         "relu^2 flow-limit loss" % (chol.n, int(chol.l.nnz), bprime.nnz),
                loss=float(loss.detach()), ms_per_step=seconds * 1e3,
@@ -359,7 +359,7 @@ def run_batch(gpu, args):
     counts = (1 + O.integers(seed, 2, grids) % 64).astype(int)
     picks = O.uniforms(seed, 3, grids)
     t0 = time.perf_counter()
-    cpu, rhs = [], []
+    cpu, rhs, mats = [], [], []
     for i in range(grids):
         g = O.make_grid(int(sizes[i]), seed * 1000 + i)
         candidates = np.flatnonzero(~g.bridges())
@@ -367,10 +367,13 @@ def run_batch(gpu, args):
         mask[candidates[int(picks[i] * len(candidates))]] = False
         bprime, _, _ = g.dc(mask)
         cpu.append(O.Cholesky(bprime))
+        mats.append(bprime)
         rhs.append(O.injections(seed * 1000 + i, 4, int(sizes[i]), int(counts[i])))
     log(f"[batch] {grids} grids, sum n = {int(sum(c.n for c in cpu))}, sum r = {int(counts.sum())}; host "
         f"{time.perf_counter() - t0:.1f} s")
     factors = [Y.GpuFactor.from_cholesky(gpu, c) for c in cpu]
+    for f, m in zip(factors, mats):
+        f.set_matrix(m)  # for the FP32-refined batch's residuals
     batch = Y.GpuBatch.new(gpu)
     for f in factors:
         batch.add(f)
@@ -385,6 +388,19 @@ def run_batch(gpu, args):
     batch_s = (time.perf_counter() - t0) / reps
     stats = batch.kernel_stats()
     solved = batch.solutions()
+    # the same batch as FP32 sweeps refined in FP64 (to 1e-12 or the FP64 floor), one graph replay
+    batch.solve_staged(Y.FP32_REFINED, 1e-12, 10)
+    batch.reset_kernel_stats()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.solve_staged(Y.FP32_REFINED, 1e-12, 10)
+    refined_s = (time.perf_counter() - t0) / reps
+    refined_stats = batch.kernel_stats()
+    refined = batch.solutions()
+    steps_ratio = batch.refinement()
+    max_ratio = max(float(r.max()) for _, r in steps_ratio if len(r))
+    mean_steps = float(np.mean(np.concatenate([s for s, _ in steps_ratio])))
+    refined_err = max(float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for x, y in zip(refined, solved) if y.size)
     # one launch per factor, the same blocks
     for f, b in zip(factors, rhs):
         f.stage(b)
@@ -407,9 +423,13 @@ def run_batch(gpu, args):
                batch_kernel_ms=stats.milliseconds / stats.launches, one_launch_per_factor_ms=single_s * 1e3,
                one_launch_per_factor_kernel_ms=single_kernel_ms,
                speedup_over_one_launch_per_factor=single_s / batch_s,
+               fp32_refined=dict(batch_ms=refined_s * 1e3, batch_kernel_ms=refined_stats.milliseconds / reps,
+                                 max_relative_residual=max_ratio, mean_corrections=mean_steps,
+                                 max_rel_diff_vs_fp64=refined_err),
                bit_exact_sampled_factors=f"{len(sample)} of {len(sample)}" if exact else "MISMATCH")
     log(f"[batch] batch {out['batch_ms']:.3f} ms (kernel {out['batch_kernel_ms']:.3f}), one launch per factor "
-        f"{out['one_launch_per_factor_ms']:.3f} ms; bit-exact {out['bit_exact_sampled_factors']}")
+        f"{out['one_launch_per_factor_ms']:.3f} ms; bit-exact {out['bit_exact_sampled_factors']}; FP32-refined "
+        f"{out['fp32_refined']['batch_ms']:.3f} ms, residual <= {max_ratio:.2e}, {mean_steps:.2f} corrections")
     return out
 
 
