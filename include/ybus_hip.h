@@ -120,6 +120,24 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx *ctx, uint64_t n, const uint64_t 
                                   const double *u_val, const uint64_t *row_perm,
                                   const uint64_t *col_perm, ybus_hip_factor **out);
 
+/* Complex factors (new: the reference's GPU path uploads real factors only, factor.rs:103-133):
+ * the arrays of ybus_hip_factor_upload_cholesky / _lu, with each value an interleaved (re, im)
+ * pair of doubles (num_complex::Complex64's layout), so 2 * entries doubles per value array.
+ * Cholesky<Complex64> is P A P^T = L L^H (cholesky.rs:206-311); the upload builds L^H as the
+ * reference stores it, L's columns conjugated below the diagonal and each diagonal kept
+ * (cholesky.rs:283-297). A complex factor takes the complex solves below and the accessors;
+ * every other factor entry point returns YBUS_HIP_INVALID_ARGUMENT for it. */
+int32_t ybus_hip_factor_upload_cholesky_complex(ybus_hip_ctx *ctx, uint64_t n, const uint64_t *l_ptr,
+                                                const uint64_t *l_idx, const double *l_val,
+                                                const uint64_t *perm, ybus_hip_factor **out);
+int32_t ybus_hip_factor_upload_lu_complex(ybus_hip_ctx *ctx, uint64_t n, const uint64_t *l_ptr,
+                                          const uint64_t *l_idx, const double *l_val,
+                                          const uint64_t *u_ptr, const uint64_t *u_idx,
+                                          const double *u_val, const uint64_t *row_perm,
+                                          const uint64_t *col_perm, ybus_hip_factor **out);
+/* 1 for a complex factor, else 0. */
+int32_t ybus_hip_factor_is_complex(const ybus_hip_factor *f);
+
 void ybus_hip_factor_free(ybus_hip_factor *f);
 
 /* GpuFactor::dim (factor.rs:181-183). */
@@ -150,6 +168,20 @@ int32_t ybus_hip_factor_solve_many(ybus_hip_factor *f, const double *rhs, uint64
  * A^T = A), FP64, in blocks of block_width() columns. */
 int32_t ybus_hip_factor_solve_transpose_many(ybus_hip_factor *f, const double *rhs,
                                              uint64_t count, double *x);
+
+/* Complex FP64 solves of `count` right-hand sides of dim() interleaved (re, im) values each
+ * (Factorization::solve_many, factorization.rs:29-92, over Complex64): forward_by_rows and
+ * back_by_rows for a Cholesky factor (cholesky.rs:468-501), substitute for an LU factor
+ * (lu.rs:770-785), in blocks of block_width() columns. Products and quotients are rounded as
+ * num-complex 0.4 computes them. The kernel statistics count, per block of c columns,
+ * S * 20 + 8 (n + 1) + 32 n c bytes. */
+int32_t ybus_hip_factor_solve_many_complex(ybus_hip_factor *f, const double *rhs, uint64_t count,
+                                           double *x);
+/* A^T x = b over a complex factor: the plain transpose for an LU factor
+ * (Lu::solve_transpose, lu.rs:502-533); for a Cholesky factor A^T = conj(A), solved as
+ * conj(A^-1 conj(b)) (no counterpart in the reference). */
+int32_t ybus_hip_factor_solve_transpose_many_complex(ybus_hip_factor *f, const double *rhs,
+                                                     uint64_t count, double *x);
 
 /* FP32 solves refined in FP64 (new): each right-hand side follows refine_normwise
  * (accuracy.rs:83-122): solve, then while ||b - A x||inf / ||b||inf > tolerance and at most

@@ -226,3 +226,43 @@ def reference_lu_perm(a: O.Matrix):
                 else:
                     row[j] = updated
     return perm
+
+
+# ---- complex cases (accuracy_test.rs:105-115 complexifies a real matrix the same way) ----
+
+def hermitian_case(buses: int, seed: int):
+    """A Hermitian positive definite matrix on a synthetic grid's B' pattern and its complex
+    Cholesky factor: (ComplexCholesky, A as a scipy CSR matrix)."""
+    import scipy.sparse as sp
+
+    from oracle import complex_solve as Z
+
+    grid = O.make_grid(buses, seed)
+    bprime, _, _ = grid.dc()
+    re, im = Z.hermitian_from(bprime, seed)
+    chol = Z.ComplexCholesky(bprime, re, im, O.Cholesky(bprime))
+    a = sp.csr_matrix((re + 1j * im, np.asarray(bprime.idx), np.asarray(bprime.ptr)), shape=bprime.shape)
+    return chol, a
+
+
+def complex_lu_case(n: int, fill_percent: int, seed: int):
+    """random_square_cuda's matrix with each stored v made v + (0.4 v - 0.1) i, and its complex LU
+    over the real matrix's pivot sequence: (ComplexLu, A dense)."""
+    from oracle import complex_solve as Z
+
+    real = random_square_cuda(n, fill_percent, seed)
+    lu = O.Lu(real)
+    dense = real.to_dense()
+    a = dense + 1j * np.where(dense != 0, 0.4 * dense - 0.1, 0.0)
+    return Z.ComplexLu(a, lu), a
+
+
+def complex_right_hand_sides(n: int, count: int, seed: int) -> np.ndarray:
+    """(count, n) complex: right_hand_sides' values as real parts, a second stream's as imaginary."""
+    return right_hand_sides(n, count, seed) + 1j * right_hand_sides(n, count, seed + 1000003)
+
+
+def complex_relative_residual(a, x: np.ndarray, b: np.ndarray) -> float:
+    """The worst column's ||b - A x||inf / ||b||inf (accuracy.rs:57-78), A dense or scipy sparse."""
+    r = b.T - a @ x.T
+    return float(np.max(np.max(np.abs(r), axis=0) / np.max(np.abs(b.T), axis=0)))

@@ -106,6 +106,12 @@ def _declare(L):
         "ybus_hip_ctx_memory": (i32, [P, C.POINTER(U64), C.POINTER(U64)]),
         "ybus_hip_factor_upload_cholesky": (i32, [P, U64, u64p, u64p, f64p, u64p, C.POINTER(P)]),
         "ybus_hip_factor_upload_lu": (i32, [P, U64, u64p, u64p, f64p, u64p, u64p, f64p, u64p, u64p, C.POINTER(P)]),
+        "ybus_hip_factor_upload_cholesky_complex": (i32, [P, U64, u64p, u64p, f64p, u64p, C.POINTER(P)]),
+        "ybus_hip_factor_upload_lu_complex": (i32, [P, U64, u64p, u64p, f64p, u64p, u64p, f64p, u64p, u64p,
+                                                    C.POINTER(P)]),
+        "ybus_hip_factor_is_complex": (i32, [P]),
+        "ybus_hip_factor_solve_many_complex": (i32, [P, f64p, U64, f64p]),
+        "ybus_hip_factor_solve_transpose_many_complex": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_free": (None, [P]),
         "ybus_hip_factor_dim": (U64, [P]),
         "ybus_hip_factor_block_width": (U64, [P]),
@@ -302,6 +308,23 @@ def _rhs_array(rhs, n: int) -> np.ndarray:
     return _f64(np.array(rows, dtype=np.float64).reshape(len(rows), n))
 
 
+def _complex_rhs(rhs, n: int) -> np.ndarray:
+    """Complex right-hand sides as a (count, n) complex128 array."""
+    b = np.ascontiguousarray(rhs if isinstance(rhs, np.ndarray) else np.array(list(rhs)), dtype=np.complex128)
+    if b.ndim == 1 and b.size == 0:
+        b = b.reshape(0, n)
+    if b.ndim != 2:
+        raise ValueError("complex right-hand sides are a (count, n) array")
+    if b.shape[1] != n and b.shape[0] > 0:
+        raise ShapeMismatch(n, b.shape[1])
+    return b
+
+
+def _interleaved(values) -> np.ndarray:
+    """Complex values as interleaved (re, im) doubles, Complex64's layout."""
+    return np.ascontiguousarray(values, dtype=np.complex128).view(np.float64)
+
+
 def _pairs(pairs: Sequence[Pair]):
     plus = np.array([-1 if p.plus is None else int(p.plus) for p in pairs], dtype=np.int64)
     minus = np.array([-1 if p.minus is None else int(p.minus) for p in pairs], dtype=np.int64)
@@ -314,10 +337,11 @@ def _pairs(pairs: Sequence[Pair]):
 class GpuFactor:
     """A factorization uploaded once, for solving many right-hand sides against it."""
 
-    def __init__(self, gpu: Gpu, handle, n: int):
+    def __init__(self, gpu: Gpu, handle, n: int, is_complex: bool = False):
         self._gpu = gpu  # keeps the context alive
         self._h = handle
         self._n = n
+        self._complex = is_complex
         self._nnz_a = -1
         self._block: Optional[GpuBlock] = None
 
@@ -349,6 +373,40 @@ class GpuFactor:
                                                    _u64(u_ptr), _u64(u_idx), _f64(u_val), _u64(row_perm),
                                                    _u64(col_perm), C.byref(h)))
         return GpuFactor(gpu, h, int(n))
+
+    @staticmethod
+    def from_cholesky_complex(gpu: Gpu, cholesky) -> "GpuFactor":
+        """Uploads a complex P A P^T = L L^H (Cholesky<Complex64>, cholesky.rs:206-311):
+        ``l`` in CSR with complex ``data``, and ``perm``."""
+        l = cholesky.l
+        return GpuFactor.from_cholesky_complex_arrays(gpu, len(cholesky.perm), l.ptr, l.idx, l.data, cholesky.perm)
+
+    @staticmethod
+    def from_cholesky_complex_arrays(gpu: Gpu, n, l_ptr, l_idx, l_val, perm) -> "GpuFactor":
+        h = P()
+        _check(library().ybus_hip_factor_upload_cholesky_complex(gpu._h, int(n), _u64(l_ptr), _u64(l_idx),
+                                                                 _interleaved(l_val), _u64(perm), C.byref(h)))
+        return GpuFactor(gpu, h, int(n), True)
+
+    @staticmethod
+    def from_lu_complex(gpu: Gpu, lu) -> "GpuFactor":
+        """Uploads a complex P A Q = L U (Lu<Complex64>): ``l``, ``u`` (CSC, complex ``data``),
+        ``row_perm`` and ``col_perm``."""
+        return GpuFactor.from_lu_complex_arrays(gpu, len(lu.row_perm), lu.l.ptr, lu.l.idx, lu.l.data, lu.u.ptr,
+                                                lu.u.idx, lu.u.data, lu.row_perm, lu.col_perm)
+
+    @staticmethod
+    def from_lu_complex_arrays(gpu: Gpu, n, l_ptr, l_idx, l_val, u_ptr, u_idx, u_val, row_perm,
+                               col_perm) -> "GpuFactor":
+        h = P()
+        _check(library().ybus_hip_factor_upload_lu_complex(gpu._h, int(n), _u64(l_ptr), _u64(l_idx),
+                                                           _interleaved(l_val), _u64(u_ptr), _u64(u_idx),
+                                                           _interleaved(u_val), _u64(row_perm), _u64(col_perm),
+                                                           C.byref(h)))
+        return GpuFactor(gpu, h, int(n), True)
+
+    def is_complex(self) -> bool:
+        return bool(library().ybus_hip_factor_is_complex(self._h))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -415,8 +473,20 @@ class GpuFactor:
         if self._block is not None:
             self._block.free()
 
+    def _solve_complex(self, rhs, transpose: bool) -> np.ndarray:
+        b = _complex_rhs(rhs, self._n)
+        x = np.zeros_like(b)
+        if b.shape[0]:
+            fn = (library().ybus_hip_factor_solve_transpose_many_complex if transpose
+                  else library().ybus_hip_factor_solve_many_complex)
+            _check(fn(self._h, b.view(np.float64), b.shape[0], x.view(np.float64)))
+        return x
+
     def solve_many(self, rhs) -> np.ndarray:
-        """A x = b for each right-hand side, FP64, as a (count, n) array (factor.rs:234-289)."""
+        """A x = b for each right-hand side, FP64, as a (count, n) array (factor.rs:234-289);
+        complex128 arrays for a complex factor."""
+        if self._complex:
+            return self._solve_complex(rhs, False)
         b = _rhs_array(rhs, self._n)
         self._release_block()
         x = np.zeros_like(b)
@@ -425,7 +495,10 @@ class GpuFactor:
         return x
 
     def solve_transpose_many(self, rhs) -> np.ndarray:
-        """A^T x = b for each right-hand side, FP64 (Lu::solve_transpose, lu.rs:502-533)."""
+        """A^T x = b for each right-hand side, FP64 (Lu::solve_transpose, lu.rs:502-533); for a
+        complex factor the plain transpose, complex128 arrays."""
+        if self._complex:
+            return self._solve_complex(rhs, True)
         b = _rhs_array(rhs, self._n)
         self._release_block()
         x = np.zeros_like(b)

@@ -43,7 +43,10 @@ Sweep schedule(const RowForm& form, const std::vector<int64_t>& src_map, const s
   s.row_ptr.assign(n + 1, 0);
   s.dep.reserve(form.dep.size());
   s.val.reserve(form.val.size());
+  const bool complex = !form.val_im.empty() || !form.diag_im.empty();
+  if (complex) s.val_im.reserve(form.val.size());
   if (!form.unit) s.diag.resize(n);
+  if (!form.unit && complex) s.diag_im.resize(n);
   for (int64_t k = 0; k < n; ++k) {
     const int64_t i = order[k];
     s.src[k] = (int32_t)src_map[i];
@@ -51,9 +54,11 @@ Sweep schedule(const RowForm& form, const std::vector<int64_t>& src_map, const s
     for (int64_t p = form.ptr[i]; p < form.ptr[i + 1]; ++p) {
       s.dep.push_back((int32_t)dst_map[form.dep[p]]);
       s.val.push_back(form.val[p]);
+      if (complex) s.val_im.push_back(form.val_im[p]);
     }
     s.row_ptr[k + 1] = (int32_t)s.dep.size();
     if (!form.unit) s.diag[k] = form.diag[i];
+    if (!form.unit && complex) s.diag_im[k] = form.diag_im[i];
   }
   return s;
 }
@@ -209,6 +214,14 @@ RowForm lu_transpose_backward(int64_t n, const uint64_t* l_ptr, const uint64_t* 
     f.ptr[j + 1] = (int64_t)f.dep.size();
   }
   return f;
+}
+
+RowForm complex_form(RowForm re, const RowForm& im, bool conjugate) {
+  re.val_im = im.val;
+  if (conjugate)
+    for (double& v : re.val_im) v = -v;
+  if (!re.unit) re.diag_im = im.diag;
+  return re;
 }
 
 Supernodes cholesky_supernodes(int64_t n, const uint64_t* l_ptr, const uint64_t* l_idx) {

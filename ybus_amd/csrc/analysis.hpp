@@ -22,6 +22,8 @@ struct RowForm {
   std::vector<int32_t> dep;
   std::vector<double> val;
   std::vector<double> diag;  // n, unless unit
+  // imaginary parts of a complex factor's val and diag (empty for a real factor)
+  std::vector<double> val_im, diag_im;
 };
 
 // A sweep scheduled for the device: rows in level order (position k), each reading its own
@@ -37,6 +39,7 @@ struct Sweep {
   std::vector<int32_t> dep;
   std::vector<double> val;
   std::vector<double> diag;  // n, unless unit
+  std::vector<double> val_im, diag_im;  // a complex factor's imaginary parts, else empty
   int32_t widest_level = 0;
 };
 
@@ -62,6 +65,13 @@ RowForm lu_backward(int64_t n, const uint64_t* u_ptr, const uint64_t* u_idx, con
 // then unit L^T backward, each column of L below its diagonal read as a row.
 RowForm lu_transpose_forward(int64_t n, const uint64_t* u_ptr, const uint64_t* u_idx, const double* u_val);
 RowForm lu_transpose_backward(int64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, const double* l_val);
+
+// A complex factor's row form from two built with the same builder over the same pattern: `re`
+// from the entries' real parts, `im` from their imaginary parts. `conjugate` negates the
+// imaginary parts of the entries off the diagonal and leaves the diagonal's: the reference's
+// L^H (cholesky.rs:283-297) conjugates L's columns below the diagonal and keeps each (real)
+// diagonal as it is.
+RowForm complex_form(RowForm re, const RowForm& im, bool conjugate);
 
 // Fundamental supernodes of a Cholesky factor (multifrontal.rs:94-138): the elimination tree
 // comes from L's pattern (the parent of column j is the first row below j in column j).

@@ -361,6 +361,87 @@ __global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, Sweep
   wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on, from_t, [&](int32_t dst, double v) { x[dst * ld + col] = v; });
 }
 
+// ---- complex FP64 solves ----
+
+// Complex arithmetic rounded as num-complex 0.4 writes it (the reference's element type):
+// a * b = (a.re b.re - a.im b.im, a.re b.im + a.im b.re), and
+// a / b = ((a.re b.re + a.im b.im) / s, (a.im b.re - a.re b.im) / s) with s = b.re b.re + b.im b.im.
+// Each product and sum is rounded on its own (the file is built without contraction).
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
+  const double s = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / s, (a.y * b.x - a.x * b.y) / s);
+}
+
+// Unknown `row` of column `col` in a complex block (real parts in row 2 row, imaginary in 2 row + 1)
+__device__ __forceinline__ double2 zload(const double* __restrict__ blk, int32_t row, int64_t ld, int32_t col) {
+  const int64_t at = 2 * (int64_t)row * ld + col;
+  return make_double2(blk[at], blk[at + ld]);
+}
+__device__ __forceinline__ void zstore(double* __restrict__ blk, int32_t row, int64_t ld, int32_t col, double2 v) {
+  const int64_t at = 2 * (int64_t)row * ld + col;
+  blk[at] = v.x;
+  blk[at + ld] = v.y;
+}
+
+// One complex sweep of a tile, row by row as forward_by_rows / back_by_rows and the LU
+// substitutions do (cholesky.rs:468-501, lu.rs:502-533,640-649,770-785): the row's right-hand
+// side, minus each term's value times its dependency in stored order, divided by the diagonal
+// unless it is unit. Lane (slot, c) takes rows lp[l] + slot + j slots of level l; a barrier ends
+// each level, and every thread takes every barrier.
+template <int TW, class Rhs, class Store>
+__device__ void complex_sweep(const ComplexSweepDev& s, const int32_t* lp, const double* __restrict__ dep_buf,
+                              int64_t ld, const Tile<TW>& g, Rhs rhs, Store store) {
+  const int32_t* __restrict__ row_ptr = s.row_ptr;
+  const int32_t* __restrict__ dep = s.dep;
+  const double2* __restrict__ val = s.val;
+  for (int32_t l = 0; l < s.levels; ++l) {
+    if (g.on) {
+      const int32_t ke = lp[l + 1];
+      for (int32_t k = lp[l] + g.slot; k < ke; k += g.slots) {
+        double2 acc = rhs(s.src[k]);
+        const int32_t pe = row_ptr[k + 1];
+        for (int32_t p = row_ptr[k]; p < pe; ++p) acc = zsub(acc, zmul(val[p], zload(dep_buf, dep[p], ld, g.col)));
+        if (s.diag) acc = zdiv(acc, s.diag[k]);
+        store(s.dst[k], acc);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int TW>
+__global__ void __launch_bounds__(1024) solve_complex_kernel(ComplexSweepDev fwd, ComplexSweepDev bwd,
+                                                             const double* __restrict__ b, double* __restrict__ t,
+                                                             double* __restrict__ x, int64_t ld, int32_t ncols,
+                                                             int32_t conjugate) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem);
+  int32_t* lpb = lpf + fwd.levels + 1;
+  for (int32_t l = threadIdx.x; l <= fwd.levels; l += blockDim.x) lpf[l] = fwd.level_ptr[l];
+  for (int32_t l = threadIdx.x; l <= bwd.levels; l += blockDim.x) lpb[l] = bwd.level_ptr[l];
+  __syncthreads();
+  const Tile<TW> g(ncols);
+  const int32_t col = g.col;
+  complex_sweep<TW>(
+      fwd, lpf, t, ld, g,
+      [&](int32_t src) {
+        double2 v = zload(b, src, ld, col);
+        if (conjugate) v.y = -v.y;
+        return v;
+      },
+      [&](int32_t dst, double2 v) { zstore(t, dst, ld, col, v); });
+  complex_sweep<TW>(
+      bwd, lpb, x, ld, g, [&](int32_t src) { return zload(t, src, ld, col); },
+      [&](int32_t dst, double2 v) { zstore(x, dst, ld, col, v); });
+  // the backward sweep read its own results unconjugated; each lane now conjugates its column
+  if (conjugate && g.on)
+    for (int32_t i = g.slot; i < bwd.n; i += g.slots) x[(2 * (int64_t)i + 1) * ld + col] = -x[(2 * (int64_t)i + 1) * ld + col];
+}
+
 // ---- FP32 sweeps refined in FP64 ----
 
 // Forward FP32 sweep of one tile: t[dst] = (float(src[src_row]) - sum val * t[dep]) / diag
@@ -1097,6 +1178,25 @@ hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepD
     if (e != hipSuccess) return e;                                                                               \
     hipLaunchKernelGGL((refine_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, d, x0, x1, \
                        r, ld, ncols, tol, max_steps, steps, ratio);                                              \
+  }
+  DISPATCH_TILE(tile, L)
+#undef L
+  return hipGetLastError();
+}
+
+hipError_t launch_solve_complex(hipStream_t stream, int tile, int threads, const ComplexSweepDev& fwd,
+                                const ComplexSweepDev& bwd, const double* b, double* t, double* x, int64_t ld,
+                                int32_t ncols, bool conjugate) {
+  if (threads < tile || threads > 1024 || threads % 64 || ld < ncols) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
+  const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
+#define L(TWV)                                                                                             \
+  {                                                                                                          \
+    hipError_t e = hipFuncSetAttribute((const void*)solve_complex_kernel<TWV>,                               \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);              \
+    if (e != hipSuccess) return e;                                                                           \
+    hipLaunchKernelGGL((solve_complex_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, b, t, x, \
+                       ld, ncols, (int32_t)conjugate);                                                       \
   }
   DISPATCH_TILE(tile, L)
 #undef L

@@ -42,6 +42,18 @@ struct SweepDev {
   bool unit = false;
 };
 
+// One scheduled sweep of a complex factor: the arrays of SweepDev, each value a (re, im) pair.
+struct ComplexSweepDev {
+  int32_t n = 0, levels = 0;
+  const int32_t* level_ptr = nullptr;
+  const int32_t* src = nullptr;
+  const int32_t* dst = nullptr;
+  const int32_t* row_ptr = nullptr;
+  const int32_t* dep = nullptr;
+  const double2* val = nullptr;
+  const double2* diag = nullptr;  // null: unit diagonal
+};
+
 // A CSR matrix on the device.
 struct CsrDev {
   int32_t rows = 0, cols = 0;
@@ -126,6 +138,13 @@ hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepD
 
 // Layout changes between count vectors of n values, one after another (the host's &[Vec<f64>])
 // and a row-interleaved block (n rows x ld, column c of row i at i * ld + c).
+// Complex FP64 solve of a block: forward sweep b -> t, backward t -> x. A complex block holds
+// unknown i's real parts in row 2i and its imaginary parts in row 2i + 1 (ld doubles a row).
+// conjugate: solve with conj(b) and return conj of the solution.
+hipError_t launch_solve_complex(hipStream_t stream, int tile, int threads, const ComplexSweepDev& fwd,
+                                const ComplexSweepDev& bwd, const double* b, double* t, double* x, int64_t ld,
+                                int32_t ncols, bool conjugate);
+
 hipError_t launch_to_block(hipStream_t stream, const double* vectors, double* block, int64_t n, int64_t count,
                            int64_t ld);
 hipError_t launch_from_block(hipStream_t stream, const double* block, double* vectors, int64_t n, int64_t count,

@@ -194,6 +194,43 @@ struct DevSweep {
   size_t entries() const { return host.dep.size(); }
 };
 
+// A complex factor's sweep on the device: level pointers, the rows' CSR arrays, and the values
+// as (re, im) pairs
+struct DevSweepC {
+  DevBuf<int32_t> level_ptr, src, dst, row_ptr, dep;
+  DevBuf<double2> val, diag;
+  int32_t n = 0, levels = 0;
+  bool unit = false;
+  void upload(const Sweep& s, hipStream_t stream) {
+    n = s.n;
+    levels = s.levels;
+    unit = s.unit;
+    std::vector<double2> v(s.val.size()), d(unit ? 0 : (size_t)n);
+    for (size_t p = 0; p < v.size(); ++p) v[p] = make_double2(s.val[p], s.val_im[p]);
+    for (size_t k = 0; k < d.size(); ++k) d[k] = make_double2(s.diag[k], s.diag_im[k]);
+    level_ptr.upload(s.level_ptr, stream);
+    src.upload(s.src, stream);
+    dst.upload(s.dst, stream);
+    row_ptr.upload(s.row_ptr, stream);
+    dep.upload(s.dep, stream);
+    val.upload(v, stream);
+    if (!unit) diag.upload(d, stream);
+  }
+  ComplexSweepDev view() const {
+    ComplexSweepDev c;
+    c.n = n;
+    c.levels = levels;
+    c.level_ptr = level_ptr.p;
+    c.src = src.p;
+    c.dst = dst.p;
+    c.row_ptr = row_ptr.p;
+    c.dep = dep.p;
+    c.val = val.p;
+    c.diag = unit ? nullptr : diag.p;
+    return c;
+  }
+};
+
 struct DevCsr {
   int32_t rows = 0, cols = 0;
   DevBuf<int32_t> ptr, col;
@@ -277,6 +314,11 @@ struct ybus_hip_factor {
   DevSweep cfwd, cbwd, ctfwd, ctbwd;
   DevBuf<int32_t> rhs_row, sol_of, t_rhs_row, t_sol_of;
   bool has_transpose = false;
+  // a complex factor: its sweeps (solve and transposed solve) and its blocks of vectors; none
+  // of the real arrays above is built
+  bool complex = false;
+  DevSweepC zfwd, zbwd, ztfwd, ztbwd;
+  DevBuf<double> zb, zt, zx, zvec;
   DevCsr a;   // the matrix itself, by user rows, for the FP64 residual
   DevCsr at;  // its transpose, by rows (A's columns), for the residual of a transposed refinement
   // workspace for `cols` columns
@@ -673,11 +715,12 @@ static bool fits_columns(const ybus_hip_factor* f, int32_t levels) {
 static void finish_factor(ybus_hip_factor* f) {
   check(hipEventCreate(&f->ev0), "hipEventCreate");
   check(hipEventCreate(&f->ev1), "hipEventCreate");
-  // factor.rs:158-160: two blocks of n doubles per column in a quarter of the free memory;
+  // factor.rs:158-160: two blocks of n values per column in a quarter of the free memory;
   // here clamped at 32768 columns, enough workgroups to fill every compute unit many times over
   size_t free_b = 0, total_b = 0;
   check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-  uint64_t w = free_b / 4 / (2 * 8 * (uint64_t)std::max<int64_t>(f->n, 1));
+  const uint64_t value = f->complex ? 16 : 8;
+  uint64_t w = free_b / 4 / (2 * value * (uint64_t)std::max<int64_t>(f->n, 1));
   f->width = std::clamp<uint64_t>(w, 1, MAX_DEFAULT_WIDTH);
   ybus_hip_factor_set_block_width(f, f->width);
 }
@@ -689,22 +732,70 @@ static std::vector<int32_t> inverse_i32(const std::vector<int64_t>& perm) {
   return inv;
 }
 
+// Checks a Cholesky factor's arrays (L in CSR, each row ending with its diagonal, and the
+// permutation) and returns its number of entries
+static uint64_t check_cholesky(uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, const double* l_val,
+                               const uint64_t* perm) {
+  if (!l_ptr || (!l_idx && l_ptr[n]) || (!l_val && l_ptr[n])) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+  check_permutation(perm, n, "perm");
+  std::string why;
+  if (!valid_compressed((int64_t)n, (int64_t)n, l_ptr, l_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "L: " + why);
+  const uint64_t entries = l_ptr[n];
+  if (entries > (uint64_t)INDEX_LIMIT || n > (uint64_t)INDEX_LIMIT)
+    fail(YBUS_HIP_TOO_LARGE, "a factor with " + std::to_string(entries) + " entries is too large for 32-bit indices");
+  for (uint64_t i = 0; i < n; ++i) {
+    if (l_ptr[i + 1] == l_ptr[i] || l_idx[l_ptr[i + 1] - 1] != i)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "each row of L must end with its diagonal");
+  }
+  return entries;
+}
+
+// Checks an LU factor's arrays (L and U in CSC, L's unit diagonal first and U's diagonal last in
+// each column, and both permutations) and returns its number of entries
+static uint64_t check_lu(uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx, const double* l_val,
+                         const uint64_t* u_ptr, const uint64_t* u_idx, const double* u_val, const uint64_t* row_perm,
+                         const uint64_t* col_perm) {
+  if (!l_ptr || !u_ptr) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+  if ((l_ptr[n] && (!l_idx || !l_val)) || (u_ptr[n] && (!u_idx || !u_val)))
+    fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+  check_permutation(row_perm, n, "row_perm");
+  check_permutation(col_perm, n, "col_perm");
+  std::string why;
+  if (!valid_compressed((int64_t)n, (int64_t)n, l_ptr, l_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "L: " + why);
+  if (!valid_compressed((int64_t)n, (int64_t)n, u_ptr, u_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "U: " + why);
+  const uint64_t entries = l_ptr[n] + u_ptr[n];
+  if (entries > (uint64_t)INDEX_LIMIT || n > (uint64_t)INDEX_LIMIT)
+    fail(YBUS_HIP_TOO_LARGE, "a factor with " + std::to_string(entries) + " entries is too large for 32-bit indices");
+  for (uint64_t j = 0; j < n; ++j) {
+    if (l_ptr[j + 1] == l_ptr[j] || l_idx[l_ptr[j]] != j)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "each column of L must start with its unit diagonal");
+    if (u_ptr[j + 1] == u_ptr[j] || u_idx[u_ptr[j + 1] - 1] != j)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "each column of U must end with its diagonal");
+  }
+  return entries;
+}
+
+// The real parts and the imaginary parts of `count` interleaved (re, im) values
+static void split_complex(const double* v, uint64_t count, std::vector<double>& re, std::vector<double>& im) {
+  re.resize(count);
+  im.resize(count);
+  for (uint64_t p = 0; p < count; ++p) {
+    re[p] = v[2 * p];
+    im[p] = v[2 * p + 1];
+  }
+}
+
+// Entry points of real factors reject a complex one
+static void require_real(const ybus_hip_factor* f) {
+  if (f->complex) fail(YBUS_HIP_INVALID_ARGUMENT, "a complex factor takes the complex solves only");
+}
+
 int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx,
                                         const double* l_val, const uint64_t* perm, ybus_hip_factor** out) {
   return guard([&] {
-    if (!ctx || !out || !l_ptr || (!l_idx && l_ptr[n]) || (!l_val && l_ptr[n]))
-      fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (!ctx || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    check_permutation(perm, n, "perm");
-    std::string why;
-    if (!valid_compressed((int64_t)n, (int64_t)n, l_ptr, l_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "L: " + why);
-    const uint64_t entries = l_ptr[n];
-    if (entries > (uint64_t)INDEX_LIMIT || n > (uint64_t)INDEX_LIMIT)
-      fail(YBUS_HIP_TOO_LARGE, "a factor with " + std::to_string(entries) + " entries is too large for 32-bit indices");
-    for (uint64_t i = 0; i < n; ++i) {
-      if (l_ptr[i + 1] == l_ptr[i] || l_idx[l_ptr[i + 1] - 1] != i)
-        fail(YBUS_HIP_INVALID_ARGUMENT, "each row of L must end with its diagonal");
-    }
+    const uint64_t entries = check_cholesky(n, l_ptr, l_idx, l_val, perm);
     check(hipSetDevice(ctx->device), "hipSetDevice");
     auto f = std::make_unique<ybus_hip_factor>();
     begin_factor(f.get(), ctx, (int64_t)n);
@@ -738,24 +829,9 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
                                   const double* l_val, const uint64_t* u_ptr, const uint64_t* u_idx, const double* u_val,
                                   const uint64_t* row_perm, const uint64_t* col_perm, ybus_hip_factor** out) {
   return guard([&] {
-    if (!ctx || !out || !l_ptr || !u_ptr) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
-    if ((l_ptr[n] && (!l_idx || !l_val)) || (u_ptr[n] && (!u_idx || !u_val)))
-      fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (!ctx || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    check_permutation(row_perm, n, "row_perm");
-    check_permutation(col_perm, n, "col_perm");
-    std::string why;
-    if (!valid_compressed((int64_t)n, (int64_t)n, l_ptr, l_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "L: " + why);
-    if (!valid_compressed((int64_t)n, (int64_t)n, u_ptr, u_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "U: " + why);
-    const uint64_t entries = l_ptr[n] + u_ptr[n];
-    if (entries > (uint64_t)INDEX_LIMIT || n > (uint64_t)INDEX_LIMIT)
-      fail(YBUS_HIP_TOO_LARGE, "a factor with " + std::to_string(entries) + " entries is too large for 32-bit indices");
-    for (uint64_t j = 0; j < n; ++j) {
-      if (l_ptr[j + 1] == l_ptr[j] || l_idx[l_ptr[j]] != j)
-        fail(YBUS_HIP_INVALID_ARGUMENT, "each column of L must start with its unit diagonal");
-      if (u_ptr[j + 1] == u_ptr[j] || u_idx[u_ptr[j + 1] - 1] != j)
-        fail(YBUS_HIP_INVALID_ARGUMENT, "each column of U must end with its diagonal");
-    }
+    const uint64_t entries = check_lu(n, l_ptr, l_idx, l_val, u_ptr, u_idx, u_val, row_perm, col_perm);
     check(hipSetDevice(ctx->device), "hipSetDevice");
     auto f = std::make_unique<ybus_hip_factor>();
     begin_factor(f.get(), ctx, (int64_t)n);
@@ -794,6 +870,82 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
   });
 }
 
+int32_t ybus_hip_factor_upload_cholesky_complex(ybus_hip_ctx* ctx, uint64_t n, const uint64_t* l_ptr,
+                                                const uint64_t* l_idx, const double* l_val, const uint64_t* perm,
+                                                ybus_hip_factor** out) {
+  return guard([&] {
+    if (!ctx || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    const uint64_t entries = check_cholesky(n, l_ptr, l_idx, l_val, perm);
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto f = std::make_unique<ybus_hip_factor>();
+    begin_factor(f.get(), ctx, (int64_t)n);
+    f->lu = false;
+    f->complex = true;
+    f->nnz_factor = 2 * (int64_t)entries;
+    std::vector<double> re, im;
+    split_complex(l_val, entries, re, im);
+    std::vector<int64_t> gather(n), ident(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      gather[i] = (int64_t)perm[i];
+      ident[i] = (int64_t)i;
+    }
+    // L forward, and L^H (L's columns conjugated below the diagonal) backward
+    const RowForm lower =
+        complex_form(cholesky_forward(n, l_ptr, l_idx, re.data()), cholesky_forward(n, l_ptr, l_idx, im.data()), false);
+    const RowForm upper = complex_form(cholesky_backward(n, l_ptr, l_idx, re.data()),
+                                       cholesky_backward(n, l_ptr, l_idx, im.data()), true);
+    f->zfwd.upload(schedule(lower, gather, ident), ctx->stream);
+    f->zbwd.upload(schedule(upper, ident, gather), ctx->stream);
+    check(hipStreamSynchronize(ctx->stream), "upload factor");
+    finish_factor(f.get());
+    *out = f.release();
+  });
+}
+
+int32_t ybus_hip_factor_upload_lu_complex(ybus_hip_ctx* ctx, uint64_t n, const uint64_t* l_ptr, const uint64_t* l_idx,
+                                          const double* l_val, const uint64_t* u_ptr, const uint64_t* u_idx,
+                                          const double* u_val, const uint64_t* row_perm, const uint64_t* col_perm,
+                                          ybus_hip_factor** out) {
+  return guard([&] {
+    if (!ctx || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    const uint64_t entries = check_lu(n, l_ptr, l_idx, l_val, u_ptr, u_idx, u_val, row_perm, col_perm);
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto f = std::make_unique<ybus_hip_factor>();
+    begin_factor(f.get(), ctx, (int64_t)n);
+    f->lu = true;
+    f->complex = true;
+    f->has_transpose = true;
+    f->nnz_factor = (int64_t)entries;
+    std::vector<double> lre, lim, ure, uim;
+    split_complex(l_val, l_ptr[n], lre, lim);
+    split_complex(u_val, u_ptr[n], ure, uim);
+    std::vector<int64_t> rp(n), cp(n), ident(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      rp[i] = (int64_t)row_perm[i];
+      cp[i] = (int64_t)col_perm[i];
+      ident[i] = (int64_t)i;
+    }
+    // the plain (unconjugated) transpose, as Lu::solve_transpose takes it (lu.rs:502-533)
+    const RowForm l = complex_form(lu_forward(n, l_ptr, l_idx, lre.data()), lu_forward(n, l_ptr, l_idx, lim.data()), false);
+    const RowForm u = complex_form(lu_backward(n, u_ptr, u_idx, ure.data()), lu_backward(n, u_ptr, u_idx, uim.data()), false);
+    const RowForm ut = complex_form(lu_transpose_forward(n, u_ptr, u_idx, ure.data()),
+                                    lu_transpose_forward(n, u_ptr, u_idx, uim.data()), false);
+    const RowForm lt = complex_form(lu_transpose_backward(n, l_ptr, l_idx, lre.data()),
+                                    lu_transpose_backward(n, l_ptr, l_idx, lim.data()), false);
+    f->zfwd.upload(schedule(l, rp, ident), ctx->stream);
+    f->zbwd.upload(schedule(u, ident, cp), ctx->stream);
+    f->ztfwd.upload(schedule(ut, cp, ident), ctx->stream);
+    f->ztbwd.upload(schedule(lt, ident, rp), ctx->stream);
+    check(hipStreamSynchronize(ctx->stream), "upload factor");
+    finish_factor(f.get());
+    *out = f.release();
+  });
+}
+
+int32_t ybus_hip_factor_is_complex(const ybus_hip_factor* f) { return f && f->complex ? 1 : 0; }
+
 void ybus_hip_factor_free(ybus_hip_factor* f) { delete f; }
 
 uint64_t ybus_hip_factor_dim(const ybus_hip_factor* f) { return f ? (uint64_t)f->n : 0; }
@@ -831,6 +983,7 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
                                    const double* val) {
   return guard([&] {
     if (!f || !ptr) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    require_real(f);
     const int64_t n = f->n;
     std::string why;
     if (!valid_compressed(n, n, ptr, idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "A: " + why);
@@ -966,6 +1119,7 @@ static void copy_out_device(ybus_hip_factor* f, const double* block, int64_t cou
 }
 
 static void require_unborrowed(const ybus_hip_factor* f) {
+  require_real(f);
   if (f->borrowed) fail(YBUS_HIP_INVALID_ARGUMENT, "a block borrows this factor: free it first");
 }
 
@@ -1012,6 +1166,67 @@ static int32_t solve_many_mode(ybus_hip_factor* f, const double* rhs, uint64_t c
       f->timings.blocks += 1;
     }
   });
+}
+
+// Complex solves of `count` right-hand sides (n interleaved (re, im) values each), in blocks of
+// block_width() columns: each block is copied in, turned into a complex block (real and
+// imaginary rows), solved by one launch, and copied out. A Cholesky factor's transposed solve
+// is conj(A^-1 conj(b)), since A^T = conj(A) for a Hermitian A.
+static int32_t solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x, bool transpose) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    if (!f->complex) fail(YBUS_HIP_INVALID_ARGUMENT, "a real factor takes the real solves only");
+    if (count == 0 || f->n == 0) return;
+    if (!rhs || !x) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
+    const int64_t ld = round_up(width, 16);
+    const int64_t rows = 2 * f->n;
+    const size_t elems = (size_t)rows * (size_t)ld;
+    f->zb.reserve(elems);
+    f->zt.reserve(elems);
+    f->zx.reserve(elems);
+    f->zvec.reserve((size_t)rows * (size_t)width);
+    const bool lu_t = transpose && f->lu;
+    const ComplexSweepDev fw = (lu_t ? f->ztfwd : f->zfwd).view(), bw = (lu_t ? f->ztbwd : f->zbwd).view();
+    for (uint64_t start = 0; start < count; start += (uint64_t)width) {
+      const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
+      const size_t len = (size_t)chunk * (size_t)rows;
+      auto t0 = std::chrono::steady_clock::now();
+      double* host = f->staging.get(len);
+      std::memcpy(host, rhs + start * (uint64_t)rows, len * sizeof(double));
+      check(hipMemcpyAsync(f->zvec.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+      check(launch_to_block(f->stream(), f->zvec.p, f->zb.p, rows, chunk, ld), "to_block");
+      check(hipStreamSynchronize(f->stream()), "upload");
+      auto t1 = std::chrono::steady_clock::now();
+      f->begin_kernel();
+      check(launch_solve_complex(f->stream(), f->tile_override ? f->tile_override : tile_width_for(chunk, f->ctx->cus),
+                                 f->threads_wide, fw, bw, f->zb.p, f->zt.p, f->zx.p, ld, (int32_t)chunk,
+                                 transpose && !f->lu),
+            "complex solve kernel");
+      // the factor's (re, im) values and 4-byte indices once, row pointers, and a 16-byte value
+      // in and out per unknown and column
+      f->end_kernel((uint64_t)f->nnz_factor * 20 + 8 * (uint64_t)(f->n + 1) + 32 * (uint64_t)f->n * (uint64_t)chunk);
+      auto t2 = std::chrono::steady_clock::now();
+      check(launch_from_block(f->stream(), f->zx.p, f->zvec.p, rows, chunk, ld), "from_block");
+      check(hipMemcpyAsync(host, f->zvec.p, len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+      check(hipStreamSynchronize(f->stream()), "download");
+      std::memcpy(x + start * (uint64_t)rows, host, len * sizeof(double));
+      f->timings.upload += std::chrono::duration<double>(t1 - t0).count();
+      f->timings.solve += std::chrono::duration<double>(t2 - t1).count();
+      f->timings.download += seconds_since(t2);
+      f->timings.blocks += 1;
+    }
+  });
+}
+
+int32_t ybus_hip_factor_solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x) {
+  return solve_many_complex(f, rhs, count, x, false);
+}
+
+int32_t ybus_hip_factor_solve_transpose_many_complex(ybus_hip_factor* f, const double* rhs, uint64_t count,
+                                                     double* x) {
+  return solve_many_complex(f, rhs, count, x, true);
 }
 
 int32_t ybus_hip_factor_solve_many(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x) {
@@ -1543,6 +1758,7 @@ int32_t ybus_hip_factor_operator(ybus_hip_factor* f, uint64_t rows, const uint64
                                  const double* val, ybus_hip_operator** out) {
   return guard([&] {
     if (!f || !ptr || !out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    require_real(f);
     std::string why;
     if (!valid_compressed((int64_t)rows, f->n, ptr, idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "operator: " + why);
     const uint64_t nnz = ptr[rows];
@@ -1739,6 +1955,7 @@ uint64_t ybus_hip_batch_factors(const ybus_hip_batch* b) { return b ? (uint64_t)
 int32_t ybus_hip_batch_add(ybus_hip_batch* b, ybus_hip_factor* f) {
   return guard([&] {
     if (!b || !f) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    require_real(f);
     if (f->ctx != b->ctx) fail(YBUS_HIP_INVALID_ARGUMENT, "the factor belongs to another context");
     if (!f->fits)
       fail(YBUS_HIP_INVALID_ARGUMENT, "a batch takes factors whose solution fits in LDS (dim " + std::to_string(f->n) +
