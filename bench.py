@@ -433,6 +433,176 @@ def run_batch(gpu, args):
     return out
 
 
+def host_screening(chol, bf, theta, plus, minus, inverse_weight, weights, rows, threads):
+    """The CPU side of N-1 screening for the outages `rows`: each outage's pair solve
+    (par_solve_many over dense pair vectors), its rank-one update of theta (pairs.rs:209-253 as
+    gpu_pairs_test.rs:48-62 states it, alpha = 0), the flows Bf theta' with the outaged branch's
+    zeroed, and the most loaded branch: (rows of the maxima, their loadings)."""
+    n = chol.n
+    dense = np.zeros((len(rows), n))
+    for j, k in enumerate(rows):
+        if plus[k] >= 0:
+            dense[j, plus[k]] = 1.0
+        if minus[k] >= 0:
+            dense[j, minus[k]] = -1.0
+    y = chol.solve_many(dense, threads)
+    p, m = plus[rows], minus[rows]
+
+    def at(v, idx):
+        return np.where(idx >= 0, v[np.arange(len(idx)), np.maximum(idx, 0)], 0.0)
+
+    x = theta[None, :] + 0.0 * y
+    w = -(at(y, p) - at(y, m)) + inverse_weight[rows]
+    z = (at(x, p) - at(x, m)) / w
+    updated = x + z[:, None] * y
+    flows = (bf_dense_product(bf, updated))
+    flows[np.arange(len(rows)), rows] = 0.0
+    loading = np.abs(flows) * weights[None, :]
+    return np.argmax(loading, axis=1), np.max(loading, axis=1)
+
+
+def bf_dense_product(bf, x):
+    """Bf x for each row of x (Bf in CSR, at most two entries a row)."""
+    out = np.zeros((x.shape[0], bf.rows))
+    ptr, idx, val = np.asarray(bf.ptr), np.asarray(bf.idx), np.asarray(bf.data)
+    for slot in range(int(np.max(np.diff(ptr)))):
+        has = np.diff(ptr) > slot
+        at = ptr[:-1][has] + slot
+        out[:, has] += val[at][None, :] * x[:, idx[at]]
+    return out
+
+
+def run_screening(gpu, args):
+    """N-1 screening on the 25,000-bus grid (the reference's outage_loadings, dc/gpu.rs:334-430):
+    base angles theta = B'^-1 p for one zero-sum N(0,1) injection vector, then for every
+    non-bridge branch k taken out alone, theta_k = theta plus the rank-one update of B' losing
+    b_k (pair (from, to), alpha = 0, inverse weight 1/b_k), flows Bf theta_k with branch k's
+    zeroed, and the most loaded branch under limits U[0.5, 2.0] pu. One step screens every
+    outage: pair_updates, product_maxima and maxima on the device, one block."""
+    import oracle as O
+    import ybus_amd as Y
+
+    buses, seed = 25000, SEED + 5
+    t0 = time.perf_counter()
+    grid = O.make_grid(buses, seed)
+    bprime, bf, _ = grid.dc()
+    chol = O.Cholesky(bprime)
+    p = O.injections(seed, 4, buses, 1)[0]
+    theta = chol.solve(p)
+    limit = 0.5 + 1.5 * O.uniforms(seed, 5, bf.rows)
+    weights = 1.0 / limit
+    plus, minus = pair_block(grid)
+    b = 1.0 / np.asarray(grid.x, dtype=np.float64)
+    inverse_weight = 1.0 / b
+    outages = np.flatnonzero(~grid.bridges())
+    o_plus, o_minus = plus[outages], minus[outages]
+    o_alpha, o_weight = np.zeros(len(outages)), inverse_weight[outages]
+    zero = outages.astype(np.int64)
+    log(f"[screening] n = {chol.n}, {len(outages)} outages of {grid.branches} branches; host "
+        f"{time.perf_counter() - t0:.1f} s")
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    factor.set_block_width(len(outages))
+    op = factor.operator(bf)
+    op.set_offset(np.zeros(bf.rows))
+    op.set_weights(weights)
+
+    def step():
+        block = factor.pair_updates_arrays(theta, o_plus, o_minus, o_alpha, o_weight)
+        loadings = block.product_maxima_arrays(op, zero)
+        angles = block.maxima_arrays()
+        singular = block.singular()
+        block.free()
+        return loadings, angles, singular
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    reps = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loadings, angles, singular = step()
+    step_s = (time.perf_counter() - t0) / reps
+    # parity: sampled outages against the CPU composition (loadings within 1e-12; the same branch
+    # unless two loadings tie to within rounding)
+    sample = outages[:: max(1, len(outages) // 64)]
+    positions = {int(k): j for j, k in enumerate(outages)}
+    rows, values = host_screening(chol, bf, theta, plus, minus, inverse_weight, weights, sample, cpu_threads())
+    worst, same_branch = 0.0, 0
+    for k, r, v in zip(sample, rows, values):
+        j = positions[int(k)]
+        worst = max(worst, abs(loadings[1][j] - v) / max(v, 1e-300))
+        same_branch += int(loadings[0][j] == int(r))
+    cpu_baseline = None
+    if not args.no_cpu:
+        cpu_sample = outages[:2000]
+        threads = cpu_threads()
+        host_screening(chol, bf, theta, plus, minus, inverse_weight, weights, cpu_sample[:64], threads)
+        t0, done = time.perf_counter(), 0
+        while time.perf_counter() - t0 < CPU_SECONDS:
+            host_screening(chol, bf, theta, plus, minus, inverse_weight, weights, cpu_sample, threads)
+            done += len(cpu_sample)
+        cpu_rate = done / (time.perf_counter() - t0)
+        cpu_baseline = dict(value=cpu_rate, unit="outages/s", cores=threads, kind="port",
+                            sample="the oracle's par_solve_many over the first 2,000 outages' pair vectors, then "
+                                   "the update, flows and maxima in numpy, repeated for >= %g s" % CPU_SECONDS)
+    out = dict(workload="N-1 screening: 25,000-bus grid, B' AMD Cholesky, every non-bridge branch out alone "
+                        "(%d outages), rank-one update, Bf flows, most loaded branch" % len(outages),
+               n=int(chol.n), nnz_l=int(chol.l.nnz), outages=int(len(outages)), value=len(outages) / step_s,
+               unit="outages/s", step_ms=step_s * 1e3, singular=int(sum(singular)),
+               parity=dict(sampled_outages=int(len(sample)), max_rel_diff_loading=worst,
+                           same_branch=f"{same_branch} of {len(sample)}"),
+               cpu_baseline=cpu_baseline)
+    log(f"[screening] {step_s * 1e3:.2f} ms per step ({out['value']:.3g} outages/s); loadings within {worst:.1e}, "
+        f"same branch {same_branch} of {len(sample)}; cpu {cpu_baseline['value'] if cpu_baseline else 0:.3g} "
+        f"outages/s")
+    return out
+
+
+def run_complex(gpu, args):
+    """A complex factor: the 10,000-bus grid's B' made Hermitian (each branch entry given a phase,
+    so A stays positive definite), factored as P A P^T = L L^H by the oracle's restated complex
+    Cholesky, and 1,024 complex right-hand sides solved per step in FP64 (host vectors in and
+    out, as solve_many takes them); the kernel time comes from HIP events around each launch."""
+    import oracle as O
+    from oracle import complex_solve as Z
+    import ybus_amd as Y
+
+    t0 = time.perf_counter()
+    grid = O.make_grid(10000, SEED)
+    bprime, _, _ = grid.dc()
+    re, im = Z.hermitian_from(bprime, 1)
+    chol = Z.ComplexCholesky(bprime, re, im, O.Cholesky(bprime))
+    count = 1024
+    b = O.normals(SEED, 9, count * chol.n).reshape(count, chol.n) + 1j * O.normals(SEED, 10, count * chol.n).reshape(
+        count, chol.n)
+    log(f"[complex] n = {chol.n}, nnz(L) = {len(chol.l_idx)}; host {time.perf_counter() - t0:.1f} s")
+    factor = Y.GpuFactor.from_cholesky_complex(gpu, chol)
+    factor.set_block_width(count)
+    for _ in range(max(1, args.warmup)):
+        x = factor.solve_many(b)
+    factor.reset_kernel_stats()
+    reps = max(3, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x = factor.solve_many(b)
+    step_s = (time.perf_counter() - t0) / reps
+    stats = factor.kernel_stats()
+    kernel_ms = stats.milliseconds / stats.launches
+    achieved = stats.bytes / stats.launches / (kernel_ms / 1e3) / 1e9
+    sample = np.arange(0, count, 128)
+    exact = x[sample].tobytes() == chol.solve_many(b[sample]).tobytes()
+    work = float(count) * len(chol.l_idx)
+    out = dict(workload="10,000-bus grid's B' made Hermitian, complex AMD Cholesky (L L^H), 1,024 complex RHS, "
+                        "FP64 complex forward+backward", n=int(chol.n), nnz_l=int(len(chol.l_idx)), rhs=count,
+               value=work / (kernel_ms / 1e3), unit="RHS·nnz(L)/s (kernel time)",
+               with_host_copies=work / step_s, step_ms=step_s * 1e3, kernel="solve_complex_kernel",
+               kernel_ms=kernel_ms, bytes_per_launch=stats.bytes / stats.launches, achieved_gbs=achieved,
+               hbm_frac=achieved / HBM_PEAK_GBS,
+               bit_exact_sampled_columns=f"{len(sample)} of {len(sample)}" if exact else "MISMATCH")
+    log(f"[complex] kernel {kernel_ms:.3f} ms ({achieved:.0f} GB/s), step with copies {step_s * 1e3:.3f} ms; "
+        f"bit-exact {out['bit_exact_sampled_columns']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,6 +612,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-complex", action="store_true")
+    ap.add_argument("--no-screening", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB on the device (to scale HBM counters) and exit")
     args = ap.parse_args()
@@ -526,6 +698,10 @@ def main():
         line["adjoint"] = run_adjoint(gpu, args)
     if not args.no_batch:
         line["batch"] = run_batch(gpu, args)
+    if not args.no_screening:
+        line["screening"] = run_screening(gpu, args)
+    if not args.no_complex:
+        line["complex"] = run_complex(gpu, args)
     print(json.dumps(line), flush=True)
 
 

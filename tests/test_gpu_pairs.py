@@ -210,3 +210,68 @@ def test_pair_blocks_give_the_cpu_bits_with_and_without_the_reach(gpu, monkeypat
         factor.set_block_width(k)
         with factor.pair_solves(pairs) as block:
             assert block.solutions().tobytes() == want.tobytes(), reach
+
+
+def host_maxima(values, weights):
+    """Per column (a row of `values`): the largest finite |v| * w, the first row reaching it (None
+    when every product is 0) and whether any value isn't finite (pairs.rs:39-50)."""
+    out = []
+    for column in values:
+        row, best, bad = None, 0.0, False
+        for i, (v, w) in enumerate(zip(column, weights)):
+            if not np.isfinite(v):
+                bad = True
+                continue
+            if abs(v) * w > best:
+                row, best = i, abs(v) * w
+        out.append(Y.ColumnMax(row, best, bad))
+    return out
+
+
+def test_maxima_split_over_row_chunks_keep_the_first_of_tied_rows(gpu):
+    """Blocks and operator products with thousands of rows: the maxima are reduced over runs of
+    rows and then combined, and a tie between rows in different runs still goes to the first."""
+    n, rows = 1200, 3000
+    cpu = O.Cholesky(random_spd(n, 1, 41))
+    rng = Lcg(43)
+    triplets = []
+    for i in range(rows):
+        cols = [rng.next() % n for _ in range(3)]
+        vals = [rng.value() for _ in range(3)]
+        if i in (10, 2500):  # two identical rows, large enough to be every column's maximum
+            cols, vals = [5, 6, 7], [400.0, -250.0, 125.0]
+        triplets += [(i, c, v) for c, v in zip(cols, vals)]
+    m = O.Matrix.from_triplets(rows, n, triplets)
+    base = np.array([rng.value() for _ in range(n)])
+    weights = np.ones(rows)
+    updates = [Y.PairUpdate(p, 0.25, 3.0) for p in pairs(n, 9, 44)]
+    zero = [None, 10, None, 7, None, None, 2500, None, 1]
+    factor = Y.GpuFactor.from_cholesky(gpu, cpu)
+    factor.set_block_width(16)
+    op = factor.operator(m)
+    op.set_offset(np.zeros(rows))
+    op.set_weights(weights)
+    block = factor.pair_updates(base, updates)
+    theta = block.solutions()
+    products = block.product(op, zero)
+    maxima = block.product_maxima(op, zero)
+    assert maxima == host_maxima(products, weights)
+    assert maxima[0].row == 10 and maxima[1].row == 2500 and maxima[6].row == 10
+    assert block.maxima() == host_maxima(theta, np.ones(n))
+    # the array forms give the same results
+    z = np.array([-1 if r is None else r for r in zero])
+    rows_a, values_a, bad_a = block.product_maxima_arrays(op, z)
+    assert [Y.ColumnMax(None if r < 0 else int(r), float(v), bool(b)) for r, v, b in zip(rows_a, values_a, bad_a)] \
+        == maxima
+    rows_b, values_b, _ = block.maxima_arrays()
+    assert list(values_b) == [mx.value for mx in block.maxima()]
+    block.free()
+    plus, minus = _pairs_arrays([u.pair for u in updates])
+    again = factor.pair_updates_arrays(base, plus, minus, [0.25] * 9, [3.0] * 9)
+    assert again.solutions().tobytes() == theta.tobytes()
+    again.free()
+
+
+def _pairs_arrays(ps):
+    return (np.array([-1 if p.plus is None else p.plus for p in ps]),
+            np.array([-1 if p.minus is None else p.minus for p in ps]))

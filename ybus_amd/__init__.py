@@ -80,6 +80,11 @@ class _ColumnMax(C.Structure):
     _fields_ = [("row", C.c_int64), ("value", C.c_double), ("not_finite", C.c_int32)]
 
 
+# ybus_hip_column_max as a numpy record, for maxima read as arrays
+_COLUMN_MAX = np.dtype([("row", np.int64), ("value", np.float64), ("not_finite", np.int32)], align=True)
+assert _COLUMN_MAX.itemsize == C.sizeof(_ColumnMax)
+
+
 class _Tuning(C.Structure):
     _fields_ = [("path", C.c_int32), ("threads", C.c_int32), ("tile", C.c_int32), ("milliseconds", C.c_double)]
 
@@ -635,6 +640,23 @@ class GpuFactor:
                                                       C.byref(h)))
         return self._new_block(h)
 
+    def pair_updates_arrays(self, base, plus, minus, alpha, inverse_weight) -> "GpuBlock":
+        """pair_updates with the columns as arrays: plus and minus (-1 for none), alpha and
+        inverse_weight, one entry per column."""
+        self._release_block()
+        base = _f64(base)
+        if base.shape[0] != self._n:
+            raise InvalidArgument(INVALID_ARGUMENT, "one base value per unknown")
+        plus = np.ascontiguousarray(plus, dtype=np.int64)
+        minus = np.ascontiguousarray(minus, dtype=np.int64)
+        alpha, weight = _f64(alpha), _f64(inverse_weight)
+        if not (len(plus) == len(minus) == len(alpha) == len(weight)):
+            raise InvalidArgument(INVALID_ARGUMENT, "one plus, minus, alpha and inverse_weight per column")
+        h = P()
+        _check(library().ybus_hip_factor_pair_updates(self._h, base, plus, minus, alpha, weight, len(plus),
+                                                      C.byref(h)))
+        return self._new_block(h)
+
     def operator(self, matrix) -> "GpuOperator":
         """A CSR matrix with one column per unknown, applied to blocks (pairs.rs:112-149)."""
         if matrix.cols != self._n:
@@ -702,6 +724,27 @@ class GpuBlock:
         arr = (_ColumnMax * max(k, 1))()
         _check(library().ybus_hip_block_maxima(self._live(), arr))
         return self._maxima(arr[:k])
+
+    def maxima_arrays(self):
+        """maxima() as arrays: (rows, -1 where every product is zero; values; not_finite)."""
+        k = self.columns()
+        arr = np.zeros(max(k, 1), dtype=_COLUMN_MAX)
+        _check(library().ybus_hip_block_maxima(self._live(), arr.ctypes.data_as(C.POINTER(_ColumnMax))))
+        arr = arr[:k]
+        return arr["row"].copy(), arr["value"].copy(), arr["not_finite"] != 0
+
+    def product_maxima_arrays(self, operator: "GpuOperator", zero_rows):
+        """product_maxima() with the rows to zero as an array (-1 for none), its maxima as arrays:
+        (rows, values, not_finite)."""
+        z = np.ascontiguousarray(zero_rows, dtype=np.int64)
+        k = self.columns()
+        if len(z) != k:
+            raise InvalidArgument(INVALID_ARGUMENT, "one entry to zero (or none) per column")
+        arr = np.zeros(max(k, 1), dtype=_COLUMN_MAX)
+        _check(library().ybus_hip_block_product_maxima(self._live(), operator._h, z,
+                                                        arr.ctypes.data_as(C.POINTER(_ColumnMax))))
+        arr = arr[:k]
+        return arr["row"].copy(), arr["value"].copy(), arr["not_finite"] != 0
 
     def product(self, operator: "GpuOperator", zero) -> np.ndarray:
         z = np.array([-1 if r is None else int(r) for r in zero], dtype=np.int64)

@@ -387,26 +387,48 @@ __device__ __forceinline__ void zstore(double* __restrict__ blk, int32_t row, in
   blk[at + ld] = v.y;
 }
 
+// Terms of a complex row taken at a time: their indices and values are loaded together, then
+// their dependencies together, so a row waits for two round trips per chunk rather than per term
+constexpr int COMPLEX_CHUNK = 4;
+
 // One complex sweep of a tile, row by row as forward_by_rows / back_by_rows and the LU
 // substitutions do (cholesky.rs:468-501, lu.rs:502-533,640-649,770-785): the row's right-hand
 // side, minus each term's value times its dependency in stored order, divided by the diagonal
-// unless it is unit. Lane (slot, c) takes rows lp[l] + slot + j slots of level l; a barrier ends
-// each level, and every thread takes every barrier.
+// unless it is unit. Lane (slot, c) takes rows lp[l] + slot + j slots of level l, loading the
+// next row's header while it works on one; a barrier ends each level, and every thread takes
+// every barrier.
 template <int TW, class Rhs, class Store>
 __device__ void complex_sweep(const ComplexSweepDev& s, const int32_t* lp, const double* __restrict__ dep_buf,
                               int64_t ld, const Tile<TW>& g, Rhs rhs, Store store) {
-  const int32_t* __restrict__ row_ptr = s.row_ptr;
+  const int4* __restrict__ head = s.head;
   const int32_t* __restrict__ dep = s.dep;
   const double2* __restrict__ val = s.val;
   for (int32_t l = 0; l < s.levels; ++l) {
-    if (g.on) {
-      const int32_t ke = lp[l + 1];
-      for (int32_t k = lp[l] + g.slot; k < ke; k += g.slots) {
-        double2 acc = rhs(s.src[k]);
-        const int32_t pe = row_ptr[k + 1];
-        for (int32_t p = row_ptr[k]; p < pe; ++p) acc = zsub(acc, zmul(val[p], zload(dep_buf, dep[p], ld, g.col)));
-        if (s.diag) acc = zdiv(acc, s.diag[k]);
-        store(s.dst[k], acc);
+    const int32_t kb = lp[l] + g.slot, ke = lp[l + 1];
+    if (g.on && kb < ke) {
+      int4 h = head[kb];
+      for (int32_t k = kb; k < ke; k += g.slots) {
+        const int4 next = head[k + g.slots < ke ? k + g.slots : k];
+        const double2 diag = s.diag ? s.diag[k] : make_double2(1.0, 0.0);
+        double2 acc = rhs(h.y);
+        for (int32_t p = h.z; p < h.w; p += COMPLEX_CHUNK) {
+          int32_t d[COMPLEX_CHUNK];
+          double2 v[COMPLEX_CHUNK], x[COMPLEX_CHUNK];
+#pragma unroll
+          for (int q = 0; q < COMPLEX_CHUNK; ++q) {
+            const int32_t at = p + q < h.w ? p + q : p;  // past the row's end: reload its first, unused
+            d[q] = dep[at];
+            v[q] = val[at];
+          }
+#pragma unroll
+          for (int q = 0; q < COMPLEX_CHUNK; ++q) x[q] = zload(dep_buf, d[q], ld, g.col);
+#pragma unroll
+          for (int q = 0; q < COMPLEX_CHUNK; ++q)
+            if (p + q < h.w) acc = zsub(acc, zmul(v[q], x[q]));
+        }
+        if (s.diag) acc = zdiv(acc, diag);
+        store(h.x, acc);
+        h = next;
       }
     }
     __syncthreads();
@@ -850,23 +872,16 @@ __global__ void from_block_kernel(const double* __restrict__ block, double* __re
 
 // ---- pairs, maxima, operator ----
 
-// ybus_cuda/src/kernels.cu:24-58, over a block by unknown: x = base + alpha y,
-// w = inverse_weight - c^T y, z = c^T x / w, out = x + z y; w == 0 flags the column singular
-__global__ void pair_update_kernel(const double* __restrict__ y, double* __restrict__ out, int32_t n, int32_t ncols,
-                                   int64_t rs, int64_t cs, const double* __restrict__ base, const int32_t* __restrict__ plus,
-                                   const int32_t* __restrict__ minus, const double* __restrict__ alpha,
-                                   const double* __restrict__ inverse_weight, int32_t* __restrict__ singular) {
-  const int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (index >= (int64_t)n * ncols) return;
-  // consecutive threads take consecutive elements of whichever index has unit stride
-  int32_t i, k;
-  if (cs == 1) {
-    i = (int32_t)(index / ncols);
-    k = (int32_t)(index % ncols);
-  } else {
-    k = (int32_t)(index / n);
-    i = (int32_t)(index % n);
-  }
+// ybus_cuda/src/kernels.cu:24-58, over a block by unknown, in two passes. Per column: w =
+// inverse_weight - c^T y and q = c^T x / w with x = base + alpha y, or the column flagged singular
+// when w == 0. Then every entry: out = x + q y (x alone for a singular column).
+__global__ void pair_coefficient_kernel(const double* __restrict__ y, int32_t ncols, int64_t rs, int64_t cs,
+                                        const double* __restrict__ base, const int32_t* __restrict__ plus,
+                                        const int32_t* __restrict__ minus, const double* __restrict__ alpha,
+                                        const double* __restrict__ inverse_weight, double* __restrict__ coef,
+                                        int32_t* __restrict__ singular) {
+  const int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= ncols) return;
   const int32_t p = plus[k], m = minus[k];
   const double a = alpha[k];
   const int64_t c = k * cs;
@@ -880,102 +895,162 @@ __global__ void pair_update_kernel(const double* __restrict__ y, double* __restr
     cx -= base[m] + a * y[m * rs + c];
   }
   const double w = -cy + inverse_weight[k];
-  const double x = base[i] + a * y[i * rs + c];
-  if (w == 0.0) {
-    if (i == 0) singular[k] = 1;
-    out[i * rs + c] = x;
-    return;
-  }
-  out[i * rs + c] = x + (cx / w) * y[i * rs + c];
+  singular[k] = w == 0.0 ? 1 : 0;
+  coef[k] = w == 0.0 ? 0.0 : cx / w;
 }
 
-// One wavefront-wide column group per workgroup: lane = column, the workgroup's waves split
-// the rows (each lane visits its rows in increasing order, so a strictly larger product wins),
-// then the waves' results combine in LDS, the smaller row winning a tie.
-__global__ void __launch_bounds__(256) column_max_kernel(const double* __restrict__ block, int32_t rows, int32_t ncols,
-                                                         int64_t rs, int64_t cs, const double* __restrict__ weight,
-                                                         double* __restrict__ best, int32_t* __restrict__ best_row,
-                                                         int32_t* __restrict__ not_finite) {
-  __shared__ double s_value[256];
-  __shared__ int32_t s_row[256], s_bad[256];
-  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
-  const int32_t col = (int32_t)blockIdx.x * 64 + lane;
-  double value = 0.0;
-  int32_t row = -1, bad = 0;
-  if (col < ncols) {
-    for (int32_t i = wave; i < rows; i += waves) {
-      const double v = block[i * rs + col * cs];
-      if (!isfinite(v)) {
-        bad = 1;
-        continue;
-      }
-      const double product = fabs(v) * weight[i];
-      if (product > value) {
-        value = product;
-        row = i;
-      }
-    }
-  }
-  s_value[threadIdx.x] = value;
-  s_row[threadIdx.x] = row;
-  s_bad[threadIdx.x] = bad;
-  __syncthreads();
-  if (wave == 0 && col < ncols) {
-    for (int w = 1; w < waves; ++w) {
-      const double v = s_value[w * 64 + lane];
-      const int32_t r = s_row[w * 64 + lane];
-      if (r >= 0 && (v > value || (v == value && (row < 0 || r < row)))) {
-        value = v;
-        row = r;
-      }
-      bad |= s_bad[w * 64 + lane];
-    }
-    best[col] = value;
-    best_row[col] = row;
-    not_finite[col] = bad;
+// Row-interleaved blocks (entry (i, c) at i ld + c): lanes over columns, each thread a run of
+// rows_per_block rows of its column
+__global__ void pair_apply_rows_kernel(const double* __restrict__ y, double* __restrict__ out, int32_t n, int32_t ncols,
+                                       int64_t ld, int32_t rows_per_block, const double* __restrict__ base,
+                                       const double* __restrict__ alpha, const double* __restrict__ coef,
+                                       const int32_t* __restrict__ singular) {
+  const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= ncols) return;
+  const double a = alpha[c], q = coef[c];
+  const bool flat = singular[c] != 0;
+  const int32_t i0 = (int32_t)blockIdx.y * rows_per_block, i1 = min(n, i0 + rows_per_block);
+  for (int32_t i = i0; i < i1; ++i) {
+    const double v = y[i * ld + c];
+    const double x = base[i] + a * v;
+    out[i * ld + c] = flat ? x : x + q * v;
   }
 }
 
-// Column-major blocks: one workgroup per column, threads over rows (each thread visits its rows
-// in increasing order, so a strictly larger product wins), combined in LDS with the smaller row
-// winning a tie: the same result as column_max_kernel.
-__global__ void __launch_bounds__(256) column_max_cols_kernel(const double* __restrict__ block, int32_t rows,
-                                                              int64_t cs, const double* __restrict__ weight,
-                                                              double* __restrict__ best, int32_t* __restrict__ best_row,
-                                                              int32_t* __restrict__ not_finite) {
-  __shared__ double s_value[256];
-  __shared__ int32_t s_row[256], s_bad[256];
-  const int32_t col = (int32_t)blockIdx.x;
-  const double* column = block + (int64_t)col * cs;
+// Column-major blocks (entry (i, c) at i + c ld): threads over rows, columns over grid.y
+__global__ void pair_apply_columns_kernel(const double* __restrict__ y, double* __restrict__ out, int32_t n,
+                                          int32_t ncols, int64_t ld, const double* __restrict__ base,
+                                          const double* __restrict__ alpha, const double* __restrict__ coef,
+                                          const int32_t* __restrict__ singular) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  for (int32_t c = (int32_t)blockIdx.y; c < ncols; c += (int32_t)gridDim.y) {
+    const double v = y[i + c * ld];
+    const double x = base[i] + alpha[c] * v;
+    out[i + c * ld] = singular[c] ? x : x + coef[c] * v;
+  }
+}
+
+// A column's running maximum of |v| * weight (kernels.cu:78-128): rows visited in increasing
+// order, so a strictly larger product wins; a value that isn't finite sets `bad` and is skipped.
+// merge() folds in another run's result, the smaller row winning a tie.
+struct MaxAcc {
   double value = 0.0;
   int32_t row = -1, bad = 0;
-  for (int32_t i = threadIdx.x; i < rows; i += blockDim.x) {
-    const double v = column[i];
+  __device__ void visit(double v, double w, int32_t i) {
     if (!isfinite(v)) {
       bad = 1;
-      continue;
+      return;
     }
-    const double product = fabs(v) * weight[i];
+    const double product = fabs(v) * w;
     if (product > value) {
       value = product;
       row = i;
     }
   }
-  s_value[threadIdx.x] = value;
-  s_row[threadIdx.x] = row;
-  s_bad[threadIdx.x] = bad;
+  __device__ void merge(double v, int32_t r, int32_t b) {
+    if (r >= 0 && (v > value || (v == value && (row < 0 || r < row)))) {
+      value = v;
+      row = r;
+    }
+    bad |= b;
+  }
+};
+
+// Entry (i, c) of a block
+struct BlockEntry {
+  const double* __restrict__ b;
+  int64_t rs, cs;
+  __device__ double operator()(int32_t i, int32_t c) const { return b[i * rs + c * cs]; }
+};
+
+// Entry (r, c) of M x + offset with row zero[c] of column c zeroed, summed from 0 in index
+// order as the host's M.mul_vec(theta) then + offset (pairs.rs:312-366)
+struct ProductEntry {
+  CsrDev m;
+  const double* __restrict__ x;
+  int64_t rs, cs;
+  const double* __restrict__ offset;
+  const int32_t* __restrict__ zero;
+  __device__ double operator()(int32_t r, int32_t c) const {
+    double sum = 0.0;
+    const int32_t pe = m.ptr[r + 1];
+    for (int32_t p = m.ptr[r]; p < pe; ++p) sum = sum + m.val[p] * x[m.col[p] * rs + c * cs];
+    return zero[c] == r ? 0.0 : sum + offset[r];
+  }
+};
+
+// Row-interleaved blocks: a workgroup takes 64 columns (one per lane) and one chunk of
+// chunk_rows rows, its waves interleaving the rows; the waves' results combine in LDS and each
+// (chunk, column) result goes to the partial arrays, chunk-major.
+template <class Entry>
+__global__ void __launch_bounds__(256) max_rows_kernel(Entry entry, int32_t rows, int32_t ncols, int32_t chunk_rows,
+                                                       const double* __restrict__ weight, double* __restrict__ pv,
+                                                       int32_t* __restrict__ pr, int32_t* __restrict__ pb) {
+  __shared__ double s_value[256];
+  __shared__ int32_t s_row[256], s_bad[256];
+  const int lane = threadIdx.x % 64, waves = blockDim.x / 64;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+  const int32_t col = (int32_t)blockIdx.x * 64 + lane;
+  const int32_t r0 = (int32_t)blockIdx.y * chunk_rows, r1 = min(rows, r0 + chunk_rows);
+  MaxAcc acc;
+  if (col < ncols)
+    for (int32_t i = r0 + wave; i < r1; i += waves) acc.visit(entry(i, col), weight[i], i);
+  s_value[threadIdx.x] = acc.value;
+  s_row[threadIdx.x] = acc.row;
+  s_bad[threadIdx.x] = acc.bad;
+  __syncthreads();
+  if (wave == 0 && col < ncols) {
+    for (int w = 1; w < waves; ++w) acc.merge(s_value[w * 64 + lane], s_row[w * 64 + lane], s_bad[w * 64 + lane]);
+    const int64_t at = (int64_t)blockIdx.y * ncols + col;
+    pv[at] = acc.value;
+    pr[at] = acc.row;
+    pb[at] = acc.bad;
+  }
+}
+
+// Each column's chunks combined in chunk order
+__global__ void max_combine_kernel(int32_t ncols, int32_t chunks, const double* __restrict__ pv,
+                                   const int32_t* __restrict__ pr, const int32_t* __restrict__ pb,
+                                   double* __restrict__ best, int32_t* __restrict__ best_row,
+                                   int32_t* __restrict__ not_finite) {
+  const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= ncols) return;
+  MaxAcc acc;
+  for (int32_t k = 0; k < chunks; ++k) {
+    const int64_t at = (int64_t)k * ncols + c;
+    acc.merge(pv[at], pr[at], pb[at]);
+  }
+  best[c] = acc.value;
+  best_row[c] = acc.row;
+  not_finite[c] = acc.bad;
+}
+
+// Column-major blocks: one workgroup per column, threads over rows (each visiting its rows in
+// increasing order), combined in LDS with the smaller row winning a tie
+template <class Entry>
+__global__ void __launch_bounds__(256) max_columns_kernel(Entry entry, int32_t rows, const double* __restrict__ weight,
+                                                          double* __restrict__ best, int32_t* __restrict__ best_row,
+                                                          int32_t* __restrict__ not_finite) {
+  __shared__ double s_value[256];
+  __shared__ int32_t s_row[256], s_bad[256];
+  const int32_t col = (int32_t)blockIdx.x;
+  MaxAcc acc;
+  for (int32_t i = threadIdx.x; i < rows; i += blockDim.x) acc.visit(entry(i, col), weight[i], i);
+  s_value[threadIdx.x] = acc.value;
+  s_row[threadIdx.x] = acc.row;
+  s_bad[threadIdx.x] = acc.bad;
   __syncthreads();
   for (int stride = blockDim.x / 2; stride > 0; stride /= 2) {
     if ((int)threadIdx.x < stride) {
-      const double v = s_value[threadIdx.x + stride];
-      const int32_t r = s_row[threadIdx.x + stride];
-      const double mv = s_value[threadIdx.x];
-      const int32_t mr = s_row[threadIdx.x];
-      if (r >= 0 && (v > mv || (v == mv && (mr < 0 || r < mr)))) {
-        s_value[threadIdx.x] = v;
-        s_row[threadIdx.x] = r;
-      }
-      s_bad[threadIdx.x] |= s_bad[threadIdx.x + stride];
+      MaxAcc mine;
+      mine.value = s_value[threadIdx.x];
+      mine.row = s_row[threadIdx.x];
+      mine.bad = s_bad[threadIdx.x];
+      mine.merge(s_value[threadIdx.x + stride], s_row[threadIdx.x + stride], s_bad[threadIdx.x + stride]);
+      s_value[threadIdx.x] = mine.value;
+      s_row[threadIdx.x] = mine.row;
+      s_bad[threadIdx.x] = mine.bad;
     }
     __syncthreads();
   }
@@ -988,24 +1063,20 @@ __global__ void __launch_bounds__(256) column_max_cols_kernel(const double* __re
 
 // out(r, c) = (sum_p M[r][p] x(p, c), summed from 0 in index order) + offset[r]: the host's
 // M.mul_vec(theta) then + offset, term for term; then the zeroed entries. Entry (i, c) of x is
-// at i * rs + c * cs and of out at r * ors + c * ocs; consecutive threads take consecutive
-// entries of out.
-__global__ void operator_kernel(CsrDev m, const double* __restrict__ x, int64_t rs, int64_t cs,
-                                const double* __restrict__ offset, const int32_t* __restrict__ zero,
-                                double* __restrict__ out, int64_t ors, int64_t ocs, int32_t ncols) {
-  const int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (index >= (int64_t)m.rows * ncols) return;
-  int32_t r, c;
-  if (ors == 1) {
-    c = (int32_t)(index / m.rows);
-    r = (int32_t)(index % m.rows);
-  } else {
-    r = (int32_t)(index / ncols);
-    c = (int32_t)(index % ncols);
-  }
-  double sum = 0.0;
-  for (int32_t p = m.ptr[r]; p < m.ptr[r + 1]; ++p) sum = sum + m.val[p] * x[m.col[p] * rs + c * cs];
-  out[r * ors + c * ocs] = zero[c] == r ? 0.0 : sum + offset[r];
+// at i * rs + c * cs and of out at r * ors + c * ocs. Row-interleaved output (ocs = 1): lanes over
+// columns and a run of rows per thread; column-major output: threads over rows.
+__global__ void operator_rows_kernel(ProductEntry e, int32_t ncols, int32_t rows_per_block, double* __restrict__ out,
+                                     int64_t ors) {
+  const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= ncols) return;
+  const int32_t r0 = (int32_t)blockIdx.y * rows_per_block, r1 = min(e.m.rows, r0 + rows_per_block);
+  for (int32_t r = r0; r < r1; ++r) out[r * ors + c] = e(r, c);
+}
+
+__global__ void operator_columns_kernel(ProductEntry e, int32_t ncols, double* __restrict__ out, int64_t ocs) {
+  const int32_t r = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= e.m.rows) return;
+  for (int32_t c = (int32_t)blockIdx.y; c < ncols; c += (int32_t)gridDim.y) out[r + c * ocs] = e(r, c);
 }
 
 // ---- batches of factors: persistent column solves fed by a device queue ----
@@ -1217,27 +1288,71 @@ hipError_t launch_from_block(hipStream_t stream, const double* block, double* ve
   return hipGetLastError();
 }
 
+// Rows per thread in the row-interleaved elementwise kernels: enough that grid.y stays within
+// its limit, at least 16 so each thread streams a run of rows
+static int32_t rows_per_thread(int32_t rows) { return std::max<int32_t>(16, (rows + 65534) / 65535); }
+
 hipError_t launch_pair_update(hipStream_t stream, const double* y, double* out, int32_t n, int32_t ncols, int64_t rs,
                               int64_t cs, const double* base, const int32_t* plus, const int32_t* minus,
-                              const double* alpha, const double* inverse_weight, int32_t* singular) {
-  const int64_t total = (int64_t)n * ncols;
-  const unsigned grid = (unsigned)((total + 255) / 256);
-  hipLaunchKernelGGL(pair_update_kernel, dim3(grid), dim3(256), 0, stream, y, out, n, ncols, rs, cs, base, plus,
-                     minus, alpha, inverse_weight, singular);
+                              const double* alpha, const double* inverse_weight, int32_t* singular, double* coef) {
+  if (n <= 0 || ncols <= 0) return hipSuccess;
+  if (cs != 1 && rs != 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pair_coefficient_kernel, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, stream, y, ncols,
+                     rs, cs, base, plus, minus, alpha, inverse_weight, coef, singular);
+  if (cs == 1) {
+    const int32_t per = rows_per_thread(n);
+    const dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)((n + per - 1) / per));
+    hipLaunchKernelGGL(pair_apply_rows_kernel, grid, dim3(256), 0, stream, y, out, n, ncols, rs, per, base, alpha,
+                       coef, singular);
+  } else {
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)std::min<int32_t>(ncols, 65535));
+    hipLaunchKernelGGL(pair_apply_columns_kernel, grid, dim3(256), 0, stream, y, out, n, ncols, cs, base, alpha, coef,
+                       singular);
+  }
+  return hipGetLastError();
+}
+
+int32_t max_chunks(int32_t rows, int32_t ncols, int cus) {
+  const int64_t groups = (ncols + 63) / 64;
+  const int64_t want = (8 * (int64_t)cus + groups - 1) / groups;
+  return (int32_t)std::clamp<int64_t>(want, 1, std::max<int64_t>(1, (rows + 255) / 256));
+}
+
+template <class Entry>
+static hipError_t launch_max(hipStream_t stream, const Entry& e, int32_t rows, int32_t ncols, bool column_major,
+                             const double* weight, int32_t chunks, double* pv, int32_t* pr, int32_t* pb, double* best,
+                             int32_t* best_row, int32_t* not_finite) {
+  if (ncols <= 0) return hipSuccess;
+  if (column_major) {
+    hipLaunchKernelGGL((max_columns_kernel<Entry>), dim3((unsigned)ncols), dim3(256), 0, stream, e, rows, weight, best,
+                       best_row, not_finite);
+    return hipGetLastError();
+  }
+  if (chunks < 1) return hipErrorInvalidValue;
+  const int32_t chunk_rows = (rows + chunks - 1) / chunks;
+  const dim3 grid((unsigned)((ncols + 63) / 64), (unsigned)chunks);
+  hipLaunchKernelGGL((max_rows_kernel<Entry>), grid, dim3(256), 0, stream, e, rows, ncols, chunk_rows, weight, pv, pr,
+                     pb);
+  hipLaunchKernelGGL(max_combine_kernel, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, stream, ncols, chunks, pv,
+                     pr, pb, best, best_row, not_finite);
   return hipGetLastError();
 }
 
 hipError_t launch_column_max(hipStream_t stream, const double* block, int32_t rows, int32_t ncols, int64_t rs,
-                             int64_t cs, const double* weight, double* best, int32_t* best_row, int32_t* not_finite) {
-  if (rs == 1) {
-    hipLaunchKernelGGL(column_max_cols_kernel, dim3(ncols), dim3(256), 0, stream, block, rows, cs, weight, best,
-                       best_row, not_finite);
-    return hipGetLastError();
-  }
-  const unsigned grid = (unsigned)((ncols + 63) / 64);
-  hipLaunchKernelGGL(column_max_kernel, dim3(grid), dim3(256), 0, stream, block, rows, ncols, rs, cs, weight, best,
-                     best_row, not_finite);
-  return hipGetLastError();
+                             int64_t cs, const double* weight, int32_t chunks, double* pv, int32_t* pr, int32_t* pb,
+                             double* best, int32_t* best_row, int32_t* not_finite) {
+  if (rs != 1 && cs != 1) return hipErrorInvalidValue;
+  return launch_max(stream, BlockEntry{block, rs, cs}, rows, ncols, rs == 1, weight, chunks, pv, pr, pb, best, best_row,
+                    not_finite);
+}
+
+hipError_t launch_product_max(hipStream_t stream, const CsrDev& m, const double* x, int64_t rs, int64_t cs,
+                              const double* offset, const int32_t* zero, const double* weight, int32_t ncols,
+                              int32_t chunks, double* pv, int32_t* pr, int32_t* pb, double* best, int32_t* best_row,
+                              int32_t* not_finite) {
+  if (rs != 1 && cs != 1) return hipErrorInvalidValue;
+  return launch_max(stream, ProductEntry{m, x, rs, cs, offset, zero}, m.rows, ncols, rs == 1, weight, chunks, pv, pr,
+                    pb, best, best_row, not_finite);
 }
 
 hipError_t prepare_batch(size_t lds_bytes) {
@@ -1292,10 +1407,18 @@ hipError_t launch_scatter_negated(hipStream_t stream, const double* acc, const i
 hipError_t launch_operator(hipStream_t stream, const CsrDev& m, const double* x, int64_t rs, int64_t cs,
                            const double* offset, const int32_t* zero, double* out, int64_t ors, int64_t ocs,
                            int32_t ncols) {
-  const int64_t total = (int64_t)m.rows * ncols;
-  const unsigned grid = (unsigned)((total + 255) / 256);
-  hipLaunchKernelGGL(operator_kernel, dim3(grid), dim3(256), 0, stream, m, x, rs, cs, offset, zero, out, ors, ocs,
-                     ncols);
+  if (m.rows <= 0 || ncols <= 0) return hipSuccess;
+  const ProductEntry e{m, x, rs, cs, offset, zero};
+  if (ocs == 1) {
+    const int32_t per = rows_per_thread(m.rows);
+    const dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)((m.rows + per - 1) / per));
+    hipLaunchKernelGGL(operator_rows_kernel, grid, dim3(256), 0, stream, e, ncols, per, out, ors);
+  } else if (ors == 1) {
+    const dim3 grid((unsigned)((m.rows + 255) / 256), (unsigned)std::min<int32_t>(ncols, 65535));
+    hipLaunchKernelGGL(operator_columns_kernel, grid, dim3(256), 0, stream, e, ncols, out, ocs);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -46,6 +46,7 @@ struct SweepDev {
 struct ComplexSweepDev {
   int32_t n = 0, levels = 0;
   const int32_t* level_ptr = nullptr;
+  const int4* head = nullptr;  // per position: dst, src, first term, end of terms
   const int32_t* src = nullptr;
   const int32_t* dst = nullptr;
   const int32_t* row_ptr = nullptr;
@@ -153,15 +154,29 @@ hipError_t launch_from_block(hipStream_t stream, const double* block, double* ve
 // Blocks below are addressed with strides: entry (i, c) at i * rs + c * cs (row-interleaved:
 // rs = ld, cs = 1; column-major: rs = 1, cs = ld).
 
-// pair_update (ybus_cuda/src/kernels.cu:24-58) on a solved block y (by unknown).
+// pair_update (ybus_cuda/src/kernels.cu:24-58) on a solved block y (by unknown), row-interleaved
+// or column-major; coef holds one double per column (the update's coefficient), and singular gets
+// 1 or 0 for every column.
 hipError_t launch_pair_update(hipStream_t stream, const double* y, double* out, int32_t n, int32_t ncols, int64_t rs,
                               int64_t cs, const double* base, const int32_t* plus, const int32_t* minus,
-                              const double* alpha, const double* inverse_weight, int32_t* singular);
+                              const double* alpha, const double* inverse_weight, int32_t* singular, double* coef);
 
 // column_max (kernels.cu:78-128): per column, the largest |v| * weight[row] (smallest row on a
-// tie, -1 if every product is zero) and whether any value isn't finite.
+// tie, -1 if every product is zero) and whether any value isn't finite. A row-interleaved block
+// is split into `chunks` runs of rows whose results (pv, pr, pb: chunks x ncols each) are then
+// combined; max_chunks gives the count that fills the device. A column-major block (rs = 1)
+// takes one workgroup per column and no partials.
+int32_t max_chunks(int32_t rows, int32_t ncols, int cus);
 hipError_t launch_column_max(hipStream_t stream, const double* block, int32_t rows, int32_t ncols, int64_t rs,
-                             int64_t cs, const double* weight, double* best, int32_t* best_row, int32_t* not_finite);
+                             int64_t cs, const double* weight, int32_t chunks, double* pv, int32_t* pr, int32_t* pb,
+                             double* best, int32_t* best_row, int32_t* not_finite);
+
+// The same maxima over M x + offset with row zero[c] of column c zeroed (the operator's product,
+// pairs.rs:312-366), computed on the fly rather than stored.
+hipError_t launch_product_max(hipStream_t stream, const CsrDev& m, const double* x, int64_t rs, int64_t cs,
+                              const double* offset, const int32_t* zero, const double* weight, int32_t ncols,
+                              int32_t chunks, double* pv, int32_t* pr, int32_t* pb, double* best, int32_t* best_row,
+                              int32_t* not_finite);
 
 // out = M x + offset for each column, then out(zero[c], c) = 0 where zero[c] >= 0; out's entry
 // (r, c) at r * ors + c * ocs.

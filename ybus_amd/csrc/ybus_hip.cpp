@@ -9,11 +9,13 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <initializer_list>
 #include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "analysis.hpp"
@@ -198,6 +200,7 @@ struct DevSweep {
 // as (re, im) pairs
 struct DevSweepC {
   DevBuf<int32_t> level_ptr, src, dst, row_ptr, dep;
+  DevBuf<int4> head;
   DevBuf<double2> val, diag;
   int32_t n = 0, levels = 0;
   bool unit = false;
@@ -208,6 +211,9 @@ struct DevSweepC {
     std::vector<double2> v(s.val.size()), d(unit ? 0 : (size_t)n);
     for (size_t p = 0; p < v.size(); ++p) v[p] = make_double2(s.val[p], s.val_im[p]);
     for (size_t k = 0; k < d.size(); ++k) d[k] = make_double2(s.diag[k], s.diag_im[k]);
+    std::vector<int4> h(n);
+    for (int32_t k = 0; k < n; ++k) h[k] = make_int4(s.dst[k], s.src[k], s.row_ptr[k], s.row_ptr[k + 1]);
+    head.upload(h, stream);
     level_ptr.upload(s.level_ptr, stream);
     src.upload(s.src, stream);
     dst.upload(s.dst, stream);
@@ -221,6 +227,7 @@ struct DevSweepC {
     c.n = n;
     c.levels = levels;
     c.level_ptr = level_ptr.p;
+    c.head = head.p;
     c.src = src.p;
     c.dst = dst.p;
     c.row_ptr = row_ptr.p;
@@ -247,6 +254,12 @@ constexpr uint64_t SINGLE_ROW_WIDTH = 64;
 constexpr int64_t INDEX_LIMIT = 0x7fffffff;
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// Host threads for the library's own host work: the machine's, at most 16
+int64_t host_threads() {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return std::clamp<int64_t>(hw ? hw : 1, 1, 16);
+}
 
 double seconds_since(std::chrono::steady_clock::time_point t) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
@@ -329,6 +342,9 @@ struct ybus_hip_factor {
   DevBuf<int32_t> plus, minus, isteps, singular;
   DevBuf<double> dratio;
   DevBuf<double> abar;  // the adjoint's running sums, one per stored entry of the matrix
+  // pair updates' coefficients, and the maxima's partial and final results (per column)
+  DevBuf<double> coef, max_pv, max_best, ones;
+  DevBuf<int32_t> max_pr, max_pb, max_row, max_bad;
   // the staged pair block's reach, per tile of `reach_tile` columns (0: none staged), and the
   // host index it is computed from (built on first use)
   bool use_reach = true;
@@ -461,27 +477,57 @@ struct ybus_hip_factor {
     }
     const int64_t k = (int64_t)p.size();
     const int64_t tiles = (k + tile - 1) / tile;
-    std::vector<int32_t> stamp(rows, -1), queue, pos;
-    std::vector<int32_t> lp((size_t)tiles * (levels + 1));
-    queue.reserve(rows);
-    for (int64_t t = 0; t < tiles; ++t) {
-      queue.clear();
-      auto visit = [&](int32_t r) {
-        if (stamp[r] != (int32_t)t) {
-          stamp[r] = (int32_t)t;
-          queue.push_back(r);
+    // each tile's reach, sorted by position, found by workers that each take a run of tiles
+    std::vector<std::vector<int32_t>> reach(tiles);
+    auto work = [&](int64_t t0, int64_t t1) {
+      std::vector<int32_t> stamp(rows, -1), queue;
+      queue.reserve(rows);
+      for (int64_t t = t0; t < t1; ++t) {
+        queue.clear();
+        auto visit = [&](int32_t r) {
+          if (stamp[r] != (int32_t)t) {
+            stamp[r] = (int32_t)t;
+            queue.push_back(r);
+          }
+        };
+        for (int64_t c = t * tile; c < std::min<int64_t>(k, (t + 1) * tile); ++c) {
+          if (p[c] >= 0) visit(row_of_src[p[c]]);
+          if (m[c] >= 0) visit(row_of_src[m[c]]);
         }
-      };
-      for (int64_t c = t * tile; c < std::min<int64_t>(k, (t + 1) * tile); ++c) {
-        if (p[c] >= 0) visit(row_of_src[p[c]]);
-        if (m[c] >= 0) visit(row_of_src[m[c]]);
+        for (size_t qi = 0; qi < queue.size(); ++qi)
+          for (int32_t e = dep_ptr[queue[qi]]; e < dep_ptr[queue[qi] + 1]; ++e) visit(dependents[e]);
+        std::vector<int32_t>& out = reach[t];
+        out.resize(queue.size());
+        for (size_t qi = 0; qi < queue.size(); ++qi) out[qi] = pos_of_row[queue[qi]];
+        std::sort(out.begin(), out.end());
       }
-      for (size_t qi = 0; qi < queue.size(); ++qi)
-        for (int32_t e = dep_ptr[queue[qi]]; e < dep_ptr[queue[qi] + 1]; ++e) visit(dependents[e]);
+    };
+    const int64_t workers = std::clamp<int64_t>(tiles / 8, 1, host_threads());
+    if (workers == 1) {
+      work(0, tiles);
+    } else {
+      std::vector<std::thread> pool;
+      std::vector<std::exception_ptr> errors(workers);
+      for (int64_t w = 0; w < workers; ++w)
+        pool.emplace_back([&, w] {
+          try {
+            work(tiles * w / workers, tiles * (w + 1) / workers);
+          } catch (...) {
+            errors[w] = std::current_exception();
+          }
+        });
+      for (std::thread& t : pool) t.join();
+      for (std::exception_ptr& e : errors)
+        if (e) std::rethrow_exception(e);
+    }
+    size_t total = 0;
+    for (const std::vector<int32_t>& r : reach) total += r.size();
+    if (total > (size_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the pair block's reach is too large");
+    std::vector<int32_t> pos, lp((size_t)tiles * (levels + 1));
+    pos.reserve(total);
+    for (int64_t t = 0; t < tiles; ++t) {
       const size_t base = pos.size();
-      for (int32_t r : queue) pos.push_back(pos_of_row[r]);
-      std::sort(pos.begin() + base, pos.end());
-      if (pos.size() > (size_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the pair block's reach is too large");
+      pos.insert(pos.end(), reach[t].begin(), reach[t].end());
       for (int32_t l = 0; l <= levels; ++l)
         lp[(size_t)t * (levels + 1) + l] =
             (int32_t)(std::lower_bound(pos.begin() + base, pos.end(), sw.level_ptr[l]) - pos.begin());
@@ -601,8 +647,8 @@ struct ybus_hip_block {
 struct ybus_hip_operator {
   ybus_hip_factor* f = nullptr;
   DevCsr m;
-  DevBuf<double> offset, weight, out, best;
-  DevBuf<int32_t> zero, best_row, not_finite;
+  DevBuf<double> offset, weight, out;
+  DevBuf<int32_t> zero;
   int64_t out_cols = 0;
 };
 
@@ -1563,9 +1609,10 @@ int32_t ybus_hip_factor_pair_updates(ybus_hip_factor* f, const double* base, con
       dbase.upload(std::vector<double>(base, base + f->n), f->stream());
       dalpha.upload(std::vector<double>(alpha, alpha + k), f->stream());
       dweight.upload(std::vector<double>(inverse_weight, inverse_weight + k), f->stream());
-      check(hipMemsetAsync(f->singular.p, 0, k * sizeof(int32_t), f->stream()), "memset");
+      f->coef.reserve(k);
       check(launch_pair_update(f->stream(), f->x.p, f->y2.p, (int32_t)f->n, (int32_t)k, f->layout.rs(),
-                               f->layout.cs(), dbase.p, f->plus.p, f->minus.p, dalpha.p, dweight.p, f->singular.p),
+                               f->layout.cs(), dbase.p, f->plus.p, f->minus.p, dalpha.p, dweight.p, f->singular.p,
+                               f->coef.p),
             "pair_update");
       std::vector<int32_t> s(k);
       check(hipMemcpyAsync(s.data(), f->singular.p, k * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
@@ -1609,22 +1656,38 @@ int32_t ybus_hip_block_solutions(ybus_hip_block* b, double* out) {
   });
 }
 
-static void maxima_of(ybus_hip_factor* f, const double* block, int32_t rows, int64_t rs, int64_t cs,
-                      const double* weight, int64_t columns, ybus_hip_column_max* out) {
-  DevBuf<double> best;
-  DevBuf<int32_t> row, bad;
-  best.reserve(columns);
-  row.reserve(columns);
-  bad.reserve(columns);
-  check(launch_column_max(f->stream(), block, rows, (int32_t)columns, rs, cs, weight, best.p, row.p, bad.p),
-        "column_max");
+// The factor's buffers for the maxima of `columns` columns over `rows` rows: the partial results
+// of a row-interleaved block's chunks, and the final ones; returns the chunk count
+static int32_t reserve_maxima(ybus_hip_factor* f, int32_t rows, int64_t columns) {
+  const int32_t chunks = max_chunks(rows, (int32_t)columns, f->ctx->cus);
+  const size_t partial = (size_t)chunks * (size_t)columns;
+  f->max_pv.reserve(partial);
+  f->max_pr.reserve(partial);
+  f->max_pb.reserve(partial);
+  f->max_best.reserve(columns);
+  f->max_row.reserve(columns);
+  f->max_bad.reserve(columns);
+  return chunks;
+}
+
+// Copies the final maxima of `columns` columns to the host
+static void read_maxima(ybus_hip_factor* f, int64_t columns, ybus_hip_column_max* out) {
   std::vector<double> hb(columns);
   std::vector<int32_t> hr(columns), hn(columns);
-  check(hipMemcpyAsync(hb.data(), best.p, columns * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
-  check(hipMemcpyAsync(hr.data(), row.p, columns * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
-  check(hipMemcpyAsync(hn.data(), bad.p, columns * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  check(hipMemcpyAsync(hb.data(), f->max_best.p, columns * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  check(hipMemcpyAsync(hr.data(), f->max_row.p, columns * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
+  check(hipMemcpyAsync(hn.data(), f->max_bad.p, columns * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
   check(hipStreamSynchronize(f->stream()), "column_max");
   for (int64_t c = 0; c < columns; ++c) out[c] = ybus_hip_column_max{hr[c], hb[c], hn[c]};
+}
+
+static void maxima_of(ybus_hip_factor* f, const double* block, int32_t rows, int64_t rs, int64_t cs,
+                      const double* weight, int64_t columns, ybus_hip_column_max* out) {
+  const int32_t chunks = reserve_maxima(f, rows, columns);
+  check(launch_column_max(f->stream(), block, rows, (int32_t)columns, rs, cs, weight, chunks, f->max_pv.p,
+                          f->max_pr.p, f->max_pb.p, f->max_best.p, f->max_row.p, f->max_bad.p),
+        "column_max");
+  read_maxima(f, columns, out);
 }
 
 int32_t ybus_hip_block_maxima(ybus_hip_block* b, ybus_hip_column_max* out) {
@@ -1637,15 +1700,13 @@ int32_t ybus_hip_block_maxima(ybus_hip_block* b, ybus_hip_column_max* out) {
       return;
     }
     check(hipSetDevice(f->ctx->device), "hipSetDevice");
-    DevBuf<double> ones;
-    ones.upload(std::vector<double>(f->n, 1.0), f->stream());
-    maxima_of(f, b->data, (int32_t)f->n, b->layout.rs(), b->layout.cs(), ones.p, b->columns, out);
+    if (f->ones.len < (size_t)f->n) f->ones.upload(std::vector<double>(f->n, 1.0), f->stream());
+    maxima_of(f, b->data, (int32_t)f->n, b->layout.rs(), b->layout.cs(), f->ones.p, b->columns, out);
   });
 }
 
-// pairs.rs:312-366: M theta + offset per column, then the zeroed entries, left in op->out in the
-// block's layout (column-major blocks: one vector of rows per column; wide: rows x columns)
-static void apply(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero) {
+// Checks each column's row to zero (-1 for none) and copies them to the device
+static void stage_zero(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero) {
   ybus_hip_factor* f = b->f;
   if (op->f != f) fail(YBUS_HIP_INVALID_ARGUMENT, "an operator of this factorization");
   if (b->columns && !zero) fail(YBUS_HIP_INVALID_ARGUMENT, "one entry to zero (or none) per column");
@@ -1657,6 +1718,13 @@ static void apply(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero)
   }
   check(hipSetDevice(f->ctx->device), "hipSetDevice");
   op->zero.upload(z, f->stream());
+}
+
+// pairs.rs:312-366: M theta + offset per column, then the zeroed entries, left in op->out in the
+// block's layout (column-major blocks: one vector of rows per column; wide: rows x columns)
+static void apply(ybus_hip_block* b, ybus_hip_operator* op, const int64_t* zero) {
+  ybus_hip_factor* f = b->f;
+  stage_zero(b, op, zero);
   op->out.reserve((size_t)std::max<int32_t>(op->m.rows, 1) * (size_t)std::max<int64_t>(b->columns, 1));
   op->out_cols = b->columns;
   if (b->columns && op->m.rows) {
@@ -1696,17 +1764,21 @@ int32_t ybus_hip_block_product_maxima(ybus_hip_block* b, ybus_hip_operator* op, 
                                       ybus_hip_column_max* out) {
   return guard([&] {
     if (!b || !op) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
-    apply(b, op, zero);
+    stage_zero(b, op, zero);
     if (!b->columns) return;
     if (!out) fail(YBUS_HIP_INVALID_ARGUMENT, "out is null");
     if (!op->m.rows) {
       for (int64_t c = 0; c < b->columns; ++c) out[c] = ybus_hip_column_max{-1, 0.0, 0};
       return;
     }
-    if (b->layout.columns)
-      maxima_of(b->f, op->out.p, op->m.rows, 1, op->m.rows, op->weight.p, b->columns, out);
-    else
-      maxima_of(b->f, op->out.p, op->m.rows, b->columns, 1, op->weight.p, b->columns, out);
+    // the product's maxima computed as it is formed, without storing it
+    ybus_hip_factor* f = b->f;
+    const int32_t chunks = reserve_maxima(f, op->m.rows, b->columns);
+    check(launch_product_max(f->stream(), op->m.view(), b->data, b->layout.rs(), b->layout.cs(), op->offset.p,
+                             op->zero.p, op->weight.p, (int32_t)b->columns, chunks, f->max_pv.p, f->max_pr.p,
+                             f->max_pb.p, f->max_best.p, f->max_row.p, f->max_bad.p),
+          "product_max");
+    read_maxima(f, b->columns, out);
   });
 }
 
