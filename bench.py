@@ -502,6 +502,9 @@ def run_screening(gpu, args):
         f"{time.perf_counter() - t0:.1f} s")
     factor = Y.GpuFactor.from_cholesky(gpu, chol)
     factor.set_block_width(len(outages))
+    tuning = factor.autotune(len(outages), [Y.Pair(None if a < 0 else int(a), None if b < 0 else int(b))
+                                            for a, b in zip(o_plus, o_minus)])
+    log(f"[screening] autotuned: {tuning}")
     op = factor.operator(bf)
     op.set_offset(np.zeros(bf.rows))
     op.set_weights(weights)
@@ -546,7 +549,8 @@ def run_screening(gpu, args):
                                    "the update, flows and maxima in numpy, repeated for >= %g s" % CPU_SECONDS)
     out = dict(workload="N-1 screening: 25,000-bus grid, B' AMD Cholesky, every non-bridge branch out alone "
                         "(%d outages), rank-one update, Bf flows, most loaded branch" % len(outages),
-               n=int(chol.n), nnz_l=int(chol.l.nnz), outages=int(len(outages)), value=len(outages) / step_s,
+               n=int(chol.n), nnz_l=int(chol.l.nnz), outages=int(len(outages)), tuning=tuning,
+               value=len(outages) / step_s,
                unit="outages/s", step_ms=step_s * 1e3, singular=int(sum(singular)),
                parity=dict(sampled_outages=int(len(sample)), max_rel_diff_loading=worst,
                            same_branch=f"{same_branch} of {len(sample)}"),

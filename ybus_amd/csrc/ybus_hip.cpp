@@ -343,7 +343,7 @@ struct ybus_hip_factor {
   DevBuf<double> dratio;
   DevBuf<double> abar;  // the adjoint's running sums, one per stored entry of the matrix
   // pair updates' coefficients, and the maxima's partial and final results (per column)
-  DevBuf<double> coef, max_pv, max_best, ones;
+  DevBuf<double> coef, max_pv, max_best, ones, upd_base, upd_alpha, upd_weight;
   DevBuf<int32_t> max_pr, max_pb, max_row, max_bad;
   // the staged pair block's reach, per tile of `reach_tile` columns (0: none staged), and the
   // host index it is computed from (built on first use)
@@ -1605,14 +1605,17 @@ int32_t ybus_hip_factor_pair_updates(ybus_hip_factor* f, const double* base, con
     if (k && f->n) f->solve64(false, true, (int64_t)k);
     auto block = new_block(f, (int64_t)k, f->y2.p);
     if (k && f->n) {
-      DevBuf<double> dbase, dalpha, dweight;
-      dbase.upload(std::vector<double>(base, base + f->n), f->stream());
-      dalpha.upload(std::vector<double>(alpha, alpha + k), f->stream());
-      dweight.upload(std::vector<double>(inverse_weight, inverse_weight + k), f->stream());
+      f->upd_base.reserve(f->n);
+      f->upd_alpha.reserve(k);
+      f->upd_weight.reserve(k);
       f->coef.reserve(k);
+      check(hipMemcpyAsync(f->upd_base.p, base, f->n * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+      check(hipMemcpyAsync(f->upd_alpha.p, alpha, k * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+      check(hipMemcpyAsync(f->upd_weight.p, inverse_weight, k * sizeof(double), hipMemcpyHostToDevice, f->stream()),
+            "H2D");
       check(launch_pair_update(f->stream(), f->x.p, f->y2.p, (int32_t)f->n, (int32_t)k, f->layout.rs(),
-                               f->layout.cs(), dbase.p, f->plus.p, f->minus.p, dalpha.p, dweight.p, f->singular.p,
-                               f->coef.p),
+                               f->layout.cs(), f->upd_base.p, f->plus.p, f->minus.p, f->upd_alpha.p, f->upd_weight.p,
+                               f->singular.p, f->coef.p),
             "pair_update");
       std::vector<int32_t> s(k);
       check(hipMemcpyAsync(s.data(), f->singular.p, k * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
