@@ -270,6 +270,8 @@ def run_adjoint(gpu, args):
     factor = Y.GpuFactor.from_cholesky(gpu, chol)
     factor.set_matrix(bprime)
     factor.set_block_width(64)
+    tuning = factor.autotune(64)  # the 64-scenario block's kernel configuration
+    log(f"[adjoint] autotuned: {tuning}")
     # Bf has at most two entries per row: f = v0 * theta[c0] + v1 * theta[c1]
     counts = np.diff(bf.ptr)
     c0 = np.where(counts > 0, bf.idx[np.minimum(bf.ptr[:-1], len(bf.idx) - 1)], 0)
@@ -328,7 +330,7 @@ def run_adjoint(gpu, args):
                         "relu^2 flow-limit loss" % (chol.n, int(chol.l.nnz), bprime.nnz),
                loss=float(loss.detach()), ms_per_step=seconds * 1e3,
                library_kernel_ms_per_step=stats.milliseconds / reps,
-               library_kernel_launches_per_step=stats.launches // reps,
+               library_kernel_launches_per_step=stats.launches // reps, tuning=tuning,
                theta_bit_exact=bool(theta_h.tobytes() == chol.solve_many(p).tobytes()),
                b_bar_bit_exact=bool(b_grad.cpu().numpy().tobytes() == want_b.tobytes()),
                a_bar_bit_exact=bool(a_grad.cpu().numpy().tobytes() == want_a.tobytes()),

@@ -290,17 +290,21 @@ enum { YBUS_HIP_PATH_COLUMNS = 0, YBUS_HIP_PATH_WIDE = 1 };
 int32_t ybus_hip_factor_path(const ybus_hip_factor *f, uint64_t columns, int32_t pairs);
 
 /* The kernel configuration a block width takes: path (YBUS_HIP_PATH_*), workgroup size, column
- * tile width (1 on the column path), and the time of the synthetic block it was chosen on. */
+ * tile width (1 on the column path), the time of the synthetic block it was chosen on, and
+ * whether the wide path sweeps the factor's partitioned form (1: a Cholesky factor's independent
+ * elimination subtrees side by side, then the top of the tree; 0: one schedule). */
 typedef struct {
   int32_t path;
   int32_t threads;
   int32_t tile;
   double milliseconds;
+  int32_t parts;
 } ybus_hip_tuning;
 
 /* Autotuner (new): times every kernel configuration a block of `columns` right-hand sides can
  * take with this factor (the column path at 256 and 512 threads where the factor fits; the wide
- * path at 256, 512 and 1024 threads with column tiles of 1 to 64), each on a synthetic FP64 block
+ * path at 256, 512 and 1024 threads with column tiles of 1 to 64, over the factor's single and,
+ * where it has one, partitioned schedule), each on a synthetic FP64 block
  * of that width, and keeps the fastest for this factor's later blocks of that width (up to the
  * next power of two) and kind: pair blocks (pairs != 0; pair_solves, pair_updates, staged pairs)
  * or blocks of vectors (solve_many, staged vectors, refinement). A pair block is timed on the

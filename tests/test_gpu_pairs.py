@@ -275,3 +275,43 @@ def test_maxima_split_over_row_chunks_keep_the_first_of_tied_rows(gpu):
 def _pairs_arrays(ps):
     return (np.array([-1 if p.plus is None else p.plus for p in ps]),
             np.array([-1 if p.minus is None else p.minus for p in ps]))
+
+
+@pytest.mark.parametrize("kind", ["pairs", "vectors"])
+def test_partitioned_sweeps_give_the_same_bits(gpu, monkeypatch, kind):
+    """A Cholesky factor's elimination tree cut into independent subtrees, swept side by side and
+    then the top of the tree: every row keeps its terms and their order, so the solutions are the
+    one-schedule solve's bits, for pair blocks (with and without the reach) and vector blocks."""
+    g = O.make_grid(3000, 777)
+    bp, _, _ = g.dc()
+    cpu = O.Cholesky(bp)
+    n = bp.rows
+    rng = np.random.default_rng(8)
+    k = 300
+    if kind == "pairs":
+        plus = rng.integers(-1, n, size=k)
+        minus = rng.integers(-1, n, size=k)
+        minus[minus == plus] = -1
+        pairs = [Y.Pair(None if p < 0 else int(p), None if m < 0 else int(m)) for p, m in zip(plus, minus)]
+        dense = np.zeros((k, n))
+        for c in range(k):
+            if plus[c] >= 0:
+                dense[c, plus[c]] = 1.0
+            if minus[c] >= 0:
+                dense[c, minus[c]] = -1.0
+    else:
+        dense = rng.standard_normal((k, n))
+    want = cpu.solve_many(dense, threads=4)
+    for parts, reach in [("8", "1"), ("8", "0"), ("33", "1"), ("0", "1")]:
+        monkeypatch.setenv("YBUS_HIP_PARTS", parts)
+        monkeypatch.setenv("YBUS_HIP_REACH", reach)
+        for tile in ["1", "8", "64"]:
+            monkeypatch.setenv("YBUS_HIP_TILE", tile)
+            factor = Y.GpuFactor.from_cholesky(gpu, cpu)
+            factor.set_block_width(k)
+            if kind == "pairs":
+                with factor.pair_solves(pairs) as block:
+                    got = block.solutions()
+            else:
+                got = factor.solve_many(dense)
+            assert got.tobytes() == want.tobytes(), (parts, reach, tile)

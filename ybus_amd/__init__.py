@@ -86,7 +86,8 @@ assert _COLUMN_MAX.itemsize == C.sizeof(_ColumnMax)
 
 
 class _Tuning(C.Structure):
-    _fields_ = [("path", C.c_int32), ("threads", C.c_int32), ("tile", C.c_int32), ("milliseconds", C.c_double)]
+    _fields_ = [("path", C.c_int32), ("threads", C.c_int32), ("tile", C.c_int32), ("milliseconds", C.c_double),
+                ("parts", C.c_int32)]
 
 
 class _KernelStats(C.Structure):
@@ -447,7 +448,7 @@ class GpuFactor:
         """Times the kernel configurations for blocks of `columns` right-hand sides and keeps the
         fastest for this factor's blocks of that width and kind: pair blocks when `pairs` (True
         for synthetic pairs, or the caller's own list of Pair to time on), else blocks of vectors.
-        Returns {'path', 'threads', 'tile', 'milliseconds'}."""
+        Returns {'path', 'threads', 'tile', 'parts', 'milliseconds'}."""
         self._release_block()
         t = _Tuning()
         plus = minus = None
@@ -458,7 +459,7 @@ class GpuFactor:
                                                   plus.ctypes.data if plus is not None else None,
                                                   minus.ctypes.data if minus is not None else None, C.byref(t)))
         return dict(path="wide" if t.path == 1 else "columns", threads=int(t.threads), tile=int(t.tile),
-                    milliseconds=float(t.milliseconds))
+                    parts=bool(t.parts), milliseconds=float(t.milliseconds))
 
     def kernel_stats(self) -> KernelStats:
         s = _KernelStats()

@@ -21,6 +21,34 @@ std::vector<int32_t> levels_of(const RowForm& form) {
   return level;
 }
 
+// Fills s's per-position arrays from `form`, position k holding row order[k]
+static void emit_rows(const RowForm& form, const std::vector<int64_t>& src_map, const std::vector<int64_t>& dst_map,
+                      const std::vector<int64_t>& order, Sweep& s) {
+  const int64_t n = form.n;
+  const bool complex = !form.val_im.empty() || !form.diag_im.empty();
+  s.src.resize(n);
+  s.dst.resize(n);
+  s.row_ptr.assign(n + 1, 0);
+  s.dep.reserve(form.dep.size());
+  s.val.reserve(form.val.size());
+  if (complex) s.val_im.reserve(form.val.size());
+  if (!form.unit) s.diag.resize(n);
+  if (!form.unit && complex) s.diag_im.resize(n);
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t i = order[k];
+    s.src[k] = (int32_t)src_map[i];
+    s.dst[k] = (int32_t)dst_map[i];
+    for (int64_t p = form.ptr[i]; p < form.ptr[i + 1]; ++p) {
+      s.dep.push_back((int32_t)dst_map[form.dep[p]]);
+      s.val.push_back(form.val[p]);
+      if (complex) s.val_im.push_back(form.val_im[p]);
+    }
+    s.row_ptr[k + 1] = (int32_t)s.dep.size();
+    if (!form.unit) s.diag[k] = form.diag[i];
+    if (!form.unit && complex) s.diag_im[k] = form.diag_im[i];
+  }
+}
+
 Sweep schedule(const RowForm& form, const std::vector<int64_t>& src_map, const std::vector<int64_t>& dst_map) {
   const int64_t n = form.n;
   const std::vector<int32_t> level = levels_of(form);
@@ -38,28 +66,140 @@ Sweep schedule(const RowForm& form, const std::vector<int64_t>& src_map, const s
   std::vector<int32_t> cursor(s.level_ptr.begin(), s.level_ptr.end() - (levels ? 1 : 0));
   std::vector<int64_t> order(n);
   for (int64_t i = 0; i < n; ++i) order[cursor[level[i]]++] = i;  // ascending rows within a level
-  s.src.resize(n);
-  s.dst.resize(n);
-  s.row_ptr.assign(n + 1, 0);
-  s.dep.reserve(form.dep.size());
-  s.val.reserve(form.val.size());
-  const bool complex = !form.val_im.empty() || !form.diag_im.empty();
-  if (complex) s.val_im.reserve(form.val.size());
-  if (!form.unit) s.diag.resize(n);
-  if (!form.unit && complex) s.diag_im.resize(n);
-  for (int64_t k = 0; k < n; ++k) {
-    const int64_t i = order[k];
-    s.src[k] = (int32_t)src_map[i];
-    s.dst[k] = (int32_t)dst_map[i];
-    for (int64_t p = form.ptr[i]; p < form.ptr[i + 1]; ++p) {
-      s.dep.push_back((int32_t)dst_map[form.dep[p]]);
-      s.val.push_back(form.val[p]);
-      if (complex) s.val_im.push_back(form.val_im[p]);
-    }
-    s.row_ptr[k + 1] = (int32_t)s.dep.size();
-    if (!form.unit) s.diag[k] = form.diag[i];
-    if (!form.unit && complex) s.diag_im[k] = form.diag_im[i];
+  emit_rows(form, src_map, dst_map, order, s);
+  return s;
+}
+
+std::vector<int32_t> tree_parts(const RowForm& lower, int32_t target, int32_t& parts) {
+  parts = 0;
+  const int64_t n = lower.n;
+  if (!lower.lower || n < 2 || target < 2) return {};
+  // dependents of each row, and each row's first (smallest) dependent
+  std::vector<int64_t> dptr(n + 1, 0);
+  for (int64_t p = 0; p < lower.ptr[n]; ++p) dptr[lower.dep[p] + 1] += 1;
+  for (int64_t r = 0; r < n; ++r) dptr[r + 1] += dptr[r];
+  std::vector<int32_t> dependents(dptr[n]);
+  {
+    std::vector<int64_t> cursor(dptr.begin(), dptr.end() - 1);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t p = lower.ptr[i]; p < lower.ptr[i + 1]; ++p) dependents[cursor[lower.dep[p]]++] = (int32_t)i;
   }
+  std::vector<int32_t> parent(n, -1), mark(n, -1);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t e = dptr[r]; e < dptr[r + 1]; ++e)
+      if (parent[r] < 0 || dependents[e] < parent[r]) parent[r] = dependents[e];
+  for (int64_t r = 0; r < n; ++r) {
+    const int32_t up = parent[r];
+    if (up < 0) continue;
+    for (int64_t e = dptr[up]; e < dptr[up + 1]; ++e) mark[dependents[e]] = (int32_t)r;
+    for (int64_t e = dptr[r]; e < dptr[r + 1]; ++e)
+      if (dependents[e] != up && mark[dependents[e]] != (int32_t)r) return {};
+  }
+  // subtree sizes (a parent comes after its children) and children lists
+  std::vector<int64_t> size(n, 1);
+  for (int64_t r = 0; r < n; ++r)
+    if (parent[r] >= 0) size[parent[r]] += size[r];
+  std::vector<int64_t> cptr(n + 1, 0);
+  for (int64_t r = 0; r < n; ++r)
+    if (parent[r] >= 0) cptr[parent[r] + 1] += 1;
+  for (int64_t r = 0; r < n; ++r) cptr[r + 1] += cptr[r];
+  std::vector<int32_t> children(cptr[n]);
+  {
+    std::vector<int64_t> cursor(cptr.begin(), cptr.end() - 1);
+    for (int64_t r = 0; r < n; ++r)
+      if (parent[r] >= 0) children[cursor[parent[r]]++] = (int32_t)r;
+  }
+  // cut the largest subtree until every one is small enough
+  const int64_t limit = std::max<int64_t>(1, n / (2 * (int64_t)target));
+  auto larger = [&](int32_t a, int32_t b) { return size[a] < size[b] || (size[a] == size[b] && a > b); };
+  std::vector<int32_t> heap;
+  for (int64_t r = 0; r < n; ++r)
+    if (parent[r] < 0) heap.push_back((int32_t)r);
+  std::make_heap(heap.begin(), heap.end(), larger);
+  std::vector<char> top(n, 0);
+  while (!heap.empty() && size[heap.front()] > limit && (int64_t)heap.size() < 16 * (int64_t)target) {
+    std::pop_heap(heap.begin(), heap.end(), larger);
+    const int32_t r = heap.back();
+    heap.pop_back();
+    top[r] = 1;
+    for (int64_t c = cptr[r]; c < cptr[r + 1]; ++c) {
+      heap.push_back(children[c]);
+      std::push_heap(heap.begin(), heap.end(), larger);
+    }
+  }
+  if (heap.size() < 2) return {};
+  // largest subtree first into the lightest part
+  std::sort(heap.begin(), heap.end(), [&](int32_t a, int32_t b) { return larger(b, a); });
+  const int32_t k = (int32_t)std::min<int64_t>(target, (int64_t)heap.size());
+  std::vector<int64_t> load(k, 0);
+  std::vector<int32_t> part_of(n, -1);
+  for (int32_t r : heap) {
+    const int32_t q = (int32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[q] += size[r];
+    part_of[r] = q;
+  }
+  // every other row: the top part, or its subtree root's part (parents come after children)
+  for (int64_t r = n; r-- > 0;) {
+    if (top[r])
+      part_of[r] = k;
+    else if (part_of[r] < 0)
+      part_of[r] = part_of[parent[r]];
+  }
+  parts = k;
+  return part_of;
+}
+
+Sweep schedule_parts(const RowForm& form, const std::vector<int64_t>& src_map, const std::vector<int64_t>& dst_map,
+                     const std::vector<int32_t>& part_of, int32_t parts, bool top_first) {
+  const int64_t n = form.n;
+  Sweep bad;
+  if ((int64_t)part_of.size() != n || parts < 1) return bad;
+  // each row's level within its part, in the form's processing order
+  std::vector<int32_t> level(n, 0);
+  auto visit = [&](int64_t i) {
+    const int32_t q = part_of[i];
+    int32_t l = 0;
+    for (int64_t p = form.ptr[i]; p < form.ptr[i + 1]; ++p) {
+      const int32_t d = form.dep[p], qd = part_of[d];
+      const bool allowed = top_first ? (qd == q || qd == parts) && !(q == parts && qd != parts)
+                                     : (qd == q || q == parts);
+      if (!allowed) return false;
+      if (qd == q) l = std::max(l, level[d] + 1);
+    }
+    level[i] = l;
+    return true;
+  };
+  if (form.lower) {
+    for (int64_t i = 0; i < n; ++i)
+      if (!visit(i)) return bad;
+  } else {
+    for (int64_t i = n; i-- > 0;)
+      if (!visit(i)) return bad;
+  }
+  // levels per part, and each part's first global level
+  std::vector<int32_t> depth(parts + 1, 0);
+  for (int64_t i = 0; i < n; ++i) depth[part_of[i]] = std::max(depth[part_of[i]], level[i] + 1);
+  Sweep s;
+  s.n = (int32_t)n;
+  s.unit = form.unit;
+  s.parts = parts;
+  s.part_level.assign(parts + 2, 0);
+  for (int32_t q = 0; q <= parts; ++q) s.part_level[q + 1] = s.part_level[q] + depth[q];
+  s.levels = s.part_level[parts + 1];
+  s.level_ptr.assign(s.levels + 1, 0);
+  std::vector<int32_t> global(n);
+  for (int64_t i = 0; i < n; ++i) {
+    global[i] = s.part_level[part_of[i]] + level[i];
+    s.level_ptr[global[i] + 1] += 1;
+  }
+  for (int32_t l = 0; l < s.levels; ++l) {
+    s.widest_level = std::max(s.widest_level, s.level_ptr[l + 1]);
+    s.level_ptr[l + 1] += s.level_ptr[l];
+  }
+  std::vector<int32_t> cursor(s.level_ptr.begin(), s.level_ptr.end() - 1);
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) order[cursor[global[i]]++] = i;  // ascending rows within a level
+  emit_rows(form, src_map, dst_map, order, s);
   return s;
 }
 

@@ -41,12 +41,34 @@ struct Sweep {
   std::vector<double> diag;  // n, unless unit
   std::vector<double> val_im, diag_im;  // a complex factor's imaginary parts, else empty
   int32_t widest_level = 0;
+  // A partitioned sweep (parts > 0): its rows are grouped into `parts` independent parts and a
+  // top part (index parts), in that order, each leveled on its own dependencies; part q's levels
+  // are level_ptr[part_level[q]] .. level_ptr[part_level[q + 1]] (parts + 2 entries).
+  int32_t parts = 0;
+  std::vector<int32_t> part_level;
 };
 
 // Schedules `form`: level[i] is one more than the deepest of row i's dependencies; rows are
 // ordered by (level, row). src_map / dst_map give each factor row's row in the source and
 // destination blocks (dependencies are mapped through dst_map).
 Sweep schedule(const RowForm& form, const std::vector<int64_t>& src_map, const std::vector<int64_t>& dst_map);
+
+// Splits the rows of a lower triangular form into independent parts and a top part, when its
+// dependency graph is a tree in this sense: every row's dependents other than the first are also
+// dependents of the first (a Cholesky factor: the elimination tree). Subtrees are cut from the
+// root down until none holds more than n / (2 target) rows, the cut rows forming the top part, and
+// the subtrees are packed into at most `target` parts, largest first into the lightest. Returns
+// each row's part (parts for the top part), or nothing when the graph is not such a tree or
+// there would be fewer than two parts.
+std::vector<int32_t> tree_parts(const RowForm& lower, int32_t target, int32_t& parts);
+
+// schedule() with the rows grouped by part_of (0 .. parts - 1, then the top part `parts`) and
+// each part leveled on its dependencies inside it. top_first: the top part is swept before the
+// others (a part's rows may depend on its own rows and the top part's; the top part's only on
+// its own); otherwise after them (a part's rows only on its own; the top part's on any). A
+// partition the form does not satisfy gives a sweep with parts = 0 and no rows.
+Sweep schedule_parts(const RowForm& form, const std::vector<int64_t>& src_map, const std::vector<int64_t>& dst_map,
+                     const std::vector<int32_t>& part_of, int32_t parts, bool top_first);
 
 // Each row's level: one more than the deepest of its dependencies (0 for none).
 std::vector<int32_t> levels_of(const RowForm& form);

@@ -464,6 +464,141 @@ __global__ void __launch_bounds__(1024) solve_complex_kernel(ComplexSweepDev fwd
     for (int32_t i = g.slot; i < bwd.n; i += g.slots) x[(2 * (int64_t)i + 1) * ld + col] = -x[(2 * (int64_t)i + 1) * ld + col];
 }
 
+// ---- partitioned FP64 solves: a sweep's independent parts side by side ----
+
+// Stages part q's level pointers (from lp_all: the sweep's, or a tile's own list over the same
+// levels) into LDS at lp and returns the sweep restricted to that part's levels. Every thread
+// must call; ends with a barrier.
+__device__ SweepDev part_of_sweep(const SweepDev& s, const int32_t* __restrict__ lp_all, int32_t q, int32_t* lp) {
+  const int32_t g0 = s.part_level[q], g1 = s.part_level[q + 1];
+  for (int32_t l = threadIdx.x; l <= g1 - g0; l += blockDim.x) lp[l] = lp_all[g0 + l];
+  __syncthreads();
+  SweepDev v = s;
+  v.levels = g1 - g0;
+  return v;
+}
+
+// Copies tile `tile`'s reach bitmap into LDS (every thread must call; ends with a barrier)
+__device__ void stage_live(const ReachDev& reach, int32_t tile, uint32_t* live) {
+  const uint32_t* __restrict__ src = reach.live + (int64_t)tile * reach.words;
+  for (int32_t w = threadIdx.x; w < reach.words; w += blockDim.x) live[w] = src[w];
+  __syncthreads();
+}
+
+// The right-hand side a forward row reads: column col of the block, or of e_plus - e_minus
+struct ForwardRhs {
+  const double* __restrict__ block;
+  int64_t ld;
+  int32_t col, plus, minus;
+  __device__ double operator()(int32_t src) const {
+    if (block) return block[src * ld + col];
+    return (src == plus ? 1.0 : 0.0) - (src == minus ? 1.0 : 0.0);
+  }
+};
+
+__device__ __forceinline__ ForwardRhs forward_rhs(const RhsDev& rhs, int64_t ld, int32_t col, bool on) {
+  ForwardRhs f{rhs.block, ld, col, -1, -1};
+  if (!rhs.block && on) {
+    f.plus = rhs.plus[col];
+    f.minus = rhs.minus[col];
+  }
+  return f;
+}
+
+// A bitmap of live rows in LDS
+struct LiveBits {
+  const uint32_t* bits;
+  __device__ __forceinline__ bool operator()(int32_t row) const { return (bits[row >> 5] >> (row & 31)) & 1u; }
+};
+struct ReachAt {
+  const int32_t* __restrict__ pos;
+  __device__ __forceinline__ int32_t operator()(int32_t k) const { return pos[k]; }
+};
+
+// Phase 1: workgroup (tile, part) sweeps the part's rows forward into t
+template <int TW>
+__global__ void __launch_bounds__(1024) parts_forward_kernel(SweepDev fwd, RhsDev rhs, ReachDev reach,
+                                                             double* __restrict__ t, int64_t ld, int32_t ncols,
+                                                             int32_t lp_words) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* lp = reinterpret_cast<int32_t*>(smem);
+  uint32_t* live = reinterpret_cast<uint32_t*>(lp + lp_words);
+  const bool pairs_reach = reach.lp != nullptr;
+  const int32_t* lp_all = pairs_reach ? reach.lp + (int64_t)blockIdx.x * (fwd.levels + 1) : fwd.level_ptr;
+  const SweepDev s = part_of_sweep(fwd, lp_all, (int32_t)blockIdx.y, lp);
+  const Tile<TW> g(ncols);
+  const ForwardRhs from = forward_rhs(rhs, ld, g.col, g.on);
+  auto store = [&](int32_t dst, double v) { t[dst * ld + g.col] = v; };
+  if (pairs_reach) {
+    stage_live(reach, (int32_t)blockIdx.x, live);
+    wide_sweep<double, TW>(s, lp, t, ld, g, g.on, from, store, ReachAt{reach.pos}, LiveBits{live});
+  } else {
+    wide_sweep<double, TW>(s, lp, t, ld, g, g.on, from, store);
+  }
+}
+
+// Phase 2: per tile, the top part forward (into t) and then backward (into x)
+template <int TW>
+__global__ void __launch_bounds__(1024) top_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, ReachDev reach,
+                                                   double* __restrict__ t, double* __restrict__ x, int64_t ld,
+                                                   int32_t ncols, int32_t lpf_words, int32_t lpb_words) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem);
+  int32_t* lpb = lpf + lpf_words;
+  uint32_t* live = reinterpret_cast<uint32_t*>(lpb + lpb_words);
+  const bool pairs_reach = reach.lp != nullptr;
+  const int32_t* lp_all = pairs_reach ? reach.lp + (int64_t)blockIdx.x * (fwd.levels + 1) : fwd.level_ptr;
+  const SweepDev sf = part_of_sweep(fwd, lp_all, fwd.parts, lpf);
+  const SweepDev sb = part_of_sweep(bwd, bwd.level_ptr, bwd.parts, lpb);
+  const Tile<TW> g(ncols);
+  const int32_t col = g.col;
+  const ForwardRhs from = forward_rhs(rhs, ld, col, g.on);
+  auto store_t = [&](int32_t dst, double v) { t[dst * ld + col] = v; };
+  auto store_x = [&](int32_t dst, double v) { x[dst * ld + col] = v; };
+  if (pairs_reach) {
+    stage_live(reach, (int32_t)blockIdx.x, live);
+    const LiveBits in_reach{live};
+    wide_sweep<double, TW>(sf, lpf, t, ld, g, g.on, from, store_t, ReachAt{reach.pos}, in_reach);
+    const int32_t pad = fwd.n;
+    auto from_t = [&](int32_t src) -> double {
+      const bool on = in_reach(src);
+      const double v = t[(on ? src : pad) * ld + col];
+      return on ? v : 0.0;
+    };
+    wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, from_t, store_x);
+  } else {
+    wide_sweep<double, TW>(sf, lpf, t, ld, g, g.on, from, store_t);
+    wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; }, store_x);
+  }
+}
+
+// Phase 3: workgroup (tile, part) sweeps the part's rows backward from t into x
+template <int TW>
+__global__ void __launch_bounds__(1024) parts_backward_kernel(SweepDev bwd, ReachDev reach, const double* __restrict__ t,
+                                                              double* __restrict__ x, int64_t ld, int32_t ncols,
+                                                              int32_t lp_words) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* lp = reinterpret_cast<int32_t*>(smem);
+  uint32_t* live = reinterpret_cast<uint32_t*>(lp + lp_words);
+  const SweepDev s = part_of_sweep(bwd, bwd.level_ptr, (int32_t)blockIdx.y, lp);
+  const Tile<TW> g(ncols);
+  const int32_t col = g.col;
+  auto store_x = [&](int32_t dst, double v) { x[dst * ld + col] = v; };
+  if (reach.lp) {
+    stage_live(reach, (int32_t)blockIdx.x, live);
+    const LiveBits in_reach{live};
+    const int32_t pad = bwd.n;
+    auto from_t = [&](int32_t src) -> double {
+      const bool on = in_reach(src);
+      const double v = t[(on ? src : pad) * ld + col];
+      return on ? v : 0.0;
+    };
+    wide_sweep<double, TW>(s, lp, x, ld, g, g.on, from_t, store_x);
+  } else {
+    wide_sweep<double, TW>(s, lp, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; }, store_x);
+  }
+}
+
 // ---- FP32 sweeps refined in FP64 ----
 
 // Forward FP32 sweep of one tile: t[dst] = (float(src[src_row]) - sum val * t[dep]) / diag
@@ -1207,6 +1342,41 @@ hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const Sweep
     if (e != hipSuccess) return e;                                                                               \
     hipLaunchKernelGGL((solve64_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, t, x, ld,    \
                        ncols);                                                                                   \
+  }
+  DISPATCH_TILE(tile, L)
+#undef L
+  return hipGetLastError();
+}
+
+hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
+                                const PartLevels& levels, const RhsDev& rhs, const ReachDev& reach, double* t,
+                                double* x, int64_t ld, int32_t ncols) {
+  if (threads < tile || threads > 1024 || threads % 64 || fwd.parts < 1 || bwd.parts < 1) return hipErrorInvalidValue;
+  const unsigned tiles = (unsigned)((ncols + tile - 1) / tile);
+  // LDS: level pointers rounded to 4 words (so the bitmap after them stays 16-byte aligned), and
+  // the tile's reach bitmap for a pair block
+  auto words = [](int32_t levels) { return (levels + 1 + 3) / 4 * 4; };
+  const size_t live = reach.lp ? (size_t)reach.words * sizeof(uint32_t) : 0;
+  const int32_t wf = words(levels.fwd_parts), wt = words(levels.fwd_top), wb = words(levels.bwd_top),
+                wp = words(levels.bwd_parts);
+  const size_t b1 = wf * sizeof(int32_t) + live, b2 = (size_t)(wt + wb) * sizeof(int32_t) + live,
+               b3 = wp * sizeof(int32_t) + live;
+#define L(TWV)                                                                                                  \
+  {                                                                                                             \
+    hipError_t e = hipFuncSetAttribute((const void*)parts_forward_kernel<TWV>,                                  \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)b1);                    \
+    if (e == hipSuccess)                                                                                        \
+      e = hipFuncSetAttribute((const void*)top_kernel<TWV>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b2); \
+    if (e == hipSuccess)                                                                                        \
+      e = hipFuncSetAttribute((const void*)parts_backward_kernel<TWV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                              (int)b3);                                                                         \
+    if (e != hipSuccess) return e;                                                                              \
+    hipLaunchKernelGGL((parts_forward_kernel<TWV>), dim3(tiles, (unsigned)fwd.parts), dim3(threads), b1, stream, fwd, \
+                       rhs, reach, t, ld, ncols, wf);                                                           \
+    hipLaunchKernelGGL((top_kernel<TWV>), dim3(tiles), dim3(threads), b2, stream, fwd, bwd, rhs, reach, t, x, ld,  \
+                       ncols, wt, wb);                                                                          \
+    hipLaunchKernelGGL((parts_backward_kernel<TWV>), dim3(tiles, (unsigned)bwd.parts), dim3(threads), b3, stream,  \
+                       bwd, reach, t, x, ld, ncols, wp);                                                        \
   }
   DISPATCH_TILE(tile, L)
 #undef L

@@ -40,6 +40,10 @@ struct SweepDev {
   const float* diag32 = nullptr;
   RecordsDev rec;  // one record per position, in processing order
   bool unit = false;
+  // a partitioned sweep: `parts` independent parts, then the top part (index parts); part q's
+  // levels are part_level[q] .. part_level[q + 1] of level_ptr (parts + 2 entries)
+  int32_t parts = 0;
+  const int32_t* part_level = nullptr;
 };
 
 // One scheduled sweep of a complex factor: the arrays of SweepDev, each value a (re, im) pair.
@@ -122,6 +126,8 @@ hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const Sweep
 struct ReachDev {
   const int32_t* pos = nullptr;
   const int32_t* lp = nullptr;
+  const uint32_t* live = nullptr;  // per tile, a bit per forward row in its reach (words each)
+  int32_t words = 0;
 };
 
 // LDS a reach solve needs (both sweeps' level pointers and a bit per forward row), and the solve
@@ -139,6 +145,19 @@ hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepD
 
 // Layout changes between count vectors of n values, one after another (the host's &[Vec<f64>])
 // and a row-interleaved block (n rows x ld, column c of row i at i * ld + c).
+// FP64 solve of a block over partitioned sweeps (fwd: parts, then the top part; bwd: the top
+// part first, then the parts), in three launches: every (tile, part) of the forward parts, then
+// per tile the top part forward and backward, then every (tile, part) of the backward parts.
+// Each row keeps its own terms and their order, so the bits are those of launch_solve64. With
+// reach.lp set the block is pairs and each tile sweeps only its reach forward, as
+// launch_solve64_reach does (reach.lp over fwd's levels, reach.live the tiles' bitmaps).
+struct PartLevels {
+  int32_t fwd_parts = 0, fwd_top = 0, bwd_top = 0, bwd_parts = 0;  // the most levels of one part, and the top's
+};
+hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
+                                const PartLevels& levels, const RhsDev& rhs, const ReachDev& reach, double* t,
+                                double* x, int64_t ld, int32_t ncols);
+
 // Complex FP64 solve of a block: forward sweep b -> t, backward t -> x. A complex block holds
 // unknown i's real parts in row 2i and its imaginary parts in row 2i + 1 (ld doubles a row).
 // conjugate: solve with conj(b) and return conj of the solution.

@@ -107,7 +107,7 @@ struct Staging {
 // A sweep on the device: its level pointers, its rows' records (analysis order, field by
 // field), and the CSR arrays holding every entry (the records inline the first few of each row)
 struct DevSweep {
-  DevBuf<int32_t> level_ptr, src, dst, row_ptr, dep;
+  DevBuf<int32_t> level_ptr, src, dst, row_ptr, dep, part_level;
   DevBuf<double> val64, diag64;
   DevBuf<float> val32, diag32;
   DevBuf<int4> head, dep4;
@@ -156,6 +156,7 @@ struct DevSweep {
     rdiag64.upload(rd, stream);
     rdiag32.upload(rd32, stream);
     level_ptr.upload(host.level_ptr, stream);
+    if (host.parts) part_level.upload(host.part_level, stream);
     src.upload(host.src, stream);
     dst.upload(host.dst, stream);
     row_ptr.upload(host.row_ptr, stream);
@@ -191,8 +192,17 @@ struct DevSweep {
     v.rec.diag64 = rdiag64.p;
     v.rec.diag32 = rdiag32.p;
     v.unit = host.unit;
+    v.parts = host.parts;
+    v.part_level = host.parts ? part_level.p : nullptr;
     return v;
   }
+  // The most levels of one of the independent parts, and the top part's
+  int32_t most_part_levels() const {
+    int32_t m = 0;
+    for (int32_t q = 0; q < host.parts; ++q) m = std::max(m, host.part_level[q + 1] - host.part_level[q]);
+    return m;
+  }
+  int32_t top_levels() const { return host.parts ? host.part_level[host.parts + 1] - host.part_level[host.parts] : 0; }
   size_t entries() const { return host.dep.size(); }
 };
 
@@ -310,6 +320,46 @@ struct Layout {
   int64_t cs() const { return columns ? ld : 1; }
 };
 
+// A forward sweep's dependency graph, for finding the reach of pair blocks: each row's dependents
+// (rows that read it), its position, the row reading each right-hand side row, and each row's
+// first dependent in processing order. When every other dependent of a row is also a dependent
+// of that first one (a Cholesky factor's columns nest this way: the elimination tree), a row's
+// reach is its path to the root through first dependents (is_path).
+struct ReachIndex {
+  std::vector<int32_t> dep_ptr, dependents, pos_of_row, row_of_src, first_dependent;
+  bool is_path = false;
+  void build(const Sweep& sw) {
+    const int32_t rows = sw.n;
+    dep_ptr.assign(rows + 1, 0);
+    pos_of_row.assign(rows, 0);
+    row_of_src.assign(rows, 0);
+    for (int32_t k = 0; k < rows; ++k) {
+      pos_of_row[sw.dst[k]] = k;
+      row_of_src[sw.src[k]] = sw.dst[k];
+      for (int32_t q = sw.row_ptr[k]; q < sw.row_ptr[k + 1]; ++q) dep_ptr[sw.dep[q] + 1] += 1;
+    }
+    for (int32_t r = 0; r < rows; ++r) dep_ptr[r + 1] += dep_ptr[r];
+    dependents.assign(dep_ptr[rows], 0);
+    std::vector<int32_t> cursor(dep_ptr.begin(), dep_ptr.end() - 1);
+    for (int32_t k = 0; k < rows; ++k)
+      for (int32_t q = sw.row_ptr[k]; q < sw.row_ptr[k + 1]; ++q) dependents[cursor[sw.dep[q]]++] = sw.dst[k];
+    first_dependent.assign(rows, -1);
+    for (int32_t r = 0; r < rows; ++r)
+      for (int32_t e = dep_ptr[r]; e < dep_ptr[r + 1]; ++e)
+        if (first_dependent[r] < 0 || pos_of_row[dependents[e]] < pos_of_row[first_dependent[r]])
+          first_dependent[r] = dependents[e];
+    std::vector<int32_t> mark(rows, -1);
+    is_path = true;
+    for (int32_t r = 0; r < rows && is_path; ++r) {
+      const int32_t up = first_dependent[r];
+      if (up < 0) continue;
+      for (int32_t e = dep_ptr[up]; e < dep_ptr[up + 1]; ++e) mark[dependents[e]] = r;
+      for (int32_t e = dep_ptr[r]; e < dep_ptr[r + 1]; ++e)
+        if (dependents[e] != up && mark[dependents[e]] != r) is_path = false;
+    }
+  }
+};
+
 struct ybus_hip_factor {
   ybus_hip_ctx* ctx = nullptr;
   int64_t n = 0;
@@ -322,6 +372,12 @@ struct ybus_hip_factor {
   size_t lds_cap = 0;
   // wide path: schedules reading and writing permuted blocks
   DevSweep fwd, bwd, tfwd, tbwd;
+  // wide path, partitioned (a Cholesky factor whose elimination tree splits): the same sweeps
+  // with their rows grouped into independent parts and a top part; FP64 solves take these
+  bool partitioned = false;
+  bool force_parts = false;  // YBUS_HIP_PARTS named a part count: untuned blocks take the parts
+  DevSweep pfwd, pbwd;
+  PartLevels part_levels;
   // column path: schedules in factor order, and each unknown's factor row for the right-hand
   // side (rhs_row) and the solution (sol_of), for the solve and the transposed solve
   DevSweep cfwd, cbwd, ctfwd, ctbwd;
@@ -350,7 +406,9 @@ struct ybus_hip_factor {
   bool use_reach = true;
   int reach_tile = 0;
   DevBuf<int32_t> reach_pos, reach_lp;
-  std::vector<int32_t> dep_ptr, dependents, pos_of_row, row_of_src;
+  DevBuf<uint32_t> reach_live;  // partitioned sweeps: per tile, a bit per forward row in its reach
+  bool reach_parts = false;  // the staged reach is over the partitioned sweep's positions
+  ReachIndex reach_index[2];  // for the wide forward sweep, and for its partitioned form
   int64_t staged = 0;
   bool staged_pairs = false;  // the staged block is pairs (plus, minus), not vectors (b)
   ybus_hip_block* borrowed = nullptr;
@@ -365,6 +423,7 @@ struct ybus_hip_factor {
   struct Tuned {
     bool columns = false;
     int threads = 512, tile = 0;
+    int parts = -1;  // the partitioned sweeps: 1 on, 0 off, -1 by the default rule
   };
   std::map<int, Tuned> tuned;
   const Tuned* trial = nullptr;  // a configuration being timed by the autotuner
@@ -446,6 +505,17 @@ struct ybus_hip_factor {
     return ncols <= column_limit;
   }
 
+  // Whether a wide block of `ncols` columns sweeps the partitioned form: as tuned, else when its
+  // tiles alone would leave compute units idle
+  bool use_parts(int64_t ncols, bool pairs) const {
+    if (!partitioned) return false;
+    if (const Tuned* t = tuned_for(ncols, pairs))
+      if (!t->columns && t->parts >= 0) return t->parts == 1;
+    if (force_parts) return true;
+    const int tile = tile_for(ncols, pairs);
+    return (ncols + tile - 1) / tile < (int64_t)ctx->cus;
+  }
+
   int tile_for(int64_t ncols, bool pairs) const {
     if (tile_override) return tile_override;
     if (const Tuned* t = tuned_for(ncols, pairs))
@@ -457,24 +527,14 @@ struct ybus_hip_factor {
   // wide forward sweep: the rows that depend, directly or through others, on a row whose
   // right-hand side is one of the tile's unknowns, as positions in level order, with each level's
   // start per tile (a breadth-first walk over the sweep's dependents from the tile's rows).
-  void stage_reach(const std::vector<int32_t>& p, const std::vector<int32_t>& m, int tile) {
-    const Sweep& sw = fwd.host;
+  void stage_reach(const std::vector<int32_t>& p, const std::vector<int32_t>& m, int tile, bool parts) {
+    const Sweep& sw = (parts ? pfwd : fwd).host;
     const int32_t rows = sw.n, levels = sw.levels;
-    if (dep_ptr.empty()) {
-      dep_ptr.assign(rows + 1, 0);
-      pos_of_row.assign(rows, 0);
-      row_of_src.assign(rows, 0);
-      for (int32_t k = 0; k < rows; ++k) {
-        pos_of_row[sw.dst[k]] = k;
-        row_of_src[sw.src[k]] = sw.dst[k];
-        for (int32_t q = sw.row_ptr[k]; q < sw.row_ptr[k + 1]; ++q) dep_ptr[sw.dep[q] + 1] += 1;
-      }
-      for (int32_t r = 0; r < rows; ++r) dep_ptr[r + 1] += dep_ptr[r];
-      dependents.assign(dep_ptr[rows], 0);
-      std::vector<int32_t> cursor(dep_ptr.begin(), dep_ptr.end() - 1);
-      for (int32_t k = 0; k < rows; ++k)
-        for (int32_t q = sw.row_ptr[k]; q < sw.row_ptr[k + 1]; ++q) dependents[cursor[sw.dep[q]]++] = sw.dst[k];
-    }
+    ReachIndex& ix = reach_index[parts ? 1 : 0];
+    if (ix.dep_ptr.empty()) ix.build(sw);
+    const std::vector<int32_t>&dep_ptr = ix.dep_ptr, &dependents = ix.dependents, &pos_of_row = ix.pos_of_row,
+                         &row_of_src = ix.row_of_src, &first_dependent = ix.first_dependent;
+    const bool reach_is_path = ix.is_path;
     const int64_t k = (int64_t)p.size();
     const int64_t tiles = (k + tile - 1) / tile;
     // each tile's reach, sorted by position, found by workers that each take a run of tiles
@@ -494,8 +554,16 @@ struct ybus_hip_factor {
           if (p[c] >= 0) visit(row_of_src[p[c]]);
           if (m[c] >= 0) visit(row_of_src[m[c]]);
         }
-        for (size_t qi = 0; qi < queue.size(); ++qi)
-          for (int32_t e = dep_ptr[queue[qi]]; e < dep_ptr[queue[qi] + 1]; ++e) visit(dependents[e]);
+        if (reach_is_path) {
+          // walk each start up its path until it meets rows already taken
+          const size_t starts = queue.size();
+          for (size_t qi = 0; qi < starts; ++qi)
+            for (int32_t r = first_dependent[queue[qi]]; r >= 0 && stamp[r] != (int32_t)t; r = first_dependent[r])
+              visit(r);
+        } else {
+          for (size_t qi = 0; qi < queue.size(); ++qi)
+            for (int32_t e = dep_ptr[queue[qi]]; e < dep_ptr[queue[qi] + 1]; ++e) visit(dependents[e]);
+        }
         std::vector<int32_t>& out = reach[t];
         out.resize(queue.size());
         for (size_t qi = 0; qi < queue.size(); ++qi) out[qi] = pos_of_row[queue[qi]];
@@ -535,8 +603,20 @@ struct ybus_hip_factor {
     if (pos.empty()) pos.push_back(0);
     reach_pos.upload(pos, stream());
     reach_lp.upload(lp, stream());
+    if (parts) {
+      // each tile's reach as a bitmap over forward rows (and the pad row, always clear)
+      const int64_t words = rows / 32 + 1;
+      std::vector<uint32_t> bits((size_t)tiles * words, 0u);
+      for (int64_t t = 0; t < tiles; ++t)
+        for (int32_t q : reach[t]) {
+          const int32_t r = sw.dst[q];
+          bits[(size_t)t * words + (r >> 5)] |= 1u << (r & 31);
+        }
+      reach_live.upload(bits, stream());
+    }
     check(hipStreamSynchronize(stream()), "stage reach");
     reach_tile = tile;
+    reach_parts = parts;
   }
 
   void begin_kernel() { check(hipEventRecord(ev0, stream()), "hipEventRecord"); }
@@ -578,7 +658,15 @@ struct ybus_hip_factor {
       const DevSweep& bw = transpose ? tbwd : bwd;
       RhsDev rhs{pairs ? nullptr : b.p, pairs ? plus.p : nullptr, pairs ? minus.p : nullptr};
       const int tile = tile_for(ncols, pairs);
-      if (pairs && !transpose && reach_tile == tile) {
+      if (use_parts(ncols, pairs)) {
+        // a Cholesky factor (transpose is the same solve): its parts side by side
+        ReachDev reach{};
+        if (pairs && reach_tile == tile && reach_parts)
+          reach = ReachDev{reach_pos.p, reach_lp.p, reach_live.p, (int32_t)(n / 32 + 1)};
+        check(launch_solve64_parts(stream(), tile, wide_threads(ncols, pairs), pfwd.view(), pbwd.view(), part_levels,
+                                   rhs, reach, t.p, x.p, layout.ld, (int32_t)ncols),
+              "solve kernel");
+      } else if (pairs && !transpose && reach_tile == tile && !reach_parts) {
         check(launch_solve64_reach(stream(), tile, wide_threads(ncols, pairs), f.view(), bw.view(), rhs,
                                    ReachDev{reach_pos.p, reach_lp.p}, t.p, x.p, layout.ld, (int32_t)ncols),
               "solve kernel");
@@ -605,9 +693,14 @@ struct ybus_hip_factor {
                               r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     } else {
+      // the refinement sweeps one schedule: a configuration tuned for the partitioned solve does
+      // not carry over, and the default rule picks its tile and workgroup size instead
+      const Tuned* tuned_cfg = tuned_for(ncols, false);
+      const bool own = tuned_cfg && !tuned_cfg->columns && tuned_cfg->parts == 1;
+      const int tile = own ? (tile_override ? tile_override : tile_width_for(ncols, ctx->cus)) : tile_for(ncols, false);
       int th = 64;
-      while (th * 2 <= wide_threads(ncols, false)) th *= 2;
-      check(launch_refine(stream(), tile_for(ncols, false), th, (t_sweeps ? tfwd : fwd).view(),
+      while (th * 2 <= (own ? threads_wide : wide_threads(ncols, false))) th *= 2;
+      check(launch_refine(stream(), tile, th, (t_sweeps ? tfwd : fwd).view(),
                           (t_sweeps ? tbwd : bwd).view(), m, b.p, t32.p, d32.p, x.p, x1.p, r.p, layout.ld,
                           (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
@@ -865,6 +958,25 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
     }
     f->fwd.upload(schedule(lower, gather, ident), ctx->stream, true);
     f->bwd.upload(schedule(upper, ident, gather), ctx->stream, true);
+    // the elimination tree's independent subtrees, for FP64 solves on the wide path: a part per
+    // 2,048 rows, at most 32 (YBUS_HIP_PARTS sets the target; 0 turns it off)
+    const char* parts_env = std::getenv("YBUS_HIP_PARTS");
+    const int32_t target =
+        parts_env ? std::max(0, std::atoi(parts_env)) : (int32_t)std::clamp<int64_t>((int64_t)n / 2048, 0, 32);
+    f->force_parts = parts_env && target >= 2;
+    int32_t parts = 0;
+    const std::vector<int32_t> part_of = target >= 2 ? tree_parts(lower, target, parts) : std::vector<int32_t>();
+    if (parts >= 2) {
+      Sweep pf = schedule_parts(lower, gather, ident, part_of, parts, false);
+      Sweep pb = schedule_parts(upper, ident, gather, part_of, parts, true);
+      if (pf.parts && pb.parts) {
+        f->pfwd.upload(std::move(pf), ctx->stream, true);
+        f->pbwd.upload(std::move(pb), ctx->stream, true);
+        f->part_levels = PartLevels{f->pfwd.most_part_levels(), f->pfwd.top_levels(), f->pbwd.top_levels(),
+                                    f->pbwd.most_part_levels()};
+        f->partitioned = true;
+      }
+    }
     check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
     *out = f.release();
@@ -1453,7 +1565,7 @@ static void stage_pairs(ybus_hip_factor* f, const int64_t* plus, const int64_t* 
     // the wide path sweeps only each tile's reach forward, where its LDS bitmap fits
     if (!f->layout.columns && f->use_reach && f->n &&
         reach_lds_bytes(f->fwd.view(), f->bwd.view()) <= f->lds_cap)
-      f->stage_reach(p, m, f->tile_for((int64_t)k, true));
+      f->stage_reach(p, m, f->tile_for((int64_t)k, true), f->use_parts((int64_t)k, true));
   }
 }
 
@@ -1503,9 +1615,11 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
     if (f->fits && f->force != 1)
       for (int th : {256, 512}) candidates.push_back({true, th, 0});
     if (f->force != 2)
-      for (int th : {256, 512, 1024})
-        for (int tile : {1, 2, 4, 8, 16, 32, 64})
-          if (tile == 1 || tile / 2 < k) candidates.push_back({false, th, tile});
+      for (int parts : {0, 1})
+        for (int th : {256, 512, 1024})
+          for (int tile : {1, 2, 4, 8, 16, 32, 64})
+            if ((tile == 1 || tile / 2 < k) && (parts == 0 || f->partitioned))
+              candidates.push_back({false, th, tile, parts});
     if (candidates.empty()) fail(YBUS_HIP_INVALID_ARGUMENT, "no kernel configuration to try");
     const int key = 2 * ybus_hip_factor::bucket(k) + (pairs ? 1 : 0);
     f->tuned.erase(key);
@@ -1552,8 +1666,9 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
     f->stats = saved;
     f->tuned[key] = best;
     f->ensure_workspace(k, false);  // drops the synthetic staging
-    if (out) *out = ybus_hip_tuning{best.columns ? YBUS_HIP_PATH_COLUMNS : YBUS_HIP_PATH_WIDE, best.threads,
-                                    best.columns ? 1 : best.tile, best_ms};
+    if (out)
+      *out = ybus_hip_tuning{best.columns ? YBUS_HIP_PATH_COLUMNS : YBUS_HIP_PATH_WIDE, best.threads,
+                             best.columns ? 1 : best.tile, best_ms, best.columns ? 0 : std::max(best.parts, 0)};
   });
 }
 
