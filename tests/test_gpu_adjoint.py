@@ -150,3 +150,20 @@ def test_gradients_match_central_finite_differences(gpu, mode):
         minus.data[q] -= h
         fd = (flow_loss(plus, bf, p, limit)[0] - flow_loss(minus, bf, p, limit)[0]) / (2 * h)
         assert abs(fd - a_bar[q]) <= 1e-4 * max(1.0, abs(a_bar[q])), (q, fd, a_bar[q], loss)
+
+
+def test_adjoint_over_partitioned_sweeps_matches_the_cpu(gpu, monkeypatch):
+    """A Cholesky factor swept as independent elimination subtrees (forward, transposed and the
+    refinement's FP64 residual path alike) gives the CPU composition's bits."""
+    monkeypatch.setenv("YBUS_HIP_PARTS", "8")
+    g = O.make_grid(3000, 99)
+    a, _, _ = g.dc()
+    cpu_factor = O.Cholesky(a)
+    factor = Y.GpuFactor.from_cholesky(gpu, cpu_factor)
+    factor.set_matrix(a)
+    x_bar = right_hand_sides(a.rows, 24, 7)
+    x = right_hand_sides(a.rows, 24, 8)
+    want_b, want_a = O.adjoint(cpu_factor, a, x_bar, x)
+    b_bar, a_bar = factor.adjoint(x_bar, x)
+    assert b_bar.tobytes() == want_b.tobytes()
+    assert a_bar.tobytes() == want_a.tobytes()
