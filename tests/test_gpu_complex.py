@@ -102,3 +102,37 @@ def test_real_entry_points_reject_a_complex_factor_and_the_reverse(gpu, hermitia
     b = np.zeros((1, 2 * chol.n))
     out = np.zeros_like(b)
     assert Y.library().ybus_hip_factor_solve_many_complex(real._h, b, 1, out) != 0
+
+
+def test_one_unknown_and_a_pivoting_two_by_two(gpu):
+    """Closed forms: a 1 x 1 Hermitian matrix (x = b / a with a real), and a 2 x 2 complex LU whose
+    row permutation swaps the rows (A = P^T L U)."""
+    from oracle import complex_solve as Z
+
+    class One:
+        n = 1
+        perm = np.array([0])
+        l_ptr = np.array([0, 1])
+        l_idx = np.array([0])
+        l_re = np.array([2.0])
+        l_im = np.array([0.0])
+
+        @property
+        def l(self):
+            return Z.Compressed(self.l_ptr, self.l_idx, np.array([2.0 + 0.0j]))
+
+    factor = Y.GpuFactor.from_cholesky_complex(gpu, One())
+    b = np.array([[4.0 + 8.0j], [-1.0 + 0.5j]])
+    # A = L L^H = 4: x = b / 4 (num-complex divides twice by the diagonal 2, exactly)
+    assert np.array_equal(factor.solve_many(b), b / 4.0)
+    # P A Q = L U with P swapping the rows: A = [[3 + i, 1], [1 - 2i, 2]] has rows (1, 0) pivoted
+    l_val = np.array([1.0 + 0.0j, (3.0 + 1.0j) / (1.0 - 2.0j), 1.0 + 0.0j])
+    u_val = np.array([1.0 - 2.0j, 2.0 + 0.0j, 1.0 - (3.0 + 1.0j) / (1.0 - 2.0j) * 2.0])
+    lu = Y.GpuFactor.from_lu_complex_arrays(gpu, 2, [0, 2, 3], [0, 1, 1], l_val, [0, 1, 3], [0, 0, 1], u_val,
+                                            [1, 0], [0, 1])
+    a = np.array([[3.0 + 1.0j, 1.0], [1.0 - 2.0j, 2.0]])
+    rhs = np.array([[1.0 + 1.0j, -2.0 + 0.5j], [0.25j, 3.0]])
+    x = lu.solve_many(rhs)
+    assert np.max(np.abs(a @ x.T - rhs.T)) <= 1e-14
+    xt = lu.solve_transpose_many(rhs)
+    assert np.max(np.abs(a.T @ xt.T - rhs.T)) <= 1e-14

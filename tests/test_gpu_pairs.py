@@ -315,3 +315,34 @@ def test_partitioned_sweeps_give_the_same_bits(gpu, monkeypatch, kind):
             else:
                 got = factor.solve_many(dense)
             assert got.tobytes() == want.tobytes(), (parts, reach, tile)
+
+
+def test_outage_updates_and_flow_maxima_over_partitioned_sweeps(gpu, monkeypatch):
+    """The N-1 screening chain (rank-one pair updates, the operator's product maxima, the angle
+    maxima) gives the same bits over a partitioned factor as over one schedule."""
+    g = O.make_grid(3000, 515)
+    bp, bf, _ = g.dc()
+    cpu = O.Cholesky(bp)
+    n = bp.rows
+    theta = cpu.solve(O.injections(515, 4, 3000, 1)[0])
+    plus = np.array([-1 if f == 0 else int(f) - 1 for f in g.branch_from])
+    minus = np.array([-1 if t == 0 else int(t) - 1 for t in g.branch_to])
+    outages = np.flatnonzero(~g.bridges())[:400]
+    x = np.asarray(g.x, dtype=np.float64)
+    weights = 1.0 / (0.5 + 1.5 * O.uniforms(515, 5, bf.rows))
+    results = []
+    for parts in ["0", "6"]:
+        monkeypatch.setenv("YBUS_HIP_PARTS", parts)
+        factor = Y.GpuFactor.from_cholesky(gpu, cpu)
+        factor.set_block_width(len(outages))
+        op = factor.operator(bf)
+        op.set_offset(np.zeros(bf.rows))
+        op.set_weights(weights)
+        block = factor.pair_updates_arrays(theta, plus[outages], minus[outages], np.zeros(len(outages)),
+                                           x[outages])
+        loads = block.product_maxima_arrays(op, outages)
+        angles = block.maxima_arrays()
+        results.append((block.solutions().tobytes(), [a.tobytes() for a in loads], [a.tobytes() for a in angles]))
+        block.free()
+    assert results[0] == results[1]
+    assert n == bp.rows
