@@ -2033,6 +2033,11 @@ struct ybus_hip_batch {
   Staging staging;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ybus_hip_kernel_stats stats{};
+  // side streams (default priority) and events that fork the classes' launches off the batch's
+  // stream inside the captured graph, so the classes run side by side, and join them back
+  static constexpr int SIDE = 3;
+  hipStream_t side[SIDE] = {};
+  hipEvent_t fork = nullptr, join[SIDE] = {};
 
   hipStream_t stream() const { return ctx->stream; }
   void drop_plans() {
@@ -2045,6 +2050,11 @@ struct ybus_hip_batch {
     drop_plans();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (fork) (void)hipEventDestroy(fork);
+    for (int i = 0; i < SIDE; ++i) {
+      if (join[i]) (void)hipEventDestroy(join[i]);
+      if (side[i]) (void)hipStreamDestroy(side[i]);
+    }
   }
 
   // Items in classes for a mode: each factor's columns go to the class of the most workgroups
@@ -2104,22 +2114,39 @@ struct ybus_hip_batch {
     plan.ready = true;
   }
 
-  // the queue counters to 0, then each class's launch; on the batch's stream
+  // the queue counters to 0, then each class's launch: the first on the batch's stream, the
+  // others forked onto side streams and joined back (in a captured graph: parallel branches, so
+  // the small classes fill the compute units the large one leaves idle at its tail)
   void enqueue(bool refined, double tol, int32_t max_steps) {
     const Plan& plan = plans[refined ? 1 : 0];
     check(launch_fill(stream(), queues.p, 0, (int32_t)plan.classes.size()), "batch queue reset");
-    for (size_t c = 0; c < plan.classes.size(); ++c) {
+    const int classes = (int)plan.classes.size();
+    if (classes > 1) {
+      if (!fork) check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+      for (int i = 0; i + 1 < classes && i < SIDE; ++i) {
+        if (!side[i]) check(hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking), "hipStreamCreate");
+        if (!join[i]) check(hipEventCreateWithFlags(&join[i], hipEventDisableTiming), "hipEventCreate");
+      }
+      check(hipEventRecord(fork, stream()), "hipEventRecord");
+    }
+    for (int c = 0; c < classes; ++c) {
       const Class& k = plan.classes[c];
+      const bool forked = c > 0 && c - 1 < SIDE;
+      const hipStream_t on = forked ? side[c - 1] : stream();
+      if (forked) check(hipStreamWaitEvent(on, fork, 0), "hipStreamWaitEvent");
       const int grid = std::max(1, std::min<int>(k.count, k.per_cu * ctx->cus));
       if (refined)
-        check(launch_batch_refine(stream(), grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count,
+        check(launch_batch_refine(on, grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count,
                                   queues.p + c, rhs.p, x.p, x1.p, r.p, tol, max_steps, steps.p, ratio.p),
               "batch kernel");
       else
-        check(launch_batch(stream(), grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count, queues.p + c,
+        check(launch_batch(on, grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count, queues.p + c,
                            rhs.p, x.p),
               "batch kernel");
+      if (forked) check(hipEventRecord(join[c - 1], on), "hipEventRecord");
     }
+    for (int c = 1; c < classes && c - 1 < SIDE; ++c)
+      check(hipStreamWaitEvent(stream(), join[c - 1], 0), "hipStreamWaitEvent");
   }
 };
 
