@@ -563,6 +563,60 @@ def run_screening(gpu, args):
     return out
 
 
+def run_flows(gpu, args, ptdf):
+    """Whole PTDF results (the reference's DcGpuSensitivity::ptdf_transfers, dc/gpu.rs:531-611):
+    the 13,500 branch transfers of the 10,000-bus grid solved as one pair block, turned into every
+    branch's flow by the Bf operator on the device (transposed there into one vector per
+    transfer), and copied to the host: 13,500 x 13,500 flows."""
+    import oracle as O
+    import ybus_amd as Y
+
+    factor, chol = ptdf["factor"], ptdf["chol"]
+    _, bf, _ = ptdf["grid"].dc()
+    plus, minus = ptdf["plus"], ptdf["minus"]
+    pairs = [Y.Pair(None if p < 0 else int(p), None if m < 0 else int(m)) for p, m in zip(plus, minus)]
+    op = factor.operator(bf)
+    op.set_offset(np.zeros(bf.rows))
+    zero = [None] * len(pairs)
+    block = factor.pair_solves(pairs)
+    flows = block.product(op, zero)
+    block.free()
+    reps = 3
+    solve_s = product_s = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        block = factor.pair_solves(pairs)
+        t1 = time.perf_counter()
+        flows = block.product(op, zero)
+        t2 = time.perf_counter()
+        block.free()
+        solve_s += t1 - t0
+        product_s += t2 - t1
+    solve_s /= reps
+    product_s /= reps
+    # parity: sampled transfers' flows against Bf times the CPU's solutions (bit for bit: both sum
+    # each branch's two terms from 0 in index order)
+    sample = np.arange(0, len(pairs), max(1, len(pairs) // 32))
+    exact = 0
+    for k in sample:
+        b = np.zeros(chol.n)
+        if plus[k] >= 0:
+            b[plus[k]] = 1.0
+        if minus[k] >= 0:
+            b[minus[k]] = -1.0
+        want = bf.mul_vec(chol.solve(b)) + 0.0
+        exact += int(flows[k].tobytes() == want.tobytes())
+    entries = len(pairs) * bf.rows
+    out = dict(workload="PTDF transfers' flows: 10,000-bus grid, 13,500 branch transfers, Bf (13,500 branches) "
+                        "applied on the device, every flow copied to the host",
+               transfers=len(pairs), branches=int(bf.rows), solve_ms=solve_s * 1e3, product_and_copy_ms=product_s * 1e3,
+               flows_per_second=entries / (solve_s + product_s), bytes_to_host=entries * 8,
+               bit_exact_sampled_transfers=f"{exact} of {len(sample)}")
+    log(f"[flows] pair solves {solve_s * 1e3:.2f} ms, product + copy of {entries * 8 / 1e9:.2f} GB "
+        f"{product_s * 1e3:.1f} ms; bit-exact {exact} of {len(sample)}")
+    return out
+
+
 def run_complex(gpu, args):
     """A complex factor: the 10,000-bus grid's B' made Hermitian (each branch entry given a phase,
     so A stays positive definite), factored as P A P^T = L L^H by the oracle's restated complex
@@ -620,6 +674,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-complex", action="store_true")
     ap.add_argument("--no-screening", action="store_true")
+    ap.add_argument("--no-flows", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB on the device (to scale HBM counters) and exit")
     args = ap.parse_args()
@@ -704,6 +759,8 @@ def main():
         line["adjoint"] = run_adjoint(gpu, args)
     if not args.no_batch:
         line["batch"] = run_batch(gpu, args)
+    if not args.no_flows:
+        line["flows"] = run_flows(gpu, args, ptdf)
     if not args.no_screening:
         line["screening"] = run_screening(gpu, args)
     if not args.no_complex:

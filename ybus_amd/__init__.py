@@ -751,7 +751,7 @@ class GpuBlock:
         z = np.array([-1 if r is None else int(r) for r in zero], dtype=np.int64)
         if len(z) != self.columns():
             raise InvalidArgument(INVALID_ARGUMENT, "one entry to zero (or none) per column")
-        out = np.zeros((self.columns(), operator.rows()))
+        out = np.empty((self.columns(), operator.rows()))
         _check(library().ybus_hip_block_product(self._live(), operator._h, z, out))
         return out
 

@@ -740,7 +740,7 @@ struct ybus_hip_block {
 struct ybus_hip_operator {
   ybus_hip_factor* f = nullptr;
   DevCsr m;
-  DevBuf<double> offset, weight, out;
+  DevBuf<double> offset, weight, out, by_column;
   DevBuf<int32_t> zero;
   int64_t out_cols = 0;
 };
@@ -1868,13 +1868,13 @@ int32_t ybus_hip_block_product(ybus_hip_block* b, ybus_hip_operator* op, const i
       check(hipStreamSynchronize(b->f->stream()), "product");
       return;
     }
-    std::vector<double> host((size_t)rows * cols);
-    check(hipMemcpyAsync(host.data(), op->out.p, host.size() * sizeof(double), hipMemcpyDeviceToHost,
+    // rows x columns on the device: transposed there into one vector per column, then copied out
+    op->by_column.reserve((size_t)rows * (size_t)cols);
+    check(launch_from_block(b->f->stream(), op->out.p, op->by_column.p, rows, cols, cols), "from_block");
+    check(hipMemcpyAsync(out, op->by_column.p, (size_t)rows * cols * sizeof(double), hipMemcpyDeviceToHost,
                          b->f->stream()),
           "D2H");
     check(hipStreamSynchronize(b->f->stream()), "product");
-    for (int64_t c = 0; c < cols; ++c)
-      for (int64_t r = 0; r < rows; ++r) out[c * rows + r] = host[r * cols + c];
   });
 }
 
