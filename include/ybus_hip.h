@@ -259,6 +259,21 @@ int32_t ybus_hip_factor_pair_updates(ybus_hip_factor *f, const double *base, con
                                      const double *inverse_weight, uint64_t k,
                                      ybus_hip_block **out);
 
+/* Outages of sets of branches (new: the reference takes a set out on the CPU, dc/outage.rs:126-241,
+ * and its GPU path only single branches): set s holds the pairs set_ptr[s] .. set_ptr[s + 1] - 1
+ * (at most 64) of plus/minus/alpha/inverse_weight, each a branch's pair with its alpha_k and
+ * 1 / b_k. All the sets' pairs are solved as one block (Y = A^-1 C, at most block_width()
+ * pairs), then per set: x = base + sum alpha_k Y_k (in order), W = D^-1 - C^T Y,
+ * z = W^-1 C^T x, and the set's column x + sum z_j Y_j (in order). W and C^T x come from the
+ * device; z is solved on the host by Gaussian elimination with partial pivoting (the reference
+ * factors W with its sparse LU, so z agrees to rounding, not bit for bit). A set whose W is
+ * singular gets x and is flagged in ybus_hip_block_singular. Returns a block of one column per
+ * set, which borrows the factor like pair_updates's. */
+int32_t ybus_hip_factor_outage_updates(ybus_hip_factor *f, const double *base, const uint64_t *set_ptr,
+                                       uint64_t sets, const int64_t *plus, const int64_t *minus,
+                                       const double *alpha, const double *inverse_weight,
+                                       ybus_hip_block **out);
+
 void ybus_hip_block_free(ybus_hip_block *b);
 /* GpuBlock::columns (pairs.rs:258-260). */
 uint64_t ybus_hip_block_columns(const ybus_hip_block *b);

@@ -346,3 +346,65 @@ def test_outage_updates_and_flow_maxima_over_partitioned_sweeps(gpu, monkeypatch
         block.free()
     assert results[0] == results[1]
     assert n == bp.rows
+
+
+def host_outage(cpu, base, group):
+    """dc/outage.rs:150-230 over the CPU's solves: Y = A^-1 C, x = base + sum alpha_k Y_k,
+    W = D^-1 - C^T Y, z = W^-1 C^T x (a dense solve), and x + sum z_j Y_j."""
+    n = len(base)
+    ys = [cpu.solve(pair_vector(n, u.pair)) for u in group]
+    x = base.copy()
+    for u, y in zip(group, ys):
+        x = x + u.alpha * y
+    m = len(group)
+    w = np.zeros((m, m))
+    for i, u in enumerate(group):
+        for j, y in enumerate(ys):
+            w[i, j] = -dot(u.pair, y) + (u.inverse_weight if i == j else 0.0)
+    ctx = np.array([dot(u.pair, x) for u in group])
+    z = np.linalg.solve(w, ctx)
+    for zj, y in zip(z, ys):
+        x = x + zj * y
+    return x
+
+
+def test_outage_sets_match_the_host_and_single_branches_match_pair_updates(gpu):
+    """Sets of one to five branches: the rank-m update agrees with the host's to rounding (its
+    small dense solve may round differently), and a set of one is pair_updates bit for bit."""
+    g = O.make_grid(400, 31)
+    bp, _, _ = g.dc()
+    cpu = O.Cholesky(bp)
+    n = bp.rows
+    base = cpu.solve(O.injections(31, 4, 400, 1)[0])
+    plus = [-1 if f == 0 else int(f) - 1 for f in g.branch_from]
+    minus = [-1 if t == 0 else int(t) - 1 for t in g.branch_to]
+    x = np.asarray(g.x, dtype=np.float64)
+    candidates = np.flatnonzero(~g.bridges())
+    rng = np.random.default_rng(12)
+
+    def update(k):
+        return Y.PairUpdate(Y.Pair(None if plus[k] < 0 else plus[k], None if minus[k] < 0 else minus[k]), 0.0,
+                            float(x[k]))
+
+    sets = [[update(int(k)) for k in rng.choice(candidates, size=m, replace=False)] for m in [1, 2, 3, 5, 1, 4]]
+    factor = Y.GpuFactor.from_cholesky(gpu, cpu)
+    factor.set_block_width(64)
+    block = factor.outage_updates(base, sets)
+    got = block.solutions()
+    assert not any(block.singular())
+    block.free()
+    for group, column in zip(sets, got):
+        want = host_outage(cpu, base, group)
+        assert max_difference(column, want) < 1e-12
+    singles = [group[0] for group in sets if len(group) == 1]
+    with factor.pair_updates(base, singles) as pu:
+        ones = pu.solutions()
+    assert got[0].tobytes() == ones[0].tobytes() and got[4].tobytes() == ones[1].tobytes()
+    # a set of two branches against removing both from B' and solving
+    dense = bp.to_dense()
+    group = sets[1]
+    for u in group:
+        c = pair_vector(n, u.pair)
+        dense = dense - np.outer(c, c) / u.inverse_weight
+    p = bp.mul_vec(base)
+    assert max_difference(got[1], np.linalg.solve(dense, p)) < 1e-9

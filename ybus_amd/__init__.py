@@ -138,6 +138,7 @@ def _declare(L):
         "ybus_hip_factor_solve_staged": (i32, [P, i32, C.c_double, U64, C.POINTER(P)]),
         "ybus_hip_factor_pair_solves": (i32, [P, i64p, i64p, U64, C.POINTER(P)]),
         "ybus_hip_factor_pair_updates": (i32, [P, f64p, i64p, i64p, f64p, f64p, U64, C.POINTER(P)]),
+        "ybus_hip_factor_outage_updates": (i32, [P, f64p, u64p, U64, i64p, i64p, f64p, f64p, C.POINTER(P)]),
         "ybus_hip_block_free": (None, [P]),
         "ybus_hip_block_columns": (U64, [P]),
         "ybus_hip_block_singular": (i32, [P, u8p]),
@@ -656,6 +657,35 @@ class GpuFactor:
         h = P()
         _check(library().ybus_hip_factor_pair_updates(self._h, base, plus, minus, alpha, weight, len(plus),
                                                       C.byref(h)))
+        return self._new_block(h)
+
+    def outage_updates(self, base, sets: Sequence[Sequence[PairUpdate]]) -> "GpuBlock":
+        """Each set of branches taken out together (a rank-m update of `base`, m = the set's
+        size, at most 64): a block of one column per set. The sets' pairs are solved as one block
+        (at most block_width() pairs in all)."""
+        flat = [u for group in sets for u in group]
+        set_ptr = np.zeros(len(sets) + 1, dtype=np.uint64)
+        set_ptr[1:] = np.cumsum([len(group) for group in sets]) if len(sets) else []
+        plus, minus = _pairs([u.pair for u in flat])
+        return self.outage_updates_arrays(base, set_ptr, plus, minus, [u.alpha for u in flat],
+                                          [u.inverse_weight for u in flat])
+
+    def outage_updates_arrays(self, base, set_ptr, plus, minus, alpha, inverse_weight) -> "GpuBlock":
+        """outage_updates with the sets as arrays: set s is pairs set_ptr[s] .. set_ptr[s + 1] - 1
+        of plus and minus (-1 for none), alpha and inverse_weight."""
+        self._release_block()
+        base = _f64(base)
+        if base.shape[0] != self._n:
+            raise InvalidArgument(INVALID_ARGUMENT, "one base value per unknown")
+        set_ptr = _u64(set_ptr)
+        plus = np.ascontiguousarray(plus, dtype=np.int64)
+        minus = np.ascontiguousarray(minus, dtype=np.int64)
+        alpha, weight = _f64(alpha), _f64(inverse_weight)
+        if len(set_ptr) < 1 or int(set_ptr[-1]) != len(plus) or not (len(plus) == len(minus) == len(alpha) == len(weight)):
+            raise InvalidArgument(INVALID_ARGUMENT, "set_ptr must end at the number of pairs")
+        h = P()
+        _check(library().ybus_hip_factor_outage_updates(self._h, base, set_ptr, len(set_ptr) - 1, plus, minus, alpha,
+                                                        weight, C.byref(h)))
         return self._new_block(h)
 
     def operator(self, matrix) -> "GpuOperator":

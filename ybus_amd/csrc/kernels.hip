@@ -1066,6 +1066,68 @@ __global__ void pair_apply_columns_kernel(const double* __restrict__ y, double* 
   }
 }
 
+// ---- outages of sets of branches: rank-m updates (dc/outage.rs:126-241) ----
+
+// x_p = base_p + alpha_k Y(p, k) for the set's pairs k in order (x_i += alpha y_i)
+__device__ __forceinline__ double outage_x(const double* __restrict__ y, int64_t rs, int64_t cs,
+                                           const double* __restrict__ base, const double* __restrict__ alpha,
+                                           int32_t k0, int32_t k1, int32_t p) {
+  double x = base[p];
+  for (int32_t k = k0; k < k1; ++k) x = x + alpha[k] * y[p * rs + k * cs];
+  return x;
+}
+
+// c^T v for pair k: +v[plus] - v[minus] (0.0 for a missing end), as pair_update sums it
+__device__ __forceinline__ double pair_dot(int32_t p, int32_t m, double vp, double vm) {
+  double d = 0.0;
+  if (p >= 0) d += vp;
+  if (m >= 0) d -= vm;
+  return d;
+}
+
+// Per set s (workgroup s): W = D^-1 - C^T Y (m x m, row-major at w_at[s]) and C^T x (m, at
+// c_at[s]), where the set's pairs are set_ptr[s] .. set_ptr[s + 1] of the block's columns.
+__global__ void outage_system_kernel(const double* __restrict__ y, int64_t rs, int64_t cs,
+                                     const double* __restrict__ base, const int32_t* __restrict__ set_ptr,
+                                     const int32_t* __restrict__ plus, const int32_t* __restrict__ minus,
+                                     const double* __restrict__ alpha, const double* __restrict__ inverse_weight,
+                                     const int32_t* __restrict__ w_at, double* __restrict__ w,
+                                     double* __restrict__ ctx) {
+  const int32_t s = (int32_t)blockIdx.x;
+  const int32_t k0 = set_ptr[s], k1 = set_ptr[s + 1], m = k1 - k0;
+  double* __restrict__ ws = w + w_at[s];
+  for (int32_t e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int32_t i = k0 + e / m, j = k0 + e % m;
+    const int32_t p = plus[i], q = minus[i];
+    const double vp = p >= 0 ? y[p * rs + j * cs] : 0.0, vq = q >= 0 ? y[q * rs + j * cs] : 0.0;
+    double v = -pair_dot(p, q, vp, vq);
+    if (i == j) v += inverse_weight[i];
+    ws[e] = v;
+  }
+  for (int32_t i = k0 + threadIdx.x; i < k1; i += blockDim.x) {
+    const int32_t p = plus[i], q = minus[i];
+    const double vp = p >= 0 ? outage_x(y, rs, cs, base, alpha, k0, k1, p) : 0.0;
+    const double vq = q >= 0 ? outage_x(y, rs, cs, base, alpha, k0, k1, q) : 0.0;
+    ctx[set_ptr[s] + (i - k0)] = pair_dot(p, q, vp, vq);
+  }
+}
+
+// out(i, s) = x_i + z_j Y(i, j) over the set's pairs j in order (x alone for a singular set)
+__global__ void outage_apply_kernel(const double* __restrict__ y, int64_t rs, int64_t cs, int32_t n,
+                                    const double* __restrict__ base, const int32_t* __restrict__ set_ptr,
+                                    const double* __restrict__ alpha, const double* __restrict__ z,
+                                    const int32_t* __restrict__ singular, double* __restrict__ out, int64_t ors,
+                                    int64_t ocs, int32_t sets) {
+  const int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (index >= (int64_t)n * sets) return;
+  const int32_t s = (int32_t)(index / n), i = (int32_t)(index % n);
+  const int32_t k0 = set_ptr[s], k1 = set_ptr[s + 1];
+  double x = outage_x(y, rs, cs, base, alpha, k0, k1, i);
+  if (!singular[s])
+    for (int32_t j = k0; j < k1; ++j) x = x + z[j] * y[i * rs + j * cs];
+  out[i * ors + s * ocs] = x;
+}
+
 // A column's running maximum of |v| * weight (kernels.cu:78-128): rows visited in increasing
 // order, so a strictly larger product wins; a value that isn't finite sets `bad` and is skipped.
 // merge() folds in another run's result, the smaller row winning a tie.
@@ -1479,6 +1541,26 @@ hipError_t launch_pair_update(hipStream_t stream, const double* y, double* out, 
     hipLaunchKernelGGL(pair_apply_columns_kernel, grid, dim3(256), 0, stream, y, out, n, ncols, cs, base, alpha, coef,
                        singular);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_outage_system(hipStream_t stream, const double* y, int64_t rs, int64_t cs, const double* base,
+                                const int32_t* set_ptr, int32_t sets, const int32_t* plus, const int32_t* minus,
+                                const double* alpha, const double* inverse_weight, const int32_t* w_at, double* w,
+                                double* ctx) {
+  if (sets <= 0) return hipSuccess;
+  hipLaunchKernelGGL(outage_system_kernel, dim3((unsigned)sets), dim3(256), 0, stream, y, rs, cs, base, set_ptr, plus,
+                     minus, alpha, inverse_weight, w_at, w, ctx);
+  return hipGetLastError();
+}
+
+hipError_t launch_outage_apply(hipStream_t stream, const double* y, int64_t rs, int64_t cs, int32_t n,
+                               const double* base, const int32_t* set_ptr, const double* alpha, const double* z,
+                               const int32_t* singular, double* out, int64_t ors, int64_t ocs, int32_t sets) {
+  if (sets <= 0 || n <= 0) return hipSuccess;
+  const int64_t total = (int64_t)n * sets;
+  hipLaunchKernelGGL(outage_apply_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, y, rs, cs, n,
+                     base, set_ptr, alpha, z, singular, out, ors, ocs, sets);
   return hipGetLastError();
 }
 

@@ -180,6 +180,18 @@ hipError_t launch_pair_update(hipStream_t stream, const double* y, double* out, 
                               int64_t cs, const double* base, const int32_t* plus, const int32_t* minus,
                               const double* alpha, const double* inverse_weight, int32_t* singular, double* coef);
 
+// Outages of sets of branches (dc/outage.rs:126-241) over a solved pair block y (one column per
+// pair; set s holds pairs set_ptr[s] .. set_ptr[s + 1]): per set, W = D^-1 - C^T Y (row-major at
+// w + w_at[s]) and C^T x with x = base + sum alpha_k Y_k (at ctx + set_ptr[s]); then, with z = W^-1
+// C^T x per set, out(:, s) = x + sum z_j Y_j (x alone where singular[s]).
+hipError_t launch_outage_system(hipStream_t stream, const double* y, int64_t rs, int64_t cs, const double* base,
+                                const int32_t* set_ptr, int32_t sets, const int32_t* plus, const int32_t* minus,
+                                const double* alpha, const double* inverse_weight, const int32_t* w_at, double* w,
+                                double* ctx);
+hipError_t launch_outage_apply(hipStream_t stream, const double* y, int64_t rs, int64_t cs, int32_t n,
+                               const double* base, const int32_t* set_ptr, const double* alpha, const double* z,
+                               const int32_t* singular, double* out, int64_t ors, int64_t ocs, int32_t sets);
+
 // column_max (kernels.cu:78-128): per column, the largest |v| * weight[row] (smallest row on a
 // tie, -1 if every product is zero) and whether any value isn't finite. A row-interleaved block
 // is split into `chunks` runs of rows whose results (pv, pr, pb: chunks x ncols each) are then

@@ -265,6 +265,9 @@ constexpr int64_t INDEX_LIMIT = 0x7fffffff;
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// The most branches one outage set may hold (its system is solved densely on the host)
+constexpr int32_t MAX_OUTAGE_SET = 64;
+
 // Host threads for the library's own host work: the machine's, at most 16
 int64_t host_threads() {
   const unsigned hw = std::thread::hardware_concurrency();
@@ -1736,6 +1739,105 @@ int32_t ybus_hip_factor_pair_updates(ybus_hip_factor* f, const double* base, con
       check(hipMemcpyAsync(s.data(), f->singular.p, k * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream()), "D2H");
       check(hipStreamSynchronize(f->stream()), "pair_update");
       for (uint64_t c = 0; c < k; ++c) block->singular[c] = s[c] != 0;
+    }
+    f->borrowed = block.get();
+    *out = block.release();
+  });
+}
+
+// Solves the m x m system w z = c (row-major w, both overwritten; z in c) by Gaussian
+// elimination with partial pivoting (the row of largest |value| below the diagonal, the first on
+// a tie); false when a pivot is zero or the result is not finite
+static bool dense_solve(int32_t m, double* w, double* c) {
+  for (int32_t j = 0; j < m; ++j) {
+    int32_t piv = j;
+    for (int32_t i = j + 1; i < m; ++i)
+      if (std::fabs(w[i * m + j]) > std::fabs(w[piv * m + j])) piv = i;
+    if (w[piv * m + j] == 0.0) return false;
+    if (piv != j) {
+      for (int32_t q = 0; q < m; ++q) std::swap(w[j * m + q], w[piv * m + q]);
+      std::swap(c[j], c[piv]);
+    }
+    for (int32_t i = j + 1; i < m; ++i) {
+      const double l = w[i * m + j] / w[j * m + j];
+      for (int32_t q = j + 1; q < m; ++q) w[i * m + q] = w[i * m + q] - l * w[j * m + q];
+      c[i] = c[i] - l * c[j];
+    }
+  }
+  for (int32_t j = m; j-- > 0;) {
+    double v = c[j];
+    for (int32_t q = j + 1; q < m; ++q) v = v - w[j * m + q] * c[q];
+    c[j] = v / w[j * m + j];
+    if (!std::isfinite(c[j])) return false;
+  }
+  return true;
+}
+
+int32_t ybus_hip_factor_outage_updates(ybus_hip_factor* f, const double* base, const uint64_t* set_ptr, uint64_t sets,
+                                       const int64_t* plus, const int64_t* minus, const double* alpha,
+                                       const double* inverse_weight, ybus_hip_block** out) {
+  return guard([&] {
+    if (!f || !out || !set_ptr) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (!base && f->n) fail(YBUS_HIP_INVALID_ARGUMENT, "one base value per unknown");
+    if (set_ptr[0] != 0) fail(YBUS_HIP_INVALID_ARGUMENT, "set_ptr[0] must be 0");
+    for (uint64_t s = 0; s < sets; ++s) {
+      if (set_ptr[s + 1] < set_ptr[s]) fail(YBUS_HIP_INVALID_ARGUMENT, "set_ptr must not decrease");
+      if (set_ptr[s + 1] - set_ptr[s] > (uint64_t)MAX_OUTAGE_SET)
+        fail(YBUS_HIP_INVALID_ARGUMENT, "at most " + std::to_string(MAX_OUTAGE_SET) + " branches a set");
+    }
+    const uint64_t k = set_ptr[sets];
+    if (k && (!alpha || !inverse_weight)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    stage_pairs(f, plus, minus, k);
+    if (k && f->n) f->solve64(false, true, (int64_t)k);
+    auto block = new_block(f, (int64_t)sets, f->y2.p);
+    if (sets && f->n) {
+      // the sets' systems on the device (W and C^T x), solved here, applied on the device
+      std::vector<int32_t> sp(sets + 1), w_at(sets);
+      size_t w_len = 0;
+      for (uint64_t s = 0; s <= sets; ++s) sp[s] = (int32_t)set_ptr[s];
+      for (uint64_t s = 0; s < sets; ++s) {
+        w_at[s] = (int32_t)w_len;
+        w_len += (size_t)(sp[s + 1] - sp[s]) * (size_t)(sp[s + 1] - sp[s]);
+      }
+      if (w_len > (size_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many outage sets");
+      f->upd_base.reserve(f->n);
+      f->upd_alpha.reserve(std::max<uint64_t>(k, 1));
+      f->upd_weight.reserve(std::max<uint64_t>(k, 1));
+      f->coef.reserve(std::max<uint64_t>(k, 1));
+      DevBuf<int32_t> d_sp, d_at, d_singular;
+      DevBuf<double> d_w, d_ctx;
+      d_sp.upload(sp, f->stream());
+      d_at.upload(w_at, f->stream());
+      d_w.reserve(std::max<size_t>(w_len, 1));
+      d_ctx.reserve(std::max<uint64_t>(k, 1));
+      check(hipMemcpyAsync(f->upd_base.p, base, f->n * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+      if (k) {
+        check(hipMemcpyAsync(f->upd_alpha.p, alpha, k * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+        check(hipMemcpyAsync(f->upd_weight.p, inverse_weight, k * sizeof(double), hipMemcpyHostToDevice, f->stream()),
+              "H2D");
+      }
+      check(launch_outage_system(f->stream(), f->x.p, f->layout.rs(), f->layout.cs(), f->upd_base.p, d_sp.p,
+                                 (int32_t)sets, f->plus.p, f->minus.p, f->upd_alpha.p, f->upd_weight.p, d_at.p, d_w.p,
+                                 d_ctx.p),
+            "outage system");
+      std::vector<double> w(w_len), z(k);
+      if (w_len)
+        check(hipMemcpyAsync(w.data(), d_w.p, w_len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+      if (k) check(hipMemcpyAsync(z.data(), d_ctx.p, k * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+      check(hipStreamSynchronize(f->stream()), "outage system");
+      std::vector<int32_t> singular(sets, 0);
+      for (uint64_t s = 0; s < sets; ++s) {
+        const int32_t m = sp[s + 1] - sp[s];
+        if (m && !dense_solve(m, w.data() + w_at[s], z.data() + sp[s])) singular[s] = 1;
+      }
+      d_singular.upload(singular, f->stream());
+      if (k) check(hipMemcpyAsync(f->coef.p, z.data(), k * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+      check(launch_outage_apply(f->stream(), f->x.p, f->layout.rs(), f->layout.cs(), (int32_t)f->n, f->upd_base.p,
+                                d_sp.p, f->upd_alpha.p, f->coef.p, d_singular.p, f->y2.p, f->layout.rs(),
+                                f->layout.cs(), (int32_t)sets),
+            "outage apply");
+      check(hipStreamSynchronize(f->stream()), "outage apply");
+      for (uint64_t s = 0; s < sets; ++s) block->singular[s] = singular[s] != 0;
     }
     f->borrowed = block.get();
     *out = block.release();
