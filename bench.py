@@ -549,6 +549,7 @@ def run_screening(gpu, args):
         cpu_baseline = dict(value=cpu_rate, unit="outages/s", cores=threads, kind="port",
                             sample="the oracle's par_solve_many over the first 2,000 outages' pair vectors, then "
                                    "the update, flows and maxima in numpy, repeated for >= %g s" % CPU_SECONDS)
+    n2 = run_double_outages(factor, op, chol, bf, theta, plus, minus, inverse_weight, weights, outages, args)
     out = dict(workload="N-1 screening: 25,000-bus grid, B' AMD Cholesky, every non-bridge branch out alone "
                         "(%d outages), rank-one update, Bf flows, most loaded branch" % len(outages),
                n=int(chol.n), nnz_l=int(chol.l.nnz), outages=int(len(outages)), tuning=tuning,
@@ -556,7 +557,7 @@ def run_screening(gpu, args):
                unit="outages/s", step_ms=step_s * 1e3, singular=int(sum(singular)),
                parity=dict(sampled_outages=int(len(sample)), max_rel_diff_loading=worst,
                            same_branch=f"{same_branch} of {len(sample)}"),
-               cpu_baseline=cpu_baseline)
+               cpu_baseline=cpu_baseline, double_outages=n2)
     log(f"[screening] {step_s * 1e3:.2f} ms per step ({out['value']:.3g} outages/s); loadings within {worst:.1e}, "
         f"same branch {same_branch} of {len(sample)}; cpu {cpu_baseline['value'] if cpu_baseline else 0:.3g} "
         f"outages/s")
@@ -614,6 +615,69 @@ def run_flows(gpu, args, ptdf):
                bit_exact_sampled_transfers=f"{exact} of {len(sample)}")
     log(f"[flows] pair solves {solve_s * 1e3:.2f} ms, product + copy of {entries * 8 / 1e9:.2f} GB "
         f"{product_s * 1e3:.1f} ms; bit-exact {exact} of {len(sample)}")
+    return out
+
+
+def run_double_outages(factor, op, chol, bf, theta, plus, minus, inverse_weight, weights, outages, args):
+    """Pairs of branches out together on the screening grid (dc/outage.rs:126-241 with two
+    branches): 8,192 random pairs of the non-bridge branches, each a rank-2 update, its flows and
+    most loaded branch; the block holds the 16,384 branch solves. Checked against the CPU's
+    composition on a sample (to rounding: the 2 x 2 systems are solved on the host)."""
+    rng = np.random.default_rng(SEED)
+    count = 8192
+    first = rng.choice(outages, size=count)
+    second = rng.choice(outages, size=count)
+    second = np.where(second == first, outages[(np.searchsorted(outages, first) + 1) % len(outages)], second)
+    ks = np.stack([first, second], axis=1).reshape(-1)
+    set_ptr = np.arange(0, 2 * count + 1, 2, dtype=np.uint64)
+    zero = first.astype(np.int64)  # the first branch's flow is zeroed (its row of the product)
+    o_plus, o_minus = plus[ks], minus[ks]
+    o_alpha, o_weight = np.zeros(len(ks)), inverse_weight[ks]
+
+    def step():
+        block = factor.outage_updates_arrays(theta, set_ptr, o_plus, o_minus, o_alpha, o_weight)
+        loads = block.product_maxima_arrays(op, zero)
+        singular = block.singular()
+        block.free()
+        return loads, singular
+
+    step()
+    reps = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loads, singular = step()
+    step_s = (time.perf_counter() - t0) / reps
+    # the CPU composition for a sample of sets
+    worst = 0.0
+    for s in range(0, count, count // 16):
+        k1, k2 = int(first[s]), int(second[s])
+        ys = []
+        for k in (k1, k2):
+            b = np.zeros(chol.n)
+            if plus[k] >= 0:
+                b[plus[k]] = 1.0
+            if minus[k] >= 0:
+                b[minus[k]] = -1.0
+            ys.append(chol.solve(b))
+
+        def cdot(k, v):
+            return (v[plus[k]] if plus[k] >= 0 else 0.0) - (v[minus[k]] if minus[k] >= 0 else 0.0)
+        w = np.array([[-cdot(ki, yj) + (inverse_weight[ki] if i == j else 0.0) for j, yj in enumerate(ys)]
+                      for i, ki in enumerate((k1, k2))])
+        if singular[s] or abs(np.linalg.det(w)) < 1e-300:
+            continue
+        z = np.linalg.solve(w, np.array([cdot(k1, theta), cdot(k2, theta)]))
+        updated = theta + z[0] * ys[0] + z[1] * ys[1]
+        flows = bf_dense_product(bf, updated[None, :])[0]
+        flows[k1] = 0.0
+        v = float(np.max(np.abs(flows) * weights))
+        worst = max(worst, abs(loads[1][s] - v) / max(v, 1e-300))
+    out = dict(workload="N-2: 8,192 random pairs of the 25,000-bus grid's non-bridge branches, each out together "
+                        "(rank-2 update), flows and most loaded branch",
+               sets=count, value=count / step_s, unit="outage pairs/s", step_ms=step_s * 1e3,
+               singular=int(sum(singular)), max_rel_diff_loading_vs_cpu=worst)
+    log(f"[screening] N-2: {step_s * 1e3:.2f} ms for {count} pairs ({out['value']:.3g} pairs/s), "
+        f"{out['singular']} singular, loadings within {worst:.1e} of the CPU")
     return out
 
 
