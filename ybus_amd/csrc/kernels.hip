@@ -1112,20 +1112,41 @@ __global__ void outage_system_kernel(const double* __restrict__ y, int64_t rs, i
   }
 }
 
-// out(i, s) = x_i + z_j Y(i, j) over the set's pairs j in order (x alone for a singular set)
-__global__ void outage_apply_kernel(const double* __restrict__ y, int64_t rs, int64_t cs, int32_t n,
-                                    const double* __restrict__ base, const int32_t* __restrict__ set_ptr,
-                                    const double* __restrict__ alpha, const double* __restrict__ z,
-                                    const int32_t* __restrict__ singular, double* __restrict__ out, int64_t ors,
-                                    int64_t ocs, int32_t sets) {
-  const int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (index >= (int64_t)n * sets) return;
-  const int32_t s = (int32_t)(index / n), i = (int32_t)(index % n);
-  const int32_t k0 = set_ptr[s], k1 = set_ptr[s + 1];
+// out(i, s) = x_i + z_j Y(i, j) over the set's pairs j in order (x alone for a singular set).
+// Row-interleaved blocks: lanes over sets, each thread a run of rows_per_block rows; column-major
+// blocks: threads over rows, sets over grid.y.
+__device__ __forceinline__ double outage_value(const double* __restrict__ y, int64_t rs, int64_t cs,
+                                               const double* __restrict__ base, const double* __restrict__ alpha,
+                                               const double* __restrict__ z, bool flat, int32_t k0, int32_t k1,
+                                               int32_t i) {
   double x = outage_x(y, rs, cs, base, alpha, k0, k1, i);
-  if (!singular[s])
+  if (!flat)
     for (int32_t j = k0; j < k1; ++j) x = x + z[j] * y[i * rs + j * cs];
-  out[i * ors + s * ocs] = x;
+  return x;
+}
+
+__global__ void outage_apply_rows_kernel(const double* __restrict__ y, int64_t ld, int32_t n, int32_t rows_per_block,
+                                         const double* __restrict__ base, const int32_t* __restrict__ set_ptr,
+                                         const double* __restrict__ alpha, const double* __restrict__ z,
+                                         const int32_t* __restrict__ singular, double* __restrict__ out,
+                                         int32_t sets) {
+  const int32_t s = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (s >= sets) return;
+  const int32_t k0 = set_ptr[s], k1 = set_ptr[s + 1];
+  const bool flat = singular[s] != 0;
+  const int32_t i0 = (int32_t)blockIdx.y * rows_per_block, i1 = min(n, i0 + rows_per_block);
+  for (int32_t i = i0; i < i1; ++i) out[i * ld + s] = outage_value(y, ld, 1, base, alpha, z, flat, k0, k1, i);
+}
+
+__global__ void outage_apply_columns_kernel(const double* __restrict__ y, int64_t ld, int32_t n,
+                                            const double* __restrict__ base, const int32_t* __restrict__ set_ptr,
+                                            const double* __restrict__ alpha, const double* __restrict__ z,
+                                            const int32_t* __restrict__ singular, double* __restrict__ out,
+                                            int32_t sets) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  for (int32_t s = (int32_t)blockIdx.y; s < sets; s += (int32_t)gridDim.y)
+    out[i + s * ld] = outage_value(y, 1, ld, base, alpha, z, singular[s] != 0, set_ptr[s], set_ptr[s + 1], i);
 }
 
 // A column's running maximum of |v| * weight (kernels.cu:78-128): rows visited in increasing
@@ -1558,9 +1579,20 @@ hipError_t launch_outage_apply(hipStream_t stream, const double* y, int64_t rs, 
                                const double* base, const int32_t* set_ptr, const double* alpha, const double* z,
                                const int32_t* singular, double* out, int64_t ors, int64_t ocs, int32_t sets) {
   if (sets <= 0 || n <= 0) return hipSuccess;
-  const int64_t total = (int64_t)n * sets;
-  hipLaunchKernelGGL(outage_apply_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, y, rs, cs, n,
-                     base, set_ptr, alpha, z, singular, out, ors, ocs, sets);
+  // the output shares the block's layout
+  if (ors != rs || ocs != cs) return hipErrorInvalidValue;
+  if (cs == 1) {
+    const int32_t per = rows_per_thread(n);
+    const dim3 grid((unsigned)((sets + 255) / 256), (unsigned)((n + per - 1) / per));
+    hipLaunchKernelGGL(outage_apply_rows_kernel, grid, dim3(256), 0, stream, y, rs, n, per, base, set_ptr, alpha, z,
+                       singular, out, sets);
+  } else if (rs == 1) {
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)std::min<int32_t>(sets, 65535));
+    hipLaunchKernelGGL(outage_apply_columns_kernel, grid, dim3(256), 0, stream, y, cs, n, base, set_ptr, alpha, z,
+                       singular, out, sets);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
