@@ -97,12 +97,14 @@ def _declare(L):
         "yo_lu_solve_many": (C.c_int, [P, U64, f64p, f64p, C.c_int]),
         "yo_lu_solve_transpose_many": (C.c_int, [P, U64, f64p, f64p, C.c_int]),
         "yo_lu_refine": (C.c_int, [P] + mat + [f64p, U64, DBL, f64p, C.POINTER(U64), C.POINTER(DBL)]),
+        "yo_lu_refactor": (C.c_int, [P] + mat + [C.POINTER(C.c_int)]),
+        "yo_lu_refactor_work": (U64, [P]),
         "yo_grid": (C.c_int, [U64, U64, C.POINTER(P)]),
         "yo_grid_sizes": (None, [P, u64p]),
         "yo_grid_copy": (None, [P, f64p, f64p, u64p, u64p, f64p]),
         "yo_grid_bridges": (None, [P, u8p]),
         "yo_dc": (C.c_int, [P, P, C.POINTER(P), C.POINTER(P), u64p]),
-        "yo_jacobian": (C.c_int, [P, U64, C.POINTER(P), C.POINTER(P)]),
+        "yo_jacobian": (C.c_int, [P, U64, DBL, U64, C.POINTER(P), C.POINTER(P)]),
         "yo_ac_sizes": (None, [P, u64p]),
         "yo_ac_copy": (None, [P, u64p, u64p, f64p, f64p, f64p, f64p]),
         "yo_uniforms": (None, [U64, U64, U64, f64p]),
@@ -324,7 +326,10 @@ class Lu:
                          C.byref(h), C.byref(piv))
         _check(st, piv.value)
         self._h = h
-        L = lib()
+        self._read()
+
+    def _read(self):
+        L, h = lib(), self._h
         s = np.zeros(3, np.uint64)
         L.yo_lu_sizes(h, s)
         n, nl, nu = (int(v) for v in s)
@@ -345,6 +350,22 @@ class Lu:
 
     def permutation(self):
         return self.row_perm
+
+    def refactor(self, a_csc: Matrix) -> str:
+        """Lu::refactor (lu.rs:662-757): "reused" (same pivots and patterns, new values) or
+        "repivoted" (factored afresh under the same column ordering); raises SparseError
+        (PatternMismatch, SingularMatrix) and leaves the factors unchanged on an error."""
+        if a_csc.row_major:
+            a_csc = a_csc.to_csc()
+        outcome = C.c_int()
+        _check(lib().yo_lu_refactor(self._h, *a_csc.args(), C.byref(outcome)))
+        self._read()
+        return "reused" if outcome.value == 0 else "repivoted"
+
+    def refactor_work(self) -> int:
+        """Multiply-subtracts of one refactor (the sum of |L(:, j)| below the diagonal over the
+        entries u_jk above U's diagonal)."""
+        return int(lib().yo_lu_refactor_work(self._h))
 
     def column_permutation(self):
         return self.col_perm
@@ -407,11 +428,13 @@ class Grid:
         bprime, bf = _take_matrix(hb), _take_matrix(hf)
         return bprime, bf, rows[: bf.rows].astype(np.int64)
 
-    def jacobian(self, seed: int):
-        """The AC Newton Jacobian (CSC) and its operating point (pv, pq, vm, va)."""
+    def jacobian(self, seed: int, shift: float = 0.0, shift_seed: int = 0):
+        """The AC Newton Jacobian (CSC) and its operating point (pv, pq, vm, va). shift > 0 moves
+        every |V| and non-slack angle by shift * U[-1, 1] (drawn from shift_seed), as a Newton
+        iterate would, keeping the Jacobian's pattern."""
         L = lib()
         hj, ha = P(), P()
-        _check(L.yo_jacobian(self._h, seed, C.byref(hj), C.byref(ha)))
+        _check(L.yo_jacobian(self._h, seed, shift, shift_seed, C.byref(hj), C.byref(ha)))
         jac = _take_matrix(hj)
         s = np.zeros(4, np.uint64)
         L.yo_ac_sizes(ha, s)

@@ -312,6 +312,16 @@ int yo_lu_solve_transpose_many(void* h, uint64_t count, const double* rhs, doubl
   return guard([&] { lu_solve_transpose_many(static_cast<LuObj*>(h)->f, rhs, count, out, threads); });
 }
 
+// lu.rs:662-757: outcome 0 = Reused, 1 = Repivoted (the handle then holds the fresh factors)
+int yo_lu_refactor(void* h, MAT_ARGS, int* outcome) {
+  return guard([&] {
+    Compressed a = MAT;
+    *outcome = (int)lu_refactor(static_cast<LuObj*>(h)->f, a);
+  });
+}
+
+uint64_t yo_lu_refactor_work(void* h) { return lu_refactor_work(static_cast<LuObj*>(h)->f); }
+
 int yo_lu_refine(void* h, MAT_ARGS, const double* b, uint64_t max_steps, double tol, double* x, uint64_t* steps,
                  double* ratio) {
   return guard([&] {
@@ -378,10 +388,10 @@ int yo_dc(void* h, const uint8_t* in_service, void** bprime, void** bf, uint64_t
   });
 }
 
-int yo_jacobian(void* h, uint64_t seed, void** jac, void** ac_out) {
+int yo_jacobian(void* h, uint64_t seed, double shift, uint64_t shift_seed, void** jac, void** ac_out) {
   return guard([&] {
     auto* o = new AcObj;
-    o->ac = make_jacobian(static_cast<GridObj*>(h)->g, seed);
+    o->ac = make_jacobian(static_cast<GridObj*>(h)->g, seed, shift, shift_seed);
     auto* j = new MatrixObj;
     j->m = o->ac.jacobian;
     *jac = j;

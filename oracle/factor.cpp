@@ -402,7 +402,70 @@ Lu gilbert_peierls(const Compressed& a, const std::vector<size_t>& q, double tau
   f.col_perm = q;
   f.pivot_threshold = tau;
   f.prefer_diagonal = prefer_diagonal;
+  f.a_ptr = a.ptr;
+  f.a_idx = a.idx;
   return f;
+}
+
+// lu.rs:662-757. Column k of the ordered matrix is scattered into x (indexed by pivot step),
+// then for each row j above the diagonal in U(:, k), in increasing order (a topological order,
+// L being lower triangular in step numbering), u_jk = x[j] and x[i] -= l_ij * u_jk over L(:, j)
+// below its diagonal. The reference fuses nested column pairs (lu.rs:693-714), applying column
+// j's term and then column j + 1's to each shared row: the same operations on each x[i] in the
+// same order as the unfused loop written here, so the same values. The reused pivot x[k] must
+// be nonzero and at least tau times the largest magnitude below it (a fold that starts from
+// |x[k]| and takes b only when b > a); otherwise the matrix is factored afresh with the same
+// column ordering. On an error f is unchanged.
+RefactorOutcome lu_refactor(Lu& f, const Compressed& a) {
+  const size_t n = f.dim();
+  if (a.row_major || a.rows != n || a.cols != n || a.ptr != f.a_ptr || a.idx != f.a_idx)
+    throw SparseError(Err::PatternMismatch, "the matrix's pattern differs from the factored one");
+  const double tau = f.pivot_threshold;
+  std::vector<double> l_val(f.l.nnz(), 0.0), u_val(f.u.nnz(), 0.0), x(n, 0.0);
+  const std::vector<size_t>& step_of = f.row_perm_inverse;
+  for (size_t k = 0; k < n; ++k) {
+    const size_t col = f.col_perm[k];
+    for (size_t p = a.ptr[col]; p < a.ptr[col + 1]; ++p) x[step_of[a.idx[p]]] = a.data[p];
+    const size_t diagonal = f.u.ptr[k + 1] - 1;
+    for (size_t p = f.u.ptr[k]; p < diagonal; ++p) {
+      const size_t j = f.u.idx[p];
+      const double u_jk = x[j];
+      x[j] = 0.0;
+      u_val[p] = u_jk;
+      for (size_t q = f.l.ptr[j] + 1; q < f.l.ptr[j + 1]; ++q) x[f.l.idx[q]] = x[f.l.idx[q]] - l_val[q] * u_jk;
+    }
+    const double pivot = x[k];
+    double largest = std::fabs(pivot);
+    for (size_t q = f.l.ptr[k] + 1; q < f.l.ptr[k + 1]; ++q) {
+      const double b = std::fabs(x[f.l.idx[q]]);
+      if (b > largest) largest = b;
+    }
+    if (pivot == 0.0 || std::fabs(pivot) < tau * largest) {
+      f = gilbert_peierls(a, f.col_perm, tau, f.prefer_diagonal);
+      return RefactorOutcome::Repivoted;
+    }
+    u_val[diagonal] = pivot;
+    x[k] = 0.0;
+    l_val[f.l.ptr[k]] = 1.0;
+    for (size_t q = f.l.ptr[k] + 1; q < f.l.ptr[k + 1]; ++q) {
+      const size_t row = f.l.idx[q];
+      l_val[q] = x[row] / pivot;
+      x[row] = 0.0;
+    }
+  }
+  f.l.data = std::move(l_val);
+  f.u.data = std::move(u_val);
+  return RefactorOutcome::Reused;
+}
+
+uint64_t lu_refactor_work(const Lu& f) {
+  uint64_t work = 0;
+  for (size_t k = 0; k < f.dim(); ++k)
+    for (size_t p = f.u.ptr[k]; p + 1 < f.u.ptr[k + 1]; ++p) {
+      const size_t j = f.u.idx[p];
+      work += f.l.ptr[j + 1] - f.l.ptr[j] - 1;
+    }
+  return work;
 }
 
 // lu.rs:640-649 (back_substitute) inside lu.rs:770-785 (substitute)

@@ -129,6 +129,7 @@ struct Lu {
   std::vector<size_t> row_perm, row_perm_inverse, col_perm;
   double pivot_threshold = 0.1;
   bool prefer_diagonal = true;
+  std::vector<size_t> a_ptr, a_idx;  // the factored matrix's pattern, which refactor requires
   size_t dim() const { return row_perm.size(); }
 };
 
@@ -144,6 +145,13 @@ Lu factor_lu(const SymbolicLu& s, const Compressed& a_csc, double pivot_threshol
 Lu gilbert_peierls(const Compressed& a, const std::vector<size_t>& q, double tau,
                    bool prefer_diagonal);                                                 // lu.rs:268-414
 void lu_substitute(const Lu& f, double* x, std::vector<double>& scratch);                 // lu.rs:770-785
+// lu.rs:662-757 (Lu::refactor): new values on the same pattern, reusing the pivots; a pivot
+// that fails the threshold factors afresh under the same column ordering (Repivoted)
+enum class RefactorOutcome : int { Reused = 0, Repivoted = 1 };
+RefactorOutcome lu_refactor(Lu& f, const Compressed& a_csc);
+// The multiply-subtracts one refactor makes: the sum over columns k of the below-diagonal
+// entries of L(:, j) for each j above the diagonal in U(:, k)
+uint64_t lu_refactor_work(const Lu& f);
 std::vector<double> lu_solve_transpose(const Lu& f, const double* b);                     // lu.rs:506-533
 
 // factorization.rs:52-92 (solve_many / par_solve_many): rhs and solutions are `count`

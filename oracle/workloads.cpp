@@ -346,7 +346,7 @@ CxCsr cx_from_triplets(size_t n, const std::vector<std::pair<size_t, size_t>>& p
 // N(0, 0.1) rad (the slack's angle 0). Ybus follows makeYbus's stamps (ac.rs:55-174: four
 // branch stamps, then one shunt stamp per bus); the Jacobian follows JacobianAssembly::new and
 // refill (ac.rs:1311-1427) with pv_pq = PV buses then PQ buses (topology.rs:277-283).
-AcJacobian make_jacobian(const Grid& g, uint64_t seed) {
+AcJacobian make_jacobian(const Grid& g, uint64_t seed, double shift, uint64_t shift_seed) {
   const size_t nb = g.buses, nl = g.from.size();
   Rng ac(seed, 3);
   std::vector<double> r(nl), bc(nl);
@@ -371,8 +371,16 @@ AcJacobian make_jacobian(const Grid& g, uint64_t seed) {
   for (size_t i = 0; i < nb; ++i) {
     out.vm[i] = 0.95 + 0.1 * ac.uniform();
     out.va[i] = i == 0 ? 0.0 : 0.1 * ac.normal();
-    v[i] = from_polar(out.vm[i], out.va[i]);
   }
+  if (shift > 0.0) {
+    Rng step(shift_seed, 4);
+    for (size_t i = 0; i < nb; ++i) {
+      out.vm[i] += shift * (2.0 * step.uniform() - 1.0);
+      const double da = shift * (2.0 * step.uniform() - 1.0);
+      if (i != 0) out.va[i] += da;
+    }
+  }
+  for (size_t i = 0; i < nb; ++i) v[i] = from_polar(out.vm[i], out.va[i]);
   // ac.rs:137-151 with tap 1 and no shift
   std::vector<std::pair<size_t, size_t>> positions;
   std::vector<Cx> stamps;

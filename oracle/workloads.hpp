@@ -42,6 +42,9 @@ struct AcJacobian {
   std::vector<double> vm, va;
   std::vector<double> r, bc;  // branch resistance and line charging, pu
 };
-AcJacobian make_jacobian(const Grid& g, uint64_t seed);
+// `shift` > 0 moves the operating point as a Newton iterate would (every |V| and every non-slack
+// angle by shift * U[-1, 1], drawn from `shift_seed`), keeping the branches and the PV/PQ split,
+// so the Jacobian keeps its pattern
+AcJacobian make_jacobian(const Grid& g, uint64_t seed, double shift = 0.0, uint64_t shift_seed = 0);
 
 }  // namespace oracle
