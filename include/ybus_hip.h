@@ -42,7 +42,8 @@ enum {
   YBUS_HIP_TOO_LARGE = 4,        /* CudaError::TooLarge: beyond 32-bit device indices */
   YBUS_HIP_DEVICE_ERROR = 5,     /* CudaError::Driver: the HIP runtime failed */
   YBUS_HIP_OUT_OF_MEMORY = 6,    /* a device allocation failed */
-  YBUS_HIP_INTERNAL = 7          /* a bound inside a kernel was exceeded (a library bug) */
+  YBUS_HIP_INTERNAL = 7,         /* a bound inside a kernel was exceeded (a library bug) */
+  YBUS_HIP_PATTERN_MISMATCH = 8  /* SparseError::PatternMismatch (lu.rs:662-667) */
 };
 
 /* Precision and direction of a device solve. */
@@ -85,6 +86,25 @@ typedef struct {
   uint64_t bytes;
 } ybus_hip_kernel_stats;
 
+/* Outcome of a device refactorization (RefactorOutcome, lu.rs:88-99): the pivots were
+ * reused and the factor now holds the new values, or a reused pivot failed the threshold and the
+ * factor is unchanged (the reference would factor afresh here; on the device that is the
+ * caller's host factorization and a new upload). */
+enum {
+  YBUS_HIP_REFACTOR_REUSED = 0,
+  YBUS_HIP_REFACTOR_NEEDS_PIVOTING = 1
+};
+
+/* Statistics of a factor's refactorizations and value updates since its upload. */
+typedef struct {
+  uint64_t calls;           /* refactorizations run on the device */
+  uint64_t reused;          /* of those, the ones whose pivots all passed */
+  uint64_t updates;         /* value updates (ybus_hip_factor_update_lu_values) */
+  double refactor_ms;       /* summed HIP-event time of the refactor kernel */
+  double refresh_ms;        /* summed HIP-event time of rewriting the sweeps' values */
+  uint64_t multiply_subtracts; /* of one refactorization */
+} ybus_hip_refactor_stats;
+
 /* The message of the calling thread's last failure (ybus_ffi/src/lib.rs:54-80): copies at most
  * len - 1 bytes and a NUL into buf, and returns the length the whole message needs, NUL
  * included. */
@@ -119,6 +139,33 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx *ctx, uint64_t n, const uint64_t 
                                   const uint64_t *u_ptr, const uint64_t *u_idx,
                                   const double *u_val, const uint64_t *row_perm,
                                   const uint64_t *col_perm, ybus_hip_factor **out);
+
+/* Lu::refactor (lu.rs:662-757) on the device, for an uploaded real LU factor: the matrix A
+ * (n x n, CSC: a_ptr n + 1, a_idx and a_val a_ptr[n] entries, rows increasing in each column)
+ * is factored with the factor's own pivots and patterns, P A Q = L U, each column k by the
+ * reference's loop (u_jk = x[j] for the rows j above the diagonal in U(:, k), in increasing
+ * order, then x[i] -= l_ij * u_jk over L(:, j) below its diagonal; the pivot x[k] reused when it
+ * is nonzero and at least pivot_threshold times the largest magnitude below it; l = x / pivot),
+ * so the new values are the CPU refactor's bit for bit. *outcome is
+ * YBUS_HIP_REFACTOR_REUSED (the factor's solves now use the new values) or
+ * YBUS_HIP_REFACTOR_NEEDS_PIVOTING (the factor is unchanged). The first call fixes the matrix
+ * pattern; a later call with another pattern, or a matrix with an entry outside the factor's
+ * pattern, is YBUS_HIP_PATTERN_MISMATCH. pivot_threshold is the factorization's threshold
+ * (LuOptions::pivot_threshold, lu.rs:248-255), in (0, 1]. A column of L with more than 512
+ * entries below its diagonal, or a column of L and U together longer than 7,328, is beyond the
+ * device refactor (YBUS_HIP_TOO_LARGE): refactor on the host and call
+ * ybus_hip_factor_update_lu_values. */
+int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor *f, uint64_t n, const uint64_t *a_ptr,
+                                    const uint64_t *a_idx, const double *a_val,
+                                    double pivot_threshold, int32_t *outcome);
+/* New values for an uploaded real LU factor with the same pattern and permutations (a host
+ * refactorization that reused its pivots): l_val and u_val in the uploaded CSC order. The
+ * factor's sweeps are rewritten on the device, with no new analysis. */
+int32_t ybus_hip_factor_update_lu_values(ybus_hip_factor *f, const double *l_val, const double *u_val);
+/* The refactorization statistics, and the new L and U values of the factor (in the uploaded
+ * CSC order: l_val L's entries, u_val U's; either may be NULL). */
+int32_t ybus_hip_factor_refactor_stats(const ybus_hip_factor *f, ybus_hip_refactor_stats *out);
+int32_t ybus_hip_factor_lu_values(ybus_hip_factor *f, double *l_val, double *u_val);
 
 /* Complex factors (new: the reference's GPU path uploads real factors only, factor.rs:103-133):
  * the arrays of ybus_hip_factor_upload_cholesky / _lu, with each value an interleaved (re, im)

@@ -1,0 +1,207 @@
+"""The device LU refactorization (Lu::refactor, lu.rs:662-757) and value updates against the
+oracle's CPU refactor: the new factor values bit for bit, then the solves (plain, transposed,
+FP32-refined) with those values, on both kernel paths; the reference's refactor cases
+(factorization_test.rs:966-1141): pivots that fail the threshold, repeated refactors, pattern
+errors leaving the factor unchanged; and a wait given up inside the kernel (a tiny spin limit
+set from the host), which must leave the factor unchanged."""
+import numpy as np
+import pytest
+
+import oracle as O
+import ybus_amd as Y
+from tests.fixtures import random_square_sparse, right_hand_sides
+from tests.gpu_common import gpu, path  # noqa: F401
+from tests.test_oracle_refactor import new_values, ring_blocks, with_values
+
+pytestmark = [pytest.mark.gpu]
+
+
+def same_bits(a, b) -> bool:
+    a, b = np.ascontiguousarray(a, dtype=np.float64), np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def cases():
+    """(name, matrix, ordering, new matrix) with pivots that survive the new values."""
+    ring = ring_blocks(60, 6)
+    rand = random_square_sparse(90, 4, 3.0, 2).to_csc()
+    grid = O.make_grid(600, 20261207)
+    jac = grid.jacobian(20261207)[0]
+    return [
+        ("ring", ring, O.AMD, new_values(ring, 106)),
+        ("random", rand, O.AMD, new_values(rand, 10)),
+        ("jacobian", jac, O.COLAMD, grid.jacobian(20261207, 0.003, 5)[0]),
+    ]
+
+
+@pytest.mark.usefixtures("path")
+@pytest.mark.parametrize("case", cases(), ids=lambda c: c[0])
+def test_refactor_matches_the_cpu_bit_for_bit_and_solves_with_it(gpu, case):
+    _, a, ordering, b = case
+    lu = O.Lu(a, ordering)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    assert factor.refactor(b) == "reused"
+    assert lu.refactor(b) == "reused"
+    l_val, u_val = factor.lu_values()
+    assert same_bits(l_val, lu.l.data) and same_bits(u_val, lu.u.data)
+    n = lu.n
+    rhs = right_hand_sides(n, 9, 3)
+    assert same_bits(factor.solve_many(rhs), lu.solve_many(rhs))
+    assert same_bits(factor.solve_transpose_many(rhs), lu.solve_transpose_many(rhs))
+    # FP32 sweeps from the new values, refined in FP64 against the new matrix, to the floor the
+    # CPU's FP64 refinement reaches with the same factor (about 1e-11 for the Jacobian)
+    factor.set_matrix(b)
+    x, steps, ratio = factor.solve_many_refined(rhs, 1e-12, 10)
+    floor = np.array([lu.solve_refined_normwise(b, r, 10, 1e-12)[2] for r in rhs])
+    assert np.all(ratio <= np.maximum(1e-12, 4.0 * floor)), (ratio, floor)
+    stats = factor.refactor_stats()
+    assert stats["calls"] == 1 and stats["reused"] == 1
+    assert stats["multiply_subtracts"] == lu.refactor_work()
+
+
+def test_refactor_repeats_bit_for_bit(gpu):
+    # factorization_test.rs:1097-1127 on the device: b, then a, then b again
+    a = ring_blocks(100, 7)
+    b = new_values(a, 107)
+    lu = O.Lu(a, O.AMD)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    assert factor.refactor(b) == "reused"
+    first = factor.lu_values()
+    assert factor.refactor(a) == "reused"
+    cpu_a = O.Lu(a, O.AMD)
+    assert cpu_a.refactor(a) == "reused"
+    l_val, u_val = factor.lu_values()
+    assert same_bits(l_val, cpu_a.l.data) and same_bits(u_val, cpu_a.u.data)
+    assert factor.refactor(b) == "reused"
+    again = factor.lu_values()
+    assert same_bits(first[0], again[0]) and same_bits(first[1], again[1])
+
+
+def test_a_pivot_below_the_threshold_leaves_the_factor_unchanged(gpu):
+    # factorization_test.rs:993-1030: the first pivot shrinks below 0.1 of its column's largest
+    t = [(0, 0, 4.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 4.0), (1, 2, 1.0), (2, 1, 1.0), (2, 2, 4.0)]
+    a = O.Matrix.from_triplets(3, 3, t, row_major=False)
+    lu = O.Lu(a, O.NATURAL)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    before = factor.lu_values()
+    for value in [0.01, 0.0]:
+        data = a.data.copy()
+        data[0] = value
+        b = with_values(a, data)
+        assert factor.refactor(b) == "needs_pivoting"
+        assert O.Lu(a, O.NATURAL).refactor(b) == "repivoted"
+        after = factor.lu_values()
+        assert same_bits(before[0], after[0]) and same_bits(before[1], after[1])
+        rhs = np.array([[1.0, 2.0, 3.0]])
+        assert same_bits(factor.solve_many(rhs), lu.solve_many(rhs))
+    stats = factor.refactor_stats()
+    assert stats["calls"] == 2 and stats["reused"] == 0
+    # the threshold is the caller's: a first pivot of 0.5 against 1 below it passes 0.1, not 1
+    data = a.data.copy()
+    data[0] = 0.5
+    b = with_values(a, data)
+    assert factor.refactor(b, 0.1) == "reused"
+    kept = factor.lu_values()
+    assert factor.refactor(b, 1.0) == "needs_pivoting"
+    after = factor.lu_values()
+    assert same_bits(kept[0], after[0]) and same_bits(kept[1], after[1])
+
+
+def test_pattern_errors_leave_the_factor_unchanged(gpu):
+    # factorization_test.rs:1048-1065
+    a = ring_blocks(20, 3)
+    lu = O.Lu(a, O.AMD)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    before = factor.lu_values()
+    # an entry the factor's pattern has no place for (pivot step, column step) outside L and U
+    pattern = set(zip(lu.l.idx.tolist(), np.repeat(np.arange(lu.n), np.diff(lu.l.ptr)).tolist()))
+    pattern |= set(zip(lu.u.idx.tolist(), np.repeat(np.arange(lu.n), np.diff(lu.u.ptr)).tolist()))
+    step_of, column_of = np.argsort(lu.row_perm), np.argsort(lu.col_perm)
+    dense = a.to_dense()
+    outside = next((int(i), int(j)) for i, j in np.argwhere(dense == 0.0)
+                   if (int(step_of[i]), int(column_of[j])) not in pattern)
+    entries = [(int(i), int(j), float(dense[i, j])) for i, j in np.argwhere(dense != 0.0)]
+    grown = O.Matrix.from_triplets(a.rows, a.cols, entries + [(outside[0], outside[1], 1.0)], row_major=False)
+    with pytest.raises(Y.PatternMismatch):
+        factor.refactor(grown)
+    after = factor.lu_values()
+    assert same_bits(before[0], after[0]) and same_bits(before[1], after[1])
+    # a good call fixes the pattern; a smaller one (inside the factor's) is then refused too
+    assert factor.refactor(new_values(a, 3)) == "reused"
+    fixed = factor.lu_values()
+    fewer = O.Matrix.from_triplets(a.rows, a.cols, [e for e in entries if e[0] != e[1] or e[0] > 0],
+                                   row_major=False)
+    with pytest.raises(Y.PatternMismatch):
+        factor.refactor(fewer)
+    with pytest.raises(Y.InvalidArgument):
+        factor.refactor(a, 0.0)
+    after = factor.lu_values()
+    assert same_bits(fixed[0], after[0]) and same_bits(fixed[1], after[1])
+    # a Cholesky factor has no LU values
+    spd = O.Cholesky(O.Matrix.from_triplets(2, 2, [(0, 0, 2.0), (1, 1, 3.0)]))
+    chol = Y.GpuFactor.from_cholesky(gpu, spd)
+    with pytest.raises(Y.InvalidArgument):
+        chol.refactor(a)
+
+
+@pytest.mark.usefixtures("path")
+def test_value_updates_from_a_host_refactor(gpu):
+    grid = O.make_grid(800, 20261208)
+    a = grid.jacobian(20261208)[0]
+    lu = O.Lu(a, O.COLAMD)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    rhs = right_hand_sides(lu.n, 5, 8)
+    for step in range(3):
+        b = grid.jacobian(20261208, 0.002, 30 + step)[0]
+        assert lu.refactor(b) == "reused"
+        factor.update_lu_values(lu.l.data, lu.u.data)
+        assert same_bits(factor.solve_many(rhs), lu.solve_many(rhs))
+        assert same_bits(factor.solve_transpose_many(rhs), lu.solve_transpose_many(rhs))
+    l_val, u_val = factor.lu_values()
+    assert same_bits(l_val, lu.l.data) and same_bits(u_val, lu.u.data)
+    assert factor.refactor_stats()["updates"] == 3
+
+
+def test_the_jacobian_of_25k_buses_refactors_bit_for_bit(gpu):
+    grid = O.make_grid(25000, 20261205)
+    a = grid.jacobian(20261205)[0]
+    b = grid.jacobian(20261205, 0.001, 7)[0]
+    lu = O.Lu(a, O.COLAMD)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    for round_ in range(2):
+        assert factor.refactor(b) == "reused"
+        assert factor.refactor(a) == "reused"
+    assert factor.refactor(b) == "reused"
+    assert lu.refactor(b) == "reused"
+    l_val, u_val = factor.lu_values()
+    assert same_bits(l_val, lu.l.data) and same_bits(u_val, lu.u.data)
+    rhs = right_hand_sides(lu.n, 4, 2)
+    assert same_bits(factor.solve_many(rhs), lu.solve_many(rhs))
+
+
+def test_a_wait_given_up_leaves_the_factor_unchanged(gpu, monkeypatch):
+    # a chain: column k of a tridiagonal matrix reads column k - 1, so with a one-poll spin limit
+    # some wait is given up (the kernel still ends: every column sets its flag); the factor keeps
+    # its values either way
+    n = 3000
+    t = [(i, i, 4.0) for i in range(n)] + [(i, i + 1, -1.0) for i in range(n - 1)] + \
+        [(i + 1, i, -1.0) for i in range(n - 1)]
+    a = O.Matrix.from_triplets(n, n, t, row_major=False)
+    lu = O.Lu(a, O.NATURAL)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    before = factor.lu_values()
+    b = new_values(a, 5)
+    monkeypatch.setenv("YBUS_HIP_REFACTOR_SPIN", "1")
+    try:
+        outcome = factor.refactor(b)
+    except Y.GpuError as e:
+        assert e.status == Y.INTERNAL
+        outcome = None
+    if outcome is None:
+        after = factor.lu_values()
+        assert same_bits(before[0], after[0]) and same_bits(before[1], after[1])
+    monkeypatch.delenv("YBUS_HIP_REFACTOR_SPIN")
+    assert factor.refactor(b) == "reused"
+    assert lu.refactor(b) == "reused"
+    l_val, u_val = factor.lu_values()
+    assert same_bits(l_val, lu.l.data) and same_bits(u_val, lu.u.data)

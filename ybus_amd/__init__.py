@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libybus_hip.so")
 
-OK, NO_DEVICE, INVALID_ARGUMENT, SHAPE_MISMATCH, TOO_LARGE, DEVICE_ERROR, OUT_OF_MEMORY, INTERNAL = range(8)
+OK, NO_DEVICE, INVALID_ARGUMENT, SHAPE_MISMATCH, TOO_LARGE, DEVICE_ERROR, OUT_OF_MEMORY, INTERNAL, PATTERN_MISMATCH = range(9)
 FP64, FP32_REFINED, TRANSPOSE = 0, 1, 2
 
 # ---- errors: CudaError (ybus_cuda/src/lib.rs:88-124) ----
@@ -55,7 +55,12 @@ class DeviceError(GpuError):
     pass
 
 
-_ERRORS = {TOO_LARGE: TooLarge, INVALID_ARGUMENT: InvalidArgument, DEVICE_ERROR: DeviceError}
+class PatternMismatch(GpuError):
+    """SparseError::PatternMismatch (lu.rs:662-667): the matrix's pattern is not the factor's."""
+
+
+_ERRORS = {TOO_LARGE: TooLarge, INVALID_ARGUMENT: InvalidArgument, DEVICE_ERROR: DeviceError,
+           PATTERN_MISMATCH: PatternMismatch}
 
 _lib = None
 
@@ -92,6 +97,11 @@ class _Tuning(C.Structure):
 
 class _KernelStats(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("milliseconds", C.c_double), ("bytes", C.c_uint64)]
+
+
+class _RefactorStats(C.Structure):
+    _fields_ = [("calls", C.c_uint64), ("reused", C.c_uint64), ("updates", C.c_uint64),
+                ("refactor_ms", C.c_double), ("refresh_ms", C.c_double), ("multiply_subtracts", C.c_uint64)]
 
 
 P = C.c_void_p
@@ -166,6 +176,10 @@ def _declare(L):
         "ybus_hip_batch_solve": (i32, [P, C.c_void_p, u64p, C.c_void_p]),
         "ybus_hip_batch_kernel_stats": (None, [P, C.POINTER(_KernelStats)]),
         "ybus_hip_batch_reset_kernel_stats": (None, [P]),
+        "ybus_hip_factor_refactor_lu": (i32, [P, U64, u64p, u64p, f64p, C.c_double, C.POINTER(C.c_int32)]),
+        "ybus_hip_factor_update_lu_values": (i32, [P, f64p, f64p]),
+        "ybus_hip_factor_refactor_stats": (i32, [P, C.POINTER(_RefactorStats)]),
+        "ybus_hip_factor_lu_values": (i32, [P, C.c_void_p, C.c_void_p]),
         "ybus_hip_analyze_cholesky": (i32, [U64, u64p, u64p, P, P, P, P]),
         "ybus_hip_analyze_lu": (i32, [U64, u64p, u64p, u64p, u64p, P, P, P, P]),
     }
@@ -351,6 +365,7 @@ class GpuFactor:
         self._complex = is_complex
         self._nnz_a = -1
         self._block: Optional[GpuBlock] = None
+        self._lu_sizes = (0, 0)
 
     @staticmethod
     def from_cholesky(gpu: Gpu, cholesky) -> "GpuFactor":
@@ -379,7 +394,9 @@ class GpuFactor:
         _check(library().ybus_hip_factor_upload_lu(gpu._h, int(n), _u64(l_ptr), _u64(l_idx), _f64(l_val),
                                                    _u64(u_ptr), _u64(u_idx), _f64(u_val), _u64(row_perm),
                                                    _u64(col_perm), C.byref(h)))
-        return GpuFactor(gpu, h, int(n))
+        factor = GpuFactor(gpu, h, int(n))
+        factor._lu_sizes = (int(np.asarray(l_ptr)[-1]), int(np.asarray(u_ptr)[-1]))
+        return factor
 
     @staticmethod
     def from_cholesky_complex(gpu: Gpu, cholesky) -> "GpuFactor":
@@ -469,6 +486,37 @@ class GpuFactor:
 
     def reset_kernel_stats(self):
         library().ybus_hip_factor_reset_kernel_stats(self._h)
+
+    def refactor(self, a, pivot_threshold: float = 0.1) -> str:
+        """Lu::refactor (lu.rs:662-757) on the device: `a` (CSC: ``ptr``, ``idx``, ``data``, as
+        ybus_sparse's CscMatrix) factored with this LU factor's pivots and patterns. Returns
+        "reused" (the factor now holds the new values, the CPU refactor's bits) or
+        "needs_pivoting" (a reused pivot failed the threshold; the factor is unchanged, and the
+        reference's fresh factorization is the caller's, followed by a new upload). Raises
+        PatternMismatch for another pattern than the first call's."""
+        if getattr(a, "row_major", False):
+            raise InvalidArgument(INVALID_ARGUMENT, "refactor takes the matrix in CSC form")
+        outcome = C.c_int32()
+        _check(library().ybus_hip_factor_refactor_lu(self._h, self._n, _u64(a.ptr), _u64(a.idx), _f64(a.data),
+                                                     float(pivot_threshold), C.byref(outcome)))
+        return "reused" if outcome.value == 0 else "needs_pivoting"
+
+    def update_lu_values(self, l_val, u_val):
+        """New L and U values (the uploaded CSC order, the same pattern and permutations), as a host
+        refactor that reused its pivots makes them: the device sweeps are rewritten in place."""
+        _check(library().ybus_hip_factor_update_lu_values(self._h, _f64(l_val), _f64(u_val)))
+
+    def refactor_stats(self) -> dict:
+        s = _RefactorStats()
+        _check(library().ybus_hip_factor_refactor_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in _RefactorStats._fields_}
+
+    def lu_values(self):
+        """The factor's current (L values, U values) in the uploaded CSC order."""
+        nl = int(self._lu_sizes[0])
+        l_val, u_val = np.zeros(nl), np.zeros(int(self._lu_sizes[1]))
+        _check(library().ybus_hip_factor_lu_values(self._h, l_val.ctypes.data, u_val.ctypes.data))
+        return l_val, u_val
 
     def set_matrix(self, a):
         """The matrix itself (CSR when ``a.row_major`` else CSC), for FP32_REFINED residuals."""

@@ -1749,4 +1749,205 @@ hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fw
   return hipGetLastError();
 }
 
+// ---- LU refactorization (lu.rs:662-757) ----
+
+// A column's completion flag, read without ordering (an acquire fence follows a successful read)
+__device__ inline uint32_t flag_of(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+size_t refactor_lds_bytes(int32_t x_cap) {
+  return (size_t)x_cap * sizeof(double) + (size_t)REFACTOR_STAGE * (sizeof(double) + sizeof(int32_t)) +
+         2 * 64 * sizeof(int32_t);
+}
+
+// One wavefront per column, columns taken from control[0] in level order, so every column a
+// wavefront waits on was taken earlier by a wavefront that is running (the wait ends). The steps
+// of a column go in batches: the next (up to 64) rows of U(:, k) whose columns of L are written
+// and whose entries fit the stage; their L values and target slots are loaded together into LDS,
+// then the steps run in order on x in LDS, each a broadcast read of u_jk and one subtract per
+// entry of L(:, j) (distinct slots, all after j's). A wait that outlasts spin_limit polls gives
+// the column up (counted in control[2]); its flag is still set, so no wavefront waits forever.
+__global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
+  extern __shared__ double refactor_lds[];
+  double* xs = refactor_lds;
+  double* sval = xs + d.x_cap;
+  int32_t* stgt = reinterpret_cast<int32_t*>(sval + REFACTOR_STAGE);
+  int32_t* s_off = stgt + REFACTOR_STAGE;
+  int32_t* s_first = s_off + 64;
+  __shared__ int32_t taken;
+  const int lane = threadIdx.x;
+  for (;;) {
+    if (lane == 0) taken = atomicAdd(&d.control[0], 1);
+    __syncthreads();
+    const int32_t c = taken;
+    __syncthreads();
+    if (c >= d.n) break;
+    const int32_t k = d.order[c];
+    const int4 col = d.column[k];
+    const int32_t u0 = col.x, nu = col.y, l0 = col.z, below = col.w - 1;
+    const int32_t steps = nu - 1, m = nu + below;
+    for (int32_t i = lane; i < m; i += 64) xs[i] = 0.0;
+    __syncthreads();
+    const int2 ar = d.a_range[k];
+    for (int32_t p = ar.x + lane; p < ar.y; p += 64) xs[d.a_slot[p]] = d.a_val[p];
+    __syncthreads();
+    int64_t e = d.upd_start[k];
+    int32_t t = 0;
+    bool abandoned = false;
+    while (t < steps) {
+      const int32_t s = t + lane;
+      int32_t len = 0, first = 0;
+      bool ready = false;
+      if (s < steps) {
+        const int32_t j = d.u_row[u0 + s];
+        first = d.l_ptr[j] + 1;
+        len = d.l_ptr[j + 1] - first;
+        ready = flag_of(&d.done[j]) == d.epoch;
+      }
+      int32_t incl = len;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+      }
+      const uint64_t good = __ballot(s < steps && ready && incl <= REFACTOR_STAGE);
+      const int32_t count = good == ~0ull ? 64 : (int32_t)__builtin_ctzll(~good);
+      if (count == 0) {
+        if (__shfl(ready ? 1 : 0, 0, 64)) {
+          // written, but longer than the stage (the host admits no such factor)
+          abandoned = true;
+          break;
+        }
+        const int32_t j = d.u_row[u0 + t];
+        uint32_t polls = 0;
+        while (flag_of(&d.done[j]) != d.epoch && polls <= d.spin_limit) {
+          __builtin_amdgcn_s_sleep(1);
+          ++polls;
+        }
+        if (polls > d.spin_limit) {
+          abandoned = true;
+          break;
+        }
+        continue;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      const int32_t total = __shfl(incl, count - 1, 64);
+      if (lane < count) {
+        s_off[lane] = incl - len;
+        s_first[lane] = first;
+      }
+      __syncthreads();
+      constexpr int R = REFACTOR_STAGE / 64;
+      double v[R];
+      int32_t g[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int32_t f = lane + 64 * r;
+        if (f < total) {
+          // the step entry f belongs to: the last one whose offset is at most f
+          int32_t lo = 0, hi = count - 1;
+          while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= f)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          v[r] = d.l_val[s_first[lo] + (f - s_off[lo])];
+          g[r] = d.target[e + f];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int32_t f = lane + 64 * r;
+        if (f < total) {
+          sval[f] = v[r];
+          stgt[f] = g[r];
+        }
+      }
+      __syncthreads();
+      for (int32_t q = 0; q < count; ++q) {
+        const double u = xs[t + q];
+        if (lane == 0) d.u_val[u0 + t + q] = u;
+        const int32_t o1 = q + 1 < count ? s_off[q + 1] : total;
+        for (int32_t f = s_off[q] + lane; f < o1; f += 64) {
+          const int32_t slot = stgt[f];
+          xs[slot] = xs[slot] - sval[f] * u;
+        }
+        __syncthreads();
+      }
+      t += count;
+      e += total;
+    }
+    if (!abandoned) {
+      // lu.rs:717-733: the reused pivot against the largest magnitude below it
+      const double pivot = xs[nu - 1];
+      double big = -1.0;
+      for (int32_t i = lane; i < below; i += 64) {
+        const double b = fabs(xs[nu + i]);
+        if (b > big) big = b;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(big, off, 64);
+        if (o > big) big = o;
+      }
+      double largest = fabs(pivot);
+      if (big > largest) largest = big;
+      if (lane == 0) {
+        if (pivot == 0.0 || fabs(pivot) < d.tau * largest) atomicMin(&d.control[1], k);
+        d.u_val[u0 + nu - 1] = pivot;
+        d.l_val[l0] = 1.0;
+      }
+      for (int32_t i = lane; i < below; i += 64) d.l_val[l0 + 1 + i] = xs[nu + i] / pivot;
+    } else if (lane == 0) {
+      atomicAdd(&d.control[2], 1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) __hip_atomic_store(&d.done[k], d.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid) {
+  if (d.n <= 0) return hipSuccess;
+  if (grid <= 0 || refactor_lds_bytes(d.x_cap) > 65536 - 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(refactor_kernel, dim3((unsigned)grid), dim3(64), refactor_lds_bytes(d.x_cap), stream, d);
+  return hipGetLastError();
+}
+
+__global__ void refresh_entries_kernel(RefreshDev r, const double* __restrict__ src) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= r.entries) return;
+  const double v = src[r.vpos[p]];
+  r.val64[p] = v;
+  r.val32[p] = (float)v;
+}
+
+__global__ void refresh_rows_kernel(RefreshDev r, const double* __restrict__ src) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= r.n) return;
+  const int32_t p0 = r.row_ptr[k], p1 = r.row_ptr[k + 1];
+  double v[RECORD_ENTRIES];
+#pragma unroll
+  for (int q = 0; q < RECORD_ENTRIES; ++q) v[q] = p0 + q < p1 ? src[r.vpos[p0 + q]] : 0.0;
+  r.val01[k] = make_double2(v[0], v[1]);
+  r.val23[k] = make_double2(v[2], v[3]);
+  r.v32[k] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+  if (r.dpos) {
+    const double dg = src[r.dpos[k]];
+    r.diag64[k] = dg;
+    r.diag32[k] = (float)dg;
+    r.rdiag64[k] = dg;
+    r.rdiag32[k] = (float)dg;
+  }
+}
+
+hipError_t launch_refresh(hipStream_t stream, const RefreshDev& r, const double* src) {
+  if (r.entries > 0)
+    hipLaunchKernelGGL(refresh_entries_kernel, dim3((unsigned)((r.entries + 255) / 256)), dim3(256), 0, stream, r,
+                       src);
+  if (r.n > 0)
+    hipLaunchKernelGGL(refresh_rows_kernel, dim3((unsigned)((r.n + 255) / 256)), dim3(256), 0, stream, r, src);
+  return hipGetLastError();
+}
+
 }  // namespace ybus_hip

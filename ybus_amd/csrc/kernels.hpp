@@ -242,4 +242,59 @@ hipError_t launch_fill(hipStream_t stream, int32_t* p, int32_t value, int32_t co
 hipError_t launch_sddmm(hipStream_t stream, const int32_t* row, const int32_t* col, int64_t nnz, const double* bbar,
                         const double* x, int64_t rs, int64_t cs, int32_t ncols, double* acc, bool first);
 
+// ---- numeric refactorization of an LU factor on the device (lu.rs:662-757) ----
+//
+// Column k of the ordered matrix P A Q is gathered into x (its slots: U(:, k)'s rows in order,
+// the diagonal last, then L(:, k)'s rows below the diagonal), then for each row j above the
+// diagonal in U(:, k), in increasing order, u_jk = x[slot of j] and x[slot] -= l_ij * u_jk over
+// L(:, j) below its diagonal, then the pivot check and l = x / pivot. Columns are taken in an
+// order where each comes after the columns it reads (by level), each by one wavefront that
+// waits on the flags of the columns it reads.
+struct RefactorDev {
+  int32_t n = 0;
+  const int32_t* order = nullptr;      // columns in level order
+  const int4* column = nullptr;        // per column: (first U entry, U entries, first L entry, L entries)
+  const int2* a_range = nullptr;       // per column: the matrix entries [x, y) of its column
+  const int32_t* a_slot = nullptr;     // per matrix entry: its slot in its column's x
+  const double* a_val = nullptr;       // the matrix's values, as stored by the caller (CSC)
+  const int32_t* u_row = nullptr;      // U's row (pivot step) of each entry
+  const int32_t* l_ptr = nullptr;      // n + 1
+  const int64_t* upd_start = nullptr;  // per column: its first entry in `target`
+  const int32_t* target = nullptr;     // per multiply-subtract: the slot it updates
+  double* l_val = nullptr;             // the new factors, in the uploaded CSC order
+  double* u_val = nullptr;
+  uint32_t* done = nullptr;  // per column: `epoch` once its values are written
+  uint32_t epoch = 0;
+  int32_t* control = nullptr;  // [0] next column to take (0 at launch), [1] the smallest column whose
+                               // pivot fails (n at launch), [2] waits given up (0 at launch)
+  double tau = 0.1;
+  int32_t x_cap = 0;          // slots of x in LDS (at least the longest column's)
+  uint32_t spin_limit = 0;    // polls of a flag before a wait is given up
+};
+// The longest L column (below its diagonal) one batch of steps stages
+constexpr int32_t REFACTOR_STAGE = 512;
+size_t refactor_lds_bytes(int32_t x_cap);
+hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid);
+
+// A sweep's value arrays rewritten from new factor values: every entry takes src[vpos[p]] (a
+// position in the new L values followed by the new U values), every diagonal src[dpos[i]], in
+// FP64 and FP32 and in the rows' inline records (their first four entries; zero past a row's end).
+struct RefreshDev {
+  int32_t n = 0;
+  int64_t entries = 0;
+  const int32_t* row_ptr = nullptr;
+  const int32_t* vpos = nullptr;
+  const int32_t* dpos = nullptr;  // null for a unit diagonal
+  double* val64 = nullptr;
+  float* val32 = nullptr;
+  double* diag64 = nullptr;
+  float* diag32 = nullptr;
+  double2* val01 = nullptr;
+  double2* val23 = nullptr;
+  float4* v32 = nullptr;
+  double* rdiag64 = nullptr;
+  float* rdiag32 = nullptr;
+};
+hipError_t launch_refresh(hipStream_t stream, const RefreshDev& r, const double* src);
+
 }  // namespace ybus_hip

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <deque>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -363,6 +364,42 @@ struct ReachIndex {
   }
 };
 
+// An LU factor's pattern as uploaded (CSC, pivot-step numbering), kept for a later refactor or
+// value update
+struct LuPattern {
+  std::vector<uint64_t> l_ptr, l_idx, u_ptr, u_idx, row_perm, col_perm;
+  std::vector<double> values;  // the uploaded values: L's, then U's
+};
+
+// One sweep's value arrays and where each value comes from in the new factor values (L's, then
+// U's, in the uploaded CSC order)
+struct SweepRefresh {
+  DevSweep* sweep = nullptr;
+  DevBuf<int32_t> vpos, dpos;
+  SweepRefresh() = default;
+  SweepRefresh(const SweepRefresh&) = delete;
+};
+
+// New values for an uploaded LU factor: the sweeps' refresh maps, and (once a matrix pattern is
+// given) the device refactorization's arrays for that pattern
+struct RefactorPlan {
+  std::deque<SweepRefresh> refresh;
+  // the factor's values (L's, then U's), and the next ones being made
+  DevBuf<double> vals, next;
+  bool numeric = false;
+  std::vector<uint64_t> a_ptr, a_idx;  // the matrix pattern the numeric plan was built for
+  DevBuf<int32_t> order, a_slot, u_row, l_ptr, target, control;
+  DevBuf<int4> column;
+  DevBuf<int2> a_range;
+  DevBuf<int64_t> upd_start;
+  DevBuf<double> a_val;
+  DevBuf<uint32_t> done;
+  uint32_t epoch = 0;
+  int32_t x_cap = 0;
+  int grid = 0;
+  ybus_hip_refactor_stats stats{};
+};
+
 struct ybus_hip_factor {
   ybus_hip_ctx* ctx = nullptr;
   int64_t n = 0;
@@ -391,6 +428,10 @@ struct ybus_hip_factor {
   bool complex = false;
   DevSweepC zfwd, zbwd, ztfwd, ztbwd;
   DevBuf<double> zb, zt, zx, zvec;
+  // a real LU factor's pattern, and the plan for new values (built by the first refactor or
+  // value update)
+  LuPattern lu_host;
+  std::unique_ptr<RefactorPlan> refactor;
   DevCsr a;   // the matrix itself, by user rows, for the FP64 residual
   DevCsr at;  // its transpose, by rows (A's columns), for the residual of a transposed refinement
   // workspace for `cols` columns
@@ -826,6 +867,14 @@ static size_t lds_capacity(const ybus_hip_ctx* ctx) {
 }
 
 // Chooses the path (the solution of one column in LDS, or wide tiles) and sets the defaults
+// Polls of a column flag before the device refactor gives a wait up: far beyond any wait of a
+// working refactor (each poll sleeps); YBUS_HIP_REFACTOR_SPIN sets it
+static uint32_t refactor_spin_limit() {
+  const char* v = std::getenv("YBUS_HIP_REFACTOR_SPIN");
+  if (v) return (uint32_t)std::max(1L, std::atol(v));
+  return 1u << 22;
+}
+
 static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   f->ctx = ctx;
   f->n = n;
@@ -1025,9 +1074,334 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
     f->tfwd.upload(schedule(ut, cp, ident), ctx->stream, true);
     f->tbwd.upload(schedule(lt, ident, rp), ctx->stream, true);
     f->has_transpose = true;
+    f->lu_host.l_ptr.assign(l_ptr, l_ptr + n + 1);
+    f->lu_host.l_idx.assign(l_idx, l_idx + l_ptr[n]);
+    f->lu_host.u_ptr.assign(u_ptr, u_ptr + n + 1);
+    f->lu_host.u_idx.assign(u_idx, u_idx + u_ptr[n]);
+    f->lu_host.row_perm.assign(row_perm, row_perm + n);
+    f->lu_host.col_perm.assign(col_perm, col_perm + n);
+    f->lu_host.values.assign(l_val, l_val + l_ptr[n]);
+    f->lu_host.values.insert(f->lu_host.values.end(), u_val, u_val + u_ptr[n]);
     check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
     *out = f.release();
+  });
+}
+
+// ---- new values for an uploaded LU factor (lu.rs:662-757) ----
+
+static void require_lu(const ybus_hip_factor* f) {
+  if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "null factor");
+  if (!f->lu || f->complex || f->lu_host.l_ptr.empty())
+    fail(YBUS_HIP_INVALID_ARGUMENT, "new values need a real LU factor");
+}
+
+// The sweeps' refresh maps: each sweep scheduled again from factors whose values are their own
+// positions (plus one), so every value slot of the uploaded layout names its source
+static RefactorPlan& refactor_plan(ybus_hip_factor* f) {
+  if (f->refactor) return *f->refactor;
+  auto plan = std::make_unique<RefactorPlan>();
+  const LuPattern& h = f->lu_host;
+  const int64_t n = f->n;
+  const uint64_t nl = h.l_ptr[n], nu = h.u_ptr[n];
+  std::vector<double> lpos(nl), upos(nu);
+  for (uint64_t p = 0; p < nl; ++p) lpos[p] = (double)(p + 1);
+  for (uint64_t p = 0; p < nu; ++p) upos[p] = (double)(nl + p + 1);
+  const RowForm l = lu_forward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data());
+  const RowForm u = lu_backward(n, h.u_ptr.data(), h.u_idx.data(), upos.data());
+  const RowForm ut = lu_transpose_forward(n, h.u_ptr.data(), h.u_idx.data(), upos.data());
+  const RowForm lt = lu_transpose_backward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data());
+  std::vector<int64_t> rp(n), cp(n), ident(n);
+  for (int64_t i = 0; i < n; ++i) {
+    rp[i] = (int64_t)h.row_perm[i];
+    cp[i] = (int64_t)h.col_perm[i];
+    ident[i] = i;
+  }
+  hipStream_t stream = f->stream();
+  auto add = [&](DevSweep& d, const Sweep& s) {
+    if (s.row_ptr != d.host.row_ptr || s.dep != d.host.dep || s.unit != d.host.unit)
+      fail(YBUS_HIP_INTERNAL, "a sweep's schedule differs from its upload");
+    plan->refresh.emplace_back();
+    SweepRefresh& r = plan->refresh.back();
+    r.sweep = &d;
+    std::vector<int32_t> vp(s.val.size());
+    for (size_t p = 0; p < vp.size(); ++p) vp[p] = (int32_t)s.val[p] - 1;
+    r.vpos.upload(vp, stream);
+    if (!s.unit) {
+      std::vector<int32_t> dp(s.diag.size());
+      for (size_t k = 0; k < dp.size(); ++k) dp[k] = (int32_t)s.diag[k] - 1;
+      r.dpos.upload(dp, stream);
+    }
+    check(hipStreamSynchronize(stream), "refresh maps");
+  };
+  add(f->fwd, schedule(l, rp, ident));
+  add(f->bwd, schedule(u, ident, cp));
+  add(f->tfwd, schedule(ut, cp, ident));
+  add(f->tbwd, schedule(lt, ident, rp));
+  if (f->fits) {
+    add(f->cfwd, schedule(l, ident, ident));
+    add(f->cbwd, schedule(u, ident, ident));
+    add(f->ctfwd, schedule(ut, ident, ident));
+    add(f->ctbwd, schedule(lt, ident, ident));
+  }
+  plan->vals.upload(h.values, stream);
+  plan->next.reserve(h.values.size());
+  check(hipStreamSynchronize(stream), "refresh maps");
+  f->refactor = std::move(plan);
+  return *f->refactor;
+}
+
+// Rewrites every sweep's values from plan.next, then makes next the factor's values; returns the
+// HIP-event milliseconds of the rewrite
+static double refresh_sweeps(ybus_hip_factor* f, RefactorPlan& plan) {
+  hipStream_t stream = f->stream();
+  check(hipEventRecord(f->ev0, stream), "hipEventRecord");
+  for (SweepRefresh& r : plan.refresh) {
+    DevSweep& d = *r.sweep;
+    RefreshDev v;
+    v.n = d.host.n;
+    v.entries = (int64_t)d.host.dep.size();
+    v.row_ptr = d.row_ptr.p;
+    v.vpos = r.vpos.p;
+    v.dpos = d.host.unit ? nullptr : r.dpos.p;
+    v.val64 = d.val64.p;
+    v.val32 = d.val32.p;
+    v.diag64 = d.diag64.p;
+    v.diag32 = d.diag32.p;
+    v.val01 = d.val01.p;
+    v.val23 = d.val23.p;
+    v.v32 = d.v32.p;
+    v.rdiag64 = d.rdiag64.p;
+    v.rdiag32 = d.rdiag32.p;
+    check(launch_refresh(stream, v, plan.next.p), "refresh sweeps");
+  }
+  check(hipEventRecord(f->ev1, stream), "hipEventRecord");
+  check(hipEventSynchronize(f->ev1), "refresh sweeps");
+  float ms = 0.0f;
+  check(hipEventElapsedTime(&ms, f->ev0, f->ev1), "hipEventElapsedTime");
+  std::swap(plan.vals.p, plan.next.p);
+  std::swap(plan.vals.len, plan.next.len);
+  return ms;
+}
+
+// Binary search for `row` in idx[lo, hi) (increasing); -1 when absent
+static int64_t find_row(const std::vector<uint64_t>& idx, uint64_t lo, uint64_t hi, uint64_t row) {
+  auto it = std::lower_bound(idx.begin() + (int64_t)lo, idx.begin() + (int64_t)hi, row);
+  return it != idx.begin() + (int64_t)hi && *it == row ? (int64_t)(it - idx.begin()) : -1;
+}
+
+// The device refactorization's arrays for the matrix pattern (a_ptr, a_idx)
+static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t* a_ptr, const uint64_t* a_idx) {
+  const LuPattern& h = f->lu_host;
+  const int64_t n = f->n;
+  const uint64_t nnz_a = a_ptr[n];
+  // each column of U: rows increasing, all above the diagonal but the last; of L: the diagonal,
+  // then rows increasing below it
+  int32_t longest_l = 0;
+  int64_t x_cap = 1;
+  for (int64_t k = 0; k < n; ++k) {
+    for (uint64_t p = h.u_ptr[k]; p + 1 < h.u_ptr[k + 1]; ++p)
+      if (h.u_idx[p] >= (uint64_t)k || (p > h.u_ptr[k] && h.u_idx[p] <= h.u_idx[p - 1]))
+        fail(YBUS_HIP_INVALID_ARGUMENT, "refactor needs U's columns in increasing row order");
+    for (uint64_t p = h.l_ptr[k] + 1; p < h.l_ptr[k + 1]; ++p)
+      if (h.l_idx[p] <= (uint64_t)k || (p > h.l_ptr[k] + 1 && h.l_idx[p] <= h.l_idx[p - 1]))
+        fail(YBUS_HIP_INVALID_ARGUMENT, "refactor needs L's columns in increasing row order");
+    longest_l = std::max<int32_t>(longest_l, (int32_t)(h.l_ptr[k + 1] - h.l_ptr[k] - 1));
+    x_cap = std::max<int64_t>(x_cap, (int64_t)(h.u_ptr[k + 1] - h.u_ptr[k] + h.l_ptr[k + 1] - h.l_ptr[k] - 1));
+  }
+  x_cap = round_up(x_cap, 8);
+  if (longest_l > REFACTOR_STAGE || refactor_lds_bytes((int32_t)std::min<int64_t>(x_cap, INDEX_LIMIT)) > 65536 - 256)
+    fail(YBUS_HIP_TOO_LARGE, "a column of the factor is too long for the device refactor (L below its diagonal: " +
+                                 std::to_string(longest_l) + ", L and U: " + std::to_string(x_cap) + ")");
+  std::vector<uint64_t> step_of(n);
+  for (int64_t s = 0; s < n; ++s) step_of[h.row_perm[s]] = (uint64_t)s;
+  // slot of pivot step `row` in column k's x, or -1
+  auto slot_of = [&](int64_t k, uint64_t row) -> int64_t {
+    const uint64_t u0 = h.u_ptr[k], nu = h.u_ptr[k + 1] - u0;
+    if (row == (uint64_t)k) return (int64_t)nu - 1;
+    if (row < (uint64_t)k) {
+      const int64_t p = find_row(h.u_idx, u0, h.u_ptr[k + 1] - 1, row);
+      return p < 0 ? -1 : p - (int64_t)u0;
+    }
+    const int64_t p = find_row(h.l_idx, h.l_ptr[k] + 1, h.l_ptr[k + 1], row);
+    return p < 0 ? -1 : (int64_t)nu + (p - (int64_t)h.l_ptr[k] - 1);
+  };
+  std::vector<int4> column(n);
+  std::vector<int2> a_range(n);
+  std::vector<int32_t> a_slot(nnz_a), level(n, 0), u_row(h.u_idx.size()), l_ptr(n + 1);
+  std::vector<int64_t> upd_start(n + 1, 0);
+  std::vector<int32_t> target;
+  for (int64_t k = 0; k < n; ++k) {
+    const uint64_t col = h.col_perm[k];
+    column[k] = make_int4((int32_t)h.u_ptr[k], (int32_t)(h.u_ptr[k + 1] - h.u_ptr[k]), (int32_t)h.l_ptr[k],
+                          (int32_t)(h.l_ptr[k + 1] - h.l_ptr[k]));
+    a_range[k] = make_int2((int32_t)a_ptr[col], (int32_t)a_ptr[col + 1]);
+    for (uint64_t p = a_ptr[col]; p < a_ptr[col + 1]; ++p) {
+      if (p > a_ptr[col] && a_idx[p] <= a_idx[p - 1])
+        fail(YBUS_HIP_INVALID_ARGUMENT, "the matrix's columns must list their rows in increasing order");
+      const int64_t slot = slot_of(k, step_of[a_idx[p]]);
+      if (slot < 0) fail(YBUS_HIP_PATTERN_MISMATCH, "the matrix has an entry outside the factor's pattern");
+      a_slot[p] = (int32_t)slot;
+    }
+    upd_start[k] = (int64_t)target.size();
+    for (uint64_t p = h.u_ptr[k]; p + 1 < h.u_ptr[k + 1]; ++p) {
+      const uint64_t j = h.u_idx[p];
+      level[k] = std::max(level[k], level[j] + 1);
+      for (uint64_t q = h.l_ptr[j] + 1; q < h.l_ptr[j + 1]; ++q) {
+        const int64_t slot = slot_of(k, h.l_idx[q]);
+        if (slot < 0) fail(YBUS_HIP_INVALID_ARGUMENT, "the factor's pattern is not closed under elimination");
+        target.push_back((int32_t)slot);
+      }
+    }
+  }
+  upd_start[n] = (int64_t)target.size();
+  for (size_t p = 0; p < u_row.size(); ++p) u_row[p] = (int32_t)h.u_idx[p];
+  for (int64_t k = 0; k <= n; ++k) l_ptr[k] = (int32_t)h.l_ptr[k];
+  // columns by (level, column)
+  const int32_t levels = n ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+  std::vector<int32_t> start(levels + 1, 0), order(n);
+  for (int64_t k = 0; k < n; ++k) ++start[level[k] + 1];
+  for (int32_t v = 0; v < levels; ++v) start[v + 1] += start[v];
+  for (int64_t k = 0; k < n; ++k) order[start[level[k]]++] = (int32_t)k;
+  hipStream_t stream = f->stream();
+  plan.order.upload(order, stream);
+  plan.column.upload(column, stream);
+  plan.a_range.upload(a_range, stream);
+  plan.a_slot.upload(a_slot, stream);
+  plan.u_row.upload(u_row, stream);
+  plan.l_ptr.upload(l_ptr, stream);
+  plan.upd_start.upload(upd_start, stream);
+  plan.target.upload(target, stream);
+  plan.a_val.reserve(nnz_a);
+  plan.done.reserve(n);
+  check(hipMemsetAsync(plan.done.p, 0, (size_t)std::max<int64_t>(n, 1) * sizeof(uint32_t), stream), "refactor flags");
+  plan.control.reserve(4);
+  check(hipStreamSynchronize(stream), "refactor plan");
+  plan.epoch = 0;
+  plan.x_cap = (int32_t)x_cap;
+  // enough wavefronts for the widest levels, a few per SIMD
+  plan.grid = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 16 * (int64_t)f->ctx->cus);
+  plan.stats.multiply_subtracts = (uint64_t)target.size();
+  plan.a_ptr.assign(a_ptr, a_ptr + n + 1);
+  plan.a_idx.assign(a_idx, a_idx + nnz_a);
+  plan.numeric = true;
+}
+
+int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64_t* a_ptr, const uint64_t* a_idx,
+                                    const double* a_val, double pivot_threshold, int32_t* outcome) {
+  return guard([&] {
+    require_lu(f);
+    if (!a_ptr || !outcome) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (a_ptr[n] && (!a_idx || !a_val)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (!(pivot_threshold > 0.0 && pivot_threshold <= 1.0))
+      fail(YBUS_HIP_INVALID_ARGUMENT, "the pivot threshold must be in (0, 1]");
+    if ((int64_t)n != f->n) fail(YBUS_HIP_PATTERN_MISMATCH, "the matrix's shape differs from the factor's");
+    std::string why;
+    if (!valid_compressed((int64_t)n, (int64_t)n, a_ptr, a_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "A: " + why);
+    if (a_ptr[n] > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the matrix has too many entries");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    RefactorPlan& plan = refactor_plan(f);
+    if (!plan.numeric) {
+      build_numeric(f, plan, a_ptr, a_idx);
+    } else if (!std::equal(plan.a_ptr.begin(), plan.a_ptr.end(), a_ptr) ||
+               !std::equal(plan.a_idx.begin(), plan.a_idx.end(), a_idx)) {
+      fail(YBUS_HIP_PATTERN_MISMATCH, "the matrix's pattern differs from the one this factor was refactored with");
+    }
+    hipStream_t stream = f->stream();
+    const uint64_t nnz_a = a_ptr[n];
+    if (nnz_a) {
+      double* host = f->staging.get(nnz_a);
+      std::memcpy(host, a_val, nnz_a * sizeof(double));
+      check(hipMemcpyAsync(plan.a_val.p, host, nnz_a * sizeof(double), hipMemcpyHostToDevice, stream), "upload A");
+    }
+    if (++plan.epoch == 0) {
+      check(hipMemsetAsync(plan.done.p, 0, (size_t)std::max<int64_t>(f->n, 1) * sizeof(uint32_t), stream),
+            "refactor flags");
+      plan.epoch = 1;
+    }
+    check(launch_fill(stream, plan.control.p, 0, 4), "refactor control");
+    check(launch_fill(stream, plan.control.p + 1, (int32_t)f->n, 1), "refactor control");
+    RefactorDev d;
+    d.n = (int32_t)f->n;
+    d.order = plan.order.p;
+    d.column = plan.column.p;
+    d.a_range = plan.a_range.p;
+    d.a_slot = plan.a_slot.p;
+    d.a_val = plan.a_val.p;
+    d.u_row = plan.u_row.p;
+    d.l_ptr = plan.l_ptr.p;
+    d.upd_start = plan.upd_start.p;
+    d.target = plan.target.p;
+    d.l_val = plan.next.p;
+    d.u_val = plan.next.p + f->lu_host.l_ptr[n];
+    d.done = plan.done.p;
+    d.epoch = plan.epoch;
+    d.control = plan.control.p;
+    d.tau = pivot_threshold;
+    d.x_cap = plan.x_cap;
+    d.spin_limit = refactor_spin_limit();
+    check(hipEventRecord(f->ev0, stream), "hipEventRecord");
+    check(launch_refactor(stream, d, plan.grid), "refactor");
+    check(hipEventRecord(f->ev1, stream), "hipEventRecord");
+    int32_t control[4] = {0, 0, 0, 0};
+    check(hipMemcpyAsync(control, plan.control.p, sizeof(control), hipMemcpyDeviceToHost, stream), "refactor control");
+    check(hipStreamSynchronize(stream), "refactor");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, f->ev0, f->ev1), "hipEventElapsedTime");
+    plan.stats.calls += 1;
+    plan.stats.refactor_ms += ms;
+    if (control[2] != 0)
+      fail(YBUS_HIP_INTERNAL, "the device refactor gave up " + std::to_string(control[2]) +
+                                  " column(s) waiting on another; the factor is unchanged");
+    if (control[0] < (int32_t)f->n) fail(YBUS_HIP_INTERNAL, "the device refactor left columns untaken");
+    if (control[1] < (int32_t)f->n) {
+      *outcome = YBUS_HIP_REFACTOR_NEEDS_PIVOTING;
+      return;
+    }
+    plan.stats.refresh_ms += refresh_sweeps(f, plan);
+    plan.stats.reused += 1;
+    *outcome = YBUS_HIP_REFACTOR_REUSED;
+  });
+}
+
+int32_t ybus_hip_factor_update_lu_values(ybus_hip_factor* f, const double* l_val, const double* u_val) {
+  return guard([&] {
+    require_lu(f);
+    const uint64_t nl = f->lu_host.l_ptr[f->n], nu = f->lu_host.u_ptr[f->n];
+    if ((nl && !l_val) || (nu && !u_val)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    RefactorPlan& plan = refactor_plan(f);
+    hipStream_t stream = f->stream();
+    double* host = f->staging.get(nl + nu);
+    if (nl) std::memcpy(host, l_val, nl * sizeof(double));
+    if (nu) std::memcpy(host + nl, u_val, nu * sizeof(double));
+    check(hipMemcpyAsync(plan.next.p, host, (nl + nu) * sizeof(double), hipMemcpyHostToDevice, stream), "upload values");
+    plan.stats.refresh_ms += refresh_sweeps(f, plan);
+    plan.stats.updates += 1;
+  });
+}
+
+int32_t ybus_hip_factor_refactor_stats(const ybus_hip_factor* f, ybus_hip_refactor_stats* out) {
+  return guard([&] {
+    require_lu(f);
+    if (!out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    *out = f->refactor ? f->refactor->stats : ybus_hip_refactor_stats{};
+  });
+}
+
+int32_t ybus_hip_factor_lu_values(ybus_hip_factor* f, double* l_val, double* u_val) {
+  return guard([&] {
+    require_lu(f);
+    const uint64_t nl = f->lu_host.l_ptr[f->n], nu = f->lu_host.u_ptr[f->n];
+    std::vector<double> v;
+    if (f->refactor) {
+      check(hipSetDevice(f->ctx->device), "hipSetDevice");
+      v.resize(nl + nu);
+      check(hipMemcpy(v.data(), f->refactor->vals.p, (nl + nu) * sizeof(double), hipMemcpyDeviceToHost), "values");
+    }
+    const std::vector<double>& src = f->refactor ? v : f->lu_host.values;
+    if (l_val && nl) std::memcpy(l_val, src.data(), nl * sizeof(double));
+    if (u_val && nu) std::memcpy(u_val, src.data() + nl, nu * sizeof(double));
   });
 }
 
