@@ -351,15 +351,18 @@ class Lu:
     def permutation(self):
         return self.row_perm
 
-    def refactor(self, a_csc: Matrix) -> str:
+    def refactor(self, a_csc: Matrix, read: bool = True) -> str:
         """Lu::refactor (lu.rs:662-757): "reused" (same pivots and patterns, new values) or
         "repivoted" (factored afresh under the same column ordering); raises SparseError
-        (PatternMismatch, SingularMatrix) and leaves the factors unchanged on an error."""
+        (PatternMismatch, SingularMatrix) and leaves the factors unchanged on an error. With
+        read=False the new factors stay in the C++ object (``l``, ``u`` and the permutations
+        here are not updated until a later refactor with read=True)."""
         if a_csc.row_major:
             a_csc = a_csc.to_csc()
         outcome = C.c_int()
         _check(lib().yo_lu_refactor(self._h, *a_csc.args(), C.byref(outcome)))
-        self._read()
+        if read:
+            self._read()
         return "reused" if outcome.value == 0 else "repivoted"
 
     def refactor_work(self) -> int:

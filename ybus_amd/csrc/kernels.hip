@@ -1751,23 +1751,37 @@ hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fw
 
 // ---- LU refactorization (lu.rs:662-757) ----
 
-// A column's completion flag, read without ordering (an acquire fence follows a successful read)
-__device__ inline uint32_t flag_of(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
+// LDS of one wavefront's column: x (x_cap doubles), the staged L values and their target slots,
+// and each staged step's first entry and L position
 size_t refactor_lds_bytes(int32_t x_cap) {
   return (size_t)x_cap * sizeof(double) + (size_t)REFACTOR_STAGE * (sizeof(double) + sizeof(int32_t)) +
          2 * 64 * sizeof(int32_t);
 }
 
+// Columns hand their L values over between workgroups anywhere on the device (other XCDs, other
+// L2s) through device-coherent accesses: the values are stored and loaded as device-scope atomics
+// (which bypass the caches a device-scope fence would have to write back or invalidate), the
+// stores are complete (vmcnt 0) before the column's flag is stored, and the loads are issued only
+// after the flag has been read.
+__device__ inline uint32_t flag_of(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double shared_value(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void share_value(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One wavefront per column, columns taken from control[0] in level order, so every column a
-// wavefront waits on was taken earlier by a wavefront that is running (the wait ends). The steps
+// wavefront waits on was taken earlier by a wavefront that is running (each wait ends). The steps
 // of a column go in batches: the next (up to 64) rows of U(:, k) whose columns of L are written
 // and whose entries fit the stage; their L values and target slots are loaded together into LDS,
 // then the steps run in order on x in LDS, each a broadcast read of u_jk and one subtract per
-// entry of L(:, j) (distinct slots, all after j's). A wait that outlasts spin_limit polls gives
-// the column up (counted in control[2]); its flag is still set, so no wavefront waits forever.
+// entry of L(:, j) (distinct slots, all after j's), the next step's entries read while this one's
+// update is made. (A single workgroup is one wavefront, so __syncthreads orders its LDS accesses
+// and costs no barrier.) A wait that outlasts spin_limit polls gives the column up (counted in
+// control[2]); its flag is still set, so no wavefront waits forever.
 __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
   extern __shared__ double refactor_lds[];
   double* xs = refactor_lds;
@@ -1775,24 +1789,20 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
   int32_t* stgt = reinterpret_cast<int32_t*>(sval + REFACTOR_STAGE);
   int32_t* s_off = stgt + REFACTOR_STAGE;
   int32_t* s_first = s_off + 64;
-  __shared__ int32_t taken;
   const int lane = threadIdx.x;
   for (;;) {
-    if (lane == 0) taken = atomicAdd(&d.control[0], 1);
-    __syncthreads();
-    const int32_t c = taken;
-    __syncthreads();
+    int32_t c = 0;
+    if (lane == 0) c = atomicAdd(&d.control[0], 1);
+    c = __shfl(c, 0, 64);
     if (c >= d.n) break;
-    const int32_t k = d.order[c];
-    const int4 col = d.column[k];
-    const int32_t u0 = col.x, nu = col.y, l0 = col.z, below = col.w - 1;
+    const int4 head = d.head[c], span = d.span[c];
+    const int32_t k = head.x, u0 = head.y, nu = head.z, l0 = head.w, below = span.x;
     const int32_t steps = nu - 1, m = nu + below;
     for (int32_t i = lane; i < m; i += 64) xs[i] = 0.0;
     __syncthreads();
-    const int2 ar = d.a_range[k];
-    for (int32_t p = ar.x + lane; p < ar.y; p += 64) xs[d.a_slot[p]] = d.a_val[p];
+    for (int32_t p = span.y + lane; p < span.z; p += 64) xs[d.a_slot[p]] = d.a_val[p];
     __syncthreads();
-    int64_t e = d.upd_start[k];
+    int32_t e = span.w;
     int32_t t = 0;
     bool abandoned = false;
     while (t < steps) {
@@ -1800,10 +1810,10 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
       int32_t len = 0, first = 0;
       bool ready = false;
       if (s < steps) {
-        const int32_t j = d.u_row[u0 + s];
-        first = d.l_ptr[j] + 1;
-        len = d.l_ptr[j + 1] - first;
-        ready = flag_of(&d.done[j]) == d.epoch;
+        const int4 st = d.step[u0 + s];
+        first = st.y;
+        len = st.z;
+        ready = flag_of(&d.done[st.x]) == d.epoch;
       }
       int32_t incl = len;
       for (int off = 1; off < 64; off <<= 1) {
@@ -1818,7 +1828,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
           abandoned = true;
           break;
         }
-        const int32_t j = d.u_row[u0 + t];
+        const int32_t j = d.step[u0 + t].x;
         uint32_t polls = 0;
         while (flag_of(&d.done[j]) != d.epoch && polls <= d.spin_limit) {
           __builtin_amdgcn_s_sleep(1);
@@ -1830,7 +1840,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
         }
         continue;
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       const int32_t total = __shfl(incl, count - 1, 64);
       if (lane < count) {
         s_off[lane] = incl - len;
@@ -1853,7 +1863,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
             else
               hi = mid - 1;
           }
-          v[r] = d.l_val[s_first[lo] + (f - s_off[lo])];
+          v[r] = shared_value(&d.l_val[s_first[lo] + (f - s_off[lo])]);
           g[r] = d.target[e + f];
         }
       }
@@ -1866,15 +1876,30 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
         }
       }
       __syncthreads();
+      // step q's first 64 entries are read one step ahead; entries past 64 (a long column of L)
+      // are read in their own step
+      int32_t o0 = s_off[0], o1 = count > 1 ? s_off[1] : total;
+      bool has = o0 + lane < o1;
+      int32_t slot = has ? stgt[o0 + lane] : 0;
+      double lv = has ? sval[o0 + lane] : 0.0;
       for (int32_t q = 0; q < count; ++q) {
         const double u = xs[t + q];
-        if (lane == 0) d.u_val[u0 + t + q] = u;
-        const int32_t o1 = q + 1 < count ? s_off[q + 1] : total;
-        for (int32_t f = s_off[q] + lane; f < o1; f += 64) {
-          const int32_t slot = stgt[f];
-          xs[slot] = xs[slot] - sval[f] * u;
+        const double xv = has ? xs[slot] : 0.0;
+        const int32_t n0 = o1, n1 = q + 2 < count ? s_off[q + 2] : total;
+        const bool next_has = q + 1 < count && n0 + lane < n1;
+        const int32_t next_slot = next_has ? stgt[n0 + lane] : 0;
+        const double next_lv = next_has ? sval[n0 + lane] : 0.0;
+        if (has) xs[slot] = xv - lv * u;
+        for (int32_t f = o0 + 64 + lane; f < o1; f += 64) {
+          const int32_t sl = stgt[f];
+          xs[sl] = xs[sl] - sval[f] * u;
         }
         __syncthreads();
+        o0 = n0;
+        o1 = n1;
+        has = next_has;
+        slot = next_slot;
+        lv = next_lv;
       }
       t += count;
       e += total;
@@ -1895,15 +1920,18 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
       if (big > largest) largest = big;
       if (lane == 0) {
         if (pivot == 0.0 || fabs(pivot) < d.tau * largest) atomicMin(&d.control[1], k);
-        d.u_val[u0 + nu - 1] = pivot;
         d.l_val[l0] = 1.0;
       }
-      for (int32_t i = lane; i < below; i += 64) d.l_val[l0 + 1 + i] = xs[nu + i] / pivot;
+      // x's U slots hold u_jk (no step writes a slot at or before its own), the last one the pivot
+      for (int32_t i = lane; i < nu; i += 64) d.u_val[u0 + i] = xs[i];
+      for (int32_t i = lane; i < below; i += 64) share_value(&d.l_val[l0 + 1 + i], xs[nu + i] / pivot);
     } else if (lane == 0) {
       atomicAdd(&d.control[2], 1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // this column's stores complete, then its flag
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (lane == 0) __hip_atomic_store(&d.done[k], d.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
   }
 }
 

@@ -252,14 +252,15 @@ hipError_t launch_sddmm(hipStream_t stream, const int32_t* row, const int32_t* c
 // waits on the flags of the columns it reads.
 struct RefactorDev {
   int32_t n = 0;
-  const int32_t* order = nullptr;      // columns in level order
-  const int4* column = nullptr;        // per column: (first U entry, U entries, first L entry, L entries)
-  const int2* a_range = nullptr;       // per column: the matrix entries [x, y) of its column
+  // per position c of the column order: (column k, its first U entry, U entries, first L entry)
+  // and (L entries below the diagonal, first and end matrix entry of its column, first target)
+  const int4* head = nullptr;
+  const int4* span = nullptr;
+  // per U entry above a diagonal (at its position in U): (row j, first L entry below j's
+  // diagonal, their count, 0)
+  const int4* step = nullptr;
   const int32_t* a_slot = nullptr;     // per matrix entry: its slot in its column's x
   const double* a_val = nullptr;       // the matrix's values, as stored by the caller (CSC)
-  const int32_t* u_row = nullptr;      // U's row (pivot step) of each entry
-  const int32_t* l_ptr = nullptr;      // n + 1
-  const int64_t* upd_start = nullptr;  // per column: its first entry in `target`
   const int32_t* target = nullptr;     // per multiply-subtract: the slot it updates
   double* l_val = nullptr;             // the new factors, in the uploaded CSC order
   double* u_val = nullptr;
@@ -273,7 +274,10 @@ struct RefactorDev {
 };
 // The longest L column (below its diagonal) one batch of steps stages
 constexpr int32_t REFACTOR_STAGE = 512;
+// LDS of one wavefront
 size_t refactor_lds_bytes(int32_t x_cap);
+// `grid` single-wavefront workgroups take the columns; control[0], [2] must hold 0 and
+// control[1] n at the launch
 hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid);
 
 // A sweep's value arrays rewritten from new factor values: every entry takes src[vpos[p]] (a

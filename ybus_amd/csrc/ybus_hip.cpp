@@ -388,10 +388,8 @@ struct RefactorPlan {
   DevBuf<double> vals, next;
   bool numeric = false;
   std::vector<uint64_t> a_ptr, a_idx;  // the matrix pattern the numeric plan was built for
-  DevBuf<int32_t> order, a_slot, u_row, l_ptr, target, control;
-  DevBuf<int4> column;
-  DevBuf<int2> a_range;
-  DevBuf<int64_t> upd_start;
+  DevBuf<int32_t> a_slot, target, control;
+  DevBuf<int4> head, span, step;
   DevBuf<double> a_val;
   DevBuf<uint32_t> done;
   uint32_t epoch = 0;
@@ -866,7 +864,6 @@ static size_t lds_capacity(const ybus_hip_ctx* ctx) {
   return (size_t)bytes;
 }
 
-// Chooses the path (the solution of one column in LDS, or wide tiles) and sets the defaults
 // Polls of a column flag before the device refactor gives a wait up: far beyond any wait of a
 // working refactor (each poll sleeps); YBUS_HIP_REFACTOR_SPIN sets it
 static uint32_t refactor_spin_limit() {
@@ -875,6 +872,7 @@ static uint32_t refactor_spin_limit() {
   return 1u << 22;
 }
 
+// Chooses the path (the solution of one column in LDS, or wide tiles) and sets the defaults
 static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   f->ctx = ctx;
   f->n = n;
@@ -1226,16 +1224,12 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
     const int64_t p = find_row(h.l_idx, h.l_ptr[k] + 1, h.l_ptr[k + 1], row);
     return p < 0 ? -1 : (int64_t)nu + (p - (int64_t)h.l_ptr[k] - 1);
   };
-  std::vector<int4> column(n);
-  std::vector<int2> a_range(n);
-  std::vector<int32_t> a_slot(nnz_a), level(n, 0), u_row(h.u_idx.size()), l_ptr(n + 1);
+  std::vector<int32_t> a_slot(nnz_a), level(n, 0);
   std::vector<int64_t> upd_start(n + 1, 0);
   std::vector<int32_t> target;
+  std::vector<int4> step(h.u_idx.size(), make_int4(0, 0, 0, 0));
   for (int64_t k = 0; k < n; ++k) {
     const uint64_t col = h.col_perm[k];
-    column[k] = make_int4((int32_t)h.u_ptr[k], (int32_t)(h.u_ptr[k + 1] - h.u_ptr[k]), (int32_t)h.l_ptr[k],
-                          (int32_t)(h.l_ptr[k + 1] - h.l_ptr[k]));
-    a_range[k] = make_int2((int32_t)a_ptr[col], (int32_t)a_ptr[col + 1]);
     for (uint64_t p = a_ptr[col]; p < a_ptr[col + 1]; ++p) {
       if (p > a_ptr[col] && a_idx[p] <= a_idx[p - 1])
         fail(YBUS_HIP_INVALID_ARGUMENT, "the matrix's columns must list their rows in increasing order");
@@ -1247,30 +1241,34 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
     for (uint64_t p = h.u_ptr[k]; p + 1 < h.u_ptr[k + 1]; ++p) {
       const uint64_t j = h.u_idx[p];
       level[k] = std::max(level[k], level[j] + 1);
+      step[p] = make_int4((int32_t)j, (int32_t)(h.l_ptr[j] + 1), (int32_t)(h.l_ptr[j + 1] - h.l_ptr[j] - 1), 0);
       for (uint64_t q = h.l_ptr[j] + 1; q < h.l_ptr[j + 1]; ++q) {
         const int64_t slot = slot_of(k, h.l_idx[q]);
         if (slot < 0) fail(YBUS_HIP_INVALID_ARGUMENT, "the factor's pattern is not closed under elimination");
         target.push_back((int32_t)slot);
       }
     }
+    if ((int64_t)target.size() > INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many multiply-subtracts for the device refactor");
   }
   upd_start[n] = (int64_t)target.size();
-  for (size_t p = 0; p < u_row.size(); ++p) u_row[p] = (int32_t)h.u_idx[p];
-  for (int64_t k = 0; k <= n; ++k) l_ptr[k] = (int32_t)h.l_ptr[k];
-  // columns by (level, column)
+  // columns by (level, column), each position's column and ranges
   const int32_t levels = n ? *std::max_element(level.begin(), level.end()) + 1 : 0;
-  std::vector<int32_t> start(levels + 1, 0), order(n);
+  std::vector<int32_t> start(levels + 1, 0);
   for (int64_t k = 0; k < n; ++k) ++start[level[k] + 1];
   for (int32_t v = 0; v < levels; ++v) start[v + 1] += start[v];
-  for (int64_t k = 0; k < n; ++k) order[start[level[k]]++] = (int32_t)k;
+  std::vector<int4> head(n), span(n);
+  for (int64_t k = 0; k < n; ++k) {
+    const int32_t c = start[level[k]]++;
+    const uint64_t col = h.col_perm[k];
+    head[c] = make_int4((int32_t)k, (int32_t)h.u_ptr[k], (int32_t)(h.u_ptr[k + 1] - h.u_ptr[k]), (int32_t)h.l_ptr[k]);
+    span[c] = make_int4((int32_t)(h.l_ptr[k + 1] - h.l_ptr[k] - 1), (int32_t)a_ptr[col], (int32_t)a_ptr[col + 1],
+                        (int32_t)upd_start[k]);
+  }
   hipStream_t stream = f->stream();
-  plan.order.upload(order, stream);
-  plan.column.upload(column, stream);
-  plan.a_range.upload(a_range, stream);
+  plan.head.upload(head, stream);
+  plan.span.upload(span, stream);
+  plan.step.upload(step, stream);
   plan.a_slot.upload(a_slot, stream);
-  plan.u_row.upload(u_row, stream);
-  plan.l_ptr.upload(l_ptr, stream);
-  plan.upd_start.upload(upd_start, stream);
   plan.target.upload(target, stream);
   plan.a_val.reserve(nnz_a);
   plan.done.reserve(n);
@@ -1323,14 +1321,11 @@ int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64
     check(launch_fill(stream, plan.control.p + 1, (int32_t)f->n, 1), "refactor control");
     RefactorDev d;
     d.n = (int32_t)f->n;
-    d.order = plan.order.p;
-    d.column = plan.column.p;
-    d.a_range = plan.a_range.p;
+    d.head = plan.head.p;
+    d.span = plan.span.p;
+    d.step = plan.step.p;
     d.a_slot = plan.a_slot.p;
     d.a_val = plan.a_val.p;
-    d.u_row = plan.u_row.p;
-    d.l_ptr = plan.l_ptr.p;
-    d.upd_start = plan.upd_start.p;
     d.target = plan.target.p;
     d.l_val = plan.next.p;
     d.u_val = plan.next.p + f->lu_host.l_ptr[n];
