@@ -1759,29 +1759,28 @@ size_t refactor_lds_bytes(int32_t x_cap) {
 }
 
 // Columns hand their L values over between workgroups anywhere on the device (other XCDs, other
-// L2s) through device-coherent accesses: the values are stored and loaded as device-scope atomics
-// (which bypass the caches a device-scope fence would have to write back or invalidate), the
-// stores are complete (vmcnt 0) before the column's flag is stored, and the loads are issued only
-// after the flag has been read.
-__device__ inline uint32_t flag_of(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// L2s) through device-coherent accesses: the values are stored and loaded as device-scope atomics,
+// which bypass the caches a device-scope fence would have to write back or invalidate. A value is
+// its own flag: every L value holds REFACTOR_PENDING until its column writes it, so a reading
+// column needs no other signal and sees each value as soon as it lands.
 __device__ inline double shared_value(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void share_value(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ inline bool pending(double v) { return (uint64_t)__double_as_longlong(v) == REFACTOR_PENDING; }
 
 // One wavefront per column, columns taken from control[0] in level order, so every column a
 // wavefront waits on was taken earlier by a wavefront that is running (each wait ends). The steps
-// of a column go in batches: the next (up to 64) rows of U(:, k) whose columns of L are written
-// and whose entries fit the stage; their L values and target slots are loaded together into LDS,
-// then the steps run in order on x in LDS, each a broadcast read of u_jk and one subtract per
-// entry of L(:, j) (distinct slots, all after j's), the next step's entries read while this one's
-// update is made. (A single workgroup is one wavefront, so __syncthreads orders its LDS accesses
-// and costs no barrier.) A wait that outlasts spin_limit polls gives the column up (counted in
-// control[2]); its flag is still set, so no wavefront waits forever.
+// of a column go in batches: the L values and target slots of the next (up to 64) rows of U(:, k)
+// that fit the stage are loaded together; the steps whose values are all written run, in order, on
+// x in LDS, each a broadcast read of u_jk and one subtract per entry of L(:, j) (distinct slots,
+// all after j's), the next step's entries read while this one's update is made; the others are
+// loaded again in the next batch. (A single workgroup is one wavefront, so __syncthreads orders
+// its LDS accesses and costs no barrier.) A wait that outlasts spin_limit polls gives the column
+// up (counted in control[2]); its L values are still written (zeros), so no wavefront waits
+// forever.
 __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
   extern __shared__ double refactor_lds[];
   double* xs = refactor_lds;
@@ -1806,56 +1805,42 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
     int32_t t = 0;
     bool abandoned = false;
     while (t < steps) {
+      // the candidates: the next steps whose entries fit the stage
       const int32_t s = t + lane;
       int32_t len = 0, first = 0;
-      bool ready = false;
       if (s < steps) {
         const int4 st = d.step[u0 + s];
         first = st.y;
         len = st.z;
-        ready = flag_of(&d.done[st.x]) == d.epoch;
       }
       int32_t incl = len;
       for (int off = 1; off < 64; off <<= 1) {
         const int32_t v = __shfl_up(incl, off, 64);
         if (lane >= off) incl += v;
       }
-      const uint64_t good = __ballot(s < steps && ready && incl <= REFACTOR_STAGE);
-      const int32_t count = good == ~0ull ? 64 : (int32_t)__builtin_ctzll(~good);
-      if (count == 0) {
-        if (__shfl(ready ? 1 : 0, 0, 64)) {
-          // written, but longer than the stage (the host admits no such factor)
-          abandoned = true;
-          break;
-        }
-        const int32_t j = d.step[u0 + t].x;
-        uint32_t polls = 0;
-        while (flag_of(&d.done[j]) != d.epoch && polls <= d.spin_limit) {
-          __builtin_amdgcn_s_sleep(1);
-          ++polls;
-        }
-        if (polls > d.spin_limit) {
-          abandoned = true;
-          break;
-        }
-        continue;
+      const uint64_t fits = __ballot(s < steps && incl <= REFACTOR_STAGE);
+      const int32_t cand = fits == ~0ull ? 64 : (int32_t)__builtin_ctzll(~fits);
+      if (cand == 0) {
+        // longer than the stage (the host admits no such factor)
+        abandoned = true;
+        break;
       }
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      const int32_t total = __shfl(incl, count - 1, 64);
-      if (lane < count) {
+      const int32_t total = __shfl(incl, cand - 1, 64);
+      if (lane < cand) {
         s_off[lane] = incl - len;
         s_first[lane] = first;
       }
       __syncthreads();
       constexpr int R = REFACTOR_STAGE / 64;
       double v[R];
-      int32_t g[R];
+      int32_t g[R], at[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int32_t f = lane + 64 * r;
+        at[r] = 0;
         if (f < total) {
           // the step entry f belongs to: the last one whose offset is at most f
-          int32_t lo = 0, hi = count - 1;
+          int32_t lo = 0, hi = cand - 1;
           while (lo < hi) {
             const int32_t mid = (lo + hi + 1) >> 1;
             if (s_off[mid] <= f)
@@ -1863,14 +1848,40 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
             else
               hi = mid - 1;
           }
-          v[r] = shared_value(&d.l_val[s_first[lo] + (f - s_off[lo])]);
+          at[r] = s_first[lo] + (f - s_off[lo]);
+          v[r] = shared_value(&d.l_val[at[r]]);
           g[r] = d.target[e + f];
         }
       }
+      // until the first step's values are all written: the steps whose entries are all written
+      // (a prefix, incl growing with the step), the unwritten values loaded again while none is
+      int32_t count = 0;
+      for (uint32_t polls = 0;; ++polls) {
+        int32_t waiting = total;  // the first entry whose value is not written yet
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int32_t f = lane + 64 * r;
+          if (f < total && pending(v[r]) && f < waiting) waiting = f;
+        }
+        for (int off = 32; off > 0; off >>= 1) waiting = min(waiting, __shfl_xor(waiting, off, 64));
+        count = __popcll(__ballot(lane < cand && incl <= waiting));
+        if (count > 0 || polls > d.spin_limit) break;
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int32_t f = lane + 64 * r;
+          if (f < total && pending(v[r])) v[r] = shared_value(&d.l_val[at[r]]);
+        }
+      }
+      if (count == 0) {
+        abandoned = true;
+        break;
+      }
+      const int32_t used = __shfl(incl, count - 1, 64);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int32_t f = lane + 64 * r;
-        if (f < total) {
+        if (f < used) {
           sval[f] = v[r];
           stgt[f] = g[r];
         }
@@ -1878,14 +1889,14 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
       __syncthreads();
       // step q's first 64 entries are read one step ahead; entries past 64 (a long column of L)
       // are read in their own step
-      int32_t o0 = s_off[0], o1 = count > 1 ? s_off[1] : total;
+      int32_t o0 = s_off[0], o1 = count > 1 ? s_off[1] : used;
       bool has = o0 + lane < o1;
       int32_t slot = has ? stgt[o0 + lane] : 0;
       double lv = has ? sval[o0 + lane] : 0.0;
       for (int32_t q = 0; q < count; ++q) {
         const double u = xs[t + q];
         const double xv = has ? xs[slot] : 0.0;
-        const int32_t n0 = o1, n1 = q + 2 < count ? s_off[q + 2] : total;
+        const int32_t n0 = o1, n1 = q + 2 < count ? s_off[q + 2] : used;
         const bool next_has = q + 1 < count && n0 + lane < n1;
         const int32_t next_slot = next_has ? stgt[n0 + lane] : 0;
         const double next_lv = next_has ? sval[n0 + lane] : 0.0;
@@ -1902,7 +1913,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
         lv = next_lv;
       }
       t += count;
-      e += total;
+      e += used;
     }
     if (!abandoned) {
       // lu.rs:717-733: the reused pivot against the largest magnitude below it
@@ -1925,14 +1936,23 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
       // x's U slots hold u_jk (no step writes a slot at or before its own), the last one the pivot
       for (int32_t i = lane; i < nu; i += 64) d.u_val[u0 + i] = xs[i];
       for (int32_t i = lane; i < below; i += 64) share_value(&d.l_val[l0 + 1 + i], xs[nu + i] / pivot);
-    } else if (lane == 0) {
-      atomicAdd(&d.control[2], 1);
+    } else {
+      if (lane == 0) atomicAdd(&d.control[2], 1);
+      for (int32_t i = lane; i < below; i += 64) share_value(&d.l_val[l0 + 1 + i], 0.0);
     }
-    // this column's stores complete, then its flag
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    if (lane == 0) __hip_atomic_store(&d.done[k], d.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
   }
+}
+
+__global__ void fill_u64_kernel(uint64_t* __restrict__ p, uint64_t value, int64_t count) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < count) p[i] = value;
+}
+
+hipError_t launch_fill_u64(hipStream_t stream, uint64_t* p, uint64_t value, int64_t count) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, p, value, count);
+  return hipGetLastError();
 }
 
 hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid) {

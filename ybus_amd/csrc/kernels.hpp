@@ -249,7 +249,7 @@ hipError_t launch_sddmm(hipStream_t stream, const int32_t* row, const int32_t* c
 // diagonal in U(:, k), in increasing order, u_jk = x[slot of j] and x[slot] -= l_ij * u_jk over
 // L(:, j) below its diagonal, then the pivot check and l = x / pivot. Columns are taken in an
 // order where each comes after the columns it reads (by level), each by one wavefront that
-// waits on the flags of the columns it reads.
+// waits for the L values it reads to be written.
 struct RefactorDev {
   int32_t n = 0;
   // per position c of the column order: (column k, its first U entry, U entries, first L entry)
@@ -262,16 +262,21 @@ struct RefactorDev {
   const int32_t* a_slot = nullptr;     // per matrix entry: its slot in its column's x
   const double* a_val = nullptr;       // the matrix's values, as stored by the caller (CSC)
   const int32_t* target = nullptr;     // per multiply-subtract: the slot it updates
-  double* l_val = nullptr;             // the new factors, in the uploaded CSC order
+  // the new factors, in the uploaded CSC order; L's entries must hold REFACTOR_PENDING at the
+  // launch (a column's values are written in place of it, which is what a reading column waits for)
+  double* l_val = nullptr;
   double* u_val = nullptr;
-  uint32_t* done = nullptr;  // per column: `epoch` once its values are written
-  uint32_t epoch = 0;
   int32_t* control = nullptr;  // [0] next column to take (0 at launch), [1] the smallest column whose
                                // pivot fails (n at launch), [2] waits given up (0 at launch)
   double tau = 0.1;
   int32_t x_cap = 0;          // slots of x in LDS (at least the longest column's)
-  uint32_t spin_limit = 0;    // polls of a flag before a wait is given up
+  uint32_t spin_limit = 0;    // polls of a value before a wait is given up
 };
+// The bits of a value not written yet: a signalling NaN, which no arithmetic result is (an
+// operation on a NaN gives a quiet one)
+constexpr uint64_t REFACTOR_PENDING = 0x7FF4C0DE0000DEADull;
+// p[0..count) = value, as a kernel
+hipError_t launch_fill_u64(hipStream_t stream, uint64_t* p, uint64_t value, int64_t count);
 // The longest L column (below its diagonal) one batch of steps stages
 constexpr int32_t REFACTOR_STAGE = 512;
 // LDS of one wavefront

@@ -391,8 +391,6 @@ struct RefactorPlan {
   DevBuf<int32_t> a_slot, target, control;
   DevBuf<int4> head, span, step;
   DevBuf<double> a_val;
-  DevBuf<uint32_t> done;
-  uint32_t epoch = 0;
   int32_t x_cap = 0;
   int grid = 0;
   ybus_hip_refactor_stats stats{};
@@ -1271,11 +1269,8 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
   plan.a_slot.upload(a_slot, stream);
   plan.target.upload(target, stream);
   plan.a_val.reserve(nnz_a);
-  plan.done.reserve(n);
-  check(hipMemsetAsync(plan.done.p, 0, (size_t)std::max<int64_t>(n, 1) * sizeof(uint32_t), stream), "refactor flags");
   plan.control.reserve(4);
   check(hipStreamSynchronize(stream), "refactor plan");
-  plan.epoch = 0;
   plan.x_cap = (int32_t)x_cap;
   // enough wavefronts for the widest levels, a few per SIMD
   plan.grid = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 16 * (int64_t)f->ctx->cus);
@@ -1312,11 +1307,10 @@ int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64
       std::memcpy(host, a_val, nnz_a * sizeof(double));
       check(hipMemcpyAsync(plan.a_val.p, host, nnz_a * sizeof(double), hipMemcpyHostToDevice, stream), "upload A");
     }
-    if (++plan.epoch == 0) {
-      check(hipMemsetAsync(plan.done.p, 0, (size_t)std::max<int64_t>(f->n, 1) * sizeof(uint32_t), stream),
-            "refactor flags");
-      plan.epoch = 1;
-    }
+    const uint64_t nl = f->lu_host.l_ptr[n];
+    check(hipEventRecord(f->ev0, stream), "hipEventRecord");
+    check(launch_fill_u64(stream, reinterpret_cast<uint64_t*>(plan.next.p), REFACTOR_PENDING, (int64_t)nl),
+          "refactor values");
     check(launch_fill(stream, plan.control.p, 0, 4), "refactor control");
     check(launch_fill(stream, plan.control.p + 1, (int32_t)f->n, 1), "refactor control");
     RefactorDev d;
@@ -1328,14 +1322,11 @@ int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64
     d.a_val = plan.a_val.p;
     d.target = plan.target.p;
     d.l_val = plan.next.p;
-    d.u_val = plan.next.p + f->lu_host.l_ptr[n];
-    d.done = plan.done.p;
-    d.epoch = plan.epoch;
+    d.u_val = plan.next.p + nl;
     d.control = plan.control.p;
     d.tau = pivot_threshold;
     d.x_cap = plan.x_cap;
     d.spin_limit = refactor_spin_limit();
-    check(hipEventRecord(f->ev0, stream), "hipEventRecord");
     check(launch_refactor(stream, d, plan.grid), "refactor");
     check(hipEventRecord(f->ev1, stream), "hipEventRecord");
     int32_t control[4] = {0, 0, 0, 0};
