@@ -823,6 +823,8 @@ int32_t ybus_hip_ctx_create(ybus_hip_ctx** out) {
     hipDeviceProp_t prop;
     check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
     ctx->name = prop.name;
+    // the marketing name comes from a table the driver may lack; the ISA name is always there
+    if (ctx->name.empty()) ctx->name = prop.gcnArchName;
     ctx->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     *out = ctx.release();
