@@ -154,7 +154,9 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx *ctx, uint64_t n, const uint64_t 
  * (LuOptions::pivot_threshold, lu.rs:248-255), in (0, 1]. A column of L with more than 512
  * entries below its diagonal, or a column of L and U together longer than 7,328, is beyond the
  * device refactor (YBUS_HIP_TOO_LARGE): refactor on the host and call
- * ybus_hip_factor_update_lu_values. */
+ * ybus_hip_factor_update_lu_values. A matrix attached with ybus_hip_factor_set_matrix that is
+ * this one (CSC, the same pattern) takes the new values too, for FP32-refined solves; one
+ * attached in another form keeps its values (attach the new matrix again). */
 int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor *f, uint64_t n, const uint64_t *a_ptr,
                                     const uint64_t *a_idx, const double *a_val,
                                     double pivot_threshold, int32_t *outcome);

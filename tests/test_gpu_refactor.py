@@ -205,3 +205,27 @@ def test_a_wait_given_up_leaves_the_factor_unchanged(gpu, monkeypatch):
     assert lu.refactor(b) == "reused"
     l_val, u_val = factor.lu_values()
     assert same_bits(l_val, lu.l.data) and same_bits(u_val, lu.u.data)
+
+
+def test_an_attached_matrix_follows_the_refactor(gpu):
+    # the matrix kept for FP32-refined solves (set_matrix) takes the refactored matrix's values
+    # when it is that matrix (CSC, the same pattern); the device's residual of each refined
+    # solution is then the CPU's residual against the new matrix, bit for bit
+    grid = O.make_grid(600, 20261209)
+    a = grid.jacobian(20261209)[0]
+    b = grid.jacobian(20261209, 0.002, 4)[0]
+    lu = O.Lu(a, O.COLAMD)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    factor.set_matrix(a)
+    rhs = right_hand_sides(lu.n, 6, 12)
+    assert factor.refactor(b) == "reused"
+    x, steps, ratio = factor.solve_many_refined(rhs, 1e-12, 10)
+    for k in range(rhs.shape[0]):
+        assert ratio[k] == b.relative_residual(x[k], rhs[k])
+    # a matrix attached in another orientation is left as it was
+    other = Y.GpuFactor.from_lu(gpu, lu)
+    other.set_matrix(a.to_csr())
+    assert other.refactor(b) == "reused"
+    x, steps, ratio = other.solve_many_refined(rhs, 1e-12, 10)
+    for k in range(rhs.shape[0]):
+        assert ratio[k] == a.relative_residual(x[k], rhs[k])

@@ -1949,6 +1949,18 @@ __global__ void fill_u64_kernel(uint64_t* __restrict__ p, uint64_t value, int64_
   if (i < count) p[i] = value;
 }
 
+__global__ void gather_kernel(const double* __restrict__ src, const int32_t* __restrict__ idx, int64_t count,
+                              double* __restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = src[idx[i]];
+}
+
+hipError_t launch_gather(hipStream_t stream, const double* src, const int32_t* idx, int64_t count, double* dst) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, src, idx, count, dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_u64(hipStream_t stream, uint64_t* p, uint64_t value, int64_t count) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, p, value, count);

@@ -277,6 +277,8 @@ struct RefactorDev {
 constexpr uint64_t REFACTOR_PENDING = 0x7FF4C0DE0000DEADull;
 // p[0..count) = value, as a kernel
 hipError_t launch_fill_u64(hipStream_t stream, uint64_t* p, uint64_t value, int64_t count);
+// dst[i] = src[idx[i]] for i < count
+hipError_t launch_gather(hipStream_t stream, const double* src, const int32_t* idx, int64_t count, double* dst);
 // The longest L column (below its diagonal) one batch of steps stages
 constexpr int32_t REFACTOR_STAGE = 512;
 // LDS of one wavefront

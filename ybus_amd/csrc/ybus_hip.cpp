@@ -257,6 +257,11 @@ struct DevCsr {
   DevBuf<int32_t> row;         // each entry's row (the adjoint's SDDMM takes entries one by one)
   std::vector<int64_t> order;  // each entry's position in the matrix as the caller stored it
   DevBuf<int32_t> order_dev;   // the same on the device
+  // the matrix as the caller gave it (its orientation and pattern), and for the transpose each
+  // entry's position in it: a refactor of the same matrix rewrites both values in place
+  bool src_row_major = true;
+  std::vector<uint64_t> src_ptr, src_idx;
+  DevBuf<int32_t> t_order;
   CsrDev view() const { return CsrDev{rows, cols, ptr.p, col.p, val.p}; }
 };
 
@@ -1348,6 +1353,12 @@ int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64
     }
     plan.stats.refresh_ms += refresh_sweeps(f, plan);
     plan.stats.reused += 1;
+    // a matrix attached for refinement that is this one (CSC, the same pattern) takes its values
+    if (f->a.nnz == nnz_a && !f->a.src_row_major && f->a.src_ptr == plan.a_ptr && f->a.src_idx == plan.a_idx) {
+      check(launch_gather(stream, plan.a_val.p, f->a.order_dev.p, (int64_t)nnz_a, f->a.val.p), "matrix values");
+      check(launch_gather(stream, plan.a_val.p, f->at.t_order.p, (int64_t)nnz_a, f->at.val.p), "matrix values");
+      check(hipStreamSynchronize(stream), "matrix values");
+    }
     *outcome = YBUS_HIP_REFACTOR_REUSED;
   });
 }
@@ -1550,7 +1561,7 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
     f->a.row.upload(rows, f->stream());
     // A^T by rows: a counting sort of A's entries by column, rows ascending within each
     {
-      std::vector<int32_t> tptr(n + 1, 0), tcol(nnz);
+      std::vector<int32_t> tptr(n + 1, 0), tcol(nnz), tord(nnz);
       std::vector<double> tval(nnz);
       for (uint64_t q = 0; q < nnz; ++q) tptr[cols[q] + 1] += 1;
       for (int64_t j = 0; j < n; ++j) tptr[j + 1] += tptr[j];
@@ -1560,13 +1571,18 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
           const int32_t t = cursor[cols[q]]++;
           tcol[t] = (int32_t)i;
           tval[t] = vals[q];
+          tord[t] = (int32_t)order[q];
         }
       f->at.rows = f->at.cols = (int32_t)n;
       f->at.nnz = nnz;
       f->at.ptr.upload(tptr, f->stream());
       f->at.col.upload(tcol, f->stream());
       f->at.val.upload(tval, f->stream());
+      f->at.t_order.upload(tord, f->stream());
     }
+    f->a.src_row_major = row_major != 0;
+    f->a.src_ptr.assign(ptr, ptr + n + 1);
+    f->a.src_idx.assign(idx, idx + nnz);
     std::vector<int32_t> order32(order.begin(), order.end());
     f->a.order_dev.upload(order32, f->stream());
     f->a.order = std::move(order);
