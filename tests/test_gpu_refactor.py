@@ -229,3 +229,62 @@ def test_an_attached_matrix_follows_the_refactor(gpu):
     x, steps, ratio = other.solve_many_refined(rhs, 1e-12, 10)
     for k in range(rhs.shape[0]):
         assert ratio[k] == a.relative_residual(x[k], rhs[k])
+
+
+def arrow(n: int, seed: int) -> O.Matrix:
+    """A dominant diagonal with a full first row and column: under natural ordering L and U fill
+    completely, so L's first columns hold up to n - 1 entries below the diagonal (CSC)."""
+    rng = np.random.default_rng(seed)
+    t = [(i, i, 4.0 + n * 0.1 + rng.random()) for i in range(n)]
+    t += [(0, j, 0.5 + rng.random()) for j in range(1, n)] + [(i, 0, 0.5 + rng.random()) for i in range(1, n)]
+    return O.Matrix.from_triplets(n, n, t, row_major=False)
+
+
+def test_columns_of_l_longer_than_a_wavefront(gpu):
+    # steps with 65 to 149 entries (a wavefront's 64 lanes take them in more than one round) and
+    # columns of thousands of entries (many batches)
+    a = arrow(150, 1)
+    lu = O.Lu(a, O.NATURAL)
+    assert max(np.diff(lu.l.ptr)) - 1 == 149
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    b = new_values(a, 4)
+    assert factor.refactor(b) == "reused"
+    assert lu.refactor(b) == "reused"
+    l_val, u_val = factor.lu_values()
+    assert same_bits(l_val, lu.l.data) and same_bits(u_val, lu.u.data)
+    rhs = right_hand_sides(150, 3, 1)
+    assert same_bits(factor.solve_many(rhs), lu.solve_many(rhs))
+
+
+def test_a_column_beyond_the_stage_is_refused(gpu):
+    # L's first column has 599 entries below its diagonal, more than one batch stages (512): the
+    # device refactor refuses the factor, which keeps its values and solves
+    a = arrow(600, 2)
+    lu = O.Lu(a, O.NATURAL)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    with pytest.raises(Y.TooLarge):
+        factor.refactor(new_values(a, 5))
+    rhs = right_hand_sides(600, 2, 3)
+    assert same_bits(factor.solve_many(rhs), lu.solve_many(rhs))
+    # a host refactor's values still go in
+    b = new_values(a, 5)
+    assert lu.refactor(b) == "reused"
+    factor.update_lu_values(lu.l.data, lu.u.data)
+    assert same_bits(factor.solve_many(rhs), lu.solve_many(rhs))
+
+
+def test_one_unknown_and_a_diagonal(gpu):
+    one = O.Matrix.from_triplets(1, 1, [(0, 0, 3.0)], row_major=False)
+    lu = O.Lu(one, O.NATURAL)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    assert factor.refactor(O.Matrix.from_triplets(1, 1, [(0, 0, -2.0)], row_major=False)) == "reused"
+    assert list(factor.lu_values()[1]) == [-2.0]
+    assert factor.refactor(O.Matrix.from_triplets(1, 1, [(0, 0, 0.0)], row_major=False)) == "needs_pivoting"
+    assert list(factor.lu_values()[1]) == [-2.0]
+    d = O.Matrix.from_triplets(5, 5, [(i, i, 1.0 + i) for i in range(5)], row_major=False)
+    lu = O.Lu(d, O.NATURAL)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    b = new_values(d, 2)
+    assert factor.refactor(b) == "reused"
+    assert lu.refactor(b) == "reused"
+    assert same_bits(factor.lu_values()[1], lu.u.data)
