@@ -71,12 +71,23 @@ def cpu_threads() -> int:
 CPU_SECONDS = 10.0
 
 
-def cpu_pair_baseline(chol, plus, minus, sample: int):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_pair_baseline(chol, plus, minus, sample: int, threads: int = 0, seconds: float = CPU_SECONDS):
     """par_solve_many on `sample` of the pair right-hand sides (dense vectors, as the reference's
-    CPU path takes them), repeated for CPU_SECONDS on this host's cores; returns (RHS per second,
-    threads, the block)."""
+    CPU path takes them), repeated for `seconds` on `threads` of this host's cores (all it may
+    use by default); returns (RHS per second, threads, the block)."""
     n = chol.n
-    threads = cpu_threads()
+    threads = threads or cpu_threads()
     dense = np.zeros((sample, n))
     for k in range(sample):
         if plus[k] >= 0:
@@ -89,7 +100,7 @@ def cpu_pair_baseline(chol, plus, minus, sample: int):
     while True:
         out = chol.solve_many(dense, threads)
         reps += 1
-        if time.perf_counter() - t0 >= CPU_SECONDS:
+        if time.perf_counter() - t0 >= seconds:
             break
     elapsed = time.perf_counter() - t0
     return reps * sample / elapsed, threads, out
@@ -167,8 +178,18 @@ def run_ptdf(gpu, args):
         f"avg over {stats.launches} launches")
     solutions = block.solutions()
     block.free()
+    # end to end, as the reference's callers use it: the pair list from the host in, every
+    # solution back in host memory (1.08 GB), the median of three
+    walls = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        with factor.pair_solves(pairs) as b:
+            b.solutions()
+        walls.append(time.perf_counter() - t1)
+    end_to_end = sorted(walls)[1]
+    log(f"[ptdf] end to end (pairs in, solutions out in host memory) {end_to_end * 1e3:.1f} ms")
     return dict(grid=grid, bprime=bprime, chol=chol, plus=plus, minus=minus, nnz_l=nnz_l, rhs=rhs_count, kernel=kernel,
-                tuning=tuning,
+                tuning=tuning, end_to_end=end_to_end,
                 elapsed=elapsed, stats=stats, solutions=solutions, factor=factor)
 
 
@@ -866,9 +887,13 @@ def main():
     cpu_baseline = None
     if not args.no_cpu:
         rate, threads, _ = cpu_pair_baseline(chol, ptdf["plus"], ptdf["minus"], min(2000, rhs_count))
+        serial, _, _ = cpu_pair_baseline(chol, ptdf["plus"], ptdf["minus"], min(250, rhs_count), 1, 2.0)
         cpu_baseline = {"value": rate * nnz_l, "unit": "RHS·nnz(L)/s", "cores": threads, "kind": "port",
                         "sample": "par_solve_many (oracle, one RHS per thread) on the first 2,000 of the 13,500 "
-                                  "pair RHS as dense vectors, repeated for >= %g s" % CPU_SECONDS}
+                                  "pair RHS as dense vectors, repeated for >= %g s; serial: solve_many on the "
+                                  "first 250, repeated for >= 2 s" % CPU_SECONDS,
+                        "serial_value": serial * nnz_l, "model": cpu_model(),
+                        "ms_per_block": rhs_count / rate * 1e3, "serial_ms_per_block": rhs_count / serial * 1e3}
 
     line = {
         "metric": METRIC,
@@ -890,6 +915,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": ptdf["kernel"], "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu_baseline,
+        "end_to_end_ms": ptdf["end_to_end"] * 1e3,
         "parity": {"sampled_columns": int(len(sample)), "bit_exact_columns": bit_exact,
                    "max_rel_diff_vs_cpu": max_rel},
     }
