@@ -184,3 +184,23 @@ def test_synthetic_jacobian_lu_matches_the_cpu(gpu):
     cpu = lu.solve_many(rhs, threads=4)
     floors = np.array([jac.relative_residual(cpu[k], rhs[k]) for k in range(64)])
     assert np.all(ratio <= np.maximum(1e-12, 2.0 * floors)), (ratio.max(), floors.max())
+
+
+def test_transfers_larger_than_the_staging_chunks(gpu):
+    # 1,000 right-hand sides of a 10,000-bus B' are 80 MB each way: the host vectors cross in
+    # three 32 MB chunks through the alternating pinned buffers, both ways, bit for bit
+    grid = O.make_grid(10000, 20261210)
+    bprime, _, _ = grid.dc()
+    chol = O.Cholesky(bprime)
+    rhs = O.injections(7, 3, grid.buses, 1000)
+    assert rhs.nbytes > 2 * 32 * 2**20
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    cpu = chol.solve_many(rhs, 8)
+    assert factor.solve_many(rhs).tobytes() == cpu.tobytes()
+    # and a pair block's solutions out of the device
+    pairs = [Y.Pair(int(k), int(k) + 1) for k in range(1000)]
+    dense = np.zeros((1000, chol.n))
+    for k in range(1000):
+        dense[k, k], dense[k, k + 1] = 1.0, -1.0
+    with factor.pair_solves(pairs) as block:
+        assert block.solutions().tobytes() == chol.solve_many(dense, 8).tobytes()

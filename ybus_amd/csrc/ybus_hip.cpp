@@ -86,12 +86,41 @@ struct DevBuf {
   }
 };
 
-// Page-locked host staging, grown as needed (factor.rs:63-89)
+int64_t host_threads();
+
+// A host-to-host copy split among the library's host threads (at least 4 MB each): a large copy
+// into memory the caller has not touched yet is bound by page faults, which threads take in
+// parallel
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  const int64_t threads = std::min<int64_t>(host_threads(), (int64_t)(bytes >> 22));
+  if (threads <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t per = (bytes + (size_t)threads - 1) / (size_t)threads;
+  std::vector<std::thread> pool;
+  for (int64_t t = 1; t < threads; ++t) {
+    const size_t lo = (size_t)t * per, hi = std::min(bytes, lo + per);
+    if (lo < hi)
+      pool.emplace_back([=] { std::memcpy((char*)dst + lo, (const char*)src + lo, hi - lo); });
+  }
+  std::memcpy(dst, src, std::min(bytes, per));
+  for (std::thread& th : pool) th.join();
+}
+
+// Page-locked host staging, grown as needed (factor.rs:63-89). Large transfers go through it in
+// two alternating chunks: the DMA of one chunk runs while the host threads copy the other
+// between the chunk and the caller's memory, so a transfer costs about the longer of the two
+// rather than their sum, and the pinned buffer stays at two chunks.
 struct Staging {
+  static constexpr size_t CHUNK = size_t(32) << 20;  // bytes
   double* p = nullptr;
   size_t len = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
   ~Staging() {
     if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
   }
   double* get(size_t n) {
     if (n > len || !p) {
@@ -102,6 +131,47 @@ struct Staging {
       len = n;
     }
     return p;
+  }
+  char* chunks(size_t bytes) {
+    for (hipEvent_t& e : ev)
+      if (!e) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    const size_t need = bytes > CHUNK ? 2 * CHUNK : bytes;
+    return reinterpret_cast<char*>(get((need + sizeof(double) - 1) / sizeof(double)));
+  }
+  // `bytes` from device memory at src into any host memory at dst, after the stream's earlier work
+  void to_host(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    char* pin = chunks(bytes);
+    const size_t count = (bytes + CHUNK - 1) / CHUNK;
+    for (size_t i = 0; i <= count; ++i) {
+      if (i < count) {
+        const size_t off = i * CHUNK;
+        check(hipMemcpyAsync(pin + (i % 2) * CHUNK, (const char*)src + off, std::min(CHUNK, bytes - off),
+                             hipMemcpyDeviceToHost, s),
+              "D2H");
+        check(hipEventRecord(ev[i % 2], s), "hipEventRecord");
+      }
+      if (i > 0) {
+        const size_t j = i - 1, off = j * CHUNK;
+        check(hipEventSynchronize(ev[j % 2]), "D2H");
+        parallel_copy((char*)dst + off, pin + (j % 2) * CHUNK, std::min(CHUNK, bytes - off));
+      }
+    }
+  }
+  // `bytes` from any host memory at src into device memory at dst; returns with the copy done
+  void to_device(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    char* pin = chunks(bytes);
+    const size_t count = (bytes + CHUNK - 1) / CHUNK;
+    for (size_t i = 0; i < count; ++i) {
+      const size_t off = i * CHUNK, n = std::min(CHUNK, bytes - off);
+      // the DMA that last read this chunk is done before the host writes it again
+      if (i >= 2) check(hipEventSynchronize(ev[i % 2]), "H2D");
+      parallel_copy(pin + (i % 2) * CHUNK, (const char*)src + off, n);
+      check(hipMemcpyAsync((char*)dst + off, pin + (i % 2) * CHUNK, n, hipMemcpyHostToDevice, s), "H2D");
+      check(hipEventRecord(ev[i % 2], s), "hipEventRecord");
+    }
+    check(hipStreamSynchronize(s), "H2D");
   }
 };
 
@@ -1595,12 +1665,10 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
 static void upload_vectors(ybus_hip_factor* f, const double* rhs, int64_t count, double* block = nullptr) {
   if (!block) block = f->b.p;
   const size_t len = (size_t)count * (size_t)f->n;
-  double* host = f->staging.get(len);
-  std::memcpy(host, rhs, len * sizeof(double));
   if (f->layout.columns) {
-    check(hipMemcpyAsync(block, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+    f->staging.to_device(f->stream(), block, rhs, len * sizeof(double));
   } else {
-    check(hipMemcpyAsync(f->vec.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+    f->staging.to_device(f->stream(), f->vec.p, rhs, len * sizeof(double));
     check(launch_to_block(f->stream(), f->vec.p, block, f->n, count, f->layout.ld), "to_block");
   }
   check(hipStreamSynchronize(f->stream()), "upload");
@@ -1609,15 +1677,12 @@ static void upload_vectors(ybus_hip_factor* f, const double* rhs, int64_t count,
 static void download_vectors(ybus_hip_factor* f, const double* block, const Layout& layout, int64_t count,
                              double* out) {
   const size_t len = (size_t)count * (size_t)f->n;
-  double* host = f->staging.get(len);
   if (layout.columns) {
-    check(hipMemcpyAsync(host, block, len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+    f->staging.to_host(f->stream(), out, block, len * sizeof(double));
   } else {
     check(launch_from_block(f->stream(), block, f->vec.p, f->n, count, layout.ld), "from_block");
-    check(hipMemcpyAsync(host, f->vec.p, len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
+    f->staging.to_host(f->stream(), out, f->vec.p, len * sizeof(double));
   }
-  check(hipStreamSynchronize(f->stream()), "download");
-  std::memcpy(out, host, len * sizeof(double));
 }
 
 // Copies count device vectors (vector c at src + c * ld) into a block of the workspace, in the
@@ -2342,19 +2407,13 @@ int32_t ybus_hip_block_product(ybus_hip_block* b, ybus_hip_operator* op, const i
     if (!out) fail(YBUS_HIP_INVALID_ARGUMENT, "out is null");
     // the caller gets one vector of rows per column
     if (b->layout.columns) {
-      check(hipMemcpyAsync(out, op->out.p, (size_t)rows * cols * sizeof(double), hipMemcpyDeviceToHost,
-                           b->f->stream()),
-            "D2H");
-      check(hipStreamSynchronize(b->f->stream()), "product");
+      b->f->staging.to_host(b->f->stream(), out, op->out.p, (size_t)rows * cols * sizeof(double));
       return;
     }
     // rows x columns on the device: transposed there into one vector per column, then copied out
     op->by_column.reserve((size_t)rows * (size_t)cols);
     check(launch_from_block(b->f->stream(), op->out.p, op->by_column.p, rows, cols, cols), "from_block");
-    check(hipMemcpyAsync(out, op->by_column.p, (size_t)rows * cols * sizeof(double), hipMemcpyDeviceToHost,
-                         b->f->stream()),
-          "D2H");
-    check(hipStreamSynchronize(b->f->stream()), "product");
+    b->f->staging.to_host(b->f->stream(), out, op->by_column.p, (size_t)rows * cols * sizeof(double));
   });
 }
 
@@ -2701,12 +2760,7 @@ int32_t ybus_hip_batch_stage(ybus_hip_batch* b, const double* rhs, const uint64_
     for (DevBuf<double>* d : {&b->rhs, &b->x, &b->x1, &b->r}) d->reserve(vals);
     b->steps.reserve(cols);
     b->ratio.reserve(cols);
-    if (values) {
-      double* host = b->staging.get((size_t)values);
-      std::memcpy(host, rhs, (size_t)values * sizeof(double));
-      check(hipMemcpyAsync(b->rhs.p, host, (size_t)values * sizeof(double), hipMemcpyHostToDevice, b->stream()),
-            "H2D");
-    }
+    if (values) b->staging.to_device(b->stream(), b->rhs.p, rhs, (size_t)values * sizeof(double));
     check(hipStreamSynchronize(b->stream()), "batch stage");
     b->staged = true;
   });
@@ -2768,10 +2822,7 @@ int32_t ybus_hip_batch_solutions(ybus_hip_batch* b, double* x) {
     if (!b->staged) fail(YBUS_HIP_INVALID_ARGUMENT, "nothing staged (ybus_hip_batch_stage)");
     if (!b->values) return;
     if (!x) fail(YBUS_HIP_INVALID_ARGUMENT, "x is null");
-    double* host = b->staging.get((size_t)b->values);
-    check(hipMemcpyAsync(host, b->x.p, (size_t)b->values * sizeof(double), hipMemcpyDeviceToHost, b->stream()), "D2H");
-    check(hipStreamSynchronize(b->stream()), "batch download");
-    std::memcpy(x, host, (size_t)b->values * sizeof(double));
+    b->staging.to_host(b->stream(), x, b->x.p, (size_t)b->values * sizeof(double));
   });
 }
 
