@@ -1379,11 +1379,7 @@ int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64
     }
     hipStream_t stream = f->stream();
     const uint64_t nnz_a = a_ptr[n];
-    if (nnz_a) {
-      double* host = f->staging.get(nnz_a);
-      std::memcpy(host, a_val, nnz_a * sizeof(double));
-      check(hipMemcpyAsync(plan.a_val.p, host, nnz_a * sizeof(double), hipMemcpyHostToDevice, stream), "upload A");
-    }
+    f->staging.to_device(stream, plan.a_val.p, a_val, nnz_a * sizeof(double));
     const uint64_t nl = f->lu_host.l_ptr[n];
     check(hipEventRecord(f->ev0, stream), "hipEventRecord");
     check(launch_fill_u64(stream, reinterpret_cast<uint64_t*>(plan.next.p), REFACTOR_PENDING, (int64_t)nl),
@@ -1441,10 +1437,8 @@ int32_t ybus_hip_factor_update_lu_values(ybus_hip_factor* f, const double* l_val
     check(hipSetDevice(f->ctx->device), "hipSetDevice");
     RefactorPlan& plan = refactor_plan(f);
     hipStream_t stream = f->stream();
-    double* host = f->staging.get(nl + nu);
-    if (nl) std::memcpy(host, l_val, nl * sizeof(double));
-    if (nu) std::memcpy(host + nl, u_val, nu * sizeof(double));
-    check(hipMemcpyAsync(plan.next.p, host, (nl + nu) * sizeof(double), hipMemcpyHostToDevice, stream), "upload values");
+    f->staging.to_device(stream, plan.next.p, l_val, nl * sizeof(double));
+    f->staging.to_device(stream, plan.next.p + nl, u_val, nu * sizeof(double));
     plan.stats.refresh_ms += refresh_sweeps(f, plan);
     plan.stats.updates += 1;
   });
@@ -1797,9 +1791,7 @@ static int32_t solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_
       const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
       const size_t len = (size_t)chunk * (size_t)rows;
       auto t0 = std::chrono::steady_clock::now();
-      double* host = f->staging.get(len);
-      std::memcpy(host, rhs + start * (uint64_t)rows, len * sizeof(double));
-      check(hipMemcpyAsync(f->zvec.p, host, len * sizeof(double), hipMemcpyHostToDevice, f->stream()), "H2D");
+      f->staging.to_device(f->stream(), f->zvec.p, rhs + start * (uint64_t)rows, len * sizeof(double));
       check(launch_to_block(f->stream(), f->zvec.p, f->zb.p, rows, chunk, ld), "to_block");
       check(hipStreamSynchronize(f->stream()), "upload");
       auto t1 = std::chrono::steady_clock::now();
@@ -1813,9 +1805,7 @@ static int32_t solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_
       f->end_kernel((uint64_t)f->nnz_factor * 20 + 8 * (uint64_t)(f->n + 1) + 32 * (uint64_t)f->n * (uint64_t)chunk);
       auto t2 = std::chrono::steady_clock::now();
       check(launch_from_block(f->stream(), f->zx.p, f->zvec.p, rows, chunk, ld), "from_block");
-      check(hipMemcpyAsync(host, f->zvec.p, len * sizeof(double), hipMemcpyDeviceToHost, f->stream()), "D2H");
-      check(hipStreamSynchronize(f->stream()), "download");
-      std::memcpy(x + start * (uint64_t)rows, host, len * sizeof(double));
+      f->staging.to_host(f->stream(), x + start * (uint64_t)rows, f->zvec.p, len * sizeof(double));
       f->timings.upload += std::chrono::duration<double>(t1 - t0).count();
       f->timings.solve += std::chrono::duration<double>(t2 - t1).count();
       f->timings.download += seconds_since(t2);
