@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <initializer_list>
 #include <map>
 #include <memory>
@@ -348,6 +349,29 @@ constexpr int32_t MAX_OUTAGE_SET = 64;
 int64_t host_threads() {
   const unsigned hw = std::thread::hardware_concurrency();
   return std::clamp<int64_t>(hw ? hw : 1, 1, 16);
+}
+
+// Runs independent host tasks together, each on its own thread (the first on the calling one),
+// and rethrows the first task's failure once all have ended
+void run_together(const std::vector<std::function<void()>>& tasks) {
+  std::vector<std::exception_ptr> errors(tasks.size());
+  std::vector<std::thread> pool;
+  for (size_t i = 1; i < tasks.size(); ++i)
+    pool.emplace_back([&, i] {
+      try {
+        tasks[i]();
+      } catch (...) {
+        errors[i] = std::current_exception();
+      }
+    });
+  try {
+    if (!tasks.empty()) tasks[0]();
+  } catch (...) {
+    errors[0] = std::current_exception();
+  }
+  for (std::thread& t : pool) t.join();
+  for (const std::exception_ptr& e : errors)
+    if (e) std::rethrow_exception(e);
 }
 
 double seconds_since(std::chrono::steady_clock::time_point t) {
@@ -1070,9 +1094,33 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
       gather[i] = (int64_t)perm[i];
       ident[i] = (int64_t)i;
     }
-    const RowForm lower = cholesky_forward(n, l_ptr, l_idx, l_val);
-    const RowForm upper = cholesky_backward(n, l_ptr, l_idx, l_val);
-    Sweep sf = schedule(lower, ident, ident), sb = schedule(upper, ident, ident);
+    // the elimination tree's independent subtrees, for FP64 solves on the wide path: a part per
+    // 2,048 rows, at most 32 (YBUS_HIP_PARTS sets the target; 0 turns it off)
+    const char* parts_env = std::getenv("YBUS_HIP_PARTS");
+    const int32_t target =
+        parts_env ? std::max(0, std::atoi(parts_env)) : (int32_t)std::clamp<int64_t>((int64_t)n / 2048, 0, 32);
+    f->force_parts = parts_env && target >= 2;
+    // the host analysis, its independent pieces on threads of their own
+    RowForm lower, upper;
+    run_together({[&] { lower = cholesky_forward(n, l_ptr, l_idx, l_val); },
+                  [&] { upper = cholesky_backward(n, l_ptr, l_idx, l_val); }});
+    Sweep sf, sb, wf, wb, pf, pb;
+    int32_t parts = 0;
+    std::vector<int32_t> part_of;
+    run_together({[&] { sf = schedule(lower, ident, ident); }, [&] { sb = schedule(upper, ident, ident); },
+                  [&] { wf = schedule(lower, gather, ident); }, [&] { wb = schedule(upper, ident, gather); },
+                  [&] {
+                    if (target >= 2) part_of = tree_parts(lower, target, parts);
+                    if (parts >= 2) pf = schedule_parts(lower, gather, ident, part_of, parts, false);
+                  },
+                  [&] {
+                    // the same parts, found again on this thread (tree_parts is deterministic)
+                    int32_t p = 0;
+                    if (target >= 2) {
+                      const std::vector<int32_t> mine = tree_parts(lower, target, p);
+                      if (p >= 2) pb = schedule_parts(upper, ident, gather, mine, p, true);
+                    }
+                  }});
     f->fits = fits_columns(f.get(), sf.levels + sb.levels);
     if (f->fits) {
       // cholesky.rs:570-579: y[i] = b[perm[i]], and x[perm[i]] = y[i]
@@ -1081,19 +1129,9 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
       f->rhs_row.upload(inverse_i32(gather), ctx->stream);
       f->sol_of.upload(inverse_i32(gather), ctx->stream);
     }
-    f->fwd.upload(schedule(lower, gather, ident), ctx->stream, true);
-    f->bwd.upload(schedule(upper, ident, gather), ctx->stream, true);
-    // the elimination tree's independent subtrees, for FP64 solves on the wide path: a part per
-    // 2,048 rows, at most 32 (YBUS_HIP_PARTS sets the target; 0 turns it off)
-    const char* parts_env = std::getenv("YBUS_HIP_PARTS");
-    const int32_t target =
-        parts_env ? std::max(0, std::atoi(parts_env)) : (int32_t)std::clamp<int64_t>((int64_t)n / 2048, 0, 32);
-    f->force_parts = parts_env && target >= 2;
-    int32_t parts = 0;
-    const std::vector<int32_t> part_of = target >= 2 ? tree_parts(lower, target, parts) : std::vector<int32_t>();
+    f->fwd.upload(std::move(wf), ctx->stream, true);
+    f->bwd.upload(std::move(wb), ctx->stream, true);
     if (parts >= 2) {
-      Sweep pf = schedule_parts(lower, gather, ident, part_of, parts, false);
-      Sweep pb = schedule_parts(upper, ident, gather, part_of, parts, true);
       if (pf.parts && pb.parts) {
         f->pfwd.upload(std::move(pf), ctx->stream, true);
         f->pbwd.upload(std::move(pb), ctx->stream, true);
@@ -1126,10 +1164,16 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
       cp[i] = (int64_t)col_perm[i];
       ident[i] = (int64_t)i;
     }
-    const RowForm l = lu_forward(n, l_ptr, l_idx, l_val), u = lu_backward(n, u_ptr, u_idx, u_val);
-    const RowForm ut = lu_transpose_forward(n, u_ptr, u_idx, u_val), lt = lu_transpose_backward(n, l_ptr, l_idx, l_val);
-    Sweep sl = schedule(l, ident, ident), su = schedule(u, ident, ident);
-    Sweep stu = schedule(ut, ident, ident), stl = schedule(lt, ident, ident);
+    // the host analysis, its independent pieces on threads of their own
+    RowForm l, u, ut, lt;
+    run_together({[&] { l = lu_forward(n, l_ptr, l_idx, l_val); }, [&] { u = lu_backward(n, u_ptr, u_idx, u_val); },
+                  [&] { ut = lu_transpose_forward(n, u_ptr, u_idx, u_val); },
+                  [&] { lt = lu_transpose_backward(n, l_ptr, l_idx, l_val); }});
+    Sweep sl, su, stu, stl, wl, wu, wut, wlt;
+    run_together({[&] { sl = schedule(l, ident, ident); }, [&] { su = schedule(u, ident, ident); },
+                  [&] { stu = schedule(ut, ident, ident); }, [&] { stl = schedule(lt, ident, ident); },
+                  [&] { wl = schedule(l, rp, ident); }, [&] { wu = schedule(u, ident, cp); },
+                  [&] { wut = schedule(ut, cp, ident); }, [&] { wlt = schedule(lt, ident, rp); }});
     f->fits = fits_columns(f.get(), std::max(sl.levels + su.levels, stu.levels + stl.levels));
     if (f->fits) {
       // lu.rs:770-785: y[i] = b[P[i]], x[Q[i]] = y[i]; transposed (lu.rs:506-533): through Q, then P
@@ -1142,10 +1186,10 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
       f->t_rhs_row.upload(inverse_i32(cp), ctx->stream);
       f->t_sol_of.upload(inverse_i32(rp), ctx->stream);
     }
-    f->fwd.upload(schedule(l, rp, ident), ctx->stream, true);
-    f->bwd.upload(schedule(u, ident, cp), ctx->stream, true);
-    f->tfwd.upload(schedule(ut, cp, ident), ctx->stream, true);
-    f->tbwd.upload(schedule(lt, ident, rp), ctx->stream, true);
+    f->fwd.upload(std::move(wl), ctx->stream, true);
+    f->bwd.upload(std::move(wu), ctx->stream, true);
+    f->tfwd.upload(std::move(wut), ctx->stream, true);
+    f->tbwd.upload(std::move(wlt), ctx->stream, true);
     f->has_transpose = true;
     f->lu_host.l_ptr.assign(l_ptr, l_ptr + n + 1);
     f->lu_host.l_idx.assign(l_idx, l_idx + l_ptr[n]);
