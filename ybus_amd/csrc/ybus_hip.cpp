@@ -1224,10 +1224,11 @@ static RefactorPlan& refactor_plan(ybus_hip_factor* f) {
   std::vector<double> lpos(nl), upos(nu);
   for (uint64_t p = 0; p < nl; ++p) lpos[p] = (double)(p + 1);
   for (uint64_t p = 0; p < nu; ++p) upos[p] = (double)(nl + p + 1);
-  const RowForm l = lu_forward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data());
-  const RowForm u = lu_backward(n, h.u_ptr.data(), h.u_idx.data(), upos.data());
-  const RowForm ut = lu_transpose_forward(n, h.u_ptr.data(), h.u_idx.data(), upos.data());
-  const RowForm lt = lu_transpose_backward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data());
+  RowForm l, u, ut, lt;
+  run_together({[&] { l = lu_forward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data()); },
+                [&] { u = lu_backward(n, h.u_ptr.data(), h.u_idx.data(), upos.data()); },
+                [&] { ut = lu_transpose_forward(n, h.u_ptr.data(), h.u_idx.data(), upos.data()); },
+                [&] { lt = lu_transpose_backward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data()); }});
   std::vector<int64_t> rp(n), cp(n), ident(n);
   for (int64_t i = 0; i < n; ++i) {
     rp[i] = (int64_t)h.row_perm[i];
@@ -1251,15 +1252,22 @@ static RefactorPlan& refactor_plan(ybus_hip_factor* f) {
     }
     check(hipStreamSynchronize(stream), "refresh maps");
   };
-  add(f->fwd, schedule(l, rp, ident));
-  add(f->bwd, schedule(u, ident, cp));
-  add(f->tfwd, schedule(ut, cp, ident));
-  add(f->tbwd, schedule(lt, ident, rp));
+  Sweep s[8];
+  run_together({[&] { s[0] = schedule(l, rp, ident); }, [&] { s[1] = schedule(u, ident, cp); },
+                [&] { s[2] = schedule(ut, cp, ident); }, [&] { s[3] = schedule(lt, ident, rp); },
+                [&] { if (f->fits) s[4] = schedule(l, ident, ident); },
+                [&] { if (f->fits) s[5] = schedule(u, ident, ident); },
+                [&] { if (f->fits) s[6] = schedule(ut, ident, ident); },
+                [&] { if (f->fits) s[7] = schedule(lt, ident, ident); }});
+  add(f->fwd, s[0]);
+  add(f->bwd, s[1]);
+  add(f->tfwd, s[2]);
+  add(f->tbwd, s[3]);
   if (f->fits) {
-    add(f->cfwd, schedule(l, ident, ident));
-    add(f->cbwd, schedule(u, ident, ident));
-    add(f->ctfwd, schedule(ut, ident, ident));
-    add(f->ctbwd, schedule(lt, ident, ident));
+    add(f->cfwd, s[4]);
+    add(f->cbwd, s[5]);
+    add(f->ctfwd, s[6]);
+    add(f->ctbwd, s[7]);
   }
   plan->vals.upload(h.values, stream);
   plan->next.reserve(h.values.size());
