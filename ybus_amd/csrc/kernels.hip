@@ -1929,13 +1929,14 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
       }
       double largest = fabs(pivot);
       if (big > largest) largest = big;
+      // the values other columns wait for go first
+      for (int32_t i = lane; i < below; i += 64) share_value(&d.l_val[l0 + 1 + i], xs[nu + i] / pivot);
       if (lane == 0) {
         if (pivot == 0.0 || fabs(pivot) < d.tau * largest) atomicMin(&d.control[1], k);
         d.l_val[l0] = 1.0;
       }
       // x's U slots hold u_jk (no step writes a slot at or before its own), the last one the pivot
       for (int32_t i = lane; i < nu; i += 64) d.u_val[u0 + i] = xs[i];
-      for (int32_t i = lane; i < below; i += 64) share_value(&d.l_val[l0 + 1 + i], xs[nu + i] / pivot);
     } else {
       if (lane == 0) atomicAdd(&d.control[2], 1);
       for (int32_t i = lane; i < below; i += 64) share_value(&d.l_val[l0 + 1 + i], 0.0);
