@@ -1753,9 +1753,15 @@ hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fw
 
 // LDS of one wavefront's column: x (x_cap doubles), the staged L values and their target slots,
 // and each staged step's first entry and L position
-size_t refactor_lds_bytes(int32_t x_cap) {
-  return (size_t)x_cap * sizeof(double) + (size_t)REFACTOR_STAGE * (sizeof(double) + sizeof(int32_t)) +
+size_t refactor_lds_bytes(int32_t x_cap, int32_t stage) {
+  return (size_t)x_cap * sizeof(double) + (size_t)stage * (sizeof(double) + sizeof(int32_t)) +
          2 * 64 * sizeof(int32_t);
+}
+
+int32_t refactor_stage_for(int32_t longest_l) {
+  int32_t stage = 64;
+  while (stage < longest_l && stage < REFACTOR_STAGE) stage *= 2;
+  return stage;
 }
 
 // Columns hand their L values over between workgroups anywhere on the device (other XCDs, other
@@ -1781,12 +1787,14 @@ __device__ inline bool pending(double v) { return (uint64_t)__double_as_longlong
 // its LDS accesses and costs no barrier.) A wait that outlasts spin_limit polls gives the column
 // up (counted in control[2]); its L values are still written (zeros), so no wavefront waits
 // forever.
+template <int R>
 __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
+  constexpr int32_t STAGE = 64 * R;
   extern __shared__ double refactor_lds[];
   double* xs = refactor_lds;
   double* sval = xs + d.x_cap;
-  int32_t* stgt = reinterpret_cast<int32_t*>(sval + REFACTOR_STAGE);
-  int32_t* s_off = stgt + REFACTOR_STAGE;
+  int32_t* stgt = reinterpret_cast<int32_t*>(sval + STAGE);
+  int32_t* s_off = stgt + STAGE;
   int32_t* s_first = s_off + 64;
   const int lane = threadIdx.x;
   for (;;) {
@@ -1818,7 +1826,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
         const int32_t v = __shfl_up(incl, off, 64);
         if (lane >= off) incl += v;
       }
-      const uint64_t fits = __ballot(s < steps && incl <= REFACTOR_STAGE);
+      const uint64_t fits = __ballot(s < steps && incl <= STAGE);
       const int32_t cand = fits == ~0ull ? 64 : (int32_t)__builtin_ctzll(~fits);
       if (cand == 0) {
         // longer than the stage (the host admits no such factor)
@@ -1831,7 +1839,6 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
         s_first[lane] = first;
       }
       __syncthreads();
-      constexpr int R = REFACTOR_STAGE / 64;
       double v[R];
       int32_t g[R], at[R];
 #pragma unroll
@@ -1968,10 +1975,17 @@ hipError_t launch_fill_u64(hipStream_t stream, uint64_t* p, uint64_t value, int6
   return hipGetLastError();
 }
 
-hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid) {
+hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid, int32_t stage) {
   if (d.n <= 0) return hipSuccess;
-  if (grid <= 0 || refactor_lds_bytes(d.x_cap) > 65536 - 256) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(refactor_kernel, dim3((unsigned)grid), dim3(64), refactor_lds_bytes(d.x_cap), stream, d);
+  const size_t bytes = refactor_lds_bytes(d.x_cap, stage);
+  if (grid <= 0 || bytes > 65536 - 256) return hipErrorInvalidValue;
+  switch (stage) {
+    case 64: hipLaunchKernelGGL(refactor_kernel<1>, dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
+    case 128: hipLaunchKernelGGL(refactor_kernel<2>, dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
+    case 256: hipLaunchKernelGGL(refactor_kernel<4>, dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
+    case 512: hipLaunchKernelGGL(refactor_kernel<8>, dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

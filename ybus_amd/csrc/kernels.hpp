@@ -279,13 +279,15 @@ constexpr uint64_t REFACTOR_PENDING = 0x7FF4C0DE0000DEADull;
 hipError_t launch_fill_u64(hipStream_t stream, uint64_t* p, uint64_t value, int64_t count);
 // dst[i] = src[idx[i]] for i < count
 hipError_t launch_gather(hipStream_t stream, const double* src, const int32_t* idx, int64_t count, double* dst);
-// The longest L column (below its diagonal) one batch of steps stages
+// The entries one batch of steps stages: 64, 128, 256 or 512 (a batch holds at least one step, so
+// this bounds the longest column of L below its diagonal; the smallest that fits is fastest)
 constexpr int32_t REFACTOR_STAGE = 512;
+int32_t refactor_stage_for(int32_t longest_l);
 // LDS of one wavefront
-size_t refactor_lds_bytes(int32_t x_cap);
-// `grid` single-wavefront workgroups take the columns; control[0], [2] must hold 0 and
-// control[1] n at the launch
-hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid);
+size_t refactor_lds_bytes(int32_t x_cap, int32_t stage);
+// `grid` single-wavefront workgroups take the columns, staging `stage` entries per batch;
+// control[0], [2] must hold 0 and control[1] n at the launch
+hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid, int32_t stage);
 
 // A sweep's value arrays rewritten from new factor values: every entry takes src[vpos[p]] (a
 // position in the new L values followed by the new U values), every diagonal src[dpos[i]], in

@@ -492,6 +492,7 @@ struct RefactorPlan {
   DevBuf<double> a_val;
   int32_t x_cap = 0;
   int grid = 0;
+  int32_t stage = REFACTOR_STAGE;  // entries staged per batch
   ybus_hip_refactor_stats stats{};
 };
 
@@ -1335,7 +1336,9 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
     x_cap = std::max<int64_t>(x_cap, (int64_t)(h.u_ptr[k + 1] - h.u_ptr[k] + h.l_ptr[k + 1] - h.l_ptr[k] - 1));
   }
   x_cap = round_up(x_cap, 8);
-  if (longest_l > REFACTOR_STAGE || refactor_lds_bytes((int32_t)std::min<int64_t>(x_cap, INDEX_LIMIT)) > 65536 - 256)
+  const int32_t stage = refactor_stage_for(longest_l);
+  if (longest_l > REFACTOR_STAGE ||
+      refactor_lds_bytes((int32_t)std::min<int64_t>(x_cap, INDEX_LIMIT), stage) > 65536 - 256)
     fail(YBUS_HIP_TOO_LARGE, "a column of the factor is too long for the device refactor (L below its diagonal: " +
                                  std::to_string(longest_l) + ", L and U: " + std::to_string(x_cap) + ")");
   std::vector<uint64_t> step_of(n);
@@ -1401,8 +1404,11 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
   plan.control.reserve(4);
   check(hipStreamSynchronize(stream), "refactor plan");
   plan.x_cap = (int32_t)x_cap;
-  // enough wavefronts for the widest levels, a few per SIMD
-  plan.grid = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 16 * (int64_t)f->ctx->cus);
+  plan.stage = stage;
+  // two wavefronts per SIMD: the time is the column graph's chain of hand-overs, which 512 to
+  // 8,192 wavefronts all cover (1.06-1.17 ms at C2, fewer being slightly faster: fewer waiting
+  // wavefronts poll)
+  plan.grid = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 8 * (int64_t)f->ctx->cus);
   plan.stats.multiply_subtracts = (uint64_t)target.size();
   plan.a_ptr.assign(a_ptr, a_ptr + n + 1);
   plan.a_idx.assign(a_idx, a_idx + nnz_a);
@@ -1452,7 +1458,7 @@ int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64
     d.tau = pivot_threshold;
     d.x_cap = plan.x_cap;
     d.spin_limit = refactor_spin_limit();
-    check(launch_refactor(stream, d, plan.grid), "refactor");
+    check(launch_refactor(stream, d, plan.grid, plan.stage), "refactor");
     check(hipEventRecord(f->ev1, stream), "hipEventRecord");
     int32_t control[4] = {0, 0, 0, 0};
     check(hipMemcpyAsync(control, plan.control.p, sizeof(control), hipMemcpyDeviceToHost, stream), "refactor control");
