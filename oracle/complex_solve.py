@@ -339,3 +339,56 @@ class ComplexLu:
         a = np.zeros((n, n), np.complex128)
         a[np.ix_(self.row_perm, self.col_perm)] = lower @ upper
         return a
+
+
+def lu_refactor(lu: ComplexLu, a_ptr, a_idx, a_re, a_im, tau: float = 0.1) -> bool:
+    """Lu::refactor (lu.rs:662-757) for Complex<f64>: the matrix (CSC, entries' real and imaginary
+    parts apart) refactored with ``lu``'s pivots and patterns. Column k of P A Q is scattered into x
+    by pivot step; for each row j above the diagonal in U(:, k), in increasing order, u_jk = x[j]
+    and x[i] = x[i] - l_ij u_jk (num-complex's product, then the difference) over L(:, j) below its
+    diagonal; the pivot x[k] is reused when it is nonzero and its magnitude (num-complex's
+    ``abs``, the hypotenuse of its parts) is at least tau times the largest magnitude below it, in
+    the reference's fold (b replaces a only when b > a); then l = x / pivot (num-complex's
+    quotient). Returns True and replaces ``lu``'s values when every pivot passes ("Reused");
+    False, leaving ``lu`` as it was, when one fails (the reference would factor afresh).
+    The magnitudes are ``math.hypot``'s, so a pivot within a rounding of the threshold is where
+    this and num-complex (libm's hypot) could decide apart: unpinned at such ties."""
+    n = lu.n
+    step_of = np.empty(n, dtype=np.int64)
+    step_of[lu.row_perm] = np.arange(n)
+    l_re, l_im = np.zeros(len(lu.l_val)), np.zeros(len(lu.l_val))
+    u_re, u_im = np.zeros(len(lu.u_val)), np.zeros(len(lu.u_val))
+    xr, xi = np.zeros(n), np.zeros(n)
+    for k in range(n):
+        col = lu.col_perm[k]
+        for p in range(a_ptr[col], a_ptr[col + 1]):
+            s = step_of[a_idx[p]]
+            xr[s], xi[s] = a_re[p], a_im[p]
+        diagonal = lu.u_ptr[k + 1] - 1
+        for p in range(lu.u_ptr[k], diagonal):
+            j = lu.u_idx[p]
+            ur, ui = xr[j], xi[j]
+            xr[j], xi[j] = 0.0, 0.0
+            u_re[p], u_im[p] = ur, ui
+            q = slice(lu.l_ptr[j] + 1, lu.l_ptr[j + 1])
+            rows = lu.l_idx[q]
+            pr, pi = zmul(l_re[q], l_im[q], ur, ui)
+            xr[rows], xi[rows] = xr[rows] - pr, xi[rows] - pi
+        pivot_r, pivot_i = xr[k], xi[k]
+        largest = math.hypot(pivot_r, pivot_i)
+        below = slice(lu.l_ptr[k] + 1, lu.l_ptr[k + 1])
+        for row in lu.l_idx[below]:
+            b = math.hypot(xr[row], xi[row])
+            if b > largest:
+                largest = b
+        if (pivot_r == 0.0 and pivot_i == 0.0) or math.hypot(pivot_r, pivot_i) < tau * largest:
+            return False
+        u_re[diagonal], u_im[diagonal] = pivot_r, pivot_i
+        xr[k], xi[k] = 0.0, 0.0
+        l_re[lu.l_ptr[k]], l_im[lu.l_ptr[k]] = 1.0, 0.0
+        rows = lu.l_idx[below]
+        l_re[below], l_im[below] = zdiv(xr[rows], xi[rows], pivot_r, pivot_i)
+        xr[rows], xi[rows] = 0.0, 0.0
+    lu.l_val = l_re + 1j * l_im
+    lu.u_val = u_re + 1j * u_im
+    return True

@@ -14,7 +14,7 @@ import pytest
 import oracle as O
 from oracle import complex_solve as Z
 from tests.fixtures import (complex_lu_case, complex_relative_residual, complex_right_hand_sides,
-                            hermitian_case, random_spd, random_square_cuda)
+                            grid_laplacian, hermitian_case, random_spd, random_square_cuda)
 
 
 def _exact(v):
@@ -127,3 +127,96 @@ def test_complex_lu_of_a_real_matrix_solves_as_the_real_lu():
     assert np.max(np.abs(x.real - real.solve_many(b))) <= 1e-12 * np.max(np.abs(x.real))
     xt = lu.solve_transpose_many(b.astype(complex))
     assert np.max(np.abs(xt.real - real.solve_transpose_many(b))) <= 1e-12 * np.max(np.abs(xt.real))
+
+
+# ---- the complex LU refactor (lu.rs:662-757 for Complex<f64>) ----
+
+
+def complex_csc(real: O.Matrix, re_scale: float, im_scale: float):
+    """The pattern of a real CSC matrix with each stored v made re_scale v + im_scale v i."""
+    csc = real if not real.row_major else real.to_csc()
+    return csc, re_scale * csc.data, im_scale * csc.data
+
+
+def textbook_complex_refactor(dense: np.ndarray, lu: Z.ComplexLu):
+    """Dense left-looking elimination of P A Q with lu's pivots in explicit real arithmetic:
+    column k takes x[j+1:] = x[j+1:] - L[j+1:, j] x[j] for j = 0 .. k - 1 in order (num-complex's
+    product, then the difference), then L[k+1:, k] = x[k+1:] / x[k] (num-complex's quotient)."""
+    pa = dense[np.ix_(lu.row_perm, lu.col_perm)]
+    n = pa.shape[0]
+    lr, li = np.eye(n), np.zeros((n, n))
+    ur, ui = np.zeros((n, n)), np.zeros((n, n))
+    for k in range(n):
+        xr, xi = pa[:, k].real.copy(), pa[:, k].imag.copy()
+        for j in range(k):
+            pr, pi = Z.zmul(lr[j + 1:, j], li[j + 1:, j], xr[j], xi[j])
+            xr[j + 1:], xi[j + 1:] = xr[j + 1:] - pr, xi[j + 1:] - pi
+        ur[:k + 1, k], ui[:k + 1, k] = xr[:k + 1], xi[:k + 1]
+        lr[k + 1:, k], li[k + 1:, k] = Z.zdiv(xr[k + 1:], xi[k + 1:], xr[k], xi[k])
+    return lr, li, ur, ui
+
+
+def stored(ptr, idx, full):
+    cols = np.repeat(np.arange(len(ptr) - 1), np.diff(ptr))
+    return full[idx, cols]
+
+
+def test_complex_refactor_closed_form():
+    # [[2i, 1], [1, 3]] with natural pivots: l21 = 1 / 2i = -i/2, u22 = 3 - (-i/2) 1 = 3 + i/2
+    real = O.Matrix.from_triplets(2, 2, [(0, 0, 2.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 3.0)], row_major=False)
+    lu = Z.ComplexLu(real.to_dense().astype(complex), O.Lu(real, O.NATURAL))
+    re = np.array([0.0, 1.0, 1.0, 3.0])
+    im = np.array([2.0, 0.0, 0.0, 0.0])
+    assert Z.lu_refactor(lu, real.ptr, real.idx, re, im)
+    assert list(lu.l_val) == [1.0, -0.5j, 1.0]
+    assert list(lu.u_val) == [2j, 1.0, 3 + 0.5j]
+
+
+def test_complex_refactor_of_a_grid_laplacian():
+    # factorization_test.rs:1032-1046: a = (v, -2v), b = (0.7v, -1.1v) on a 5 x 5 grid Laplacian
+    real = grid_laplacian(5).to_csc()
+    lu_real = O.Lu(real, O.AMD)
+    a = real.to_dense() * (1.0 - 2.0j)
+    lu = Z.ComplexLu(a, lu_real)
+    csc, re, im = complex_csc(real, 0.7, -1.1)
+    assert Z.lu_refactor(lu, csc.ptr, csc.idx, re, im)
+    b = real.to_dense() * (0.7 - 1.1j)
+    x = np.array([complex(i, 1.0) for i in range(25)])
+    solved = lu.solve_many((b @ x)[None, :])[0]
+    assert np.abs(solved - x).max() < 1e-10
+
+
+@pytest.mark.parametrize("n,fill,seed", [(40, 8, 3), (70, 5, 9)])
+def test_complex_refactor_equals_the_dense_textbook_elimination_bit_for_bit(n, fill, seed):
+    lu, a = complex_lu_case(n, fill, seed)
+    real = random_square_cuda(n, fill, seed)
+    # new values on the same pattern: each stored entry scaled by a complex factor near 1
+    csc = real
+    rng = np.random.default_rng(seed)
+    scale = 1.0 + 0.02 * (rng.random(csc.nnz) - 0.5) + 0.01j * (rng.random(csc.nnz) - 0.5)
+    cols = np.repeat(np.arange(n), np.diff(csc.ptr))
+    values = a[csc.idx, cols] * scale
+    assert Z.lu_refactor(lu, csc.ptr, csc.idx, values.real.copy(), values.imag.copy())
+    dense = np.zeros((n, n), complex)
+    dense[csc.idx, cols] = values
+    lr, li, ur, ui = textbook_complex_refactor(dense, lu)
+    for got, want in [(lu.l_val.real, stored(lu.l_ptr, lu.l_idx, lr)), (lu.l_val.imag, stored(lu.l_ptr, lu.l_idx, li)),
+                      (lu.u_val.real, stored(lu.u_ptr, lu.u_idx, ur)), (lu.u_val.imag, stored(lu.u_ptr, lu.u_idx, ui))]:
+        assert got.tobytes() == want.tobytes()
+    # and it factors the new matrix
+    pa = dense[np.ix_(lu.row_perm, lu.col_perm)]
+    l_full = np.zeros((n, n), complex)
+    l_full[lu.l_idx, np.repeat(np.arange(n), np.diff(lu.l_ptr))] = lu.l_val
+    u_full = np.zeros((n, n), complex)
+    u_full[lu.u_idx, np.repeat(np.arange(n), np.diff(lu.u_ptr))] = lu.u_val
+    assert np.abs(l_full @ u_full - pa).max() < 1e-12 * np.abs(pa).max()
+
+
+def test_complex_refactor_refuses_a_small_pivot_and_keeps_the_factor():
+    real = O.Matrix.from_triplets(2, 2, [(0, 0, 2.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 3.0)], row_major=False)
+    lu = Z.ComplexLu(real.to_dense().astype(complex), O.Lu(real, O.NATURAL))
+    before = (lu.l_val.copy(), lu.u_val.copy())
+    # the first pivot 0.01 + 0.01i against 1 below it
+    assert not Z.lu_refactor(lu, real.ptr, real.idx, np.array([0.01, 1.0, 1.0, 3.0]), np.array([0.01, 0, 0, 0]))
+    assert np.array_equal(lu.l_val, before[0]) and np.array_equal(lu.u_val, before[1])
+    assert not Z.lu_refactor(lu, real.ptr, real.idx, np.array([0.0, 1.0, 1.0, 3.0]), np.zeros(4))
