@@ -160,12 +160,22 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx *ctx, uint64_t n, const uint64_t 
 int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor *f, uint64_t n, const uint64_t *a_ptr,
                                     const uint64_t *a_idx, const double *a_val,
                                     double pivot_threshold, int32_t *outcome);
+/* The same for an uploaded complex LU factor (Lu<Complex64>: the reference's refactor is generic
+ * over the element type): a_val holds (re, im) pairs, 2 * a_ptr[n] doubles. The arithmetic is
+ * num-complex 0.4's (the product, then the difference; its quotient), and a pivot's magnitude is
+ * its norm, hypot(re, im). */
+int32_t ybus_hip_factor_refactor_lu_complex(ybus_hip_factor *f, uint64_t n, const uint64_t *a_ptr,
+                                            const uint64_t *a_idx, const double *a_val,
+                                            double pivot_threshold, int32_t *outcome);
 /* New values for an uploaded real LU factor with the same pattern and permutations (a host
  * refactorization that reused its pivots): l_val and u_val in the uploaded CSC order. The
- * factor's sweeps are rewritten on the device, with no new analysis. */
+ * factor's sweeps are rewritten on the device, with no new analysis. The _complex form takes a
+ * complex factor's values as (re, im) pairs. */
 int32_t ybus_hip_factor_update_lu_values(ybus_hip_factor *f, const double *l_val, const double *u_val);
-/* The refactorization statistics, and the new L and U values of the factor (in the uploaded
- * CSC order: l_val L's entries, u_val U's; either may be NULL). */
+int32_t ybus_hip_factor_update_lu_values_complex(ybus_hip_factor *f, const double *l_val,
+                                                 const double *u_val);
+/* The refactorization statistics, and the L and U values of the factor (in the uploaded CSC
+ * order: l_val L's entries, u_val U's, (re, im) pairs for a complex factor; either may be NULL). */
 int32_t ybus_hip_factor_refactor_stats(const ybus_hip_factor *f, ybus_hip_refactor_stats *out);
 int32_t ybus_hip_factor_lu_values(ybus_hip_factor *f, double *l_val, double *u_val);
 

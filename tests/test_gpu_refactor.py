@@ -288,3 +288,96 @@ def test_one_unknown_and_a_diagonal(gpu):
     assert factor.refactor(b) == "reused"
     assert lu.refactor(b) == "reused"
     assert same_bits(factor.lu_values()[1], lu.u.data)
+
+
+# ---- complex LU factors (the reference's refactor is generic over the element type) ----
+
+
+class ComplexCsc:
+    """A complex CSC matrix as the refactor takes it: ptr, idx and complex data."""
+
+    def __init__(self, ptr, idx, data):
+        self.ptr, self.idx, self.data = ptr, idx, np.asarray(data, dtype=np.complex128)
+        self.row_major = False
+
+
+def complex_case(n: int, fill: int, seed: int):
+    from tests.fixtures import complex_lu_case, random_square_cuda
+
+    lu, a = complex_lu_case(n, fill, seed)
+    real = random_square_cuda(n, fill, seed)
+    rng = np.random.default_rng(seed)
+    scale = 1.0 + 0.02 * (rng.random(real.nnz) - 0.5) + 0.01j * (rng.random(real.nnz) - 0.5)
+    cols = np.repeat(np.arange(n), np.diff(real.ptr))
+    return lu, ComplexCsc(real.ptr, real.idx, a[real.idx, cols] * scale)
+
+
+def complex_jacobian_case(buses: int, seed: int):
+    """A Newton Jacobian made complex (each value v as v (1 - 0.5 i)) over its real COLAMD LU, and
+    the same pattern with every value turned by 0.9 + 0.05 i."""
+    from oracle import complex_solve as Z
+
+    jac = O.make_grid(buses, seed).jacobian(seed)[0]
+    lu = Z.ComplexLu(jac.to_dense() * (1.0 - 0.5j), O.Lu(jac, O.COLAMD))
+    return lu, ComplexCsc(jac.ptr, jac.idx, jac.data * (1.0 - 0.5j) * (0.9 + 0.05j))
+
+
+@pytest.mark.parametrize("case", ["random-40", "random-120", "jacobian"])
+def test_complex_refactor_matches_the_cpu_bit_for_bit(gpu, case):
+    from oracle import complex_solve as Z
+    from tests.fixtures import complex_right_hand_sides
+
+    if case == "random-40":
+        lu, b = complex_case(40, 8, 3)
+    elif case == "random-120":
+        lu, b = complex_case(120, 4, 11)
+    else:
+        lu, b = complex_jacobian_case(150, 20261212)
+    n, seed = lu.n, 7
+    factor = Y.GpuFactor.from_lu_complex(gpu, lu)
+    assert factor.refactor(b) == "reused"
+    assert Z.lu_refactor(lu, b.ptr, b.idx, b.data.real.copy(), b.data.imag.copy())
+    l_val, u_val = factor.lu_values()
+    assert same_bits(l_val.view(np.float64), lu.l_val.view(np.float64))
+    assert same_bits(u_val.view(np.float64), lu.u_val.view(np.float64))
+    rhs = complex_right_hand_sides(n, 4, seed)
+    assert same_bits(factor.solve_many(rhs).view(np.float64), lu.solve_many(rhs).view(np.float64))
+    assert same_bits(factor.solve_transpose_many(rhs).view(np.float64), lu.solve_transpose_many(rhs).view(np.float64))
+
+
+def test_complex_refactor_of_a_grid_laplacian(gpu):
+    # factorization_test.rs:1032-1046 on the device: a = (v, -2v), then b = (0.7v, -1.1v)
+    from oracle import complex_solve as Z
+    from tests.fixtures import grid_laplacian
+
+    real = grid_laplacian(5).to_csc()
+    lu = Z.ComplexLu(real.to_dense() * (1.0 - 2.0j), O.Lu(real, O.AMD))
+    factor = Y.GpuFactor.from_lu_complex(gpu, lu)
+    b = ComplexCsc(real.ptr, real.idx, real.data * (0.7 - 1.1j))
+    assert factor.refactor(b) == "reused"
+    x = np.array([complex(i, 1.0) for i in range(25)])
+    solved = factor.solve_many(((real.to_dense() * (0.7 - 1.1j)) @ x)[None, :])[0]
+    assert np.abs(solved - x).max() < 1e-10
+
+
+def test_complex_refactor_refuses_a_small_pivot_and_the_other_kind(gpu):
+    from oracle import complex_solve as Z
+
+    real = O.Matrix.from_triplets(2, 2, [(0, 0, 2.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 3.0)], row_major=False)
+    lu = Z.ComplexLu(real.to_dense().astype(complex), O.Lu(real, O.NATURAL))
+    factor = Y.GpuFactor.from_lu_complex(gpu, lu)
+    before = factor.lu_values()
+    small = ComplexCsc(real.ptr, real.idx, [0.01 + 0.01j, 1.0, 1.0, 3.0])
+    assert factor.refactor(small) == "needs_pivoting"
+    after = factor.lu_values()
+    assert same_bits(before[0].view(np.float64), after[0].view(np.float64))
+    assert same_bits(before[1].view(np.float64), after[1].view(np.float64))
+    exact = ComplexCsc(real.ptr, real.idx, [2j, 1.0, 1.0, 3.0])
+    assert factor.refactor(exact) == "reused"
+    assert list(factor.lu_values()[0]) == [1.0, -0.5j, 1.0]
+    assert list(factor.lu_values()[1]) == [2j, 1.0, 3 + 0.5j]
+    # a real factor takes the real entry points only, and the reverse
+    real_factor = Y.GpuFactor.from_lu(gpu, O.Lu(real, O.NATURAL))
+    ones = np.ones(8)
+    assert Y.library().ybus_hip_factor_update_lu_values_complex(real_factor._h, ones, ones) == Y.INVALID_ARGUMENT
+    assert Y.library().ybus_hip_factor_update_lu_values(factor._h, ones, ones) == Y.INVALID_ARGUMENT

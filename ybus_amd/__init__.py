@@ -178,6 +178,8 @@ def _declare(L):
         "ybus_hip_batch_reset_kernel_stats": (None, [P]),
         "ybus_hip_factor_refactor_lu": (i32, [P, U64, u64p, u64p, f64p, C.c_double, C.POINTER(C.c_int32)]),
         "ybus_hip_factor_update_lu_values": (i32, [P, f64p, f64p]),
+        "ybus_hip_factor_refactor_lu_complex": (i32, [P, U64, u64p, u64p, f64p, C.c_double, C.POINTER(C.c_int32)]),
+        "ybus_hip_factor_update_lu_values_complex": (i32, [P, f64p, f64p]),
         "ybus_hip_factor_refactor_stats": (i32, [P, C.POINTER(_RefactorStats)]),
         "ybus_hip_factor_lu_values": (i32, [P, C.c_void_p, C.c_void_p]),
         "ybus_hip_analyze_cholesky": (i32, [U64, u64p, u64p, P, P, P, P]),
@@ -427,7 +429,9 @@ class GpuFactor:
                                                            _interleaved(l_val), _u64(u_ptr), _u64(u_idx),
                                                            _interleaved(u_val), _u64(row_perm), _u64(col_perm),
                                                            C.byref(h)))
-        return GpuFactor(gpu, h, int(n), True)
+        factor = GpuFactor(gpu, h, int(n), True)
+        factor._lu_sizes = (int(np.asarray(l_ptr)[-1]), int(np.asarray(u_ptr)[-1]))
+        return factor
 
     def is_complex(self) -> bool:
         return bool(library().ybus_hip_factor_is_complex(self._h))
@@ -497,14 +501,27 @@ class GpuFactor:
         if getattr(a, "row_major", False):
             raise InvalidArgument(INVALID_ARGUMENT, "refactor takes the matrix in CSC form")
         outcome = C.c_int32()
-        _check(library().ybus_hip_factor_refactor_lu(self._h, self._n, _u64(a.ptr), _u64(a.idx), _f64(a.data),
-                                                     float(pivot_threshold), C.byref(outcome)))
+        if self._complex:
+            # a complex matrix: data complex128, or ``re`` and ``im`` arrays beside the pattern
+            values = getattr(a, "data", None)
+            if values is None or not np.iscomplexobj(values):
+                values = np.asarray(a.re) + 1j * np.asarray(a.im)
+            _check(library().ybus_hip_factor_refactor_lu_complex(
+                self._h, self._n, _u64(a.ptr), _u64(a.idx), _interleaved(values), float(pivot_threshold),
+                C.byref(outcome)))
+        else:
+            _check(library().ybus_hip_factor_refactor_lu(self._h, self._n, _u64(a.ptr), _u64(a.idx), _f64(a.data),
+                                                         float(pivot_threshold), C.byref(outcome)))
         return "reused" if outcome.value == 0 else "needs_pivoting"
 
     def update_lu_values(self, l_val, u_val):
         """New L and U values (the uploaded CSC order, the same pattern and permutations), as a host
         refactor that reused its pivots makes them: the device sweeps are rewritten in place."""
-        _check(library().ybus_hip_factor_update_lu_values(self._h, _f64(l_val), _f64(u_val)))
+        if self._complex:
+            _check(library().ybus_hip_factor_update_lu_values_complex(self._h, _interleaved(l_val),
+                                                                      _interleaved(u_val)))
+        else:
+            _check(library().ybus_hip_factor_update_lu_values(self._h, _f64(l_val), _f64(u_val)))
 
     def refactor_stats(self) -> dict:
         s = _RefactorStats()
@@ -513,9 +530,11 @@ class GpuFactor:
 
     def lu_values(self):
         """The factor's current (L values, U values) in the uploaded CSC order."""
-        nl = int(self._lu_sizes[0])
-        l_val, u_val = np.zeros(nl), np.zeros(int(self._lu_sizes[1]))
+        parts = 2 if self._complex else 1
+        l_val, u_val = np.zeros(parts * int(self._lu_sizes[0])), np.zeros(parts * int(self._lu_sizes[1]))
         _check(library().ybus_hip_factor_lu_values(self._h, l_val.ctypes.data, u_val.ctypes.data))
+        if self._complex:
+            return l_val[0::2] + 1j * l_val[1::2], u_val[0::2] + 1j * u_val[1::2]
         return l_val, u_val
 
     def set_matrix(self, a):

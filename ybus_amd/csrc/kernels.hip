@@ -1751,11 +1751,11 @@ hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fw
 
 // ---- LU refactorization (lu.rs:662-757) ----
 
-// LDS of one wavefront's column: x (x_cap doubles), the staged L values and their target slots,
+// LDS of one wavefront's column: x (x_cap values), the staged L values and their target slots,
 // and each staged step's first entry and L position
-size_t refactor_lds_bytes(int32_t x_cap, int32_t stage) {
-  return (size_t)x_cap * sizeof(double) + (size_t)stage * (sizeof(double) + sizeof(int32_t)) +
-         2 * 64 * sizeof(int32_t);
+size_t refactor_lds_bytes(int32_t x_cap, int32_t stage, bool complex) {
+  const size_t value = complex ? 2 * sizeof(double) : sizeof(double);
+  return (size_t)x_cap * value + (size_t)stage * (value + sizeof(int32_t)) + 2 * 64 * sizeof(int32_t);
 }
 
 int32_t refactor_stage_for(int32_t longest_l) {
@@ -1777,6 +1777,43 @@ __device__ inline void share_value(double* p, double v) {
 }
 __device__ inline bool pending(double v) { return (uint64_t)__double_as_longlong(v) == REFACTOR_PENDING; }
 
+// The refactor's arithmetic on its element type. Real: x - l u and x / p in FP64, |x| the absolute
+// value. Complex (num-complex 0.4's Complex<f64>, as the reference's Lu<Complex64>): the product
+// then the difference, its quotient formula, and |x| = hypot(re, im) (ComplexFloat::abs is the
+// norm); a complex value crosses between wavefronts as its two parts, each its own device-scope
+// access, and counts as written only once both are.
+struct RealValues {
+  using V = double;
+  static __device__ V zero() { return 0.0; }
+  static __device__ V one() { return 1.0; }
+  static __device__ V sub_mul(V x, V l, V u) { return x - l * u; }
+  static __device__ V quotient(V x, V p) { return x / p; }
+  static __device__ double magnitude(V x) { return fabs(x); }
+  static __device__ bool is_zero(V x) { return x == 0.0; }
+  static __device__ V load(const V* p) { return shared_value(p); }
+  static __device__ void store(V* p, V v) { share_value(p, v); }
+  static __device__ bool unwritten(V v) { return pending(v); }
+};
+struct ComplexValues {
+  using V = double2;
+  static __device__ V zero() { return make_double2(0.0, 0.0); }
+  static __device__ V one() { return make_double2(1.0, 0.0); }
+  static __device__ V sub_mul(V x, V l, V u) { return zsub(x, zmul(l, u)); }
+  static __device__ V quotient(V x, V p) { return zdiv(x, p); }
+  static __device__ double magnitude(V x) { return hypot(x.x, x.y); }
+  static __device__ bool is_zero(V x) { return x.x == 0.0 && x.y == 0.0; }
+  static __device__ V load(const V* p) {
+    const double* q = reinterpret_cast<const double*>(p);
+    return make_double2(shared_value(q), shared_value(q + 1));
+  }
+  static __device__ void store(V* p, V v) {
+    double* q = reinterpret_cast<double*>(p);
+    share_value(q, v.x);
+    share_value(q + 1, v.y);
+  }
+  static __device__ bool unwritten(V v) { return pending(v.x) || pending(v.y); }
+};
+
 // One wavefront per column, columns taken from control[0] in level order, so every column a
 // wavefront waits on was taken earlier by a wavefront that is running (each wait ends). The steps
 // of a column go in batches: the L values and target slots of the next (up to 64) rows of U(:, k)
@@ -1787,15 +1824,19 @@ __device__ inline bool pending(double v) { return (uint64_t)__double_as_longlong
 // its LDS accesses and costs no barrier.) A wait that outlasts spin_limit polls gives the column
 // up (counted in control[2]); its L values are still written (zeros), so no wavefront waits
 // forever.
-template <int R>
+template <int R, class Val>
 __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
+  using V = typename Val::V;
   constexpr int32_t STAGE = 64 * R;
   extern __shared__ double refactor_lds[];
-  double* xs = refactor_lds;
-  double* sval = xs + d.x_cap;
+  V* xs = reinterpret_cast<V*>(refactor_lds);
+  V* sval = xs + d.x_cap;
   int32_t* stgt = reinterpret_cast<int32_t*>(sval + STAGE);
   int32_t* s_off = stgt + STAGE;
   int32_t* s_first = s_off + 64;
+  const V* a_val = reinterpret_cast<const V*>(d.a_val);
+  V* l_val = reinterpret_cast<V*>(d.l_val);
+  V* u_val = reinterpret_cast<V*>(d.u_val);
   const int lane = threadIdx.x;
   for (;;) {
     int32_t c = 0;
@@ -1805,9 +1846,9 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
     const int4 head = d.head[c], span = d.span[c];
     const int32_t k = head.x, u0 = head.y, nu = head.z, l0 = head.w, below = span.x;
     const int32_t steps = nu - 1, m = nu + below;
-    for (int32_t i = lane; i < m; i += 64) xs[i] = 0.0;
+    for (int32_t i = lane; i < m; i += 64) xs[i] = Val::zero();
     __syncthreads();
-    for (int32_t p = span.y + lane; p < span.z; p += 64) xs[d.a_slot[p]] = d.a_val[p];
+    for (int32_t p = span.y + lane; p < span.z; p += 64) xs[d.a_slot[p]] = a_val[p];
     __syncthreads();
     int32_t e = span.w;
     int32_t t = 0;
@@ -1839,7 +1880,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
         s_first[lane] = first;
       }
       __syncthreads();
-      double v[R];
+      V v[R];
       int32_t g[R], at[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1856,7 +1897,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
               hi = mid - 1;
           }
           at[r] = s_first[lo] + (f - s_off[lo]);
-          v[r] = shared_value(&d.l_val[at[r]]);
+          v[r] = Val::load(&l_val[at[r]]);
           g[r] = d.target[e + f];
         }
       }
@@ -1868,7 +1909,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int32_t f = lane + 64 * r;
-          if (f < total && pending(v[r]) && f < waiting) waiting = f;
+          if (f < total && Val::unwritten(v[r]) && f < waiting) waiting = f;
         }
         for (int off = 32; off > 0; off >>= 1) waiting = min(waiting, __shfl_xor(waiting, off, 64));
         count = __popcll(__ballot(lane < cand && incl <= waiting));
@@ -1877,7 +1918,7 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int32_t f = lane + 64 * r;
-          if (f < total && pending(v[r])) v[r] = shared_value(&d.l_val[at[r]]);
+          if (f < total && Val::unwritten(v[r])) v[r] = Val::load(&l_val[at[r]]);
         }
       }
       if (count == 0) {
@@ -1899,18 +1940,18 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
       int32_t o0 = s_off[0], o1 = count > 1 ? s_off[1] : used;
       bool has = o0 + lane < o1;
       int32_t slot = has ? stgt[o0 + lane] : 0;
-      double lv = has ? sval[o0 + lane] : 0.0;
+      V lv = has ? sval[o0 + lane] : Val::zero();
       for (int32_t q = 0; q < count; ++q) {
-        const double u = xs[t + q];
-        const double xv = has ? xs[slot] : 0.0;
+        const V u = xs[t + q];
+        const V xv = has ? xs[slot] : Val::zero();
         const int32_t n0 = o1, n1 = q + 2 < count ? s_off[q + 2] : used;
         const bool next_has = q + 1 < count && n0 + lane < n1;
         const int32_t next_slot = next_has ? stgt[n0 + lane] : 0;
-        const double next_lv = next_has ? sval[n0 + lane] : 0.0;
-        if (has) xs[slot] = xv - lv * u;
+        const V next_lv = next_has ? sval[n0 + lane] : Val::zero();
+        if (has) xs[slot] = Val::sub_mul(xv, lv, u);
         for (int32_t f = o0 + 64 + lane; f < o1; f += 64) {
           const int32_t sl = stgt[f];
-          xs[sl] = xs[sl] - sval[f] * u;
+          xs[sl] = Val::sub_mul(xs[sl], sval[f], u);
         }
         __syncthreads();
         o0 = n0;
@@ -1924,29 +1965,29 @@ __global__ __launch_bounds__(64) void refactor_kernel(RefactorDev d) {
     }
     if (!abandoned) {
       // lu.rs:717-733: the reused pivot against the largest magnitude below it
-      const double pivot = xs[nu - 1];
+      const V pivot = xs[nu - 1];
       double big = -1.0;
       for (int32_t i = lane; i < below; i += 64) {
-        const double b = fabs(xs[nu + i]);
+        const double b = Val::magnitude(xs[nu + i]);
         if (b > big) big = b;
       }
       for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(big, off, 64);
         if (o > big) big = o;
       }
-      double largest = fabs(pivot);
+      double largest = Val::magnitude(pivot);
       if (big > largest) largest = big;
       // the values other columns wait for go first
-      for (int32_t i = lane; i < below; i += 64) share_value(&d.l_val[l0 + 1 + i], xs[nu + i] / pivot);
+      for (int32_t i = lane; i < below; i += 64) Val::store(&l_val[l0 + 1 + i], Val::quotient(xs[nu + i], pivot));
       if (lane == 0) {
-        if (pivot == 0.0 || fabs(pivot) < d.tau * largest) atomicMin(&d.control[1], k);
-        d.l_val[l0] = 1.0;
+        if (Val::is_zero(pivot) || Val::magnitude(pivot) < d.tau * largest) atomicMin(&d.control[1], k);
+        l_val[l0] = Val::one();
       }
       // x's U slots hold u_jk (no step writes a slot at or before its own), the last one the pivot
-      for (int32_t i = lane; i < nu; i += 64) d.u_val[u0 + i] = xs[i];
+      for (int32_t i = lane; i < nu; i += 64) u_val[u0 + i] = xs[i];
     } else {
       if (lane == 0) atomicAdd(&d.control[2], 1);
-      for (int32_t i = lane; i < below; i += 64) share_value(&d.l_val[l0 + 1 + i], 0.0);
+      for (int32_t i = lane; i < below; i += 64) Val::store(&l_val[l0 + 1 + i], Val::zero());
     }
     __syncthreads();
   }
@@ -1975,18 +2016,24 @@ hipError_t launch_fill_u64(hipStream_t stream, uint64_t* p, uint64_t value, int6
   return hipGetLastError();
 }
 
-hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid, int32_t stage) {
-  if (d.n <= 0) return hipSuccess;
-  const size_t bytes = refactor_lds_bytes(d.x_cap, stage);
-  if (grid <= 0 || bytes > 65536 - 256) return hipErrorInvalidValue;
+template <class Val>
+static hipError_t launch_refactor_as(hipStream_t stream, const RefactorDev& d, int grid, int32_t stage, size_t bytes) {
   switch (stage) {
-    case 64: hipLaunchKernelGGL(refactor_kernel<1>, dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
-    case 128: hipLaunchKernelGGL(refactor_kernel<2>, dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
-    case 256: hipLaunchKernelGGL(refactor_kernel<4>, dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
-    case 512: hipLaunchKernelGGL(refactor_kernel<8>, dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
+    case 64: hipLaunchKernelGGL((refactor_kernel<1, Val>), dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
+    case 128: hipLaunchKernelGGL((refactor_kernel<2, Val>), dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
+    case 256: hipLaunchKernelGGL((refactor_kernel<4, Val>), dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
+    case 512: hipLaunchKernelGGL((refactor_kernel<8, Val>), dim3((unsigned)grid), dim3(64), bytes, stream, d); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid, int32_t stage, bool complex) {
+  if (d.n <= 0) return hipSuccess;
+  const size_t bytes = refactor_lds_bytes(d.x_cap, stage, complex);
+  if (grid <= 0 || bytes > 65536 - 256) return hipErrorInvalidValue;
+  return complex ? launch_refactor_as<ComplexValues>(stream, d, grid, stage, bytes)
+                 : launch_refactor_as<RealValues>(stream, d, grid, stage, bytes);
 }
 
 __global__ void refresh_entries_kernel(RefreshDev r, const double* __restrict__ src) {
@@ -2014,6 +2061,23 @@ __global__ void refresh_rows_kernel(RefreshDev r, const double* __restrict__ src
     r.rdiag64[k] = dg;
     r.rdiag32[k] = (float)dg;
   }
+}
+
+__global__ void refresh_complex_kernel(int32_t n, int64_t entries, const int32_t* __restrict__ vpos,
+                                       const int32_t* __restrict__ dpos, const double2* __restrict__ src,
+                                       double2* __restrict__ val, double2* __restrict__ diag) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < entries) val[i] = src[vpos[i]];
+  if (dpos && i < n) diag[i] = src[dpos[i]];
+}
+
+hipError_t launch_refresh_complex(hipStream_t stream, int32_t n, int64_t entries, const int32_t* vpos,
+                                  const int32_t* dpos, const double2* src, double2* val, double2* diag) {
+  const int64_t count = std::max<int64_t>(entries, dpos ? n : 0);
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(refresh_complex_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, n, entries,
+                     vpos, dpos, src, val, diag);
+  return hipGetLastError();
 }
 
 hipError_t launch_refresh(hipStream_t stream, const RefreshDev& r, const double* src) {

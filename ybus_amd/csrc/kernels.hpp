@@ -284,10 +284,12 @@ hipError_t launch_gather(hipStream_t stream, const double* src, const int32_t* i
 constexpr int32_t REFACTOR_STAGE = 512;
 int32_t refactor_stage_for(int32_t longest_l);
 // LDS of one wavefront
-size_t refactor_lds_bytes(int32_t x_cap, int32_t stage);
+size_t refactor_lds_bytes(int32_t x_cap, int32_t stage, bool complex);
 // `grid` single-wavefront workgroups take the columns, staging `stage` entries per batch;
-// control[0], [2] must hold 0 and control[1] n at the launch
-hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid, int32_t stage);
+// control[0], [2] must hold 0 and control[1] n at the launch. A complex factor's values (the
+// matrix's, L's and U's) are (re, im) pairs of doubles, and each part of L's holds
+// REFACTOR_PENDING at the launch.
+hipError_t launch_refactor(hipStream_t stream, const RefactorDev& d, int grid, int32_t stage, bool complex);
 
 // A sweep's value arrays rewritten from new factor values: every entry takes src[vpos[p]] (a
 // position in the new L values followed by the new U values), every diagonal src[dpos[i]], in
@@ -309,5 +311,9 @@ struct RefreshDev {
   float* rdiag32 = nullptr;
 };
 hipError_t launch_refresh(hipStream_t stream, const RefreshDev& r, const double* src);
+// A complex sweep's values (one (re, im) pair per entry, and per row unless its diagonal is unit)
+// from new factor values: val[p] = src[vpos[p]], diag[i] = src[dpos[i]]
+hipError_t launch_refresh_complex(hipStream_t stream, int32_t n, int64_t entries, const int32_t* vpos,
+                                  const int32_t* dpos, const double2* src, double2* val, double2* diag);
 
 }  // namespace ybus_hip

@@ -467,13 +467,14 @@ struct ReachIndex {
 // value update
 struct LuPattern {
   std::vector<uint64_t> l_ptr, l_idx, u_ptr, u_idx, row_perm, col_perm;
-  std::vector<double> values;  // the uploaded values: L's, then U's
+  std::vector<double> values;  // the uploaded values: L's, then U's ((re, im) pairs when complex)
 };
 
 // One sweep's value arrays and where each value comes from in the new factor values (L's, then
 // U's, in the uploaded CSC order)
 struct SweepRefresh {
-  DevSweep* sweep = nullptr;
+  DevSweep* sweep = nullptr;     // a real factor's sweep
+  DevSweepC* zsweep = nullptr;   // or a complex factor's
   DevBuf<int32_t> vpos, dpos;
   SweepRefresh() = default;
   SweepRefresh(const SweepRefresh&) = delete;
@@ -1208,10 +1209,13 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
 
 // ---- new values for an uploaded LU factor (lu.rs:662-757) ----
 
-static void require_lu(const ybus_hip_factor* f) {
+// An LU factor with its pattern kept, real or complex as `complex` says
+static void require_lu(const ybus_hip_factor* f, bool complex) {
   if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "null factor");
-  if (!f->lu || f->complex || f->lu_host.l_ptr.empty())
-    fail(YBUS_HIP_INVALID_ARGUMENT, "new values need a real LU factor");
+  if (!f->lu || f->lu_host.l_ptr.empty()) fail(YBUS_HIP_INVALID_ARGUMENT, "new values need an LU factor");
+  if (f->complex != complex)
+    fail(YBUS_HIP_INVALID_ARGUMENT, complex ? "a real factor takes the real refactor and value update"
+                                            : "a complex factor takes the complex refactor and value update");
 }
 
 // The sweeps' refresh maps: each sweep scheduled again from factors whose values are their own
@@ -1225,11 +1229,6 @@ static RefactorPlan& refactor_plan(ybus_hip_factor* f) {
   std::vector<double> lpos(nl), upos(nu);
   for (uint64_t p = 0; p < nl; ++p) lpos[p] = (double)(p + 1);
   for (uint64_t p = 0; p < nu; ++p) upos[p] = (double)(nl + p + 1);
-  RowForm l, u, ut, lt;
-  run_together({[&] { l = lu_forward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data()); },
-                [&] { u = lu_backward(n, h.u_ptr.data(), h.u_idx.data(), upos.data()); },
-                [&] { ut = lu_transpose_forward(n, h.u_ptr.data(), h.u_idx.data(), upos.data()); },
-                [&] { lt = lu_transpose_backward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data()); }});
   std::vector<int64_t> rp(n), cp(n), ident(n);
   for (int64_t i = 0; i < n; ++i) {
     rp[i] = (int64_t)h.row_perm[i];
@@ -1237,6 +1236,50 @@ static RefactorPlan& refactor_plan(ybus_hip_factor* f) {
     ident[i] = i;
   }
   hipStream_t stream = f->stream();
+  if (f->complex) {
+    // complex sweeps: the same schedules over forms whose real parts are the positions
+    std::vector<double> zl(nl, 0.0), zu(nu, 0.0);
+    RowForm zlf, zuf, zutf, zltf;
+    run_together({[&] { zlf = complex_form(lu_forward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data()),
+                                           lu_forward(n, h.l_ptr.data(), h.l_idx.data(), zl.data()), false); },
+                  [&] { zuf = complex_form(lu_backward(n, h.u_ptr.data(), h.u_idx.data(), upos.data()),
+                                           lu_backward(n, h.u_ptr.data(), h.u_idx.data(), zu.data()), false); },
+                  [&] { zutf = complex_form(lu_transpose_forward(n, h.u_ptr.data(), h.u_idx.data(), upos.data()),
+                                            lu_transpose_forward(n, h.u_ptr.data(), h.u_idx.data(), zu.data()), false); },
+                  [&] { zltf = complex_form(lu_transpose_backward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data()),
+                                            lu_transpose_backward(n, h.l_ptr.data(), h.l_idx.data(), zl.data()), false); }});
+    Sweep zs[4];
+    run_together({[&] { zs[0] = schedule(zlf, rp, ident); }, [&] { zs[1] = schedule(zuf, ident, cp); },
+                  [&] { zs[2] = schedule(zutf, cp, ident); }, [&] { zs[3] = schedule(zltf, ident, rp); }});
+    DevSweepC* targets[4] = {&f->zfwd, &f->zbwd, &f->ztfwd, &f->ztbwd};
+    for (int q = 0; q < 4; ++q) {
+      const Sweep& z = zs[q];
+      if (z.n != targets[q]->n || z.levels != targets[q]->levels || z.unit != targets[q]->unit)
+        fail(YBUS_HIP_INTERNAL, "a sweep's schedule differs from its upload");
+      plan->refresh.emplace_back();
+      SweepRefresh& r = plan->refresh.back();
+      r.zsweep = targets[q];
+      std::vector<int32_t> vp(z.val.size());
+      for (size_t p = 0; p < vp.size(); ++p) vp[p] = (int32_t)z.val[p] - 1;
+      r.vpos.upload(vp, stream);
+      if (!z.unit) {
+        std::vector<int32_t> dp(z.diag.size());
+        for (size_t k = 0; k < dp.size(); ++k) dp[k] = (int32_t)z.diag[k] - 1;
+        r.dpos.upload(dp, stream);
+      }
+      check(hipStreamSynchronize(stream), "refresh maps");
+    }
+    plan->vals.upload(h.values, stream);
+    plan->next.reserve(h.values.size());
+    check(hipStreamSynchronize(stream), "refresh maps");
+    f->refactor = std::move(plan);
+    return *f->refactor;
+  }
+  RowForm l, u, ut, lt;
+  run_together({[&] { l = lu_forward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data()); },
+                [&] { u = lu_backward(n, h.u_ptr.data(), h.u_idx.data(), upos.data()); },
+                [&] { ut = lu_transpose_forward(n, h.u_ptr.data(), h.u_idx.data(), upos.data()); },
+                [&] { lt = lu_transpose_backward(n, h.l_ptr.data(), h.l_idx.data(), lpos.data()); }});
   auto add = [&](DevSweep& d, const Sweep& s) {
     if (s.row_ptr != d.host.row_ptr || s.dep != d.host.dep || s.unit != d.host.unit)
       fail(YBUS_HIP_INTERNAL, "a sweep's schedule differs from its upload");
@@ -1283,6 +1326,13 @@ static double refresh_sweeps(ybus_hip_factor* f, RefactorPlan& plan) {
   hipStream_t stream = f->stream();
   check(hipEventRecord(f->ev0, stream), "hipEventRecord");
   for (SweepRefresh& r : plan.refresh) {
+    if (r.zsweep) {
+      DevSweepC& z = *r.zsweep;
+      check(launch_refresh_complex(stream, z.n, (int64_t)z.val.len, r.vpos.p, z.unit ? nullptr : r.dpos.p,
+                                   reinterpret_cast<const double2*>(plan.next.p), z.val.p, z.unit ? nullptr : z.diag.p),
+            "refresh sweeps");
+      continue;
+    }
     DevSweep& d = *r.sweep;
     RefreshDev v;
     v.n = d.host.n;
@@ -1338,7 +1388,7 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
   x_cap = round_up(x_cap, 8);
   const int32_t stage = refactor_stage_for(longest_l);
   if (longest_l > REFACTOR_STAGE ||
-      refactor_lds_bytes((int32_t)std::min<int64_t>(x_cap, INDEX_LIMIT), stage) > 65536 - 256)
+      refactor_lds_bytes((int32_t)std::min<int64_t>(x_cap, INDEX_LIMIT), stage, f->complex) > 65536 - 256)
     fail(YBUS_HIP_TOO_LARGE, "a column of the factor is too long for the device refactor (L below its diagonal: " +
                                  std::to_string(longest_l) + ", L and U: " + std::to_string(x_cap) + ")");
   std::vector<uint64_t> step_of(n);
@@ -1400,7 +1450,7 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
   plan.step.upload(step, stream);
   plan.a_slot.upload(a_slot, stream);
   plan.target.upload(target, stream);
-  plan.a_val.reserve(nnz_a);
+  plan.a_val.reserve((f->complex ? 2 : 1) * nnz_a);
   plan.control.reserve(4);
   check(hipStreamSynchronize(stream), "refactor plan");
   plan.x_cap = (int32_t)x_cap;
@@ -1415,96 +1465,115 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
   plan.numeric = true;
 }
 
+// The device refactor of a real or complex LU factor (values as (re, im) pairs when complex)
+static void refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64_t* a_ptr, const uint64_t* a_idx,
+                        const double* a_val, double pivot_threshold, int32_t* outcome, bool complex) {
+  require_lu(f, complex);
+  if (!a_ptr || !outcome) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+  if (a_ptr[n] && (!a_idx || !a_val)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+  if (!(pivot_threshold > 0.0 && pivot_threshold <= 1.0))
+    fail(YBUS_HIP_INVALID_ARGUMENT, "the pivot threshold must be in (0, 1]");
+  if ((int64_t)n != f->n) fail(YBUS_HIP_PATTERN_MISMATCH, "the matrix's shape differs from the factor's");
+  std::string why;
+  if (!valid_compressed((int64_t)n, (int64_t)n, a_ptr, a_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "A: " + why);
+  if (a_ptr[n] > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the matrix has too many entries");
+  check(hipSetDevice(f->ctx->device), "hipSetDevice");
+  RefactorPlan& plan = refactor_plan(f);
+  if (!plan.numeric) {
+    build_numeric(f, plan, a_ptr, a_idx);
+  } else if (!std::equal(plan.a_ptr.begin(), plan.a_ptr.end(), a_ptr) ||
+             !std::equal(plan.a_idx.begin(), plan.a_idx.end(), a_idx)) {
+    fail(YBUS_HIP_PATTERN_MISMATCH, "the matrix's pattern differs from the one this factor was refactored with");
+  }
+  hipStream_t stream = f->stream();
+  const uint64_t parts = complex ? 2 : 1;  // doubles per value
+  const uint64_t nnz_a = a_ptr[n];
+  f->staging.to_device(stream, plan.a_val.p, a_val, parts * nnz_a * sizeof(double));
+  const uint64_t nl = f->lu_host.l_ptr[n];
+  check(hipEventRecord(f->ev0, stream), "hipEventRecord");
+  check(launch_fill_u64(stream, reinterpret_cast<uint64_t*>(plan.next.p), REFACTOR_PENDING, (int64_t)(parts * nl)),
+        "refactor values");
+  check(launch_fill(stream, plan.control.p, 0, 4), "refactor control");
+  check(launch_fill(stream, plan.control.p + 1, (int32_t)f->n, 1), "refactor control");
+  RefactorDev d;
+  d.n = (int32_t)f->n;
+  d.head = plan.head.p;
+  d.span = plan.span.p;
+  d.step = plan.step.p;
+  d.a_slot = plan.a_slot.p;
+  d.a_val = plan.a_val.p;
+  d.target = plan.target.p;
+  d.l_val = plan.next.p;
+  d.u_val = plan.next.p + parts * nl;
+  d.control = plan.control.p;
+  d.tau = pivot_threshold;
+  d.x_cap = plan.x_cap;
+  d.spin_limit = refactor_spin_limit();
+  check(launch_refactor(stream, d, plan.grid, plan.stage, complex), "refactor");
+  check(hipEventRecord(f->ev1, stream), "hipEventRecord");
+  int32_t control[4] = {0, 0, 0, 0};
+  check(hipMemcpyAsync(control, plan.control.p, sizeof(control), hipMemcpyDeviceToHost, stream), "refactor control");
+  check(hipStreamSynchronize(stream), "refactor");
+  float ms = 0.0f;
+  check(hipEventElapsedTime(&ms, f->ev0, f->ev1), "hipEventElapsedTime");
+  plan.stats.calls += 1;
+  plan.stats.refactor_ms += ms;
+  if (control[2] != 0)
+    fail(YBUS_HIP_INTERNAL, "the device refactor gave up " + std::to_string(control[2]) +
+                                " column(s) waiting on another; the factor is unchanged");
+  if (control[0] < (int32_t)f->n) fail(YBUS_HIP_INTERNAL, "the device refactor left columns untaken");
+  if (control[1] < (int32_t)f->n) {
+    *outcome = YBUS_HIP_REFACTOR_NEEDS_PIVOTING;
+    return;
+  }
+  plan.stats.refresh_ms += refresh_sweeps(f, plan);
+  plan.stats.reused += 1;
+  // a matrix attached for refinement that is this one (CSC, the same pattern) takes its values
+  if (!complex && f->a.nnz == nnz_a && !f->a.src_row_major && f->a.src_ptr == plan.a_ptr &&
+      f->a.src_idx == plan.a_idx) {
+    check(launch_gather(stream, plan.a_val.p, f->a.order_dev.p, (int64_t)nnz_a, f->a.val.p), "matrix values");
+    check(launch_gather(stream, plan.a_val.p, f->at.t_order.p, (int64_t)nnz_a, f->at.val.p), "matrix values");
+    check(hipStreamSynchronize(stream), "matrix values");
+  }
+  *outcome = YBUS_HIP_REFACTOR_REUSED;
+}
+
 int32_t ybus_hip_factor_refactor_lu(ybus_hip_factor* f, uint64_t n, const uint64_t* a_ptr, const uint64_t* a_idx,
                                     const double* a_val, double pivot_threshold, int32_t* outcome) {
-  return guard([&] {
-    require_lu(f);
-    if (!a_ptr || !outcome) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
-    if (a_ptr[n] && (!a_idx || !a_val)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
-    if (!(pivot_threshold > 0.0 && pivot_threshold <= 1.0))
-      fail(YBUS_HIP_INVALID_ARGUMENT, "the pivot threshold must be in (0, 1]");
-    if ((int64_t)n != f->n) fail(YBUS_HIP_PATTERN_MISMATCH, "the matrix's shape differs from the factor's");
-    std::string why;
-    if (!valid_compressed((int64_t)n, (int64_t)n, a_ptr, a_idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "A: " + why);
-    if (a_ptr[n] > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the matrix has too many entries");
-    check(hipSetDevice(f->ctx->device), "hipSetDevice");
-    RefactorPlan& plan = refactor_plan(f);
-    if (!plan.numeric) {
-      build_numeric(f, plan, a_ptr, a_idx);
-    } else if (!std::equal(plan.a_ptr.begin(), plan.a_ptr.end(), a_ptr) ||
-               !std::equal(plan.a_idx.begin(), plan.a_idx.end(), a_idx)) {
-      fail(YBUS_HIP_PATTERN_MISMATCH, "the matrix's pattern differs from the one this factor was refactored with");
-    }
-    hipStream_t stream = f->stream();
-    const uint64_t nnz_a = a_ptr[n];
-    f->staging.to_device(stream, plan.a_val.p, a_val, nnz_a * sizeof(double));
-    const uint64_t nl = f->lu_host.l_ptr[n];
-    check(hipEventRecord(f->ev0, stream), "hipEventRecord");
-    check(launch_fill_u64(stream, reinterpret_cast<uint64_t*>(plan.next.p), REFACTOR_PENDING, (int64_t)nl),
-          "refactor values");
-    check(launch_fill(stream, plan.control.p, 0, 4), "refactor control");
-    check(launch_fill(stream, plan.control.p + 1, (int32_t)f->n, 1), "refactor control");
-    RefactorDev d;
-    d.n = (int32_t)f->n;
-    d.head = plan.head.p;
-    d.span = plan.span.p;
-    d.step = plan.step.p;
-    d.a_slot = plan.a_slot.p;
-    d.a_val = plan.a_val.p;
-    d.target = plan.target.p;
-    d.l_val = plan.next.p;
-    d.u_val = plan.next.p + nl;
-    d.control = plan.control.p;
-    d.tau = pivot_threshold;
-    d.x_cap = plan.x_cap;
-    d.spin_limit = refactor_spin_limit();
-    check(launch_refactor(stream, d, plan.grid, plan.stage), "refactor");
-    check(hipEventRecord(f->ev1, stream), "hipEventRecord");
-    int32_t control[4] = {0, 0, 0, 0};
-    check(hipMemcpyAsync(control, plan.control.p, sizeof(control), hipMemcpyDeviceToHost, stream), "refactor control");
-    check(hipStreamSynchronize(stream), "refactor");
-    float ms = 0.0f;
-    check(hipEventElapsedTime(&ms, f->ev0, f->ev1), "hipEventElapsedTime");
-    plan.stats.calls += 1;
-    plan.stats.refactor_ms += ms;
-    if (control[2] != 0)
-      fail(YBUS_HIP_INTERNAL, "the device refactor gave up " + std::to_string(control[2]) +
-                                  " column(s) waiting on another; the factor is unchanged");
-    if (control[0] < (int32_t)f->n) fail(YBUS_HIP_INTERNAL, "the device refactor left columns untaken");
-    if (control[1] < (int32_t)f->n) {
-      *outcome = YBUS_HIP_REFACTOR_NEEDS_PIVOTING;
-      return;
-    }
-    plan.stats.refresh_ms += refresh_sweeps(f, plan);
-    plan.stats.reused += 1;
-    // a matrix attached for refinement that is this one (CSC, the same pattern) takes its values
-    if (f->a.nnz == nnz_a && !f->a.src_row_major && f->a.src_ptr == plan.a_ptr && f->a.src_idx == plan.a_idx) {
-      check(launch_gather(stream, plan.a_val.p, f->a.order_dev.p, (int64_t)nnz_a, f->a.val.p), "matrix values");
-      check(launch_gather(stream, plan.a_val.p, f->at.t_order.p, (int64_t)nnz_a, f->at.val.p), "matrix values");
-      check(hipStreamSynchronize(stream), "matrix values");
-    }
-    *outcome = YBUS_HIP_REFACTOR_REUSED;
-  });
+  return guard([&] { refactor_lu(f, n, a_ptr, a_idx, a_val, pivot_threshold, outcome, false); });
+}
+
+int32_t ybus_hip_factor_refactor_lu_complex(ybus_hip_factor* f, uint64_t n, const uint64_t* a_ptr,
+                                            const uint64_t* a_idx, const double* a_val, double pivot_threshold,
+                                            int32_t* outcome) {
+  return guard([&] { refactor_lu(f, n, a_ptr, a_idx, a_val, pivot_threshold, outcome, true); });
+}
+
+static void update_lu_values(ybus_hip_factor* f, const double* l_val, const double* u_val, bool complex) {
+  require_lu(f, complex);
+  const uint64_t parts = complex ? 2 : 1;
+  const uint64_t nl = parts * f->lu_host.l_ptr[f->n], nu = parts * f->lu_host.u_ptr[f->n];
+  if ((nl && !l_val) || (nu && !u_val)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+  check(hipSetDevice(f->ctx->device), "hipSetDevice");
+  RefactorPlan& plan = refactor_plan(f);
+  hipStream_t stream = f->stream();
+  f->staging.to_device(stream, plan.next.p, l_val, nl * sizeof(double));
+  f->staging.to_device(stream, plan.next.p + nl, u_val, nu * sizeof(double));
+  plan.stats.refresh_ms += refresh_sweeps(f, plan);
+  plan.stats.updates += 1;
 }
 
 int32_t ybus_hip_factor_update_lu_values(ybus_hip_factor* f, const double* l_val, const double* u_val) {
-  return guard([&] {
-    require_lu(f);
-    const uint64_t nl = f->lu_host.l_ptr[f->n], nu = f->lu_host.u_ptr[f->n];
-    if ((nl && !l_val) || (nu && !u_val)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
-    check(hipSetDevice(f->ctx->device), "hipSetDevice");
-    RefactorPlan& plan = refactor_plan(f);
-    hipStream_t stream = f->stream();
-    f->staging.to_device(stream, plan.next.p, l_val, nl * sizeof(double));
-    f->staging.to_device(stream, plan.next.p + nl, u_val, nu * sizeof(double));
-    plan.stats.refresh_ms += refresh_sweeps(f, plan);
-    plan.stats.updates += 1;
-  });
+  return guard([&] { update_lu_values(f, l_val, u_val, false); });
+}
+
+int32_t ybus_hip_factor_update_lu_values_complex(ybus_hip_factor* f, const double* l_val, const double* u_val) {
+  return guard([&] { update_lu_values(f, l_val, u_val, true); });
 }
 
 int32_t ybus_hip_factor_refactor_stats(const ybus_hip_factor* f, ybus_hip_refactor_stats* out) {
   return guard([&] {
-    require_lu(f);
+    require_lu(f, f && f->complex);
     if (!out) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     *out = f->refactor ? f->refactor->stats : ybus_hip_refactor_stats{};
   });
@@ -1512,8 +1581,9 @@ int32_t ybus_hip_factor_refactor_stats(const ybus_hip_factor* f, ybus_hip_refact
 
 int32_t ybus_hip_factor_lu_values(ybus_hip_factor* f, double* l_val, double* u_val) {
   return guard([&] {
-    require_lu(f);
-    const uint64_t nl = f->lu_host.l_ptr[f->n], nu = f->lu_host.u_ptr[f->n];
+    require_lu(f, f && f->complex);
+    const uint64_t parts = f->complex ? 2 : 1;
+    const uint64_t nl = parts * f->lu_host.l_ptr[f->n], nu = parts * f->lu_host.u_ptr[f->n];
     std::vector<double> v;
     if (f->refactor) {
       check(hipSetDevice(f->ctx->device), "hipSetDevice");
@@ -1594,6 +1664,14 @@ int32_t ybus_hip_factor_upload_lu_complex(ybus_hip_ctx* ctx, uint64_t n, const u
     f->zbwd.upload(schedule(u, ident, cp), ctx->stream);
     f->ztfwd.upload(schedule(ut, cp, ident), ctx->stream);
     f->ztbwd.upload(schedule(lt, ident, rp), ctx->stream);
+    f->lu_host.l_ptr.assign(l_ptr, l_ptr + n + 1);
+    f->lu_host.l_idx.assign(l_idx, l_idx + l_ptr[n]);
+    f->lu_host.u_ptr.assign(u_ptr, u_ptr + n + 1);
+    f->lu_host.u_idx.assign(u_idx, u_idx + u_ptr[n]);
+    f->lu_host.row_perm.assign(row_perm, row_perm + n);
+    f->lu_host.col_perm.assign(col_perm, col_perm + n);
+    f->lu_host.values.assign(l_val, l_val + 2 * l_ptr[n]);
+    f->lu_host.values.insert(f->lu_host.values.end(), u_val, u_val + 2 * u_ptr[n]);
     check(hipStreamSynchronize(ctx->stream), "upload factor");
     finish_factor(f.get());
     *out = f.release();
