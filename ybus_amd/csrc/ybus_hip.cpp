@@ -1456,8 +1456,8 @@ static void build_numeric(ybus_hip_factor* f, RefactorPlan& plan, const uint64_t
   plan.x_cap = (int32_t)x_cap;
   plan.stage = stage;
   // two wavefronts per SIMD: the time is the column graph's chain of hand-overs, which 512 to
-  // 8,192 wavefronts all cover (1.06-1.17 ms at C2, fewer being slightly faster: fewer waiting
-  // wavefronts poll)
+  // 8,192 wavefronts all cover (1.06-1.17 ms on a 25,000-bus Jacobian; fewer are slightly faster,
+  // as fewer waiting wavefronts poll)
   plan.grid = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 8 * (int64_t)f->ctx->cus);
   plan.stats.multiply_subtracts = (uint64_t)target.size();
   plan.a_ptr.assign(a_ptr, a_ptr + n + 1);
