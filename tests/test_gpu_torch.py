@@ -89,3 +89,57 @@ def test_autograd_in_fp32_refinement_agrees_with_fp64(gpu):
         grads.append((x.detach(), b.grad, a_values.grad))
     for exact, refined in zip(grads[0], grads[1]):
         assert torch.max(torch.abs(refined - exact)) <= 1e-8 * torch.max(torch.abs(exact))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_autograd_recomputing_x_gives_the_stored_x_gradients(gpu, mode):
+    """The backward that solves for x again gives the bits of the one that kept x, in FP64 (and
+    then the oracle's adjoint) and in FP32 refinement alike."""
+    from ybus_amd.autograd import solve
+
+    g = O.make_grid(900, 53)
+    bp, _, _ = g.dc()
+    chol = O.Cholesky(bp)
+    factor = Y.GpuFactor.from_cholesky(gpu, chol)
+    factor.set_matrix(bp)
+    p = O.injections(53, 2, 900, 11)
+    w = cuda(right_hand_sides(bp.rows, 11, 6))
+    grads = []
+    for recompute in [False, True]:
+        a_values = cuda(bp.data).requires_grad_(True)
+        b = cuda(p).requires_grad_(True)
+        x = solve(factor, a_values, b, mode, recompute=recompute)
+        (x * w).sum().backward()
+        grads.append((x.detach().cpu().numpy(), b.grad.cpu().numpy(), a_values.grad.cpu().numpy()))
+    for kept, again in zip(grads[0], grads[1]):
+        assert again.tobytes() == kept.tobytes()
+    if mode == 0:
+        want_b, want_a = O.adjoint(chol, bp, w.cpu().numpy(), chol.solve_many(p))
+        assert grads[1][1].tobytes() == want_b.tobytes()
+        assert grads[1][2].tobytes() == want_a.tobytes()
+
+
+def test_autograd_recomputes_from_the_batch_size_up(gpu, monkeypatch):
+    """Below the threshold the graph keeps x, so b may change afterwards; from it up the graph
+    keeps b, and changing b in place before the backward is refused by autograd's version check
+    rather than giving a gradient of the wrong x."""
+    import ybus_amd.autograd as AG
+
+    a = random_spd(120, 4, 8)
+    factor = Y.GpuFactor.from_cholesky(gpu, O.Cholesky(a))
+    factor.set_matrix(a)
+    rhs = right_hand_sides(120, 6, 2)
+    monkeypatch.setattr(AG, "RECOMPUTE_VALUES", 6 * 120 + 1)
+    b = cuda(rhs).requires_grad_(True)
+    a_values = cuda(a.data).requires_grad_(True)
+    x = AG.solve(factor, a_values, b)
+    with torch.no_grad():
+        b.add_(1.0)
+    x.sum().backward()  # x was kept
+    monkeypatch.setattr(AG, "RECOMPUTE_VALUES", 6 * 120)
+    b = cuda(rhs).requires_grad_(True)
+    x = AG.solve(factor, a_values, b)
+    with torch.no_grad():
+        b.add_(1.0)
+    with pytest.raises(RuntimeError, match="modified by an inplace operation"):
+        x.sum().backward()

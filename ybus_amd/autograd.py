@@ -8,6 +8,12 @@ b_bar x^T) on A's stored pattern, both computed on the GPU without leaving devic
 solve does not read it (the factor was made from those values on the host), it only carries A's
 gradient. Results are bit for bit the CPU's transposed solve and outer-product sums.
 
+The backward needs x. For a large batch the graph keeps b (which the caller holds anyway) and
+the backward solves for x again instead of keeping a second (count, dim) tensor alive between
+the passes: ``recompute=None`` does so from RECOMPUTE_VALUES values of x up (128 MiB of FP64),
+True and False force either way. Every sum in the device solves runs in a fixed order, so the
+second solve gives the forward's bits and the gradients are the same either way.
+
 PyTorch ships its own HIP runtime: import torch before the first call that loads
 libybus_hip.so (creating a ybus_amd.Gpu does), so that the process has one HIP runtime.
 """
@@ -17,28 +23,40 @@ import torch
 
 
 FP64, FP32_REFINED = 0, 1
+RECOMPUTE_VALUES = 1 << 24
 
 
 class _Solve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, factor, a_values, b, mode, tolerance, max_steps):
+    def forward(ctx, factor, a_values, b, mode, tolerance, max_steps, recompute):
         x = factor.solve_device(b.contiguous(), mode=mode, tolerance=tolerance, max_steps=max_steps)
+        if recompute is None:
+            recompute = x.numel() >= RECOMPUTE_VALUES
         ctx.factor = factor
         ctx.settings = (mode, tolerance, max_steps)
-        ctx.save_for_backward(x)
+        ctx.recompute = recompute
+        # b is the caller's own tensor (an in-place change to it after this call makes the
+        # backward raise, by autograd's version check); x would be one more tensor of its size
+        ctx.save_for_backward(b if recompute else x)
         return x
 
     @staticmethod
     def backward(ctx, x_bar):
-        (x,) = ctx.saved_tensors
+        (kept,) = ctx.saved_tensors
         mode, tolerance, max_steps = ctx.settings
+        if ctx.recompute:
+            x = ctx.factor.solve_device(kept.contiguous(), mode=mode, tolerance=tolerance,
+                                        max_steps=max_steps)
+        else:
+            x = kept
         b_bar, a_bar = ctx.factor.adjoint_device(x_bar, x, mode, tolerance, max_steps)
-        return None, a_bar, b_bar, None, None, None
+        return None, a_bar, b_bar, None, None, None, None
 
 
 def solve(factor, a_values: torch.Tensor, b: torch.Tensor, mode: int = FP64, tolerance: float = 1e-12,
-          max_steps: int = 10) -> torch.Tensor:
+          max_steps: int = 10, recompute: bool | None = None) -> torch.Tensor:
     """x = A^-1 b row by row, differentiable in A's stored values and in b (see the module). With
     mode=FP32_REFINED both the solve and the backward's transposed solve are FP32 sweeps refined in
-    FP64 to `tolerance` (refine_normwise's rule); FP64 gives the CPU's bits."""
-    return _Solve.apply(factor, a_values, b, mode, tolerance, max_steps)
+    FP64 to `tolerance` (refine_normwise's rule); FP64 gives the CPU's bits. `recompute` chooses
+    whether the backward solves for x again rather than keeping it (None: for large batches)."""
+    return _Solve.apply(factor, a_values, b, mode, tolerance, max_steps, recompute)
