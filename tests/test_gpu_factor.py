@@ -204,3 +204,33 @@ def test_transfers_larger_than_the_staging_chunks(gpu):
         dense[k, k], dense[k, k + 1] = 1.0, -1.0
     with factor.pair_solves(pairs) as block:
         assert block.solutions().tobytes() == chol.solve_many(dense, 8).tobytes()
+
+
+def test_handles_left_open_at_exit_end_the_process_quietly():
+    """A process that ends with a context, factor, block, operator and batch still open exits
+    with status 0 and prints nothing: the handles are left to the process's exit rather than freed
+    from a half-torn-down interpreter."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = (
+        "import torch, oracle as O, ybus_amd as Y\n"
+        "from tests.fixtures import random_spd\n"
+        "gpu = Y.Gpu.new()\n"
+        "a = random_spd(80, 3, 5)\n"
+        "factor = Y.GpuFactor.from_cholesky(gpu, O.Cholesky(a))\n"
+        "block = factor.pair_solves([Y.Pair(1, 2), Y.Pair(3, None)])\n"
+        "operator = factor.operator(a)\n"
+        "batch = Y.GpuBatch.new(gpu)\n"
+        "batch.add(factor)\n"
+        "print('open', flush=True)\n"
+    )
+    # the library's own defaults (a path forced by this module's fixture would make the batch refuse)
+    env = {k: v for k, v in os.environ.items() if not k.startswith("YBUS_HIP_")}
+    out = subprocess.run([sys.executable, "-c", script], cwd=root, env=env, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "open"
+    assert "Exception ignored" not in out.stderr and "Traceback" not in out.stderr, out.stderr

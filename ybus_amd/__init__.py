@@ -10,7 +10,9 @@ loudly.
 from __future__ import annotations
 
 import ctypes as C
+import mmap
 import os
+import sys
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -310,10 +312,34 @@ class Gpu:
         _check(library().ybus_hip_ctx_memory(self._h, C.byref(free), C.byref(total)))
         return int(free.value), int(total.value)
 
-    def __del__(self):
-        if getattr(self, "_h", None):
+    # Handles still open when the interpreter ends are left to the process's exit, which
+    # releases the device memory: the module's globals are gone by then, and the objects' order
+    # of collection is arbitrary (a factor could outlive its context).
+    def __del__(self, _ending=sys.is_finalizing):
+        if getattr(self, "_h", None) and not _ending():
             library().ybus_hip_ctx_free(self._h)
             self._h = None
+
+
+# A result of this size or more is copied into transparent huge pages where the kernel offers them
+HUGE_RESULT_BYTES = 64 << 20
+
+
+def _result_array(shape, dtype=np.float64) -> np.ndarray:
+    """A zeroed array for a result the library copies in from the device. From HUGE_RESULT_BYTES
+    up it is an anonymous mapping advised to use transparent huge pages: the copy's first touch
+    of each page then faults in 2 MiB rather than 4 KiB, which over a gigabyte of solutions is
+    several milliseconds (the mapping is released with the array)."""
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    if nbytes < HUGE_RESULT_BYTES or not hasattr(mmap, "MADV_HUGEPAGE"):
+        return np.zeros(shape, dtype)
+    m = mmap.mmap(-1, nbytes, flags=mmap.MAP_PRIVATE | mmap.MAP_ANONYMOUS)
+    try:
+        m.madvise(mmap.MADV_HUGEPAGE)
+    except OSError:
+        pass  # no transparent huge pages here: small pages, zeroed all the same
+    return np.frombuffer(m, dtype=dtype).reshape(shape)
 
 
 def _rhs_array(rhs, n: int) -> np.ndarray:
@@ -436,8 +462,8 @@ class GpuFactor:
     def is_complex(self) -> bool:
         return bool(library().ybus_hip_factor_is_complex(self._h))
 
-    def __del__(self):
-        if getattr(self, "_h", None):
+    def __del__(self, _ending=sys.is_finalizing):
+        if getattr(self, "_h", None) and not _ending():
             if self._block is not None:
                 self._block.free()
             library().ybus_hip_factor_free(self._h)
@@ -549,7 +575,7 @@ class GpuFactor:
 
     def _solve_complex(self, rhs, transpose: bool) -> np.ndarray:
         b = _complex_rhs(rhs, self._n)
-        x = np.zeros_like(b)
+        x = _result_array(b.shape, b.dtype)
         if b.shape[0]:
             fn = (library().ybus_hip_factor_solve_transpose_many_complex if transpose
                   else library().ybus_hip_factor_solve_many_complex)
@@ -563,7 +589,7 @@ class GpuFactor:
             return self._solve_complex(rhs, False)
         b = _rhs_array(rhs, self._n)
         self._release_block()
-        x = np.zeros_like(b)
+        x = _result_array(b.shape)
         if b.shape[0]:
             _check(library().ybus_hip_factor_solve_many(self._h, b, b.shape[0], x))
         return x
@@ -575,7 +601,7 @@ class GpuFactor:
             return self._solve_complex(rhs, True)
         b = _rhs_array(rhs, self._n)
         self._release_block()
-        x = np.zeros_like(b)
+        x = _result_array(b.shape)
         if b.shape[0]:
             _check(library().ybus_hip_factor_solve_transpose_many(self._h, b, b.shape[0], x))
         return x
@@ -585,7 +611,7 @@ class GpuFactor:
         accuracy.rs:83-122): (solutions, steps per column, relative residual per column)."""
         b = _rhs_array(rhs, self._n)
         self._release_block()
-        x = np.zeros_like(b)
+        x = _result_array(b.shape)
         steps = np.zeros(b.shape[0], np.uint64)
         ratio = np.zeros(b.shape[0])
         if b.shape[0]:
@@ -780,8 +806,9 @@ class GpuBlock:
             if self._factor._block is self:
                 self._factor._block = None
 
-    def __del__(self):
-        self.free()
+    def __del__(self, _ending=sys.is_finalizing):
+        if not _ending():
+            self.free()
 
     def __enter__(self):
         return self
@@ -809,7 +836,7 @@ class GpuBlock:
         return steps.astype(np.int64), ratio
 
     def solutions(self) -> np.ndarray:
-        out = np.zeros((self.columns(), self._factor.dim()))
+        out = _result_array((self.columns(), self._factor.dim()))
         _check(library().ybus_hip_block_solutions(self._live(), out))
         return out
 
@@ -848,7 +875,7 @@ class GpuBlock:
         z = np.array([-1 if r is None else int(r) for r in zero], dtype=np.int64)
         if len(z) != self.columns():
             raise InvalidArgument(INVALID_ARGUMENT, "one entry to zero (or none) per column")
-        out = np.empty((self.columns(), operator.rows()))
+        out = _result_array((self.columns(), operator.rows()))
         _check(library().ybus_hip_block_product(self._live(), operator._h, z, out))
         return out
 
@@ -870,8 +897,8 @@ class GpuOperator:
         self._h = handle
         self._rows = rows
 
-    def __del__(self):
-        if getattr(self, "_h", None):
+    def __del__(self, _ending=sys.is_finalizing):
+        if getattr(self, "_h", None) and not _ending():
             library().ybus_hip_operator_free(self._h)
             self._h = None
 
@@ -910,8 +937,8 @@ class GpuBatch:
         _check(library().ybus_hip_batch_create(gpu._h, C.byref(h)))
         return GpuBatch(gpu, h)
 
-    def __del__(self):
-        if getattr(self, "_h", None):
+    def __del__(self, _ending=sys.is_finalizing):
+        if getattr(self, "_h", None) and not _ending():
             library().ybus_hip_batch_free(self._h)
             self._h = None
 
@@ -956,7 +983,7 @@ class GpuBatch:
         if self._staged is None:
             raise InvalidArgument(INVALID_ARGUMENT, "nothing staged")
         total = sum(c * n for c, n in self._staged)
-        flat = np.zeros(total)
+        flat = _result_array((total,))
         _check(library().ybus_hip_batch_solutions(self._h, flat.ctypes.data if total else None))
         out, at = [], 0
         for c, n in self._staged:
