@@ -823,6 +823,28 @@ def run_refactor(gpu, args):
     return out
 
 
+def copy_peak_gbs(values: int = 1 << 28, repeats: int = 10) -> float:
+    """The HBM bandwidth a plain device-to-device copy reaches on this GPU (2 GiB of FP64 read
+    and written per copy, timed with events on PyTorch's stream), the practical ceiling beside the
+    published 8 TB/s."""
+    import torch
+
+    src = torch.ones(values, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(repeats):
+        dst.copy_(src)
+    end.record()
+    end.synchronize()
+    ms = start.elapsed_time(end) / repeats
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * values * 8 / (ms / 1e3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -849,10 +871,9 @@ def main():
         copy_calibration(args.copy_calibration)
         return
 
-    if not args.no_adjoint:
-        # PyTorch ships its own HIP runtime: it must be loaded before libybus_hip.so, so that both
-        # use that one runtime (loaded the other way round, PyTorch finds no GPU)
-        import torch  # noqa: F401
+    # PyTorch ships its own HIP runtime: it must be loaded before libybus_hip.so, so that both use
+    # that one runtime (loaded the other way round, PyTorch finds no GPU)
+    import torch  # noqa: F401
 
     import ybus_amd as Y
 
@@ -869,6 +890,8 @@ def main():
     bytes_per_launch = stats.bytes / stats.launches
     achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
     traffic, traffic_source = profiled_traffic(ptdf["kernel"])
+    copy_gbs = copy_peak_gbs()
+    log(f"[bench] device copy of 2 GiB: {copy_gbs:.0f} GB/s (read + written)")
 
     # parity: sampled columns against the CPU (bit for bit)
     sample = np.arange(0, rhs_count, max(1, rhs_count // 256))
@@ -913,7 +936,8 @@ def main():
                    "nnz_l": nnz_l, "rhs": rhs_count, "parallelism": "single GPU", "tuning": ptdf["tuning"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
-                     "kernel": ptdf["kernel"], "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch},
+                     "kernel": ptdf["kernel"], "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch,
+                     "copy_peak": copy_gbs, "frac_of_copy_peak": achieved / copy_gbs},
         "cpu_baseline": cpu_baseline,
         "end_to_end_ms": ptdf["end_to_end"] * 1e3,
         "parity": {"sampled_columns": int(len(sample)), "bit_exact_columns": bit_exact,
