@@ -15,6 +15,8 @@
 // solves give ybus_sparse's bits exactly.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace ybus_hip {
@@ -172,13 +174,21 @@ struct Gathered {
 };
 
 // `live(d)` tells whether row d of dep_buf holds a value; a dead row reads as 0 (its load goes to
-// the pad row, which every tile keeps in cache).
-template <typename T, int TW, class Rhs, class Live>
+// the pad row, which every tile keeps in cache). With Skip (std::true_type) a row issues only the
+// loads of the terms it has: the pad row's loads hit in cache but still take the vector memory
+// path's bandwidth, which a sweep of short rows (about two terms each in a grid's Cholesky factor)
+// is bound by; a sweep of long rows (an LU's) is faster issuing all four without the branches.
+template <typename T, int TW, class Rhs, class Live, class Skip = std::false_type>
 __device__ __forceinline__ void gather(const Record<T>& r, const T* __restrict__ dep_buf, int64_t ld, int32_t pad,
-                                       const Tile<TW>& g, Rhs& rhs, Live& live, Gathered<T>& o) {
+                                       const Tile<TW>& g, Rhs& rhs, Live& live, Gathered<T>& o, Skip = Skip()) {
   o.rhs = rhs(r.src);
+  const int32_t len = r.p1 - r.p0;
 #pragma unroll
   for (int i = 0; i < RECORD_ENTRIES; ++i) {
+    if (Skip::value && len <= i) {
+      o.x[i] = (T)0;
+      continue;
+    }
     const bool on = live(r.dep[i]);
     const T v = dep_buf[(on ? r.dep[i] : pad) * ld + g.col];
     o.x[i] = on ? v : (T)0;
@@ -224,16 +234,17 @@ __device__ __forceinline__ T finish(const SweepDev& s, const T* __restrict__ val
 // for the gathers. A wait for a load waits for every memory operation issued before it, stores
 // included, so storing after the next gathers keeps the store's latency off the wait; the
 // records alternate between two registers sets, so no register copy waits on loads in flight.
-template <typename T, int TW, class Rhs, class Store, class At, class Live>
+template <typename T, int TW, class Rhs, class Store, class At, class Live, class Skip>
 __device__ __forceinline__ void wide_level(const SweepDev& s, const T* __restrict__ val, int32_t begin, int32_t end,
                                            const T* __restrict__ dep_buf, int64_t ld, const Tile<TW>& g,
-                                           const Record<T>& first, Rhs& rhs, Store& store, At& at, Live& live) {
+                                           const Record<T>& first, Rhs& rhs, Store& store, At& at, Live& live,
+                                           Skip skip) {
   const int32_t pad = s.n;
   int32_t k = begin + g.slot;
   if (k >= end) return;
   const int32_t step = g.slots;
   Gathered<T> o;
-  gather<T, TW>(first, dep_buf, ld, pad, g, rhs, live, o);
+  gather<T, TW>(first, dep_buf, ld, pad, g, rhs, live, o, skip);
   if (k + step >= end) {
     store(first.dst, finish<T, TW>(s, val, first, o, dep_buf, ld, pad, g, live));
     return;
@@ -244,14 +255,14 @@ __device__ __forceinline__ void wide_level(const SweepDev& s, const T* __restric
   int32_t dst = first.dst;
   for (k += step;; k += 2 * step) {
     // b holds row k
-    gather<T, TW>(b, dep_buf, ld, pad, g, rhs, live, o);
+    gather<T, TW>(b, dep_buf, ld, pad, g, rhs, live, o, skip);
     store(dst, v);
     fetch_clamped<T>(s, k + step, end, c, at);
     v = finish<T, TW>(s, val, b, o, dep_buf, ld, pad, g, live);
     dst = b.dst;
     if (k + step >= end) break;
     // c holds row k + step
-    gather<T, TW>(c, dep_buf, ld, pad, g, rhs, live, o);
+    gather<T, TW>(c, dep_buf, ld, pad, g, rhs, live, o, skip);
     store(dst, v);
     await_record<T>(b);
     fetch_clamped<T>(s, k + 2 * step, end, b, at);
@@ -269,9 +280,11 @@ __device__ __forceinline__ void wide_level(const SweepDev& s, const T* __restric
 // is fetched two levels ahead (two records alternate, so no register copies wait on loads in
 // flight). A barrier ends each level: it orders the global stores of a workgroup's waves, which
 // share one CU's L1. Every thread takes every barrier; `act` only masks the rows.
-template <typename T, int TW, class Rhs, class Store, class At = Identity, class Live = AllLive>
+template <typename T, int TW, class Rhs, class Store, class At = Identity, class Live = AllLive,
+          class Skip = std::false_type>
 __device__ void wide_sweep(const SweepDev& s, const int32_t* lp, const T* __restrict__ dep_buf, int64_t ld,
-                           const Tile<TW>& g, bool act, Rhs rhs, Store store, At at = At(), Live live = Live()) {
+                           const Tile<TW>& g, bool act, Rhs rhs, Store store, At at = At(), Live live = Live(),
+                           Skip skip = Skip()) {
   const T* __restrict__ val = values_of<T>(s);
   const int32_t levels = s.levels;
   if (levels == 0 || lp[levels] == lp[0]) return;
@@ -279,12 +292,12 @@ __device__ void wide_sweep(const SweepDev& s, const int32_t* lp, const T* __rest
   fetch_first<T>(s, lp, 0, g.slot, a, at);
   fetch_first<T>(s, lp, 1, g.slot, b, at);
   for (int32_t l = 0; l < levels; l += 2) {
-    if (act) wide_level<T, TW>(s, val, lp[l], lp[l + 1], dep_buf, ld, g, a, rhs, store, at, live);
+    if (act) wide_level<T, TW>(s, val, lp[l], lp[l + 1], dep_buf, ld, g, a, rhs, store, at, live, skip);
     __syncthreads();
     await_record<T>(a);
     fetch_first<T>(s, lp, l + 2, g.slot, a, at);
     if (l + 1 >= levels) break;
-    if (act) wide_level<T, TW>(s, val, lp[l + 1], lp[l + 2], dep_buf, ld, g, b, rhs, store, at, live);
+    if (act) wide_level<T, TW>(s, val, lp[l + 1], lp[l + 2], dep_buf, ld, g, b, rhs, store, at, live, skip);
     __syncthreads();
     await_record<T>(b);
     fetch_first<T>(s, lp, l + 3, g.slot, b, at);
@@ -351,14 +364,15 @@ __global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, Sweep
     return (src == plus ? 1.0 : 0.0) - (src == minus ? 1.0 : 0.0);
   };
   wide_sweep<double, TW>(fwd, lpf, t, ld, g, g.on, from_pair, [&](int32_t dst, double v) { t[dst * ld + col] = v; },
-                         at, in_reach);
+                         at, in_reach, std::true_type());
   const int32_t pad = fwd.n;
   auto from_t = [&](int32_t src) -> double {
     const bool on = in_reach(src);
     const double v = t[(on ? src : pad) * ld + col];
     return on ? v : 0.0;
   };
-  wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on, from_t, [&](int32_t dst, double v) { x[dst * ld + col] = v; });
+  wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on, from_t, [&](int32_t dst, double v) { x[dst * ld + col] = v; },
+                         Identity(), AllLive(), std::true_type());
 }
 
 // ---- complex FP64 solves ----
