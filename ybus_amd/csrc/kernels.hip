@@ -479,6 +479,9 @@ __global__ void __launch_bounds__(1024) solve_complex_kernel(ComplexSweepDev fwd
 }
 
 // ---- partitioned FP64 solves: a sweep's independent parts side by side ----
+//
+// Only Cholesky factors are partitioned, and their rows are short: every sweep here issues only
+// each row's own loads (gather's Skip).
 
 // Stages part q's level pointers (from lp_all: the sweep's, or a tile's own list over the same
 // levels) into LDS at lp and returns the sweep restricted to that part's levels. Every thread
@@ -545,9 +548,9 @@ __global__ void __launch_bounds__(1024) parts_forward_kernel(SweepDev fwd, RhsDe
   auto store = [&](int32_t dst, double v) { t[dst * ld + g.col] = v; };
   if (pairs_reach) {
     stage_live(reach, (int32_t)blockIdx.x, live);
-    wide_sweep<double, TW>(s, lp, t, ld, g, g.on, from, store, ReachAt{reach.pos}, LiveBits{live});
+    wide_sweep<double, TW>(s, lp, t, ld, g, g.on, from, store, ReachAt{reach.pos}, LiveBits{live}, std::true_type());
   } else {
-    wide_sweep<double, TW>(s, lp, t, ld, g, g.on, from, store);
+    wide_sweep<double, TW>(s, lp, t, ld, g, g.on, from, store, Identity(), AllLive(), std::true_type());
   }
 }
 
@@ -572,17 +575,18 @@ __global__ void __launch_bounds__(1024) top_kernel(SweepDev fwd, SweepDev bwd, R
   if (pairs_reach) {
     stage_live(reach, (int32_t)blockIdx.x, live);
     const LiveBits in_reach{live};
-    wide_sweep<double, TW>(sf, lpf, t, ld, g, g.on, from, store_t, ReachAt{reach.pos}, in_reach);
+    wide_sweep<double, TW>(sf, lpf, t, ld, g, g.on, from, store_t, ReachAt{reach.pos}, in_reach, std::true_type());
     const int32_t pad = fwd.n;
     auto from_t = [&](int32_t src) -> double {
       const bool on = in_reach(src);
       const double v = t[(on ? src : pad) * ld + col];
       return on ? v : 0.0;
     };
-    wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, from_t, store_x);
+    wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, from_t, store_x, Identity(), AllLive(), std::true_type());
   } else {
-    wide_sweep<double, TW>(sf, lpf, t, ld, g, g.on, from, store_t);
-    wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; }, store_x);
+    wide_sweep<double, TW>(sf, lpf, t, ld, g, g.on, from, store_t, Identity(), AllLive(), std::true_type());
+    wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; }, store_x, Identity(),
+                           AllLive(), std::true_type());
   }
 }
 
@@ -607,9 +611,10 @@ __global__ void __launch_bounds__(1024) parts_backward_kernel(SweepDev bwd, Reac
       const double v = t[(on ? src : pad) * ld + col];
       return on ? v : 0.0;
     };
-    wide_sweep<double, TW>(s, lp, x, ld, g, g.on, from_t, store_x);
+    wide_sweep<double, TW>(s, lp, x, ld, g, g.on, from_t, store_x, Identity(), AllLive(), std::true_type());
   } else {
-    wide_sweep<double, TW>(s, lp, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; }, store_x);
+    wide_sweep<double, TW>(s, lp, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; }, store_x, Identity(),
+                           AllLive(), std::true_type());
   }
 }
 
