@@ -434,8 +434,11 @@ __device__ void complex_sweep(const ComplexSweepDev& s, const int32_t* lp, const
             d[q] = dep[at];
             v[q] = val[at];
           }
+          // only the row's own terms are gathered: a reload past its end would hit in cache but
+          // still take the vector memory path's bandwidth
 #pragma unroll
-          for (int q = 0; q < COMPLEX_CHUNK; ++q) x[q] = zload(dep_buf, d[q], ld, g.col);
+          for (int q = 0; q < COMPLEX_CHUNK; ++q)
+            x[q] = p + q < h.w ? zload(dep_buf, d[q], ld, g.col) : make_double2(0.0, 0.0);
 #pragma unroll
           for (int q = 0; q < COMPLEX_CHUNK; ++q)
             if (p + q < h.w) acc = zsub(acc, zmul(v[q], x[q]));
