@@ -281,6 +281,19 @@ int32_t ybus_hip_factor_adjoint(ybus_hip_factor *f, int32_t mode, double toleran
 int32_t ybus_hip_factor_solve_device(ybus_hip_factor *f, int32_t mode, double tolerance,
                                      uint64_t max_steps, const double *b, uint64_t ld_b,
                                      uint64_t count, double *x, uint64_t ld_x);
+/* ybus_hip_factor_solve_device in YBUS_HIP_FP64 or YBUS_HIP_TRANSPOSE that allocates no device
+ * memory, on this or any later call. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its workspace is a fixed device array of YBUS_HIP_SCRATCH_BYTES (64 MiB), which every device the
+ * library runs on holds until the process ends; the vectors are solved in chunks of as many of
+ * them as that array holds (at most block_width()), with the bits of ybus_hip_factor_solve_device.
+ * A factor whose workspace for a single vector exceeds the array is YBUS_HIP_TOO_LARGE. Every call
+ * on one device shares that device's array, so calls that overlap on one GPU (from several threads
+ * or contexts) race; the array keeps the last call's contents, on which no later call's results
+ * depend. The factor's own workspace and staged block are left as they are. */
+#define YBUS_HIP_SCRATCH_BYTES (64u << 20)
+int32_t ybus_hip_factor_solve_device_noalloc(ybus_hip_factor *f, int32_t mode, const double *b, uint64_t ld_b,
+                                             uint64_t count, double *x, uint64_t ld_x);
 /* ybus_hip_factor_adjoint on device vectors (x_bar, x and b_bar share the stride ld). */
 int32_t ybus_hip_factor_adjoint_device(ybus_hip_factor *f, int32_t mode, double tolerance,
                                        uint64_t max_steps, const double *x_bar, const double *x,

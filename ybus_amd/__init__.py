@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libybus_hip.so")
 
 OK, NO_DEVICE, INVALID_ARGUMENT, SHAPE_MISMATCH, TOO_LARGE, DEVICE_ERROR, OUT_OF_MEMORY, INTERNAL, PATTERN_MISMATCH = range(9)
 FP64, FP32_REFINED, TRANSPOSE = 0, 1, 2
+SCRATCH_BYTES = 64 << 20  # YBUS_HIP_SCRATCH_BYTES: the no-allocation workspace per device
 
 # ---- errors: CudaError (ybus_cuda/src/lib.rs:88-124) ----
 
@@ -142,6 +143,7 @@ def _declare(L):
         "ybus_hip_factor_solve_many_refined": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
         "ybus_hip_factor_adjoint": (i32, [P, i32, C.c_double, U64, f64p, f64p, U64, f64p, f64p]),
         "ybus_hip_factor_solve_device": (i32, [P, i32, C.c_double, U64, C.c_void_p, U64, U64, C.c_void_p, U64]),
+        "ybus_hip_factor_solve_device_noalloc": (i32, [P, i32, C.c_void_p, U64, U64, C.c_void_p, U64]),
         "ybus_hip_factor_adjoint_device": (i32, [P, i32, C.c_double, U64, C.c_void_p, C.c_void_p, U64, U64,
                                                  C.c_void_p, C.c_void_p]),
         "ybus_hip_factor_autotune": (i32, [P, U64, i32, C.c_void_p, C.c_void_p, C.POINTER(_Tuning)]),
@@ -667,6 +669,20 @@ class GpuFactor:
             mode = TRANSPOSE if transpose else FP64
         _check(library().ybus_hip_factor_solve_device(self._h, int(mode), float(tolerance), int(max_steps), pb,
                                                         ldb, b.shape[0], px, ldx))
+        return x
+
+    def solve_device_noalloc(self, b, x, transpose: bool = False):
+        """solve_device (FP64, or the transposed solve) into the given GPU tensor x, allocating no
+        device memory: the library's workspace is a fixed array of SCRATCH_BYTES per device,
+        shared by every such call on it (calls that overlap on one GPU race)."""
+        import torch
+
+        pb, ldb = self._device_vectors(b)
+        px, ldx = self._device_vectors(x, b.shape[0])
+        self._release_block()
+        torch.cuda.current_stream().synchronize()
+        _check(library().ybus_hip_factor_solve_device_noalloc(self._h, TRANSPOSE if transpose else FP64, pb, ldb,
+                                                                b.shape[0], px, ldx))
         return x
 
     def adjoint_device(self, x_bar, x, mode: int = FP64, tolerance: float = 1e-12, max_steps: int = 10):

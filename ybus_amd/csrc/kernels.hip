@@ -15,6 +15,8 @@
 // solves give ybus_sparse's bits exactly.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -2109,6 +2111,26 @@ hipError_t launch_refresh(hipStream_t stream, const RefreshDev& r, const double*
   if (r.n > 0)
     hipLaunchKernelGGL(refresh_rows_kernel, dim3((unsigned)((r.n + 255) / 256)), dim3(256), 0, stream, r, src);
   return hipGetLastError();
+}
+
+// The no-allocation workspace: one array per device, its address looked up once per device
+static __device__ __align__(256) double scratch[SCRATCH_BYTES / sizeof(double)];
+
+hipError_t scratch_address(int device, void** out) {
+  static std::mutex lock;
+  static std::map<int, void*> address;
+  std::lock_guard<std::mutex> hold(lock);
+  auto it = address.find(device);
+  if (it != address.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  address[device] = p;
+  *out = p;
+  return hipSuccess;
 }
 
 }  // namespace ybus_hip

@@ -316,4 +316,13 @@ hipError_t launch_refresh(hipStream_t stream, const RefreshDev& r, const double*
 hipError_t launch_refresh_complex(hipStream_t stream, int32_t n, int64_t entries, const int32_t* vpos,
                                   const int32_t* dpos, const double2* src, double2* val, double2* diag);
 
+// The workspace of the entry point that allocates nothing (ybus_hip_factor_solve_device_noalloc):
+// a file-scope device array of SCRATCH_BYTES in kernels.hip, which every device the library runs
+// on holds until the process ends. scratch_address gives its address on `device`, which must be
+// the current device; the first call per device looks it up (hipGetSymbolAddress), later calls
+// return the address kept for that device. Host code passes that address, never the array's
+// name, to kernels and copies.
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+hipError_t scratch_address(int device, void** out);
+
 }  // namespace ybus_hip

@@ -17,6 +17,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <optional>
 #include <thread>
 #include <vector>
 
@@ -60,23 +61,26 @@ int32_t guard(F f) {
   }
 }
 
-// Device memory that frees itself
+// Device memory that frees itself, or (lent) a view of memory it does not own
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t len = 0;
+  bool lent = false;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !lent) (void)hipFree(p);
     p = nullptr;
     len = 0;
+    lent = false;
   }
-  // At least `n` elements (contents not kept when it grows)
+  // At least `n` elements (contents not kept when it grows); a lent view never grows
   void reserve(size_t n) {
     if (n <= len && p) return;
+    if (lent) fail(YBUS_HIP_INTERNAL, "a lent workspace buffer was asked to grow");
     release();
     check(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
     len = n;
@@ -404,6 +408,8 @@ struct ybus_hip_ctx {
   hipStream_t stream = nullptr;
   std::string name;
   int cus = 256;
+  // this device's no-allocation workspace (SCRATCH_BYTES), from scratch_address at creation
+  void* scratch = nullptr;
   ~ybus_hip_ctx() {
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -928,6 +934,7 @@ int32_t ybus_hip_ctx_create(ybus_hip_ctx** out) {
     if (ctx->name.empty()) ctx->name = prop.gcnArchName;
     ctx->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(scratch_address(ctx->device, &ctx->scratch), "hipGetSymbolAddress (no-allocation workspace)");
     *out = ctx.release();
   });
 }
@@ -2053,6 +2060,88 @@ int32_t ybus_hip_factor_solve_device(ybus_hip_factor* f, int32_t mode, double to
         f->refine(chunk, tolerance, (int32_t)max_steps);
       else
         f->solve64(mode == YBUS_HIP_TRANSPOSE, false, chunk);
+      copy_out_device(f, f->x.p, chunk, x + start * ld_x, (int64_t)ld_x);
+    }
+    check(hipStreamSynchronize(f->stream()), "solve");
+  });
+}
+
+static_assert(SCRATCH_BYTES == YBUS_HIP_SCRATCH_BYTES, "the header states the workspace's size");
+
+// A workspace buffer of the factor lent a slice of the no-allocation workspace for one call, and
+// given back its own memory (and length) when the call ends, however it ends
+struct Lend {
+  DevBuf<double>& buf;
+  double* p;
+  size_t len;
+  bool lent;
+  Lend(DevBuf<double>& b, double* slice, size_t elems) : buf(b), p(b.p), len(b.len), lent(b.lent) {
+    buf.p = slice;
+    buf.len = elems;
+    buf.lent = true;
+  }
+  ~Lend() {
+    buf.p = p;
+    buf.len = len;
+    buf.lent = lent;
+  }
+};
+
+int32_t ybus_hip_factor_solve_device_noalloc(ybus_hip_factor* f, int32_t mode, const double* b, uint64_t ld_b,
+                                             uint64_t count, double* x, uint64_t ld_x) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    require_unborrowed(f);
+    if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_TRANSPOSE)
+      fail(YBUS_HIP_INVALID_ARGUMENT, "mode: YBUS_HIP_FP64 or YBUS_HIP_TRANSPOSE");
+    check_device_vectors(f, count, std::min(ld_b, ld_x), {b, x});
+    if (count == 0 || f->n == 0) return;
+    if (!f->ctx->scratch) fail(YBUS_HIP_INTERNAL, "the context has no no-allocation workspace");
+    const size_t n = (size_t)f->n;
+    const size_t align = 256 / sizeof(double);
+    auto up = [&](size_t v) { return (v + align - 1) / align * align; };
+    // the widest chunk (at most block_width() and count columns) whose buffers fit the array: the
+    // column path keeps b and x (n per column each); the wide path b (n x ld), t and x (each with
+    // the pad row) and the vectors' staging (n per column)
+    Layout layout;
+    size_t need = 0;
+    int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
+    for (;; width = width / 2) {
+      if (width == 0)
+        fail(YBUS_HIP_TOO_LARGE, "one vector's workspace exceeds the no-allocation workspace of " +
+                                     std::to_string(SCRATCH_BYTES >> 20) + " MiB");
+      layout.columns = f->use_columns(width, false);
+      layout.ld = layout.columns ? (int64_t)n : round_up(width, 16);
+      const size_t elems = n * (size_t)(layout.columns ? width : layout.ld);
+      need = layout.columns ? 2 * up(elems) : up(elems) + 2 * up(elems + (size_t)layout.ld) + up(n * (size_t)width);
+      if (need * sizeof(double) <= SCRATCH_BYTES) break;
+    }
+    const size_t elems = n * (size_t)(layout.columns ? width : layout.ld);
+    const size_t padded = layout.columns ? elems : elems + (size_t)layout.ld;
+    double* slice_b = static_cast<double*>(f->ctx->scratch);
+    double* slice_x = slice_b + up(elems);
+    double* slice_t = slice_x + up(padded);
+    double* slice_v = slice_t + up(padded);
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    // the factor's own workspace (and what it has staged) is left as it is: its buffers are lent
+    // the slices for this call and its layout is put back after it
+    Lend lb(f->b, slice_b, elems), lx(f->x, slice_x, padded);
+    std::optional<Lend> lt, lv;
+    if (!layout.columns) {
+      lt.emplace(f->t, slice_t, padded);
+      lv.emplace(f->vec, slice_v, n * (size_t)width);
+    }
+    const Layout own = f->layout;
+    struct Restore {
+      ybus_hip_factor* f;
+      Layout layout;
+      ~Restore() { f->layout = layout; }
+    } restore{f, own};
+    f->layout = layout;
+    for (uint64_t start = 0; start < count; start += (uint64_t)width) {
+      const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
+      copy_in_device(f, b + start * ld_b, (int64_t)ld_b, chunk, f->b.p);
+      f->solve64(mode == YBUS_HIP_TRANSPOSE, false, chunk);
       copy_out_device(f, f->x.p, chunk, x + start * ld_x, (int64_t)ld_x);
     }
     check(hipStreamSynchronize(f->stream()), "solve");
