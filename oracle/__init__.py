@@ -23,6 +23,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 NATURAL, RCM, AMD, COLAMD = 0, 1, 2, 3
+# mistakes the tests can plant in AMD and COLAMD (oracle.hpp, Mutation)
+FIFO_LISTS, LAST_REPRESENTATIVE, NO_AGGRESSIVE_ABSORPTION = 1, 2, 4
 
 ERRORS = {
     1: "NotSquare", 2: "NotSymmetric", 3: "NotPositiveDefinite", 4: "SingularMatrix",
@@ -81,6 +83,7 @@ def _declare(L):
         "yo_mul_vec": (C.c_int, mat + [f64p, f64p]),
         "yo_relative_residual": (C.c_int, mat + [f64p, f64p, C.POINTER(DBL)]),
         "yo_ordering": (C.c_int, mat + [C.c_int, u64p]),
+        "yo_ordering_mutated": (C.c_int, mat + [C.c_int, C.c_uint, u64p]),
         "yo_cholesky": (C.c_int, mat + [C.c_int, P, C.c_int, C.POINTER(P), C.POINTER(U64)]),
         "yo_chol_sizes": (None, [P, u64p]),
         "yo_chol_copy_symbolic": (None, [P, u64p, i64p, u64p, u64p, u64p, u64p]),
@@ -202,9 +205,15 @@ class Matrix:
         _check(lib().yo_relative_residual(*self.args(), _f64(x), _f64(b), C.byref(r)))
         return r.value
 
-    def ordering(self, kind: int) -> np.ndarray:
+    def ordering(self, kind: int, mutations: int = 0) -> np.ndarray:
+        """The ordering's permutation (ordering.rs:46-63). `mutations` plants mistakes in AMD or
+        COLAMD (FIFO_LISTS, LAST_REPRESENTATIVE, NO_AGGRESSIVE_ABSORPTION), for tests that show
+        their literal permutations catch them."""
         perm = np.zeros(self.cols if kind == COLAMD else self.rows, dtype=np.uint64)
-        _check(lib().yo_ordering(*self.args(), kind, perm))
+        if mutations:
+            _check(lib().yo_ordering_mutated(*self.args(), kind, mutations, perm))
+        else:
+            _check(lib().yo_ordering(*self.args(), kind, perm))
         return perm.astype(np.int64)
 
     def to_dense(self) -> np.ndarray:

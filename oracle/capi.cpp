@@ -150,6 +150,14 @@ int yo_ordering(MAT_ARGS, int kind, uint64_t* perm) {
   });
 }
 
+// The ordering with planted mistakes (Mutation flags), for the tests that show its literals catch them
+int yo_ordering_mutated(MAT_ARGS, int kind, unsigned mutations, uint64_t* perm) {
+  return guard([&] {
+    std::vector<size_t> p = ordering(MAT, (Ordering)kind, mutations);
+    for (size_t i = 0; i < p.size(); ++i) perm[i] = p[i];
+  });
+}
+
 // ---- Cholesky ----
 
 int yo_cholesky(MAT_ARGS, int kind, const uint64_t* perm, int symbolic_only, void** out, uint64_t* pivot) {

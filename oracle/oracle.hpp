@@ -87,12 +87,20 @@ struct Adjacency {
   size_t degree(size_t i) const { return start[i + 1] - start[i]; }
 };
 
+// Mistakes the tests plant in AMD and COLAMD to show that the literal permutations they pin catch
+// them (every other use passes none): degree lists or score buckets taken first in, first out;
+// supervariables or supercolumns merged into the last of their kind rather than the first; no
+// aggressive absorption.
+enum Mutation : unsigned { FIFO_LISTS = 1, LAST_REPRESENTATIVE = 2, NO_AGGRESSIVE_ABSORPTION = 4 };
+
 // ordering.rs:46-63 (CompressedMatrix::ordering)
-std::vector<size_t> ordering(const Compressed& a, Ordering kind);
-std::vector<size_t> reverse_cuthill_mckee(const Adjacency& g);       // ordering.rs:135-234
-std::vector<size_t> approximate_minimum_degree(const Adjacency& g);  // ordering.rs:306-492
-std::vector<size_t> column_approximate_minimum_degree(              // colamd.rs:91-261
-    size_t n_rows, size_t n_cols, const std::vector<std::pair<size_t, size_t>>& entries);
+std::vector<size_t> ordering(const Compressed& a, Ordering kind, unsigned mutations = 0);
+std::vector<size_t> reverse_cuthill_mckee(const Adjacency& g);  // ordering.rs:135-234
+std::vector<size_t> approximate_minimum_degree(const Adjacency& g,
+                                               unsigned mutations = 0);  // ordering.rs:306-492
+std::vector<size_t> column_approximate_minimum_degree(                  // colamd.rs:91-261
+    size_t n_rows, size_t n_cols, const std::vector<std::pair<size_t, size_t>>& entries,
+    unsigned mutations = 0);
 
 // cholesky.rs:27-39 (SymbolicCholesky)
 struct SymbolicCholesky {

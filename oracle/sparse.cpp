@@ -219,14 +219,14 @@ Adjacency Adjacency::of(const Compressed& a) {
 }
 
 // ordering.rs:46-63
-std::vector<size_t> ordering(const Compressed& a, Ordering kind) {
+std::vector<size_t> ordering(const Compressed& a, Ordering kind, unsigned mutations) {
   if (kind == Ordering::Colamd) {
     std::vector<std::pair<size_t, size_t>> entries;
     entries.reserve(a.nnz());
     for (size_t major = 0; major < a.major_dim(); ++major)
       for (size_t p = a.ptr[major]; p < a.ptr[major + 1]; ++p)
         entries.emplace_back(a.row_major ? major : a.idx[p], a.row_major ? a.idx[p] : major);
-    return column_approximate_minimum_degree(a.rows, a.cols, entries);
+    return column_approximate_minimum_degree(a.rows, a.cols, entries, mutations);
   }
   if (!a.is_square()) throw SparseError(Err::NotSquare, "not square");
   switch (kind) {
@@ -238,7 +238,7 @@ std::vector<size_t> ordering(const Compressed& a, Ordering kind) {
     case Ordering::Rcm:
       return reverse_cuthill_mckee(Adjacency::of(a));
     case Ordering::Amd:
-      return approximate_minimum_degree(Adjacency::of(a));
+      return approximate_minimum_degree(Adjacency::of(a), mutations);
     default:
       throw SparseError(Err::InvalidArgument, "unknown ordering");
   }
@@ -328,11 +328,21 @@ std::vector<size_t> reverse_cuthill_mckee(const Adjacency& g) {
 namespace {
 enum class Node { Variable, Merged, Element, Absorbed, Dense };
 
-// ordering.rs:252-285: degree lists, each a doubly linked list; insert at the head (LIFO)
+// ordering.rs:252-285: degree lists, each a doubly linked list; insert at the head (LIFO). With
+// `fifo` (a planted mistake) a node goes to the tail instead.
 struct DegreeLists {
   std::vector<size_t> head, next, previous;
+  bool fifo = false;
   explicit DegreeLists(size_t n) : head(n + 1, NONE), next(n, NONE), previous(n, NONE) {}
   void insert(size_t node, size_t degree) {
+    if (fifo && head[degree] != NONE) {
+      size_t last = head[degree];
+      while (next[last] != NONE) last = next[last];
+      next[last] = node;
+      previous[node] = last;
+      next[node] = NONE;
+      return;
+    }
     next[node] = head[degree];
     previous[node] = NONE;
     if (head[degree] != NONE) previous[head[degree]] = node;
@@ -353,7 +363,7 @@ struct DegreeLists {
 // variable whose lists equal an earlier unmerged one's (:449-470). Groups are disjoint and
 // list equality is an equivalence, so the merges are the same whatever order the groups are
 // visited in; here a std::map keyed by the same wrapping hash visits them in key order.
-std::vector<size_t> approximate_minimum_degree(const Adjacency& g) {
+std::vector<size_t> approximate_minimum_degree(const Adjacency& g, unsigned mutations) {
   const size_t n = g.len();
   std::vector<size_t> order;
   order.reserve(n);
@@ -372,6 +382,7 @@ std::vector<size_t> approximate_minimum_degree(const Adjacency& g) {
   std::vector<size_t> degree(n);
   for (size_t i = 0; i < n; ++i) degree[i] = variables[i].size();
   DegreeLists lists(n);
+  lists.fifo = (mutations & FIFO_LISTS) != 0;
   size_t remaining = 0;
   for (size_t i = n; i-- > 0;) {
     if (kind[i] == Node::Variable) {
@@ -436,7 +447,7 @@ std::vector<size_t> approximate_minimum_degree(const Adjacency& g) {
       size_t from_elements = 0;
       for (size_t e : elements[i]) {
         if (kind[e] != Node::Element) continue;
-        if (outside[e] == 0) {
+        if (outside[e] == 0 && !(mutations & NO_AGGRESSIVE_ABSORPTION)) {
           kind[e] = Node::Absorbed;
           element_list[e].clear();
         } else {
@@ -461,8 +472,9 @@ std::vector<size_t> approximate_minimum_degree(const Adjacency& g) {
     }
 
     for (auto& entry : by_hash) {
-      const std::vector<size_t>& group = entry.second;
+      std::vector<size_t>& group = entry.second;
       if (group.size() <= 1) continue;
+      if (mutations & LAST_REPRESENTATIVE) std::reverse(group.begin(), group.end());
       for (size_t a = 0; a < group.size(); ++a) {
         size_t i = group[a];
         if (kind[i] != Node::Variable) continue;
@@ -508,14 +520,25 @@ std::vector<size_t> approximate_minimum_degree(const Adjacency& g) {
 namespace {
 enum class Status { Live, Eliminated, Merged, SetAside };
 
-// colamd.rs:35-87: score buckets, LIFO within a bucket, with a lowest-score pointer
+// colamd.rs:35-87: score buckets, LIFO within a bucket, with a lowest-score pointer. With `fifo`
+// (a planted mistake) a column goes to the tail of its bucket instead.
 struct Buckets {
   std::vector<size_t> head, next, prev, score;
   size_t lowest;
+  bool fifo = false;
   explicit Buckets(size_t n_cols)
       : head(n_cols + 1, NONE), next(n_cols, NONE), prev(n_cols, NONE), score(n_cols, 0), lowest(n_cols) {}
   void insert(size_t col, size_t s) {
     score[col] = s;
+    lowest = std::min(lowest, s);
+    if (fifo && head[s] != NONE) {
+      size_t last = head[s];
+      while (next[last] != NONE) last = next[last];
+      next[last] = col;
+      prev[col] = last;
+      next[col] = NONE;
+      return;
+    }
     prev[col] = NONE;
     next[col] = head[s];
     if (head[s] != NONE) prev[head[s]] = col;
@@ -544,7 +567,8 @@ struct Buckets {
 // (n_cols + 1)) and merges a column into the first earlier representative with identical rows
 // (:224-250); as with AMD the map's visiting order cannot change the outcome.
 std::vector<size_t> column_approximate_minimum_degree(size_t n_rows, size_t n_cols,
-                                                      const std::vector<std::pair<size_t, size_t>>& entries) {
+                                                      const std::vector<std::pair<size_t, size_t>>& entries,
+                                                      unsigned mutations) {
   std::vector<std::vector<size_t>> col_rows(n_cols);
   for (auto& e : entries) col_rows[e.second].push_back(e.first);
   for (auto& rows : col_rows) {
@@ -586,6 +610,7 @@ std::vector<size_t> column_approximate_minimum_degree(size_t n_rows, size_t n_co
   for (auto s : status)
     if (s == Status::Live) live_weight += 1;
   Buckets buckets(n_cols);
+  buckets.fifo = (mutations & FIFO_LISTS) != 0;
   for (size_t col = 0; col < n_cols; ++col) {
     if (status[col] == Status::Live) {
       size_t score = 0;
@@ -647,7 +672,8 @@ std::vector<size_t> column_approximate_minimum_degree(size_t n_rows, size_t n_co
     }
     for (size_t col : pivot_cols)
       for (size_t row : col_rows[col])
-        if (row != p && row_mark[row] == stamp && outside[row] == 0) row_alive[row] = false;
+        if (row != p && row_mark[row] == stamp && outside[row] == 0 && !(mutations & NO_AGGRESSIVE_ABSORPTION))
+          row_alive[row] = false;
     std::vector<long long> scores;
     scores.reserve(pivot_cols.size());
     for (size_t col : pivot_cols) {
@@ -667,21 +693,29 @@ std::vector<size_t> column_approximate_minimum_degree(size_t n_rows, size_t n_co
       for (size_t r : col_rows[col]) sum += r;
       auto key = std::make_pair(col_rows[col].size(), sum % (n_cols + 1));
       auto& candidates = representatives[key];
-      size_t found = NONE;
-      for (size_t other : candidates)
-        if (col_rows[pivot_cols[other]] == col_rows[col]) {
-          found = other;
+      size_t found = NONE, slot = 0;
+      for (; slot < candidates.size(); ++slot)
+        if (col_rows[pivot_cols[candidates[slot]]] == col_rows[col]) {
+          found = candidates[slot];
           break;
         }
       if (found != NONE) {
-        size_t into = pivot_cols[found];
-        weight[into] += weight[col];
-        std::vector<size_t> moved{col};
-        moved.insert(moved.end(), members[col].begin(), members[col].end());
-        members[col].clear();
+        // merged into the earlier column (with LAST_REPRESENTATIVE, a planted mistake, the earlier
+        // one into this one, which takes its place among the candidates)
+        size_t keep = found, gone = index;
+        if (mutations & LAST_REPRESENTATIVE) {
+          keep = index;
+          gone = found;
+          candidates[slot] = index;
+        }
+        const size_t into = pivot_cols[keep], merged = pivot_cols[gone];
+        weight[into] += weight[merged];
+        std::vector<size_t> moved{merged};
+        moved.insert(moved.end(), members[merged].begin(), members[merged].end());
+        members[merged].clear();
         members[into].insert(members[into].end(), moved.begin(), moved.end());
-        status[col] = Status::Merged;
-        scores[found] -= (long long)weight[col];
+        status[merged] = Status::Merged;
+        scores[keep] -= (long long)weight[merged];
       } else {
         candidates.push_back(index);
       }
