@@ -41,31 +41,27 @@ def log(msg: str):
 
 # ---- CPU-baseline leg: the reference's host side (ybus_sparse), restated by the oracle ----
 
-def reference_host_pipeline_dc(buses: int, seed: int):
-    """Grid, B' and its AMD Cholesky factor, as ybus_sparse would make them on the host."""
-    import oracle as O
-
-    grid = O.make_grid(buses, seed)
-    bprime, bf, _ = grid.dc()
-    t0 = time.perf_counter()
-    chol = O.Cholesky(bprime)
-    factor_s = time.perf_counter() - t0
-    return grid, bprime, chol, factor_s
-
-
-def reference_host_pipeline_ac(buses: int, seed: int):
-    import oracle as O
-
-    grid = O.make_grid(buses, seed)
-    jac, _ = grid.jacobian(seed)
-    t0 = time.perf_counter()
-    lu = O.Lu(jac, O.COLAMD)
-    factor_s = time.perf_counter() - t0
-    return jac, lu, factor_s
 
 
 def cpu_threads() -> int:
-    return max(1, min(16, os.cpu_count() or 1))
+    """The host threads the CPU baseline runs on: every core this process may use (its CPU
+    affinity), capped by OMP_NUM_THREADS where the environment sets it (the share of the host's
+    cores given to this GPU's jobs)."""
+    return min(cpu_affinity(), thread_cap() or cpu_affinity())
+
+
+def cpu_affinity() -> int:
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        return max(1, os.cpu_count() or 1)
+
+
+def thread_cap() -> int:
+    try:
+        return max(0, int(os.environ.get("OMP_NUM_THREADS", "0")))
+    except ValueError:
+        return 0
 
 
 CPU_SECONDS = 10.0
@@ -137,20 +133,22 @@ def profiled_traffic(kernel: str):
 
 def pair_block(grid):
     """One pair per branch: +1 at its from bus, -1 at its to bus, the slack (bus 0) dropped."""
-    plus = np.array([-1 if f == 0 else int(f) - 1 for f in grid.branch_from], dtype=np.int64)
-    minus = np.array([-1 if t == 0 else int(t) - 1 for t in grid.branch_to], dtype=np.int64)
-    return plus, minus
+    from oracle import configs
+
+    return configs.pair_lists(grid)
 
 
 def run_ptdf(gpu, args):
     import ybus_amd as Y
 
-    buses = 10000
-    grid, bprime, chol, factor_s = reference_host_pipeline_dc(buses, SEED + 1)
-    plus, minus = pair_block(grid)
+    from oracle import configs
+
+    t0 = time.perf_counter()
+    grid, bprime, chol, plus, minus = configs.ptdf()
+    factor_s = time.perf_counter() - t0
     rhs_count = len(plus)
     nnz_l = int(chol.l.nnz)
-    log(f"[ptdf] n={chol.n} nnz(L)={nnz_l} RHS={rhs_count}; host factor {factor_s * 1e3:.1f} ms")
+    log(f"[ptdf] n={chol.n} nnz(L)={nnz_l} RHS={rhs_count}; host grid and factor {factor_s * 1e3:.1f} ms")
 
     factor = Y.GpuFactor.from_cholesky(gpu, chol)
     factor.set_block_width(rhs_count)
@@ -197,10 +195,13 @@ def run_precision(gpu, args):
     import oracle as O
     import ybus_amd as Y
 
-    jac, lu, factor_s = reference_host_pipeline_ac(25000, SEED + 2)
+    from oracle import configs
+
+    t0 = time.perf_counter()
+    jac, lu, rhs = configs.jacobian()
+    factor_s = time.perf_counter() - t0
     nnz_lu = int(lu.l.nnz + lu.u.nnz)
-    rhs = O.normals(SEED + 2, 9, 512 * jac.rows).reshape(512, jac.rows)
-    log(f"[precision] n={jac.rows} nnz(L+U)={nnz_lu}; host COLAMD LU {factor_s * 1e3:.1f} ms")
+    log(f"[precision] n={jac.rows} nnz(L+U)={nnz_lu}; host Jacobian and COLAMD LU {factor_s * 1e3:.1f} ms")
     factor = Y.GpuFactor.from_lu(gpu, lu)
     factor.set_matrix(jac)
     factor.set_block_width(512)
@@ -237,6 +238,8 @@ def run_precision(gpu, args):
             f"residual <= {out[name]['max_relative_residual']:.2e}")
         if mode == Y.FP64:
             x64 = x
+        else:
+            x32 = x
     # the FP32 sweeps alone: no correction (max_steps = 0), so one FP32 solve and one residual
     for _ in range(max(1, args.warmup)):
         factor.solve_staged(Y.FP32_REFINED, 1e-12, 0).free()
@@ -246,9 +249,29 @@ def run_precision(gpu, args):
     st = factor.kernel_stats()
     out["fp32_solve_and_residual_kernel_ms"] = st.milliseconds / st.launches
     log(f"[precision] one FP32 solve + FP64 residual: {out['fp32_solve_and_residual_kernel_ms']:.3f} ms")
-    cpu = lu.solve_many(rhs[:64], 4)
-    err = float(np.max(np.abs(x64[:64] - cpu)) / np.max(np.abs(cpu)))
-    out["fp64_vs_cpu_max_rel_diff"] = err
+    # the contraction per correction: each column's relative residual after k corrections (at most
+    # k; a column that has stopped keeps its best iterate), the median over the block
+    contraction = []
+    for k in range(0, 6):
+        with factor.solve_staged(Y.FP32_REFINED, 1e-12, k) as block:
+            contraction.append(float(np.median(block.refinement()[1])))
+    out["median_residual_after_k_corrections"] = contraction
+    # against the CPU's FP64 solutions (the oracle), every column
+    cpu = lu.solve_many(rhs, cpu_threads())
+    scale = np.max(np.abs(cpu), axis=1)
+    out["fp64_vs_cpu_max_rel_diff"] = float(np.max(np.max(np.abs(x64 - cpu), axis=1) / scale))
+    out["fp64_bit_exact_columns"] = int(sum(a.tobytes() == b.tobytes() for a, b in zip(x64, cpu)))
+    out["fp32_refined_vs_cpu_fp64_max_rel_diff"] = float(np.max(np.max(np.abs(x32 - cpu), axis=1) / scale))
+    # conditioning: the oracle's 1-norm estimate (accuracy.rs:207-279) and the infinity-norm one
+    # that bounds ||x32 - x64|| by ||A^-1||inf times the residuals
+    inv1 = configs.inverse_norm_1_estimate(lu.n, lu.solve, lu.solve_transpose)
+    invinf = configs.lu_inverse_norm_inf_estimate(lu)
+    out["condition_estimate_1"] = configs.norm_1(jac) * inv1
+    out["condition_estimate_inf"] = configs.norm_inf(jac) * invinf
+    # what the 1.7x target would need: (k + 1) FP32 solves with their residuals in the FP64
+    # solve's time / 1.7, k the mean corrections
+    k = out["fp32_refined"]["mean_steps"]
+    out["fp32_solve_ms_needed_for_1_7x"] = out["fp64"]["kernel_ms"] / 1.7 / (k + 1.0)
     out["speedup_fp32_refined_over_fp64"] = out["fp32_refined"]["value"] / out["fp64"]["value"]
     out["workload"] = "25,000-bus AC Newton Jacobian (n=%d), COLAMD LU, 512 N(0,1) RHS" % jac.rows
     out["tuning"] = tuning
@@ -282,12 +305,10 @@ def run_adjoint(gpu, args):
     import ybus_amd as Y
     from ybus_amd.autograd import solve
 
-    buses, seed = 70000, SEED + 4
-    grid = O.make_grid(buses, seed)
-    bprime, bf, _ = grid.dc()
-    chol = O.Cholesky(bprime)
-    p = O.injections(seed, 4, buses, 64)
-    limit = 0.5 + 1.5 * O.uniforms(seed, 5, bf.rows)
+    from oracle import configs
+
+    case = configs.adjoint()
+    bprime, bf, chol, p, limit = case.bprime, case.bf, case.chol, case.p, case.limit
     factor = Y.GpuFactor.from_cholesky(gpu, chol)
     factor.set_matrix(bprime)
     factor.set_block_width(64)
@@ -377,21 +398,11 @@ def run_batch(gpu, args):
     import oracle as O
     import ybus_amd as Y
 
-    seed, grids = SEED + 3, 256
-    sizes = np.round(np.exp(np.log(500.0) + O.uniforms(seed, 1, grids) * np.log(20000.0 / 500.0))).astype(int)
-    counts = (1 + O.integers(seed, 2, grids) % 64).astype(int)
-    picks = O.uniforms(seed, 3, grids)
+    from oracle import configs
+
     t0 = time.perf_counter()
-    cpu, rhs, mats = [], [], []
-    for i in range(grids):
-        g = O.make_grid(int(sizes[i]), seed * 1000 + i)
-        candidates = np.flatnonzero(~g.bridges())
-        mask = np.ones(g.branches, dtype=bool)
-        mask[candidates[int(picks[i] * len(candidates))]] = False
-        bprime, _, _ = g.dc(mask)
-        cpu.append(O.Cholesky(bprime))
-        mats.append(bprime)
-        rhs.append(O.injections(seed * 1000 + i, 4, int(sizes[i]), int(counts[i])))
+    case = configs.batch()
+    grids, counts, cpu, rhs, mats = len(case.factors), case.counts, case.factors, case.rhs, case.matrices
     log(f"[batch] {grids} grids, sum n = {int(sum(c.n for c in cpu))}, sum r = {int(counts.sum())}; host "
         f"{time.perf_counter() - t0:.1f} s")
     factors = [Y.GpuFactor.from_cholesky(gpu, c) for c in cpu]
@@ -437,8 +448,10 @@ def run_batch(gpu, args):
     single_s = (time.perf_counter() - t0) / reps
     single_kernel_ms = sum(f.kernel_stats().milliseconds for f in factors) / reps
     work = float(sum(int(r) * int(c.l.nnz) for r, c in zip(counts, cpu)))
-    sample = list(range(0, grids, 16))
-    exact = all(solved[i].tobytes() == cpu[i].solve_many(rhs[i]).tobytes() for i in sample)
+    # every factor's block against the CPU's solves (the oracle), and the refined ones' distance
+    cpu_x = [c.solve_many(b, cpu_threads()) for c, b in zip(cpu, rhs)]
+    exact = sum(x.tobytes() == w.tobytes() for x, w in zip(solved, cpu_x))
+    refined_vs_cpu = max(float(np.max(np.abs(x - w)) / np.max(np.abs(w))) for x, w in zip(refined, cpu_x) if w.size)
     out = dict(workload="256 grids, n_i log-uniform in [499, 19,999] (sum n = %d), r_i in 1..64 (sum r = %d), "
                         "AMD Cholesky, one non-bridge branch out per grid" % (int(sum(c.n for c in cpu)),
                                                                               int(counts.sum())),
@@ -448,10 +461,10 @@ def run_batch(gpu, args):
                speedup_over_one_launch_per_factor=single_s / batch_s,
                fp32_refined=dict(batch_ms=refined_s * 1e3, batch_kernel_ms=refined_stats.milliseconds / reps,
                                  max_relative_residual=max_ratio, mean_corrections=mean_steps,
-                                 max_rel_diff_vs_fp64=refined_err),
-               bit_exact_sampled_factors=f"{len(sample)} of {len(sample)}" if exact else "MISMATCH")
+                                 max_rel_diff_vs_fp64=refined_err, max_rel_diff_vs_cpu_fp64=refined_vs_cpu),
+               bit_exact_factors=f"{exact} of {grids}")
     log(f"[batch] batch {out['batch_ms']:.3f} ms (kernel {out['batch_kernel_ms']:.3f}), one launch per factor "
-        f"{out['one_launch_per_factor_ms']:.3f} ms; bit-exact {out['bit_exact_sampled_factors']}; FP32-refined "
+        f"{out['one_launch_per_factor_ms']:.3f} ms; bit-exact {out['bit_exact_factors']}; FP32-refined "
         f"{out['fp32_refined']['batch_ms']:.3f} ms, residual <= {max_ratio:.2e}, {mean_steps:.2f} corrections")
     return out
 
@@ -893,25 +906,25 @@ def main():
     copy_gbs = copy_peak_gbs()
     log(f"[bench] device copy of 2 GiB: {copy_gbs:.0f} GB/s (read + written)")
 
-    # parity: sampled columns against the CPU (bit for bit)
-    sample = np.arange(0, rhs_count, max(1, rhs_count // 256))
+    # parity: every column against the CPU's solve of its dense vector (bit for bit)
+    from oracle import configs
+
     chol = ptdf["chol"]
-    dense = np.zeros((len(sample), chol.n))
-    for j, k in enumerate(sample):
-        if ptdf["plus"][k] >= 0:
-            dense[j, ptdf["plus"][k]] = 1.0
-        if ptdf["minus"][k] >= 0:
-            dense[j, ptdf["minus"][k]] = -1.0
-    cpu_cols = chol.solve_many(dense, cpu_threads())
-    got = ptdf["solutions"][sample]
-    bit_exact = int(np.sum([a.tobytes() == b.tobytes() for a, b in zip(got, cpu_cols)]))
-    max_rel = float(np.max(np.abs(got - cpu_cols)) / max(np.max(np.abs(cpu_cols)), 1e-300))
+    bit_exact, max_rel = 0, 0.0
+    for start in range(0, rhs_count, 1500):
+        stop = min(rhs_count, start + 1500)
+        cpu_cols = chol.solve_many(configs.dense_pairs(ptdf["plus"], ptdf["minus"], chol.n, start, stop), cpu_threads())
+        got = ptdf["solutions"][start:stop]
+        bit_exact += int(np.sum([a.tobytes() == b.tobytes() for a, b in zip(got, cpu_cols)]))
+        max_rel = max(max_rel, float(np.max(np.abs(got - cpu_cols)) / max(np.max(np.abs(cpu_cols)), 1e-300)))
 
     cpu_baseline = None
     if not args.no_cpu:
         rate, threads, _ = cpu_pair_baseline(chol, ptdf["plus"], ptdf["minus"], min(2000, rhs_count))
         serial, _, _ = cpu_pair_baseline(chol, ptdf["plus"], ptdf["minus"], min(250, rhs_count), 1, 2.0)
         cpu_baseline = {"value": rate * nnz_l, "unit": "RHS·nnz(L)/s", "cores": threads, "kind": "port",
+                        "cores_available": cpu_affinity(), "os_cpu_count": os.cpu_count(),
+                        "omp_num_threads": thread_cap() or None,
                         "sample": "par_solve_many (oracle, one RHS per thread) on the first 2,000 of the 13,500 "
                                   "pair RHS as dense vectors, repeated for >= %g s; serial: solve_many on the "
                                   "first 250, repeated for >= 2 s" % CPU_SECONDS,
@@ -940,8 +953,7 @@ def main():
                      "copy_peak": copy_gbs, "frac_of_copy_peak": achieved / copy_gbs},
         "cpu_baseline": cpu_baseline,
         "end_to_end_ms": ptdf["end_to_end"] * 1e3,
-        "parity": {"sampled_columns": int(len(sample)), "bit_exact_columns": bit_exact,
-                   "max_rel_diff_vs_cpu": max_rel},
+        "parity": {"columns": rhs_count, "bit_exact_columns": bit_exact, "max_rel_diff_vs_cpu": max_rel},
     }
     if not args.no_precision:
         line["precision"] = run_precision(gpu, args)

@@ -186,3 +186,12 @@ def norm_inf(a) -> float:
     sums = np.zeros(a.rows)
     np.add.at(sums, rows, np.abs(a.data))
     return float(sums.max()) if a.rows else 0.0
+
+
+def norm_1(a) -> float:
+    """The largest absolute column sum of an oracle matrix (CompressedMatrix::norm_1)."""
+    major = np.repeat(np.arange(len(a.ptr) - 1), np.diff(np.asarray(a.ptr, dtype=np.int64)))
+    cols = np.asarray(a.idx, dtype=np.int64) if a.row_major else major
+    sums = np.zeros(a.cols)
+    np.add.at(sums, cols, np.abs(a.data))
+    return float(sums.max()) if a.cols else 0.0

@@ -60,6 +60,8 @@ def test_the_infinity_norm_estimate_swaps_the_solves():
     assert truth / 3.0 <= estimate <= truth * (1.0 + 1e-10)
     assert configs.norm_inf(a) == float(np.max(np.sum(np.abs(a.to_dense()), axis=1)))
     assert configs.norm_inf(a.to_csc()) == configs.norm_inf(a)
+    assert configs.norm_1(a) == float(np.max(np.sum(np.abs(a.to_dense()), axis=0)))
+    assert configs.norm_1(a.to_csc()) == configs.norm_1(a)
 
 
 def test_dense_pairs_and_pair_lists():
