@@ -70,6 +70,79 @@ Sweep schedule(const RowForm& form, const std::vector<int64_t>& src_map, const s
   return s;
 }
 
+Sweep split_rows(const Sweep& s, int32_t cap, double continuation_diag) {
+  const int32_t levels = s.levels;
+  if (!s.val_im.empty() || s.parts || cap < 1) return s;
+  // the level at which each destination row is final
+  std::vector<int32_t> final_at(s.n + 1, 0);
+  for (int32_t l = 0; l < levels; ++l)
+    for (int32_t k = s.level_ptr[l]; k < s.level_ptr[l + 1]; ++k) final_at[s.dst[k]] = l;
+  struct Segment {
+    int32_t k, p0, p1;
+    bool first, last;
+  };
+  std::vector<std::vector<Segment>> at(levels);
+  for (int32_t l = 0; l < levels; ++l) {
+    for (int32_t k = s.level_ptr[l]; k < s.level_ptr[l + 1]; ++k) {
+      const int32_t p0 = s.row_ptr[k], p1 = s.row_ptr[k + 1];
+      if (p1 - p0 <= cap) {
+        at[l].push_back({k, p0, p1, true, true});
+        continue;
+      }
+      int32_t end = p1, level = l;
+      bool last = true;
+      while (end > p0) {
+        int32_t begin = std::max(p0, end - cap);
+        if (level == 0) {
+          begin = p0;
+        } else {
+          // every term left for the earlier segments runs at level - 1 at the latest, so its
+          // dependency must be final by level - 2: this segment starts at the first term whose
+          // dependency is final only at level - 1 or later, if it is not already before it
+          for (int32_t q = p0; q < begin; ++q)
+            if (final_at[s.dep[q]] >= level - 1) {
+              begin = q;
+              break;
+            }
+        }
+        at[level].push_back({k, begin, end, begin == p0, last});
+        last = false;
+        end = begin;
+        --level;
+      }
+    }
+  }
+  Sweep out;
+  out.n = s.n;
+  out.levels = levels;
+  out.unit = s.unit;
+  out.split = true;
+  out.level_ptr.assign(levels + 1, 0);
+  for (int32_t l = 0; l < levels; ++l) {
+    out.level_ptr[l + 1] = out.level_ptr[l] + (int32_t)at[l].size();
+    out.widest_level = std::max(out.widest_level, (int32_t)at[l].size());
+  }
+  const int32_t positions = out.level_ptr[levels];
+  out.src.reserve(positions);
+  out.dst.reserve(positions);
+  out.row_ptr.reserve(positions + 1);
+  out.row_ptr.push_back(0);
+  out.dep.reserve(s.dep.size());
+  out.val.reserve(s.val.size());
+  if (!s.unit) out.diag.reserve(positions);
+  for (int32_t l = 0; l < levels; ++l) {
+    for (const Segment& g : at[l]) {
+      out.src.push_back(g.first ? s.src[g.k] : ~s.dst[g.k]);
+      out.dst.push_back(s.dst[g.k]);
+      out.dep.insert(out.dep.end(), s.dep.begin() + g.p0, s.dep.begin() + g.p1);
+      out.val.insert(out.val.end(), s.val.begin() + g.p0, s.val.begin() + g.p1);
+      out.row_ptr.push_back((int32_t)out.dep.size());
+      if (!s.unit) out.diag.push_back(g.last ? s.diag[g.k] : continuation_diag);
+    }
+  }
+  return out;
+}
+
 std::vector<int32_t> tree_parts(const RowForm& lower, int32_t target, int32_t& parts) {
   parts = 0;
   const int64_t n = lower.n;

@@ -46,6 +46,11 @@ struct Sweep {
   // are level_ptr[part_level[q]] .. level_ptr[part_level[q + 1]] (parts + 2 entries).
   int32_t parts = 0;
   std::vector<int32_t> part_level;
+  // A split sweep (split_rows): a position is a segment of a row's terms rather than a whole
+  // row, so there may be more positions than rows; a segment that continues a row reads its
+  // right-hand side from the row's own slot of the destination block (src = ~dst).
+  bool split = false;
+  int32_t positions() const { return level_ptr.empty() ? 0 : level_ptr.back(); }
 };
 
 // Schedules `form`: level[i] is one more than the deepest of row i's dependencies; rows are
@@ -69,6 +74,20 @@ std::vector<int32_t> tree_parts(const RowForm& lower, int32_t target, int32_t& p
 // partition the form does not satisfy gives a sweep with parts = 0 and no rows.
 Sweep schedule_parts(const RowForm& form, const std::vector<int64_t>& src_map, const std::vector<int64_t>& dst_map,
                      const std::vector<int32_t>& part_of, int32_t parts, bool top_first);
+
+// A scheduled sweep with every row of more than `cap` terms cut into chained segments, so that
+// a long row's terms no longer wait, one chunk after another, at the row's own level. The terms
+// keep their order: segment j of a row takes a run of its terms after segment j - 1's, starting
+// from the row's right-hand side (the first segment) or from the partial value the previous one
+// left in the row's own slot (src = ~dst), and only the last divides by the diagonal (the others
+// carry continuation_diag: 1, an exact division, in a solve's sweep). The segments are placed
+// from the last back: the last at the row's own level, each earlier one a level before the next,
+// each of at most `cap` terms unless a term before it would then wait for a dependency that is
+// not final by the level before (the segment then reaches back to that term), so no level and
+// no dependent moves. The rows'
+// bits do not change: every term is subtracted in its row's order, each result rounded as
+// before.
+Sweep split_rows(const Sweep& s, int32_t cap, double continuation_diag = 1.0);
 
 // Each row's level: one more than the deepest of its dependencies (0 for none).
 std::vector<int32_t> levels_of(const RowForm& form);

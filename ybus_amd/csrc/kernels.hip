@@ -308,7 +308,9 @@ __device__ void wide_sweep(const SweepDev& s, const int32_t* lp, const T* __rest
 
 // ---- FP64 solve: forward into t, barrier, backward into x ----
 
-template <int TW>
+// Split: the sweeps are split (SweepDev::split), so a segment with src < 0 starts from the
+// partial value in its row's own slot of the block it writes (t forward, x backward).
+template <int TW, bool Split>
 __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, double* __restrict__ t,
                                                        double* __restrict__ x, int64_t ld, int32_t ncols) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -321,12 +323,14 @@ __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bw
   const int32_t plus = !rhs.block && g.on ? rhs.plus[col] : -1;
   const int32_t minus = !rhs.block && g.on ? rhs.minus[col] : -1;
   auto from_rhs = [&](int32_t src) -> double {
+    if (Split && src < 0) return t[(~src) * ld + col];
     if (rhs.block) return rhs.block[src * ld + col];
     // e_plus - e_minus as the host builds it: 1.0, -1.0 or 0.0
     return (src == plus ? 1.0 : 0.0) - (src == minus ? 1.0 : 0.0);
   };
   wide_sweep<double, TW>(fwd, lpf, t, ld, g, g.on, from_rhs, [&](int32_t dst, double v) { t[dst * ld + col] = v; });
-  wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; },
+  wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on,
+                         [&](int32_t src) { return Split && src < 0 ? x[(~src) * ld + col] : t[src * ld + col]; },
                          [&](int32_t dst, double v) { x[dst * ld + col] = v; });
 }
 
@@ -336,7 +340,9 @@ __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bw
 // right-hand side is +0.0, its terms are products with +0.0, and its diagonal is positive or
 // unit), so it is neither computed nor stored: a bitmap in LDS marks the reach, a dependency
 // outside it reads as 0, and the backward sweep reads its right-hand side only inside it.
-template <int TW>
+// SkipLoads: each row issues only its own terms' loads (a factor of short rows: a grid's
+// Cholesky factor), or all four inline loads without branches (an LU's long rows)
+template <int TW, bool SkipLoads>
 __global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, ReachDev reach,
                                                              double* __restrict__ t, double* __restrict__ x, int64_t ld,
                                                              int32_t ncols) {
@@ -365,8 +371,9 @@ __global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, Sweep
   auto from_pair = [&](int32_t src) -> double {
     return (src == plus ? 1.0 : 0.0) - (src == minus ? 1.0 : 0.0);
   };
+  using Skip = std::integral_constant<bool, SkipLoads>;
   wide_sweep<double, TW>(fwd, lpf, t, ld, g, g.on, from_pair, [&](int32_t dst, double v) { t[dst * ld + col] = v; },
-                         at, in_reach, std::true_type());
+                         at, in_reach, Skip());
   const int32_t pad = fwd.n;
   auto from_t = [&](int32_t src) -> double {
     const bool on = in_reach(src);
@@ -374,7 +381,7 @@ __global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, Sweep
     return on ? v : 0.0;
   };
   wide_sweep<double, TW>(bwd, lpb, x, ld, g, g.on, from_t, [&](int32_t dst, double v) { x[dst * ld + col] = v; },
-                         Identity(), AllLive(), std::true_type());
+                         Identity(), AllLive(), Skip());
 }
 
 // ---- complex FP64 solves ----
@@ -626,43 +633,100 @@ __global__ void __launch_bounds__(1024) parts_backward_kernel(SweepDev bwd, Reac
 // ---- FP32 sweeps refined in FP64 ----
 
 // Forward FP32 sweep of one tile: t[dst] = (float(src[src_row]) - sum val * t[dep]) / diag
-template <int TW>
+// (Split: a segment with src < 0 continues from t at its row)
+template <int TW, bool Split>
 __device__ void forward32(const SweepDev& s, const int32_t* lp, const double* __restrict__ src, float* __restrict__ t,
                           int64_t ld, const Tile<TW>& g, bool act) {
   const int32_t col = g.col;
-  wide_sweep<float, TW>(s, lp, t, ld, g, act, [&](int32_t r) { return (float)src[r * ld + col]; },
+  wide_sweep<float, TW>(s, lp, t, ld, g, act,
+                        [&](int32_t r) { return Split && r < 0 ? t[(~r) * ld + col] : (float)src[r * ld + col]; },
                         [&](int32_t dst, float v) { t[dst * ld + col] = v; });
 }
 
-// Backward FP32 sweep of one tile into d (its own dependencies), updating the FP64 iterate:
-// xn[dst] = xc[dst] + double(d) (or double(d) alone when xc is null)
-template <int TW>
+// Backward FP32 sweep of one tile into d, the correction (its own dependencies), and the FP64
+// iterate xn = xc + double(d) (double(d) alone when xc is null). Unsplit, each row's store forms
+// xn too. Split, the sweep stores d alone and combine() forms xn after it: a segment is not its
+// row's last, and with many segments per level a store that loads xc waits behind the next
+// row's gathers (loads retire in order); a segment with src < 0 continues from d at its row.
+template <int TW, bool Split>
 __device__ void backward32(const SweepDev& s, const int32_t* lp, const float* __restrict__ t, float* __restrict__ d,
                            const double* __restrict__ xc, double* __restrict__ xn, int64_t ld, const Tile<TW>& g,
                            bool act) {
   const int32_t col = g.col;
-  wide_sweep<float, TW>(s, lp, d, ld, g, act, [&](int32_t r) { return t[r * ld + col]; },
+  wide_sweep<float, TW>(s, lp, d, ld, g, act,
+                        [&](int32_t r) { return Split && r < 0 ? d[(~r) * ld + col] : t[r * ld + col]; },
                         [&](int32_t dst, float v) {
                           const int64_t at = dst * ld + col;
                           d[at] = v;
-                          xn[at] = xc ? xc[at] + (double)v : (double)v;
+                          if (!Split) xn[at] = xc ? xc[at] + (double)v : (double)v;
                         });
 }
 
+// The new iterate xn = xc + double(d) (double(d) alone when xc is null) for the tile's columns:
+// every row independent, four rows' loads in flight per thread
+template <int TW>
+__device__ void combine(const double* __restrict__ xc, const float* __restrict__ d, double* __restrict__ xn,
+                        int32_t rows, int64_t ld, const Tile<TW>& g, bool act) {
+  if (!act) return;
+  int32_t i = g.slot;
+  for (; i + 3 * g.slots < rows; i += 4 * g.slots) {
+    int64_t at[4];
+    double c[4];
+    float v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      at[q] = (int64_t)(i + q * g.slots) * ld + g.col;
+      c[q] = xc ? xc[at[q]] : 0.0;
+      v[q] = d[at[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xn[at[q]] = xc ? c[q] + (double)v[q] : (double)v[q];
+  }
+  for (; i < rows; i += g.slots) {
+    const int64_t at = (int64_t)i * ld + g.col;
+    xn[at] = xc ? xc[at] + (double)d[at] : (double)d[at];
+  }
+}
+
 // r = b - A x for the tile's columns (each row summed from 0 in index order, as mul_vec's
-// dot_kernel does, product.rs:17-27); returns this thread's partial max |r|
+// dot_kernel does, product.rs:17-27); returns this thread's partial max |r|. A row's terms are
+// taken RESIDUAL_CHUNK at a time: their columns and values loaded together, then their entries of
+// x together, then summed in order, so a row waits for two rounds of loads per chunk rather than
+// per term; the next row's bounds are loaded during this one.
 template <int TW>
 __device__ double residual(const CsrDev& a, const double* __restrict__ b, const double* __restrict__ x,
                            double* __restrict__ r, int64_t ld, const Tile<TW>& g, bool act) {
   double m = 0.0;
   if (!act) return m;
-  for (int32_t i = g.slot; i < a.rows; i += g.slots) {
+  constexpr int RESIDUAL_CHUNK = 4;
+  int32_t i = g.slot;
+  int32_t p0 = i < a.rows ? a.ptr[i] : 0, p1 = i < a.rows ? a.ptr[i + 1] : 0;
+  for (; i < a.rows; i += g.slots) {
+    const int32_t next = i + g.slots;
+    const int32_t n0 = next < a.rows ? a.ptr[next] : 0, n1 = next < a.rows ? a.ptr[next + 1] : 0;
+    const int64_t at = i * ld + g.col;
+    const double bi = b[at];
     double sum = 0.0;
-    const int32_t pe = a.ptr[i + 1];
-    for (int32_t p = a.ptr[i]; p < pe; ++p) sum = sum + a.val[p] * x[a.col[p] * ld + g.col];
-    const double v = b[i * ld + g.col] - sum;
-    r[i * ld + g.col] = v;
+    for (int32_t p = p0; p < p1; p += RESIDUAL_CHUNK) {
+      int32_t cc[RESIDUAL_CHUNK];
+      double vv[RESIDUAL_CHUNK], xx[RESIDUAL_CHUNK];
+#pragma unroll
+      for (int q = 0; q < RESIDUAL_CHUNK; ++q) {
+        const int32_t pq = p + q < p1 ? p + q : p1 - 1;
+        cc[q] = a.col[pq];
+        vv[q] = a.val[pq];
+      }
+#pragma unroll
+      for (int q = 0; q < RESIDUAL_CHUNK; ++q) xx[q] = x[cc[q] * ld + g.col];
+#pragma unroll
+      for (int q = 0; q < RESIDUAL_CHUNK; ++q)
+        if (p + q < p1) sum = sum + vv[q] * xx[q];
+    }
+    const double v = bi - sum;
+    r[at] = v;
     m = fold_max(m, fabs(v));
+    p0 = n0;
+    p1 = n1;
   }
   return m;
 }
@@ -671,7 +735,7 @@ __device__ __forceinline__ double ratio_of(double r_norm, double b_norm) {
   return b_norm > 0.0 ? r_norm / b_norm : (r_norm > 0.0 ? __builtin_inf() : 0.0);
 }
 
-template <int TW>
+template <int TW, bool Split>
 __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd, CsrDev a, const double* __restrict__ b,
                                                       float* __restrict__ t, float* __restrict__ d,
                                                       double* __restrict__ x0, double* __restrict__ x1,
@@ -695,8 +759,13 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
   reduce_max<TW>(scratch, m, s_bnorm, g);
 
   // x0 = solve(b), its residual and ratio
-  forward32<TW>(fwd, lpf, b, t, ld, g, g.on);
-  backward32<TW>(bwd, lpb, t, d, nullptr, x0, ld, g, g.on);
+  forward32<TW, Split>(fwd, lpf, b, t, ld, g, g.on);
+  backward32<TW, Split>(bwd, lpb, t, d, nullptr, x0, ld, g, g.on);
+  if (Split) {
+    __syncthreads();
+    combine<TW>(nullptr, d, x0, a.rows, ld, g, g.on);
+    __syncthreads();
+  }
   m = residual<TW>(a, b, x0, r, ld, g, g.on);
   reduce_max<TW>(scratch, m, s_rnorm, g);
   if (g.slot == 0) {
@@ -719,8 +788,13 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
     for (int c = 0; c < TW; ++c) any = any || s_active[c];
     if (!any) break;
     const bool act = g.on && s_active[g.lane];
-    forward32<TW>(fwd, lpf, r, t, ld, g, act);
-    backward32<TW>(bwd, lpb, t, d, cur, nxt, ld, g, act);
+    forward32<TW, Split>(fwd, lpf, r, t, ld, g, act);
+    backward32<TW, Split>(bwd, lpb, t, d, cur, nxt, ld, g, act);
+    if (Split) {
+      __syncthreads();
+      combine<TW>(cur, d, nxt, a.rows, ld, g, act);
+      __syncthreads();
+    }
     m = residual<TW>(a, b, nxt, r, ld, g, act);
     reduce_max<TW>(scratch, m, s_rnorm, g);
     if (g.slot == 0 && act) {
@@ -1442,16 +1516,23 @@ hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const Sweep
   if (threads < tile || threads > 1024 || threads % 64) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
   const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
-#define L(TWV)                                                                                                 \
+#define S(TWV, SPLIT)                                                                                            \
   {                                                                                                              \
-    hipError_t e = hipFuncSetAttribute((const void*)solve64_kernel<TWV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                       (int)bytes);                                                              \
+    hipError_t e = hipFuncSetAttribute((const void*)solve64_kernel<TWV, SPLIT>,                                  \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
     if (e != hipSuccess) return e;                                                                               \
-    hipLaunchKernelGGL((solve64_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, t, x, ld,    \
-                       ncols);                                                                                   \
+    hipLaunchKernelGGL((solve64_kernel<TWV, SPLIT>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, t, x, \
+                       ld, ncols);                                                                               \
   }
+#define L(TWV)         \
+  if (split)           \
+    S(TWV, true)       \
+  else                 \
+    S(TWV, false)
+  const bool split = fwd.split || bwd.split;
   DISPATCH_TILE(tile, L)
 #undef L
+#undef S
   return hipGetLastError();
 }
 
@@ -1496,20 +1577,26 @@ size_t reach_lds_bytes(const SweepDev& fwd, const SweepDev& bwd) {
 
 hipError_t launch_solve64_reach(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                                 const RhsDev& rhs, const ReachDev& reach, double* t, double* x, int64_t ld,
-                                int32_t ncols) {
+                                int32_t ncols, bool skip_loads) {
   if (threads < tile || threads > 1024 || threads % 64) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
   const size_t bytes = reach_lds_bytes(fwd, bwd);
-#define L(TWV)                                                                                                  \
+#define S(TWV, SKIP)                                                                                            \
   {                                                                                                             \
-    hipError_t e = hipFuncSetAttribute((const void*)solve64_reach_kernel<TWV>,                                  \
+    hipError_t e = hipFuncSetAttribute((const void*)solve64_reach_kernel<TWV, SKIP>,                            \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
     if (e != hipSuccess) return e;                                                                              \
-    hipLaunchKernelGGL((solve64_reach_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, reach, \
-                       t, x, ld, ncols);                                                                        \
+    hipLaunchKernelGGL((solve64_reach_kernel<TWV, SKIP>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, \
+                       reach, t, x, ld, ncols);                                                                 \
   }
+#define L(TWV)         \
+  if (skip_loads)      \
+    S(TWV, true)       \
+  else                 \
+    S(TWV, false)
   DISPATCH_TILE(tile, L)
 #undef L
+#undef S
   return hipGetLastError();
 }
 
@@ -1519,16 +1606,23 @@ hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepD
   if (threads < tile || threads > 1024 || threads % 64 || (threads & (threads - 1))) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
   const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
-#define L(TWV)                                                                                                 \
+#define S(TWV, SPLIT)                                                                                            \
   {                                                                                                              \
-    hipError_t e = hipFuncSetAttribute((const void*)refine_kernel<TWV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                       (int)bytes);                                                              \
+    hipError_t e = hipFuncSetAttribute((const void*)refine_kernel<TWV, SPLIT>,                                   \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
     if (e != hipSuccess) return e;                                                                               \
-    hipLaunchKernelGGL((refine_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, d, x0, x1, \
-                       r, ld, ncols, tol, max_steps, steps, ratio);                                              \
+    hipLaunchKernelGGL((refine_kernel<TWV, SPLIT>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, d, \
+                       x0, x1, r, ld, ncols, tol, max_steps, steps, ratio);                                      \
   }
+#define L(TWV)         \
+  if (split)           \
+    S(TWV, true)       \
+  else                 \
+    S(TWV, false)
+  const bool split = fwd.split || bwd.split;
   DISPATCH_TILE(tile, L)
 #undef L
+#undef S
   return hipGetLastError();
 }
 
@@ -2079,7 +2173,8 @@ __global__ void refresh_rows_kernel(RefreshDev r, const double* __restrict__ src
   r.val23[k] = make_double2(v[2], v[3]);
   r.v32[k] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
   if (r.dpos) {
-    const double dg = src[r.dpos[k]];
+    // a split sweep's continuation segment divides by 1 (no source position)
+    const double dg = r.dpos[k] >= 0 ? src[r.dpos[k]] : 1.0;
     r.diag64[k] = dg;
     r.diag32[k] = (float)dg;
     r.rdiag64[k] = dg;

@@ -40,6 +40,9 @@ struct SweepDev {
   const float* diag32 = nullptr;
   RecordsDev rec;  // one record per position, in processing order
   bool unit = false;
+  // positions are segments of rows (a split sweep): a segment with src < 0 continues row ~src
+  // from the partial value the previous segment left in its slot of the destination block
+  bool split = false;
   // a partitioned sweep: `parts` independent parts, then the top part (index parts); part q's
   // levels are part_level[q] .. part_level[q + 1] of level_ptr (parts + 2 entries)
   int32_t parts = 0;
@@ -132,10 +135,12 @@ struct ReachDev {
 
 // LDS a reach solve needs (both sweeps' level pointers and a bit per forward row), and the solve
 // itself: a pair block (rhs.plus/minus) whose forward sweep visits only each tile's reach.
+// skip_loads: each row issues only its own inline terms' loads (short rows, a Cholesky factor's),
+// else all four without branches (long rows, an LU factor's).
 size_t reach_lds_bytes(const SweepDev& fwd, const SweepDev& bwd);
 hipError_t launch_solve64_reach(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                                 const RhsDev& rhs, const ReachDev& reach, double* t, double* x, int64_t ld,
-                                int32_t ncols);
+                                int32_t ncols, bool skip_loads);
 
 // FP32 sweeps refined in FP64 to `tol` (refine_normwise's rule), one workgroup per column tile;
 // the best iterate ends in x0. steps/ratio get each column's corrections and relative residual.

@@ -199,12 +199,14 @@ struct DevSweep {
   void upload(Sweep s, hipStream_t stream, bool wide) {
     host = std::move(s);
     const int32_t n = host.n;
-    std::vector<int4> hd(n), dp(n);
-    std::vector<double2> a(n), b(n);
-    std::vector<float4> c(n);
-    std::vector<double> rd(n, 1.0);
-    std::vector<float> rd32(n, 1.0f);
-    for (int32_t k = 0; k < n; ++k) {
+    // one record per position (a split sweep has a position per segment of a row)
+    const int32_t count = host.positions();
+    std::vector<int4> hd(count), dp(count);
+    std::vector<double2> a(count), b(count);
+    std::vector<float4> c(count);
+    std::vector<double> rd(count, 1.0);
+    std::vector<float> rd32(count, 1.0f);
+    for (int32_t k = 0; k < count; ++k) {
       if (!host.unit) {
         rd[k] = host.diag[k];
         rd32[k] = (float)host.diag[k];
@@ -268,6 +270,7 @@ struct DevSweep {
     v.rec.diag64 = rdiag64.p;
     v.rec.diag32 = rdiag32.p;
     v.unit = host.unit;
+    v.split = host.split;
     v.parts = host.parts;
     v.part_level = host.parts ? part_level.p : nullptr;
     return v;
@@ -515,6 +518,10 @@ struct ybus_hip_factor {
   size_t lds_cap = 0;
   // wide path: schedules reading and writing permuted blocks
   DevSweep fwd, bwd, tfwd, tbwd;
+  // an LU factor's wide sweeps with their long rows split into chained segments (split_rows),
+  // which the wide solves of vectors and the FP32 refinement take (0: not built)
+  int32_t split_cap = 0;
+  DevSweep sfwd, sbwd, stfwd, stbwd;
   // wide path, partitioned (a Cholesky factor whose elimination tree splits): the same sweeps
   // with their rows grouped into independent parts and a top part; FP64 solves take these
   bool partitioned = false;
@@ -787,6 +794,9 @@ struct ybus_hip_factor {
     return factor + columns_bytes;
   }
 
+  // Whether the wide solves take the split sweeps (an LU factor with long rows)
+  bool split_sweeps() const { return lu && !complex && split_cap > 0; }
+
   // FP64 solve (or transposed solve) of the block: from b, or from pairs, into x
   void solve64(bool transpose, bool pairs, int64_t ncols) {
     if (transpose && !has_transpose) transpose = false;  // A^T = A for a Cholesky factor
@@ -815,10 +825,14 @@ struct ybus_hip_factor {
               "solve kernel");
       } else if (pairs && !transpose && reach_tile == tile && !reach_parts) {
         check(launch_solve64_reach(stream(), tile, wide_threads(ncols, pairs), f.view(), bw.view(), rhs,
-                                   ReachDev{reach_pos.p, reach_lp.p}, t.p, x.p, layout.ld, (int32_t)ncols),
+                                   ReachDev{reach_pos.p, reach_lp.p}, t.p, x.p, layout.ld, (int32_t)ncols, !lu),
               "solve kernel");
       } else {
-        check(launch_solve64(stream(), tile, wide_threads(ncols, pairs), f.view(), bw.view(), rhs, t.p, x.p, layout.ld,
+        // an LU factor's split sweeps, where built
+        const bool sp = split_sweeps();
+        const DevSweep& fs = sp ? (transpose ? stfwd : sfwd) : f;
+        const DevSweep& bs = sp ? (transpose ? stbwd : sbwd) : bw;
+        check(launch_solve64(stream(), tile, wide_threads(ncols, pairs), fs.view(), bs.view(), rhs, t.p, x.p, layout.ld,
                              (int32_t)ncols),
               "solve kernel");
       }
@@ -847,8 +861,10 @@ struct ybus_hip_factor {
       const int tile = own ? (tile_override ? tile_override : tile_width_for(ncols, ctx->cus)) : tile_for(ncols, false);
       int th = 64;
       while (th * 2 <= (own ? threads_wide : wide_threads(ncols, false))) th *= 2;
-      check(launch_refine(stream(), tile, th, (t_sweeps ? tfwd : fwd).view(),
-                          (t_sweeps ? tbwd : bwd).view(), m, b.p, t32.p, d32.p, x.p, x1.p, r.p, layout.ld,
+      const bool sp = split_sweeps();
+      const DevSweep& fs = sp ? (t_sweeps ? stfwd : sfwd) : (t_sweeps ? tfwd : fwd);
+      const DevSweep& bs = sp ? (t_sweeps ? stbwd : sbwd) : (t_sweeps ? tbwd : bwd);
+      check(launch_refine(stream(), tile, th, fs.view(), bs.view(), m, b.p, t32.p, d32.p, x.p, x1.p, r.p, layout.ld,
                           (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
             "refine kernel");
     }
@@ -995,6 +1011,12 @@ static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   if (reach && std::string(reach) == "0") f->use_reach = false;
   const char* path = std::getenv("YBUS_HIP_PATH");
   if (path) f->force = std::string(path) == "wide" ? 1 : std::string(path) == "columns" ? 2 : 0;
+  // an LU factor's rows longer than this many terms are split into segments for the wide solves:
+  // segments of the four inline terms, which wait for one round of loads each (measured on the
+  // 25,000-bus Jacobian, FP64: 4.57 ms unsplit, 2.98 ms in segments of 4, 3.27 of 12, 3.58 of 20)
+  f->split_cap = RECORD_ENTRIES;
+  const char* split = std::getenv("YBUS_HIP_SPLIT");
+  if (split) f->split_cap = std::max(0, std::atoi(split));
   f->column_limit = 2 * (int64_t)ctx->cus;
   f->lds_cap = lds_capacity(ctx);
 }
@@ -1195,6 +1217,23 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
       f->t_rhs_row.upload(inverse_i32(cp), ctx->stream);
       f->t_sol_of.upload(inverse_i32(rp), ctx->stream);
     }
+    if (f->split_cap > 0) {
+      Sweep sp[4];
+      const int32_t cap = f->split_cap;
+      run_together({[&] { sp[0] = split_rows(wl, cap); }, [&] { sp[1] = split_rows(wu, cap); },
+                    [&] { sp[2] = split_rows(wut, cap); }, [&] { sp[3] = split_rows(wlt, cap); }});
+      // built only when some row is long enough to be split
+      bool any = false;
+      for (const Sweep& q : sp) any = any || q.positions() > q.n;
+      if (any) {
+        f->sfwd.upload(std::move(sp[0]), ctx->stream, true);
+        f->sbwd.upload(std::move(sp[1]), ctx->stream, true);
+        f->stfwd.upload(std::move(sp[2]), ctx->stream, true);
+        f->stbwd.upload(std::move(sp[3]), ctx->stream, true);
+      } else {
+        f->split_cap = 0;
+      }
+    }
     f->fwd.upload(std::move(wl), ctx->stream, true);
     f->bwd.upload(std::move(wu), ctx->stream, true);
     f->tfwd.upload(std::move(wut), ctx->stream, true);
@@ -1314,6 +1353,18 @@ static RefactorPlan& refactor_plan(ybus_hip_factor* f) {
   add(f->bwd, s[1]);
   add(f->tfwd, s[2]);
   add(f->tbwd, s[3]);
+  if (f->split_sweeps()) {
+    // the split sweeps' maps come from splitting the position schedules the same way; a
+    // continuation segment's diagonal (1) maps to no position (-1)
+    Sweep sp[4];
+    const int32_t cap = f->split_cap;
+    run_together({[&] { sp[0] = split_rows(s[0], cap, 0.0); }, [&] { sp[1] = split_rows(s[1], cap, 0.0); },
+                  [&] { sp[2] = split_rows(s[2], cap, 0.0); }, [&] { sp[3] = split_rows(s[3], cap, 0.0); }});
+    add(f->sfwd, sp[0]);
+    add(f->sbwd, sp[1]);
+    add(f->stfwd, sp[2]);
+    add(f->stbwd, sp[3]);
+  }
   if (f->fits) {
     add(f->cfwd, s[4]);
     add(f->cbwd, s[5]);
@@ -1342,7 +1393,7 @@ static double refresh_sweeps(ybus_hip_factor* f, RefactorPlan& plan) {
     }
     DevSweep& d = *r.sweep;
     RefreshDev v;
-    v.n = d.host.n;
+    v.n = d.host.positions();
     v.entries = (int64_t)d.host.dep.size();
     v.row_ptr = d.row_ptr.p;
     v.vpos = r.vpos.p;
