@@ -522,6 +522,9 @@ struct ybus_hip_factor {
   // which the wide solves of vectors and the FP32 refinement take (0: not built)
   int32_t split_cap = 0;
   DevSweep sfwd, sbwd, stfwd, stbwd;
+  // the column path's sweeps split the same way (in place in LDS, a segment continues from its
+  // row's own slot without any change to the kernel); the cap they were split with (0: none)
+  int32_t column_split_cap = 0;
   // wide path, partitioned (a Cholesky factor whose elimination tree splits): the same sweeps
   // with their rows grouped into independent parts and a top part; FP64 solves take these
   bool partitioned = false;
@@ -1154,7 +1157,11 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
                   }});
     f->fits = fits_columns(f.get(), sf.levels + sb.levels);
     if (f->fits) {
-      // cholesky.rs:570-579: y[i] = b[perm[i]], and x[perm[i]] = y[i]
+      // cholesky.rs:570-579: y[i] = b[perm[i]], and x[perm[i]] = y[i]; long rows split into
+      // segments whose terms are all inline (split_rows), so no level loads terms on demand
+      if (f->split_cap > 0)
+        run_together({[&] { sf = split_rows(sf, f->split_cap); }, [&] { sb = split_rows(sb, f->split_cap); }});
+      f->column_split_cap = f->split_cap;
       f->cfwd.upload(std::move(sf), ctx->stream, false);
       f->cbwd.upload(std::move(sb), ctx->stream, false);
       f->rhs_row.upload(inverse_i32(gather), ctx->stream);
@@ -1207,7 +1214,14 @@ int32_t ybus_hip_factor_upload_lu(ybus_hip_ctx* ctx, uint64_t n, const uint64_t*
                   [&] { wut = schedule(ut, cp, ident); }, [&] { wlt = schedule(lt, ident, rp); }});
     f->fits = fits_columns(f.get(), std::max(sl.levels + su.levels, stu.levels + stl.levels));
     if (f->fits) {
-      // lu.rs:770-785: y[i] = b[P[i]], x[Q[i]] = y[i]; transposed (lu.rs:506-533): through Q, then P
+      // lu.rs:770-785: y[i] = b[P[i]], x[Q[i]] = y[i]; transposed (lu.rs:506-533): through Q, then P;
+      // long rows split into segments whose terms are all inline
+      if (f->split_cap > 0) {
+        const int32_t cap = f->split_cap;
+        run_together({[&] { sl = split_rows(sl, cap); }, [&] { su = split_rows(su, cap); },
+                      [&] { stu = split_rows(stu, cap); }, [&] { stl = split_rows(stl, cap); }});
+      }
+      f->column_split_cap = f->split_cap;
       f->cfwd.upload(std::move(sl), ctx->stream, false);
       f->cbwd.upload(std::move(su), ctx->stream, false);
       f->ctfwd.upload(std::move(stu), ctx->stream, false);
@@ -1366,6 +1380,11 @@ static RefactorPlan& refactor_plan(ybus_hip_factor* f) {
     add(f->stbwd, sp[3]);
   }
   if (f->fits) {
+    if (f->column_split_cap > 0) {
+      const int32_t cap = f->column_split_cap;
+      run_together({[&] { s[4] = split_rows(s[4], cap, 0.0); }, [&] { s[5] = split_rows(s[5], cap, 0.0); },
+                    [&] { s[6] = split_rows(s[6], cap, 0.0); }, [&] { s[7] = split_rows(s[7], cap, 0.0); }});
+    }
     add(f->cfwd, s[4]);
     add(f->cbwd, s[5]);
     add(f->ctfwd, s[6]);
@@ -2840,7 +2859,7 @@ struct ybus_hip_batch {
   ybus_hip_kernel_stats stats{};
   // side streams (default priority) and events that fork the classes' launches off the batch's
   // stream inside the captured graph, so the classes run side by side, and join them back
-  static constexpr int SIDE = 3;
+  static constexpr int SIDE = 7;
   hipStream_t side[SIDE] = {};
   hipEvent_t fork = nullptr, join[SIDE] = {};
 
@@ -2862,9 +2881,42 @@ struct ybus_hip_batch {
     }
   }
 
+  // The classes' shapes, most workgroups per compute unit first: (workgroups per CU, threads per
+  // workgroup). Small factors run many narrow workgroups to a compute unit (a column's levels are
+  // mostly narrow, and a workgroup's barrier waits for all its waves); measured on the 256-grid
+  // batch, FP64: 2.31 ms with shapes 4:256, 2:512, 1:1024, 1.96 ms with these; the FP32
+  // refinement, whose residual and iterate passes use every thread, keeps wider workgroups for its
+  // large factors (9.6 ms against 10.9). YBUS_HIP_BATCH overrides both as "k:threads,k:threads,...".
+  static std::vector<std::pair<int, int>> class_shapes(bool refined) {
+    std::vector<std::pair<int, int>> shapes =
+        refined ? std::vector<std::pair<int, int>>{{16, 64}, {8, 128}, {4, 256}, {2, 512}, {1, 1024}}
+                : std::vector<std::pair<int, int>>{{16, 64}, {8, 128}, {4, 256}, {2, 256}, {1, 512}};
+    if (const char* e = std::getenv("YBUS_HIP_BATCH")) {
+      std::vector<std::pair<int, int>> mine;
+      std::string spec(e);
+      size_t at = 0;
+      while (at < spec.size()) {
+        size_t comma = spec.find(',', at);
+        if (comma == std::string::npos) comma = spec.size();
+        const std::string item = spec.substr(at, comma - at);
+        const size_t colon = item.find(':');
+        if (colon != std::string::npos) {
+          const int k = std::atoi(item.substr(0, colon).c_str()), t = std::atoi(item.substr(colon + 1).c_str());
+          if (k >= 1 && k <= 64 && t >= 64 && t <= 1024 && t % 64 == 0 && (t & (t - 1)) == 0) mine.push_back({k, t});
+        }
+        at = comma + 1;
+      }
+      if (!mine.empty()) {
+        std::sort(mine.begin(), mine.end(), [](auto a, auto b) { return a.first > b.first; });
+        shapes = mine;
+      }
+    }
+    return shapes;
+  }
+
   // Items in classes for a mode: each factor's columns go to the class of the most workgroups
-  // per compute unit (up to 4, 1024 / k threads each) whose LDS they fit; longest first within
-  // a class (a column's time is set by its levels, then by its rows)
+  // per compute unit whose LDS they fit; longest first within a class (a column's time is set by
+  // its levels, then by its rows)
   void make_plan(bool refined) {
     Plan& plan = plans[refined ? 1 : 0];
     struct Item {
@@ -2882,8 +2934,9 @@ struct ybus_hip_batch {
       const int32_t levels = f->cfwd.host.levels + f->cbwd.host.levels;
       int per_cu = 0;
       size_t lds = 0;
-      for (int k = 4; k >= 1 && !per_cu; --k) {
-        const int threads = k == 1 ? 1024 : k == 2 ? 512 : 256;
+      for (const auto& kt : class_shapes(refined)) {
+        const int k = kt.first, threads = kt.second;
+        if (per_cu) break;
         const size_t bytes = refined ? refine_lds_bytes(n, threads, levels) : solve_lds_bytes(n, false, levels);
         if ((size_t)k * bytes <= f->lds_cap) {
           per_cu = k;
@@ -2906,7 +2959,8 @@ struct ybus_hip_batch {
       if (plan.classes.empty() || plan.classes.back().per_cu != all[k].per_cu) {
         Class c;
         c.per_cu = all[k].per_cu;
-        c.threads = c.per_cu == 1 ? 1024 : c.per_cu == 2 ? 512 : 256;
+        for (const auto& kt : class_shapes(refined))
+          if (kt.first == c.per_cu) c.threads = kt.second;
         c.first = (int32_t)k;
         plan.classes.push_back(c);
       }
