@@ -379,20 +379,25 @@ int32_t ybus_hip_factor_path(const ybus_hip_factor *f, uint64_t columns, int32_t
 /* The kernel configuration a block width takes: path (YBUS_HIP_PATH_*), workgroup size, column
  * tile width (1 on the column path), the time of the synthetic block it was chosen on, and
  * whether the wide path sweeps the factor's partitioned form (1: a Cholesky factor's independent
- * elimination subtrees side by side, then the top of the tree; 0: one schedule). */
+ * elimination subtrees side by side, then the top of the tree; 0: one schedule), and for a
+ * partitioned block of vectors the top of the tree's own tile width and workgroup size (the
+ * parts' own where the top takes the same). */
 typedef struct {
   int32_t path;
   int32_t threads;
   int32_t tile;
   double milliseconds;
   int32_t parts;
+  int32_t top_tile;
+  int32_t top_threads;
 } ybus_hip_tuning;
 
 /* Autotuner (new): times every kernel configuration a block of `columns` right-hand sides can
  * take with this factor (the column path at 256 and 512 threads where the factor fits; the wide
  * path at 256, 512 and 1024 threads with column tiles of 1 to 64, over the factor's single and,
  * where it has one, partitioned schedule), each on a synthetic FP64 block
- * of that width, and keeps the fastest for this factor's later blocks of that width (up to the
+ * of that width, then, when a partitioned block of vectors wins, the same tile widths and sizes
+ * for its top-of-tree kernel alone, and keeps the fastest for this factor's later blocks of that width (up to the
  * next power of two) and kind: pair blocks (pairs != 0; pair_solves, pair_updates, staged pairs)
  * or blocks of vectors (solve_many, staged vectors, refinement). A pair block is timed on the
  * caller's own `columns` pairs when plus and minus are given (the pairs' reach sets the forward

@@ -317,6 +317,44 @@ def test_partitioned_sweeps_give_the_same_bits(gpu, monkeypatch, kind):
             assert got.tobytes() == want.tobytes(), (parts, reach, tile)
 
 
+@pytest.mark.parametrize("parts", ["8", "32"])
+def test_the_top_kernel_takes_its_own_configuration(gpu, monkeypatch, parts):
+    """The partitioned solve's top-of-tree kernel on a tile width and workgroup size of its own
+    (YBUS_HIP_TOP=tile[:threads], or the autotuner's second stage for blocks of vectors) reads the
+    parts' rows in the same block layout, and with the top held in LDS slots (the default where
+    they fit; YBUS_HIP_TOP_LDS=0 keeps the block in global memory) every segment still subtracts
+    its terms in order and divides as the sweep does: a block of vectors keeps the CPU's bits,
+    over split sweeps and unsplit ones, at tile widths whose slots fit LDS and ones that do not."""
+    g = O.make_grid(3000, 778)
+    bp, _, _ = g.dc()
+    cpu = O.Cholesky(bp)
+    k = 200
+    dense = np.random.default_rng(9).standard_normal((k, bp.rows))
+    want = cpu.solve_many(dense, threads=4)
+    monkeypatch.setenv("YBUS_HIP_PARTS", parts)
+    monkeypatch.setenv("YBUS_HIP_PATH", "wide")
+    for split in ["0", "4"]:
+        monkeypatch.setenv("YBUS_HIP_SPLIT", split)
+        for lds in ["0", "1"]:
+            monkeypatch.setenv("YBUS_HIP_TOP_LDS", lds)
+            for tile, top in [("8", "1:256"), ("8", "64:1024"), ("1", "16:64"), ("64", "2"), ("4", "4:128")]:
+                monkeypatch.setenv("YBUS_HIP_TILE", tile)
+                monkeypatch.setenv("YBUS_HIP_TOP", top)
+                factor = Y.GpuFactor.from_cholesky(gpu, cpu)
+                factor.set_block_width(k)
+                case = (split, lds, tile, top)
+                assert factor.solve_many(dense).tobytes() == want.tobytes(), case
+                assert factor.solve_transpose_many(dense).tobytes() == want.tobytes(), case
+        monkeypatch.delenv("YBUS_HIP_TILE")
+        monkeypatch.delenv("YBUS_HIP_TOP")
+        monkeypatch.delenv("YBUS_HIP_TOP_LDS")
+        factor = Y.GpuFactor.from_cholesky(gpu, cpu)
+        factor.set_block_width(k)
+        tuned = factor.autotune(k)
+        assert tuned["top_tile"] in (1, 2, 4, 8, 16, 32, 64) and tuned["top_threads"] >= tuned["top_tile"], tuned
+        assert factor.solve_many(dense).tobytes() == want.tobytes(), (split, tuned)
+
+
 def test_outage_updates_and_flow_maxima_over_partitioned_sweeps(gpu, monkeypatch):
     """The N-1 screening chain (rank-one pair updates, the operator's product maxima, the angle
     maxima) gives the same bits over a partitioned factor as over one schedule."""

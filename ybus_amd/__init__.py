@@ -95,7 +95,7 @@ assert _COLUMN_MAX.itemsize == C.sizeof(_ColumnMax)
 
 class _Tuning(C.Structure):
     _fields_ = [("path", C.c_int32), ("threads", C.c_int32), ("tile", C.c_int32), ("milliseconds", C.c_double),
-                ("parts", C.c_int32)]
+                ("parts", C.c_int32), ("top_tile", C.c_int32), ("top_threads", C.c_int32)]
 
 
 class _KernelStats(C.Structure):
@@ -498,7 +498,8 @@ class GpuFactor:
         """Times the kernel configurations for blocks of `columns` right-hand sides and keeps the
         fastest for this factor's blocks of that width and kind: pair blocks when `pairs` (True
         for synthetic pairs, or the caller's own list of Pair to time on), else blocks of vectors.
-        Returns {'path', 'threads', 'tile', 'parts', 'milliseconds'}."""
+        Returns {'path', 'threads', 'tile', 'parts', 'top_tile', 'top_threads', 'milliseconds'}
+        (top_*: the partitioned solve's top-of-tree kernel)."""
         self._release_block()
         t = _Tuning()
         plus = minus = None
@@ -509,7 +510,8 @@ class GpuFactor:
                                                   plus.ctypes.data if plus is not None else None,
                                                   minus.ctypes.data if minus is not None else None, C.byref(t)))
         return dict(path="wide" if t.path == 1 else "columns", threads=int(t.threads), tile=int(t.tile),
-                    parts=bool(t.parts), milliseconds=float(t.milliseconds))
+                    parts=bool(t.parts), top_tile=int(t.top_tile), top_threads=int(t.top_threads),
+                    milliseconds=float(t.milliseconds))
 
     def kernel_stats(self) -> KernelStats:
         s = _KernelStats()

@@ -72,11 +72,22 @@ Sweep schedule(const RowForm& form, const std::vector<int64_t>& src_map, const s
 
 Sweep split_rows(const Sweep& s, int32_t cap, double continuation_diag) {
   const int32_t levels = s.levels;
-  if (!s.val_im.empty() || s.parts || cap < 1) return s;
+  if (!s.val_im.empty() || cap < 1) return s;
   // the level at which each destination row is final
   std::vector<int32_t> final_at(s.n + 1, 0);
   for (int32_t l = 0; l < levels; ++l)
     for (int32_t k = s.level_ptr[l]; k < s.level_ptr[l + 1]; ++k) final_at[s.dst[k]] = l;
+  // each level's part: its first level and the end of its levels. A partitioned sweep's parts run
+  // apart (each in its own workgroups, the top part before or after all the others), so a row's
+  // segments stay within its part's levels, and a dependency in another part is final before
+  // the part starts
+  std::vector<int32_t> part_begin(levels, 0), part_end(levels, levels);
+  if (s.parts)
+    for (int32_t q = 0; q <= s.parts; ++q)
+      for (int32_t l = s.part_level[q]; l < s.part_level[q + 1]; ++l) {
+        part_begin[l] = s.part_level[q];
+        part_end[l] = s.part_level[q + 1];
+      }
   struct Segment {
     int32_t k, p0, p1;
     bool first, last;
@@ -89,18 +100,24 @@ Sweep split_rows(const Sweep& s, int32_t cap, double continuation_diag) {
         at[l].push_back({k, p0, p1, true, true});
         continue;
       }
+      const int32_t first = part_begin[l], end_level = part_end[l];
+      // a dependency's level as this part sees it
+      auto final_of = [&](int32_t d) {
+        const int32_t f = final_at[d];
+        return f >= first && f < end_level ? f : first - 1;
+      };
       int32_t end = p1, level = l;
       bool last = true;
       while (end > p0) {
         int32_t begin = std::max(p0, end - cap);
-        if (level == 0) {
+        if (level == first) {
           begin = p0;
         } else {
           // every term left for the earlier segments runs at level - 1 at the latest, so its
           // dependency must be final by level - 2: this segment starts at the first term whose
           // dependency is final only at level - 1 or later, if it is not already before it
           for (int32_t q = p0; q < begin; ++q)
-            if (final_at[s.dep[q]] >= level - 1) {
+            if (final_of(s.dep[q]) >= level - 1) {
               begin = q;
               break;
             }
@@ -117,6 +134,8 @@ Sweep split_rows(const Sweep& s, int32_t cap, double continuation_diag) {
   out.levels = levels;
   out.unit = s.unit;
   out.split = true;
+  out.parts = s.parts;
+  out.part_level = s.part_level;
   out.level_ptr.assign(levels + 1, 0);
   for (int32_t l = 0; l < levels; ++l) {
     out.level_ptr[l + 1] = out.level_ptr[l] + (int32_t)at[l].size();
@@ -139,6 +158,72 @@ Sweep split_rows(const Sweep& s, int32_t cap, double continuation_diag) {
       out.row_ptr.push_back((int32_t)out.dep.size());
       if (!s.unit) out.diag.push_back(g.last ? s.diag[g.k] : continuation_diag);
     }
+  }
+  return out;
+}
+
+TopSlots top_slots(const Sweep& f, const Sweep& b) {
+  TopSlots out;
+  const TopSlots bad;
+  if (f.parts < 1 || b.parts != f.parts || f.n != b.n) return bad;
+  auto range = [](const Sweep& s, TopSlots::Part& p) {
+    p.level0 = s.part_level[s.parts];
+    p.levels = s.part_level[s.parts + 1] - p.level0;
+    p.seg0 = s.level_ptr[p.level0];
+    p.segs = s.level_ptr[p.level0 + p.levels] - p.seg0;
+    p.term0 = s.row_ptr[p.seg0];
+    p.terms = s.row_ptr[p.seg0 + p.segs] - p.term0;
+    p.slot.resize(p.segs);
+    p.dep.resize(p.terms);
+  };
+  range(f, out.fwd);
+  range(b, out.bwd);
+  const int32_t n = f.n;
+  // forward: the top rows in the order of their first segments, then the halo
+  std::vector<int32_t> slot_t(n, -1), halo_t(n, -1);
+  for (int32_t k = out.fwd.seg0; k < out.fwd.seg0 + out.fwd.segs; ++k) {
+    const int32_t d = f.dst[k];
+    if (slot_t[d] < 0) {
+      if (f.src[k] < 0) return bad;  // a continuation before its row's first segment
+      slot_t[d] = out.rows++;
+      out.rhs_row.push_back(f.src[k]);
+    }
+    out.fwd.slot[k - out.fwd.seg0] = slot_t[d];
+  }
+  for (int32_t q = 0; q < out.fwd.terms; ++q) {
+    const int32_t d = f.dep[out.fwd.term0 + q];
+    if (slot_t[d] >= 0) {
+      out.fwd.dep[q] = slot_t[d];
+    } else {
+      if (halo_t[d] < 0) {
+        halo_t[d] = out.halo++;
+        out.halo_row.push_back(d);
+      }
+      out.fwd.dep[q] = out.rows + halo_t[d];
+    }
+  }
+  // backward: a first segment reads its row's forward value (so takes that row's slot) and
+  // names the row's solution row; its terms read solution rows of the top
+  std::vector<int32_t> slot_x(n, -1);
+  out.out_row.assign(out.rows, -1);
+  for (int32_t k = out.bwd.seg0; k < out.bwd.seg0 + out.bwd.segs; ++k) {
+    const int32_t d = b.dst[k];
+    if (b.src[k] >= 0) {
+      const int32_t sl = slot_t[b.src[k]];
+      if (sl < 0 || slot_x[d] >= 0 || out.out_row[sl] >= 0) return bad;
+      slot_x[d] = sl;
+      out.out_row[sl] = d;
+    } else if (slot_x[d] < 0) {
+      return bad;
+    }
+    out.bwd.slot[k - out.bwd.seg0] = slot_x[d];
+  }
+  for (int32_t sl = 0; sl < out.rows; ++sl)
+    if (out.out_row[sl] < 0) return bad;
+  for (int32_t q = 0; q < out.bwd.terms; ++q) {
+    const int32_t sl = slot_x[b.dep[out.bwd.term0 + q]];
+    if (sl < 0) return bad;
+    out.bwd.dep[q] = sl;
   }
   return out;
 }

@@ -89,6 +89,28 @@ Sweep schedule_parts(const RowForm& form, const std::vector<int64_t>& src_map, c
 // before.
 Sweep split_rows(const Sweep& s, int32_t cap, double continuation_diag = 1.0);
 
+// The top parts of a partitioned forward sweep and of its backward sweep (top first, so the
+// backward top depends on top rows only), renumbered for a solve that holds them in LDS: each
+// top row takes a slot, in the order its first forward segment comes in, and each row of the
+// other parts that a forward top segment depends on takes a halo slot after them. Each segment
+// then names its row's slot and each term its dependency's slot. A first forward segment starts
+// from its row's right-hand side (rhs_row), a first backward one from its row's forward value
+// (the same slot), and a continuing segment from the partial value in its slot, so a segment
+// needs no source of its own. rows = 0 when the sweeps do not have this form.
+struct TopSlots {
+  struct Part {
+    int32_t level0 = 0, levels = 0, seg0 = 0, segs = 0, term0 = 0, terms = 0;  // in the sweep's arrays
+    std::vector<int32_t> slot;  // segs
+    std::vector<int32_t> dep;   // terms
+  };
+  int32_t rows = 0, halo = 0;
+  std::vector<int32_t> rhs_row;   // rows: each top row's forward right-hand side row
+  std::vector<int32_t> halo_row;  // halo: the forward block's row each halo slot holds
+  std::vector<int32_t> out_row;   // rows: each top row's row of the backward (solution) block
+  Part fwd, bwd;
+};
+TopSlots top_slots(const Sweep& fwd, const Sweep& bwd);
+
 // Each row's level: one more than the deepest of its dependencies (0 for none).
 std::vector<int32_t> levels_of(const RowForm& form);
 

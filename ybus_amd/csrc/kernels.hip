@@ -544,8 +544,10 @@ struct ReachAt {
   __device__ __forceinline__ int32_t operator()(int32_t k) const { return pos[k]; }
 };
 
-// Phase 1: workgroup (tile, part) sweeps the part's rows forward into t
-template <int TW>
+// Phase 1: workgroup (tile, part) sweeps the part's rows forward into t. Split: the sweeps are
+// split (blocks of vectors only), so a segment with src < 0 continues from its row's slot of the
+// block it writes.
+template <int TW, bool Split>
 __global__ void __launch_bounds__(1024) parts_forward_kernel(SweepDev fwd, RhsDev rhs, ReachDev reach,
                                                              double* __restrict__ t, int64_t ld, int32_t ncols,
                                                              int32_t lp_words) {
@@ -562,12 +564,13 @@ __global__ void __launch_bounds__(1024) parts_forward_kernel(SweepDev fwd, RhsDe
     stage_live(reach, (int32_t)blockIdx.x, live);
     wide_sweep<double, TW>(s, lp, t, ld, g, g.on, from, store, ReachAt{reach.pos}, LiveBits{live}, std::true_type());
   } else {
-    wide_sweep<double, TW>(s, lp, t, ld, g, g.on, from, store, Identity(), AllLive(), std::true_type());
+    auto rhs_of = [&](int32_t src) { return Split && src < 0 ? t[(~src) * ld + g.col] : from(src); };
+    wide_sweep<double, TW>(s, lp, t, ld, g, g.on, rhs_of, store, Identity(), AllLive(), std::true_type());
   }
 }
 
 // Phase 2: per tile, the top part forward (into t) and then backward (into x)
-template <int TW>
+template <int TW, bool Split>
 __global__ void __launch_bounds__(1024) top_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, ReachDev reach,
                                                    double* __restrict__ t, double* __restrict__ x, int64_t ld,
                                                    int32_t ncols, int32_t lpf_words, int32_t lpb_words) {
@@ -596,14 +599,175 @@ __global__ void __launch_bounds__(1024) top_kernel(SweepDev fwd, SweepDev bwd, R
     };
     wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, from_t, store_x, Identity(), AllLive(), std::true_type());
   } else {
-    wide_sweep<double, TW>(sf, lpf, t, ld, g, g.on, from, store_t, Identity(), AllLive(), std::true_type());
-    wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; }, store_x, Identity(),
-                           AllLive(), std::true_type());
+    auto rhs_t = [&](int32_t src) { return Split && src < 0 ? t[(~src) * ld + col] : from(src); };
+    auto rhs_x = [&](int32_t src) { return Split && src < 0 ? x[(~src) * ld + col] : t[src * ld + col]; };
+    wide_sweep<double, TW>(sf, lpf, t, ld, g, g.on, rhs_t, store_t, Identity(), AllLive(), std::true_type());
+    wide_sweep<double, TW>(sb, lpb, x, ld, g, g.on, rhs_x, store_x, Identity(), AllLive(), std::true_type());
+  }
+}
+
+// Phase 2 with the top in LDS (TopSlotsDev): the top is a few hundred rows over tens of levels,
+// each level a handful of segments, so a level's cost is its latency. Here every value a top
+// segment reads is a slot in LDS. The workgroup first loads the top rows' right-hand sides, the
+// halo (the parts' rows the forward top reads, final after phase 1) and the metadata image, every
+// thread issuing all its loads before its stores. Then each column of the tile is swept by one
+// wavefront alone (TW columns over min(TW, waves) wavefronts), so the levels need no workgroup
+// barrier: LDS operations of one wavefront complete in order, and a wavefront fence keeps the
+// compiler from moving a level's reads above the last one's writes. The forward top sweeps the
+// slots, the backward top sweeps them again in place (a backward row starts from its own forward
+// value and reads only rows it follows, whose slots already hold their solutions), and the
+// workgroup stores the solutions. Each segment subtracts its terms in the sweep's order and
+// divides as the sweep does, so the bits are those of top_kernel.
+struct TopStage {
+  int32_t levels, segs;
+  const TopRecord* rec;
+  const double* diag;
+  const int32_t* lp;
+  const double* ov_val;
+  const int32_t* ov_dep;
+};
+
+__device__ __forceinline__ TopStage top_stage(const TopPartDev& p, const unsigned char* meta,
+                                              const TopLdsLayout::Part& at) {
+  return TopStage{p.levels, p.segs,
+                  reinterpret_cast<const TopRecord*>(meta + at.rec),
+                  reinterpret_cast<const double*>(meta + at.diag),
+                  reinterpret_cast<const int32_t*>(meta + at.lp),
+                  reinterpret_cast<const double*>(meta + at.ov_val),
+                  reinterpret_cast<const int32_t*>(meta + at.ov_dep)};
+}
+
+__device__ __forceinline__ void wavefront_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A segment's record and diagonal, as registers
+struct TopSeg {
+  TopRecord r;
+  double diag;
+};
+__device__ __forceinline__ TopSeg top_seg(const TopStage& st, int32_t k) {
+  k = k < st.segs ? k : st.segs - 1;
+  return TopSeg{st.rec[k], st.diag[k]};
+}
+
+// One segment of column `col`: its slot's value minus its terms in order, over its diagonal
+template <int TW>
+__device__ __forceinline__ void top_segment(const TopStage& st, const TopSeg& g, bool unit, double* v, int32_t col) {
+  const int32_t slot = g.r.head.x, terms = g.r.head.y;
+  const double x0 = v[g.r.dep.x * TW + col], x1 = v[g.r.dep.y * TW + col], x2 = v[g.r.dep.z * TW + col],
+               x3 = v[g.r.dep.w * TW + col];
+  double a = v[slot * TW + col];
+  if (terms > 0) a = a - g.r.v01.x * x0;
+  if (terms > 1) a = a - g.r.v01.y * x1;
+  if (terms > 2) a = a - g.r.v23.x * x2;
+  if (terms > 3) a = a - g.r.v23.y * x3;
+  for (int32_t q = g.r.head.z, end = g.r.head.z + terms - RECORD_ENTRIES; q < end; ++q)
+    a = a - st.ov_val[q] * v[st.ov_dep[q] * TW + col];
+  v[slot * TW + col] = unit ? a : a / g.diag;
+}
+
+// One sweep's top over the slots of column `col` of the tile (TW columns side by side): this
+// lane takes segments first, first + step, ... of each level. A level's first record is read
+// before the ordering point that ends the level before it (records do not change), so after
+// that point a level waits only for the values it reads.
+template <int TW>
+__device__ void top_slots_sweep(const TopStage& st, bool unit, double* v, int32_t col, int32_t first, int32_t step) {
+  if (st.levels == 0 || st.segs == 0) return;
+  int32_t lo = st.lp[0], hi = st.lp[1];
+  TopSeg g = top_seg(st, lo + first < hi ? lo + first : lo);
+  for (int32_t l = 0; l < st.levels; ++l) {
+    const int32_t next_hi = st.lp[l + 2 <= st.levels ? l + 2 : st.levels];
+    for (int32_t k = lo + first; k < hi;) {
+      top_segment<TW>(st, g, unit, v, col);
+      k += step;
+      if (k < hi) g = top_seg(st, k);
+    }
+    lo = hi;
+    hi = next_hi;
+    if (l + 1 < st.levels) g = top_seg(st, lo + first < hi ? lo + first : lo);
+    wavefront_order();
+  }
+}
+
+template <int TW>
+__global__ void __launch_bounds__(1024) top_lds_kernel(SweepDev fwd, SweepDev bwd, TopSlotsDev top, RhsDev rhs,
+                                                       const double* __restrict__ t, double* __restrict__ x,
+                                                       int64_t ld, int32_t ncols) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int U = 8;  // loads in flight per thread
+  const TopLdsLayout o = top_lds_layout(top, TW);
+  double* v = reinterpret_cast<double*>(smem + o.values);
+  unsigned char* meta = smem + o.meta;
+  const int32_t tid = threadIdx.x, threads = blockDim.x;
+  const int32_t col0 = (int32_t)blockIdx.x * TW;
+  // the slots' values: item i is slot i / TW of column i % TW
+  {
+    const int32_t items = (top.rows + top.halo) * TW;
+    for (int32_t base = 0; base < items; base += U * threads) {
+      double r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t i = base + u * threads + tid;
+        r[u] = 0.0;
+        if (i < items) {
+          const int32_t sl = i / TW, col = col0 + i % TW;
+          if (col < ncols) {
+            if (sl < top.rows) {
+              const ForwardRhs from = forward_rhs(rhs, ld, col, true);
+              r[u] = from(top.rhs_row[sl]);
+            } else {
+              r[u] = t[top.halo_row[sl - top.rows] * ld + col];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t i = base + u * threads + tid;
+        if (i < items) v[i] = r[u];
+      }
+    }
+  }
+  // the metadata image
+  {
+    int4* dst = reinterpret_cast<int4*>(meta);
+    for (int32_t base = 0; base < top.meta_units; base += U * threads) {
+      int4 r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t i = base + u * threads + tid;
+        r[u] = i < top.meta_units ? top.meta[i] : make_int4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t i = base + u * threads + tid;
+        if (i < top.meta_units) dst[i] = r[u];
+      }
+    }
+  }
+  __syncthreads();
+  // wavefront w < sweepers takes columns w * per .. (w + 1) * per - 1 of the tile
+  const int32_t wave = tid / 64, lane = tid % 64;
+  int32_t sweepers = 1;
+  while (sweepers * 2 <= TW && sweepers * 2 <= threads / 64) sweepers *= 2;
+  const int32_t per = TW / sweepers;
+  if (wave < sweepers) {
+    const int32_t col = wave * per + lane % per, first = lane / per, step = 64 / per;
+    top_slots_sweep<TW>(top_stage(top.fwd, meta, o.fwd), fwd.unit, v, col, first, step);
+    top_slots_sweep<TW>(top_stage(top.bwd, meta, o.bwd), bwd.unit, v, col, first, step);
+  }
+  __syncthreads();
+  for (int32_t i = tid; i < top.rows * TW; i += threads) {
+    const int32_t col = col0 + i % TW;
+    if (col < ncols) x[top.out_row[i / TW] * ld + col] = v[i];
   }
 }
 
 // Phase 3: workgroup (tile, part) sweeps the part's rows backward from t into x
-template <int TW>
+template <int TW, bool Split>
 __global__ void __launch_bounds__(1024) parts_backward_kernel(SweepDev bwd, ReachDev reach, const double* __restrict__ t,
                                                               double* __restrict__ x, int64_t ld, int32_t ncols,
                                                               int32_t lp_words) {
@@ -625,8 +789,8 @@ __global__ void __launch_bounds__(1024) parts_backward_kernel(SweepDev bwd, Reac
     };
     wide_sweep<double, TW>(s, lp, x, ld, g, g.on, from_t, store_x, Identity(), AllLive(), std::true_type());
   } else {
-    wide_sweep<double, TW>(s, lp, x, ld, g, g.on, [&](int32_t src) { return t[src * ld + col]; }, store_x, Identity(),
-                           AllLive(), std::true_type());
+    auto rhs_x = [&](int32_t src) { return Split && src < 0 ? x[(~src) * ld + col] : t[src * ld + col]; };
+    wide_sweep<double, TW>(s, lp, x, ld, g, g.on, rhs_x, store_x, Identity(), AllLive(), std::true_type());
   }
 }
 
@@ -1536,11 +1700,21 @@ hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const Sweep
   return hipGetLastError();
 }
 
-hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
-                                const PartLevels& levels, const RhsDev& rhs, const ReachDev& reach, double* t,
-                                double* x, int64_t ld, int32_t ncols) {
-  if (threads < tile || threads > 1024 || threads % 64 || fwd.parts < 1 || bwd.parts < 1) return hipErrorInvalidValue;
-  const unsigned tiles = (unsigned)((ncols + tile - 1) / tile);
+hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, int top_tile, int top_threads,
+                                const SweepDev& fwd, const SweepDev& bwd, const PartLevels& levels, const RhsDev& rhs,
+                                const ReachDev& reach, const TopSlotsDev* top_lds, double* t, double* x, int64_t ld,
+                                int32_t ncols) {
+  auto valid = [](int tw, int th) {
+    return tw >= 1 && tw <= 64 && (tw & (tw - 1)) == 0 && th >= tw && th <= 1024 && th % 64 == 0;
+  };
+  if (!valid(tile, threads) || !valid(top_tile, top_threads) || fwd.parts < 1 || bwd.parts < 1)
+    return hipErrorInvalidValue;
+  // split sweeps come with blocks of vectors only (a pair block's reach is over the unsplit
+  // ones), and a pair block's reach bitmaps are per tile, so its top kernel keeps the parts' tile
+  const bool split = fwd.split || bwd.split;
+  if (reach.lp && (split || top_tile != tile || top_lds)) return hipErrorInvalidValue;
+  if (top_lds && top_lds->rows < 1) return hipErrorInvalidValue;
+  const unsigned tiles = (unsigned)((ncols + tile - 1) / tile), top_tiles = (unsigned)((ncols + top_tile - 1) / top_tile);
   // LDS: level pointers rounded to 4 words (so the bitmap after them stays 16-byte aligned), and
   // the tile's reach bitmap for a pair block
   auto words = [](int32_t levels) { return (levels + 1 + 3) / 4 * 4; };
@@ -1549,25 +1723,53 @@ hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, const
                 wp = words(levels.bwd_parts);
   const size_t b1 = wf * sizeof(int32_t) + live, b2 = (size_t)(wt + wb) * sizeof(int32_t) + live,
                b3 = wp * sizeof(int32_t) + live;
-#define L(TWV)                                                                                                  \
+#define LAUNCH(KERNEL, TWV, SPLIT, BYTES, GRID, THREADS, ...)                                                    \
   {                                                                                                             \
-    hipError_t e = hipFuncSetAttribute((const void*)parts_forward_kernel<TWV>,                                  \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)b1);                    \
-    if (e == hipSuccess)                                                                                        \
-      e = hipFuncSetAttribute((const void*)top_kernel<TWV>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b2); \
-    if (e == hipSuccess)                                                                                        \
-      e = hipFuncSetAttribute((const void*)parts_backward_kernel<TWV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                              (int)b3);                                                                         \
+    const hipError_t e = hipFuncSetAttribute((const void*)KERNEL<TWV, SPLIT>,                                    \
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)(BYTES));         \
     if (e != hipSuccess) return e;                                                                              \
-    hipLaunchKernelGGL((parts_forward_kernel<TWV>), dim3(tiles, (unsigned)fwd.parts), dim3(threads), b1, stream, fwd, \
-                       rhs, reach, t, ld, ncols, wf);                                                           \
-    hipLaunchKernelGGL((top_kernel<TWV>), dim3(tiles), dim3(threads), b2, stream, fwd, bwd, rhs, reach, t, x, ld,  \
-                       ncols, wt, wb);                                                                          \
-    hipLaunchKernelGGL((parts_backward_kernel<TWV>), dim3(tiles, (unsigned)bwd.parts), dim3(threads), b3, stream,  \
-                       bwd, reach, t, x, ld, ncols, wp);                                                        \
+    hipLaunchKernelGGL((KERNEL<TWV, SPLIT>), GRID, dim3(THREADS), BYTES, stream, __VA_ARGS__);                   \
   }
-  DISPATCH_TILE(tile, L)
-#undef L
+#define LAUNCH_LDS(TWV)                                                                                          \
+  {                                                                                                             \
+    const size_t bytes = top_lds_layout(*top_lds, TWV).total;                                                   \
+    const hipError_t e = hipFuncSetAttribute((const void*)top_lds_kernel<TWV>,                                  \
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);           \
+    if (e != hipSuccess) return e;                                                                              \
+    hipLaunchKernelGGL((top_lds_kernel<TWV>), dim3(top_tiles), dim3(top_threads), bytes, stream, fwd, bwd,       \
+                       *top_lds, rhs, t, x, ld, ncols);                                                         \
+  }
+#define FWD(TWV)                                                                                                 \
+  if (split)                                                                                                    \
+    LAUNCH(parts_forward_kernel, TWV, true, b1, dim3(tiles, (unsigned)fwd.parts), threads, fwd, rhs, reach, t,  \
+           ld, ncols, wf)                                                                                       \
+  else                                                                                                          \
+    LAUNCH(parts_forward_kernel, TWV, false, b1, dim3(tiles, (unsigned)fwd.parts), threads, fwd, rhs, reach, t, \
+           ld, ncols, wf)
+#define TOP(TWV)                                                                                                 \
+  if (top_lds)                                                                                                  \
+    LAUNCH_LDS(TWV)                                                                                             \
+  else if (split)                                                                                               \
+    LAUNCH(top_kernel, TWV, true, b2, dim3(top_tiles), top_threads, fwd, bwd, rhs, reach, t, x, ld, ncols, wt,  \
+           wb)                                                                                                  \
+  else                                                                                                          \
+    LAUNCH(top_kernel, TWV, false, b2, dim3(top_tiles), top_threads, fwd, bwd, rhs, reach, t, x, ld, ncols, wt, \
+           wb)
+#define BWD(TWV)                                                                                                 \
+  if (split)                                                                                                    \
+    LAUNCH(parts_backward_kernel, TWV, true, b3, dim3(tiles, (unsigned)bwd.parts), threads, bwd, reach, t, x,   \
+           ld, ncols, wp)                                                                                       \
+  else                                                                                                          \
+    LAUNCH(parts_backward_kernel, TWV, false, b3, dim3(tiles, (unsigned)bwd.parts), threads, bwd, reach, t, x,  \
+           ld, ncols, wp)
+  DISPATCH_TILE(tile, FWD)
+  DISPATCH_TILE(top_tile, TOP)
+  DISPATCH_TILE(tile, BWD)
+#undef BWD
+#undef TOP
+#undef FWD
+#undef LAUNCH_LDS
+#undef LAUNCH
   return hipGetLastError();
 }
 

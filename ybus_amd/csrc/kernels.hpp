@@ -155,13 +155,77 @@ hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepD
 // per tile the top part forward and backward, then every (tile, part) of the backward parts.
 // Each row keeps its own terms and their order, so the bits are those of launch_solve64. With
 // reach.lp set the block is pairs and each tile sweeps only its reach forward, as
-// launch_solve64_reach does (reach.lp over fwd's levels, reach.live the tiles' bitmaps).
+// launch_solve64_reach does (reach.lp over fwd's levels, reach.live the tiles' bitmaps). The top
+// kernel takes a configuration of its own (top_tile columns a workgroup, top_threads): its few
+// long levels want more, narrower workgroups than the parts. A pair block keeps top_tile = tile.
 struct PartLevels {
   int32_t fwd_parts = 0, fwd_top = 0, bwd_top = 0, bwd_parts = 0;  // the most levels of one part, and the top's
 };
-hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
-                                const PartLevels& levels, const RhsDev& rhs, const ReachDev& reach, double* t,
-                                double* x, int64_t ld, int32_t ncols);
+
+// The top parts of the two sweeps held in LDS (the host analysis's top_slots): slots for the top
+// rows, then a halo slot for each row of the parts a forward top segment reads. The top's
+// metadata is one image in device memory, laid out as top_lds_layout places it after the slots'
+// values, so a workgroup stages it with one coalesced copy. Per sweep: a 64-byte record per
+// segment (TopRecord: its slot, its term count, where its terms past the fourth start in the
+// overflow arrays, its first four terms' slots and values), the segments' diagonals, the level
+// starts, and the overflow terms' values and slots. (Only Cholesky factors are partitioned, and
+// their values are fixed at upload, so the image keeps copies.)
+struct TopRecord {
+  int4 head;   // slot, terms, first overflow term, 0
+  int4 dep;    // the first four terms' slots (unused ones: the segment's own slot)
+  double2 v01, v23;  // their values (unused ones: 0)
+};
+struct TopPartDev {
+  int32_t levels = 0, segs = 0, overflow = 0;
+};
+struct TopSlotsDev {
+  int32_t rows = 0, halo = 0;
+  const int32_t* rhs_row = nullptr;   // rows: each top row's forward right-hand side row
+  const int32_t* halo_row = nullptr;  // halo: the forward block's row each halo slot holds
+  const int32_t* out_row = nullptr;   // rows: each top row's solution row
+  const int4* meta = nullptr;         // the metadata image, meta_units 16-byte units
+  int32_t meta_units = 0;
+  TopPartDev fwd, bwd;
+};
+// Byte offsets in LDS for a tile of `tile` columns: the slots' values (slot-major, the tile's
+// columns side by side), then the metadata image (its arrays' offsets from its start), each
+// array 16-byte aligned
+struct TopLdsLayout {
+  size_t values = 0, meta = 0, meta_bytes = 0, total = 0;
+  struct Part {
+    size_t rec = 0, diag = 0, lp = 0, ov_val = 0, ov_dep = 0;
+  } fwd, bwd;
+};
+__host__ __device__ inline TopLdsLayout top_lds_layout(const TopSlotsDev& top, int tile) {
+  TopLdsLayout o;
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    const size_t here = at;
+    at += (bytes + 15) / 16 * 16;
+    return here;
+  };
+  auto part = [&](const TopPartDev& p, TopLdsLayout::Part& q) {
+    q.rec = take((size_t)p.segs * sizeof(TopRecord));
+    q.diag = take((size_t)p.segs * sizeof(double));
+    q.lp = take((size_t)(p.levels + 1) * sizeof(int32_t));
+    q.ov_val = take((size_t)p.overflow * sizeof(double));
+    q.ov_dep = take((size_t)p.overflow * sizeof(int32_t));
+  };
+  part(top.fwd, o.fwd);
+  part(top.bwd, o.bwd);
+  o.meta_bytes = at;
+  o.values = 0;
+  o.meta = ((size_t)(top.rows + top.halo) * (size_t)tile * sizeof(double) + 15) / 16 * 16;
+  o.total = o.meta + o.meta_bytes;
+  return o;
+}
+
+// top_lds: the top kernel over TopSlotsDev in LDS (null: the top kernel over the block in global
+// memory). Blocks of vectors, and pair blocks without a reach, only.
+hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, int top_tile, int top_threads,
+                                const SweepDev& fwd, const SweepDev& bwd, const PartLevels& levels, const RhsDev& rhs,
+                                const ReachDev& reach, const TopSlotsDev* top_lds, double* t, double* x, int64_t ld,
+                                int32_t ncols);
 
 // Complex FP64 solve of a block: forward sweep b -> t, backward t -> x. A complex block holds
 // unknown i's real parts in row 2i and its imaginary parts in row 2i + 1 (ld doubles a row).

@@ -285,6 +285,72 @@ struct DevSweep {
   size_t entries() const { return host.dep.size(); }
 };
 
+// The top parts of a partitioned pair of sweeps renumbered into LDS slots (top_slots), with the
+// metadata image the LDS top kernel copies (laid out by top_lds_layout)
+struct DevTopSlots {
+  DevBuf<int32_t> rhs_row, halo_row, out_row;
+  DevBuf<int4> meta;
+  TopSlotsDev dev;
+  void upload(const TopSlots& h, const Sweep& f, const Sweep& b, hipStream_t stream) {
+    TopSlotsDev d;
+    d.rows = h.rows;
+    d.halo = h.halo;
+    // terms past a segment's fourth go to the overflow arrays
+    auto overflow = [](const TopSlots::Part& p, const Sweep& s) {
+      int32_t m = 0;
+      for (int32_t k = 0; k < p.segs; ++k)
+        m += std::max(0, s.row_ptr[p.seg0 + k + 1] - s.row_ptr[p.seg0 + k] - RECORD_ENTRIES);
+      return m;
+    };
+    d.fwd = TopPartDev{h.fwd.levels, h.fwd.segs, overflow(h.fwd, f)};
+    d.bwd = TopPartDev{h.bwd.levels, h.bwd.segs, overflow(h.bwd, b)};
+    const TopLdsLayout o = top_lds_layout(d, 1);
+    std::vector<int4> image((o.meta_bytes + 15) / 16, make_int4(0, 0, 0, 0));
+    unsigned char* base = reinterpret_cast<unsigned char*>(image.data());
+    auto fill = [&](const TopSlots::Part& p, const Sweep& s, const TopLdsLayout::Part& at) {
+      TopRecord* rec = reinterpret_cast<TopRecord*>(base + at.rec);
+      double* diag = reinterpret_cast<double*>(base + at.diag);
+      int32_t* lp = reinterpret_cast<int32_t*>(base + at.lp);
+      double* ov_val = reinterpret_cast<double*>(base + at.ov_val);
+      int32_t* ov_dep = reinterpret_cast<int32_t*>(base + at.ov_dep);
+      int32_t ov = 0;
+      for (int32_t k = 0; k < p.segs; ++k) {
+        const int32_t q0 = s.row_ptr[p.seg0 + k] - p.term0, q1 = s.row_ptr[p.seg0 + k + 1] - p.term0;
+        int32_t dep[RECORD_ENTRIES];
+        double val[RECORD_ENTRIES];
+        for (int e = 0; e < RECORD_ENTRIES; ++e) {
+          const bool inside = q0 + e < q1;
+          dep[e] = inside ? p.dep[q0 + e] : p.slot[k];
+          val[e] = inside ? s.val[p.term0 + q0 + e] : 0.0;
+        }
+        rec[k].head = make_int4(p.slot[k], q1 - q0, ov, 0);
+        rec[k].dep = make_int4(dep[0], dep[1], dep[2], dep[3]);
+        rec[k].v01 = make_double2(val[0], val[1]);
+        rec[k].v23 = make_double2(val[2], val[3]);
+        for (int32_t q = q0 + RECORD_ENTRIES; q < q1; ++q, ++ov) {
+          ov_val[ov] = s.val[p.term0 + q];
+          ov_dep[ov] = p.dep[q];
+        }
+        diag[k] = s.unit ? 1.0 : s.diag[p.seg0 + k];
+      }
+      for (int32_t l = 0; l <= p.levels; ++l) lp[l] = s.level_ptr[p.level0 + l] - p.seg0;
+    };
+    fill(h.fwd, f, o.fwd);
+    fill(h.bwd, b, o.bwd);
+    rhs_row.upload(h.rhs_row, stream);
+    halo_row.upload(h.halo_row, stream);
+    out_row.upload(h.out_row, stream);
+    meta.upload(image, stream);
+    check(hipStreamSynchronize(stream), "upload top slots");
+    d.rhs_row = rhs_row.p;
+    d.halo_row = halo_row.p;
+    d.out_row = out_row.p;
+    d.meta = meta.p;
+    d.meta_units = (int32_t)image.size();
+    dev = d;
+  }
+};
+
 // A complex factor's sweep on the device: level pointers, the rows' CSR arrays, and the values
 // as (re, im) pairs
 struct DevSweepC {
@@ -531,6 +597,13 @@ struct ybus_hip_factor {
   bool force_parts = false;  // YBUS_HIP_PARTS named a part count: untuned blocks take the parts
   DevSweep pfwd, pbwd;
   PartLevels part_levels;
+  // the partitioned sweeps with long rows split within their parts (blocks of vectors)
+  bool parts_split = false;
+  DevSweep spfwd, spbwd;
+  // the top parts of the sweeps blocks of vectors take, in LDS slots (rows 0: none);
+  // YBUS_HIP_TOP_LDS=0 keeps the top kernel over the block in global memory
+  DevTopSlots top_slots;
+  bool use_top_lds = true;
   // column path: schedules in factor order, and each unknown's factor row for the right-hand
   // side (rhs_row) and the solution (sol_of), for the solve and the transposed solve
   DevSweep cfwd, cbwd, ctfwd, ctbwd;
@@ -575,12 +648,15 @@ struct ybus_hip_factor {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tile_override = 0, threads_wide = 512, threads_columns = 512;
   bool env_threads = false;  // YBUS_HIP_THREADS set: it overrides the tuned workgroup size
+  // YBUS_HIP_TOP=tile[:threads]: the partitioned top kernel's configuration for blocks of vectors
+  int top_override = 0, top_override_threads = 0;
   // kernel configurations chosen by ybus_hip_factor_autotune, per block width (keyed by the
   // power of two at or above it)
   struct Tuned {
     bool columns = false;
     int threads = 512, tile = 0;
     int parts = -1;  // the partitioned sweeps: 1 on, 0 off, -1 by the default rule
+    int top_tile = 0, top_threads = 0;  // the partitioned top kernel's own (0: the parts')
   };
   std::map<int, Tuned> tuned;
   const Tuned* trial = nullptr;  // a configuration being timed by the autotuner
@@ -823,8 +899,24 @@ struct ybus_hip_factor {
         ReachDev reach{};
         if (pairs && reach_tile == tile && reach_parts)
           reach = ReachDev{reach_pos.p, reach_lp.p, reach_live.p, (int32_t)(n / 32 + 1)};
-        check(launch_solve64_parts(stream(), tile, wide_threads(ncols, pairs), pfwd.view(), pbwd.view(), part_levels,
-                                   rhs, reach, t.p, x.p, layout.ld, (int32_t)ncols),
+        const bool sp = parts_split && !pairs;
+        const int threads = wide_threads(ncols, pairs);
+        int top_tile = tile, top_threads = threads;
+        if (!pairs && top_override) {
+          top_tile = top_override;
+          if (top_override_threads) top_threads = top_override_threads;
+        } else if (const Tuned* c = tuned_for(ncols, pairs)) {
+          if (!pairs && c->top_tile && !tile_override && !env_threads) {
+            top_tile = c->top_tile;
+            top_threads = c->top_threads;
+          }
+        }
+        // blocks of vectors: the top in LDS where its slots fit a workgroup's LDS at the top's tile
+        const bool lds_top = !pairs && use_top_lds && top_slots.dev.rows > 0 &&
+                             top_lds_layout(top_slots.dev, top_tile).total <= lds_cap;
+        check(launch_solve64_parts(stream(), tile, threads, top_tile, top_threads, (sp ? spfwd : pfwd).view(),
+                                   (sp ? spbwd : pbwd).view(), part_levels, rhs, reach,
+                                   lds_top ? &top_slots.dev : nullptr, t.p, x.p, layout.ld, (int32_t)ncols),
               "solve kernel");
       } else if (pairs && !transpose && reach_tile == tile && !reach_parts) {
         check(launch_solve64_reach(stream(), tile, wide_threads(ncols, pairs), f.view(), bw.view(), rhs,
@@ -1010,6 +1102,11 @@ static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   }
   const char* tile = std::getenv("YBUS_HIP_TILE");
   if (tile) f->tile_override = std::atoi(tile);
+  if (const char* lds = std::getenv("YBUS_HIP_TOP_LDS")) f->use_top_lds = std::string(lds) != "0";
+  if (const char* top = std::getenv("YBUS_HIP_TOP")) {
+    f->top_override = std::atoi(top);
+    if (const char* colon = std::strchr(top, ':')) f->top_override_threads = std::atoi(colon + 1);
+  }
   const char* reach = std::getenv("YBUS_HIP_REACH");
   if (reach && std::string(reach) == "0") f->use_reach = false;
   const char* path = std::getenv("YBUS_HIP_PATH");
@@ -1171,6 +1268,22 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
     f->bwd.upload(std::move(wb), ctx->stream, true);
     if (parts >= 2) {
       if (pf.parts && pb.parts) {
+        // the same parts with their long rows split within each part (blocks of vectors take
+        // these; pair blocks keep the rows whole, as their reach is found over them)
+        if (f->split_cap > 0) {
+          Sweep spf, spb;
+          run_together({[&] { spf = split_rows(pf, f->split_cap); }, [&] { spb = split_rows(pb, f->split_cap); }});
+          if (spf.positions() > spf.n || spb.positions() > spb.n) {
+            f->spfwd.upload(std::move(spf), ctx->stream, true);
+            f->spbwd.upload(std::move(spb), ctx->stream, true);
+            f->parts_split = true;
+          }
+        }
+        // the top of the sweeps blocks of vectors take, renumbered into LDS slots
+        const Sweep& tf = f->parts_split ? f->spfwd.host : pf;
+        const Sweep& tb = f->parts_split ? f->spbwd.host : pb;
+        const TopSlots ts = top_slots(tf, tb);
+        if (ts.rows > 0) f->top_slots.upload(ts, tf, tb, ctx->stream);
         f->pfwd.upload(std::move(pf), ctx->stream, true);
         f->pbwd.upload(std::move(pb), ctx->stream, true);
         f->part_levels = PartLevels{f->pfwd.most_part_levels(), f->pfwd.top_levels(), f->pbwd.top_levels(),
@@ -2360,30 +2473,53 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
     check(hipEventCreate(&e0), "hipEventCreate");
     check(hipEventCreate(&e1), "hipEventCreate");
     const ybus_hip_kernel_stats saved = f->stats;
+    // the best of three timed solves after a warm one
+    auto time_config = [&](const ybus_hip_factor::Tuned& c) {
+      f->trial = &c;
+      if (pairs) {
+        stage_pairs(f, plus.data(), minus.data(), (uint64_t)k);
+      } else {
+        f->ensure_workspace(k, false);
+        upload_vectors(f, vectors.data(), k);
+      }
+      f->solve64(false, pairs != 0, k);  // warm
+      double ms = 0.0;
+      for (int rep = 0; rep < 3; ++rep) {
+        check(hipEventRecord(e0, f->stream()), "hipEventRecord");
+        f->solve64(false, pairs != 0, k);
+        check(hipEventRecord(e1, f->stream()), "hipEventRecord");
+        check(hipEventSynchronize(e1), "autotune");
+        float t = 0.0f;
+        check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+        ms = rep == 0 ? t : std::min(ms, (double)t);
+      }
+      return ms;
+    };
     try {
       for (const ybus_hip_factor::Tuned& c : candidates) {
-        f->trial = &c;
-        if (pairs) {
-          stage_pairs(f, plus.data(), minus.data(), (uint64_t)k);
-        } else {
-          f->ensure_workspace(k, false);
-          upload_vectors(f, vectors.data(), k);
-        }
-        f->solve64(false, pairs != 0, k);  // warm
-        double ms = 0.0;
-        for (int rep = 0; rep < 3; ++rep) {
-          check(hipEventRecord(e0, f->stream()), "hipEventRecord");
-          f->solve64(false, pairs != 0, k);
-          check(hipEventRecord(e1, f->stream()), "hipEventRecord");
-          check(hipEventSynchronize(e1), "autotune");
-          float t = 0.0f;
-          check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
-          ms = rep == 0 ? t : std::min(ms, (double)t);
-        }
+        const double ms = time_config(c);
         if (best_ms == 0.0 || ms < best_ms) {
           best = c;
           best_ms = ms;
         }
+      }
+      // a partitioned block of vectors: the top of the tree (few, long levels) on its own
+      // configuration, the parts' kept
+      if (!pairs && !best.columns && best.parts == 1) {
+        const ybus_hip_factor::Tuned parts_best = best;
+        for (int th : {64, 128, 256, 512, 1024})
+          for (int tile : {1, 2, 4, 8, 16, 32, 64}) {
+            if (th < tile || (tile != 1 && tile / 2 >= k) || (tile == parts_best.tile && th == parts_best.threads))
+              continue;
+            ybus_hip_factor::Tuned c = parts_best;
+            c.top_tile = tile;
+            c.top_threads = th;
+            const double ms = time_config(c);
+            if (ms < best_ms) {
+              best = c;
+              best_ms = ms;
+            }
+          }
       }
     } catch (...) {
       f->trial = nullptr;
@@ -2398,8 +2534,13 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
     f->tuned[key] = best;
     f->ensure_workspace(k, false);  // drops the synthetic staging
     if (out)
-      *out = ybus_hip_tuning{best.columns ? YBUS_HIP_PATH_COLUMNS : YBUS_HIP_PATH_WIDE, best.threads,
-                             best.columns ? 1 : best.tile, best_ms, best.columns ? 0 : std::max(best.parts, 0)};
+      *out = ybus_hip_tuning{best.columns ? YBUS_HIP_PATH_COLUMNS : YBUS_HIP_PATH_WIDE,
+                             best.threads,
+                             best.columns ? 1 : best.tile,
+                             best_ms,
+                             best.columns ? 0 : std::max(best.parts, 0),
+                             best.top_tile ? best.top_tile : (best.columns ? 0 : best.tile),
+                             best.top_tile ? best.top_threads : (best.columns ? 0 : best.threads)};
   });
 }
 
