@@ -584,6 +584,21 @@ def run_screening(gpu, args):
                             sample="the oracle's par_solve_many over the first 2,000 outages' pair vectors, then "
                                    "the update, flows and maxima in numpy, repeated for >= %g s" % CPU_SECONDS)
     n2 = run_double_outages(factor, op, chol, bf, theta, plus, minus, inverse_weight, weights, outages, args)
+    # the caller's contract over every branch, bridges included (dc/gpu.rs:386-430): an outage
+    # whose update is singular or whose angles or loadings fail the suspect rule goes to the
+    # caller's own solve; here that solve only counts them, untimed
+    from ybus_amd import screening
+
+    suspects = []
+    t0 = time.perf_counter()
+    screening.outage_loadings(factor, op, theta, plus, minus, np.zeros(grid.branches), inverse_weight,
+                              np.arange(grid.branches), suspects.append)
+    every_s = time.perf_counter() - t0
+    bridge_rows = set(np.flatnonzero(grid.bridges()).tolist())
+    contract = dict(branches=int(grid.branches), suspect_outages=len(suspects),
+                    suspects_are_the_bridges=set(suspects) == bridge_rows, screen_ms=every_s * 1e3)
+    log(f"[screening] every branch: {len(suspects)} suspect outages of {grid.branches} "
+        f"(the bridges: {contract['suspects_are_the_bridges']}), {every_s * 1e3:.1f} ms")
     out = dict(workload="N-1 screening: 25,000-bus grid, B' AMD Cholesky, every non-bridge branch out alone "
                         "(%d outages), rank-one update, Bf flows, most loaded branch" % len(outages),
                n=int(chol.n), nnz_l=int(chol.l.nnz), outages=int(len(outages)), tuning=tuning,
@@ -591,7 +606,7 @@ def run_screening(gpu, args):
                unit="outages/s", step_ms=step_s * 1e3, singular=int(sum(singular)),
                parity=dict(sampled_outages=int(len(sample)), max_rel_diff_loading=worst,
                            same_branch=f"{same_branch} of {len(sample)}"),
-               cpu_baseline=cpu_baseline, double_outages=n2)
+               cpu_baseline=cpu_baseline, double_outages=n2, suspect_contract=contract)
     log(f"[screening] {step_s * 1e3:.2f} ms per step ({out['value']:.3g} outages/s); loadings within {worst:.1e}, "
         f"same branch {same_branch} of {len(sample)}; cpu {cpu_baseline['value'] if cpu_baseline else 0:.3g} "
         f"outages/s")
