@@ -131,6 +131,37 @@ def flow_loss(bf_csr, x: np.ndarray, limit: np.ndarray, gradient: bool = True):
     return losses, x_bar
 
 
+def supernode_widths(n: int, ptr, idx):
+    """The fundamental supernodes of a lower triangular factor stored by columns (CSC, each
+    column's rows ascending, the diagonal first), as (widths, stored entries of each): column
+    j + 1 continues column j's supernode when j + 1 is j's parent in the elimination tree (the
+    first row below j's diagonal), j + 1 has no other child, and column j + 1's rows below its
+    diagonal are column j's rows below j + 1 (multifrontal.rs:94-138's rule, which Cholesky's
+    supernodes() restates; this form also reads an LU factor's L, or U's rows as the columns of
+    U^T)."""
+    ptr = np.asarray(ptr, dtype=np.int64)
+    idx = np.asarray(idx, dtype=np.int64)
+    parent = np.full(n, -1, dtype=np.int64)
+    below = [None] * n
+    for j in range(n):
+        rows = idx[ptr[j]:ptr[j + 1]]
+        rows = rows[rows > j]
+        below[j] = rows
+        if len(rows):
+            parent[j] = rows.min()
+    children = np.bincount(parent[parent >= 0], minlength=n)
+    starts = [0]
+    for j in range(n - 1):
+        a, b = below[j], below[j + 1]
+        joined = (parent[j] == j + 1 and children[j + 1] == 1 and len(a) == len(b) + 1
+                  and np.array_equal(a[1:], b))
+        if not joined:
+            starts.append(j + 1)
+    widths = np.diff(np.append(starts, n))
+    entries = np.add.reduceat(np.diff(ptr), starts) if n else np.zeros(0, dtype=np.int64)
+    return widths, entries
+
+
 def inverse_norm_1_estimate(n: int, solve, solve_transpose) -> float:
     """Hager's and Higham's estimate of ||A^-1||_1 from solves with A and A^T, keeping the
     largest estimate seen (accuracy.rs:207-279, LAPACK's xLACON): never more than the true value.

@@ -90,3 +90,34 @@ def test_the_configurations_are_the_ones_measured():
     case = configs.adjoint()
     assert (case.chol.n, int(case.chol.l.nnz), case.bprime.nnz, case.p.shape) == (69999, 195210, 258991,
                                                                                   (64, 69999))
+
+
+def test_supernode_widths_restate_the_cholesky_supernodes():
+    """configs.supernode_widths on L's columns gives the pinned supernodes() widths."""
+    for buses, seed in ((400, 3), (2000, 11)):
+        chol = O.Cholesky(O.make_grid(buses, seed).dc()[0])
+        csc = chol.l.to_csc()
+        widths, entries = configs.supernode_widths(chol.n, csc.ptr, csc.idx)
+        want = [end - start for start, end, _, _ in chol.supernodes()[0]]
+        assert widths.tolist() == want
+        assert int(entries.sum()) == int(chol.l.nnz)
+
+
+@pytest.mark.slow
+def test_the_factors_supernodes_are_too_narrow_for_dense_updates():
+    """The basis of the choice not to build supernodal MFMA updates: in every configuration's
+    factor the supernodes of eight or more columns hold at most 3.1 % of the stored entries (0 %
+    for the grids' Cholesky factors), and none is wider than 16 columns."""
+    def shares(n, ptr, idx):
+        widths, entries = configs.supernode_widths(n, ptr, idx)
+        return int(widths.max()), entries[widths >= 8].sum() / entries.sum()
+
+    _, _, chol, _, _ = configs.ptdf()
+    csc = chol.l.to_csc()
+    assert shares(chol.n, csc.ptr, csc.idx) == (6, 0.0)
+    _, lu, _ = configs.jacobian(count=1)
+    widest, share = shares(lu.n, lu.l.ptr, lu.l.idx)
+    assert widest == 12 and 0.025 < share < 0.027
+    ut = lu.u.to_csr()
+    widest, share = shares(lu.n, ut.ptr, ut.idx)
+    assert widest == 16 and 0.030 < share < 0.032
