@@ -293,6 +293,26 @@ def copy_calibration(mib: int):
     log(f"[copy] {mib} MiB copied 5 times")
 
 
+def gather_calibration(mib: int, row_bytes: int = 128):
+    """A gather of known size, five times: a table of `mib` MiB as rows of `row_bytes` bytes, every
+    row read once in a random order (torch.index_select over a seeded permutation) and written
+    in that order. It scales FETCH_SIZE for a kernel whose reads are scattered whole lines, as
+    the wide sweeps' dependency gathers are (TW 16 doubles: one 128-byte line per row and tile).
+    Reads: the table plus the 8-byte indices; writes: the table's bytes."""
+    import torch
+
+    width = row_bytes // 8
+    rows = mib * (1 << 20) // row_bytes
+    table = torch.ones((rows, width), dtype=torch.float64, device="cuda")
+    order = torch.randperm(rows, generator=torch.Generator().manual_seed(SEED)).to("cuda")
+    out = torch.empty_like(table)
+    for _ in range(5):
+        torch.index_select(table, 0, order, out=out)
+    torch.cuda.synchronize()
+    log(f"[gather] {rows} rows of {row_bytes} B gathered 5 times: reads {rows * row_bytes + rows * 8} B "
+        f"(table and indices), writes {rows * row_bytes} B")
+
+
 def run_adjoint(gpu, args):
     """The adjoint on the 70,000-bus grid: 64 zero-sum N(0,1) injection scenarios, branch limits
     U[0.5, 2.0] pu, loss = sum relu(|f| - limit)^2 with f = Bf theta, theta = B'^-1 p. The whole
@@ -888,6 +908,9 @@ def main():
     ap.add_argument("--no-refactor", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB on the device (to scale HBM counters) and exit")
+    ap.add_argument("--gather-calibration", type=int, default=0, metavar="MIB",
+                    help="only gather MIB MiB as random whole rows on the device (to scale HBM counters) and exit")
+    ap.add_argument("--gather-row-bytes", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the factor-solve path is measured on one GPU's share "
@@ -897,6 +920,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.gather_calibration:
+        gather_calibration(args.gather_calibration, args.gather_row_bytes)
         return
 
     # PyTorch ships its own HIP runtime: it must be loaded before libybus_hip.so, so that both use
