@@ -15,6 +15,7 @@
 // solves give ybus_sparse's bits exactly.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -30,14 +31,26 @@ struct Tile {
   int lane, slot, slots;
   int32_t col;
   bool on;
-  __device__ Tile(int32_t ncols) {
+  __device__ Tile(int32_t ncols, int32_t tile = (int32_t)blockIdx.x) {
     lane = (int)threadIdx.x % TW;
     slot = (int)threadIdx.x / TW;
     slots = (int)blockDim.x / TW;
-    col = (int32_t)blockIdx.x * TW + lane;
+    col = tile * TW + lane;
     on = col < ncols;
   }
 };
+
+// The tile that workgroup blockIdx.x of a one-dimensional grid takes. The GPU hands workgroups
+// to its XCDs in turn (workgroup i to XCD i mod xcds), and each XCD has an L2 of its own; tiles
+// narrower than a 128-byte line share that line in every row of the block, so runs of
+// consecutive tiles are given to the workgroups of one XCD (ceil(g / xcds) tiles to each of the
+// first g mod xcds XCDs, floor to the rest). A bijection on [0, g); xcds = 1 is the identity.
+__device__ __forceinline__ int32_t tile_of_workgroup(int32_t xcds) {
+  const int32_t b = (int32_t)blockIdx.x, g = (int32_t)gridDim.x;
+  if (xcds <= 1) return b;
+  const int32_t xcd = b % xcds, idx = b / xcds, q = g / xcds, r = g % xcds;
+  return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + idx;
+}
 
 // Bytes of an n-entry vector in LDS, rounded up to 16 so what follows it stays aligned
 __host__ __device__ __forceinline__ size_t lds_vector_bytes(int32_t n, size_t elem) {
@@ -312,13 +325,14 @@ __device__ void wide_sweep(const SweepDev& s, const int32_t* lp, const T* __rest
 // partial value in its row's own slot of the block it writes (t forward, x backward).
 template <int TW, bool Split>
 __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, double* __restrict__ t,
-                                                       double* __restrict__ x, int64_t ld, int32_t ncols) {
+                                                       double* __restrict__ x, int64_t ld, int32_t ncols,
+                                                       int32_t xcds) {
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* lpf = reinterpret_cast<int32_t*>(smem);
   int32_t* lpb = lpf + fwd.levels + 1;
   stage_levels(fwd, lpf);
   stage_levels(bwd, lpb);
-  const Tile<TW> g(ncols);
+  const Tile<TW> g(ncols, tile_of_workgroup(xcds));
   const int32_t col = g.col;
   const int32_t plus = !rhs.block && g.on ? rhs.plus[col] : -1;
   const int32_t minus = !rhs.block && g.on ? rhs.minus[col] : -1;
@@ -345,13 +359,14 @@ __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bw
 template <int TW, bool SkipLoads>
 __global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, SweepDev bwd, RhsDev rhs, ReachDev reach,
                                                              double* __restrict__ t, double* __restrict__ x, int64_t ld,
-                                                             int32_t ncols) {
+                                                             int32_t ncols, int32_t xcds) {
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* lpf = reinterpret_cast<int32_t*>(smem);
   int32_t* lpb = lpf + fwd.levels + 1;
   uint32_t* live = reinterpret_cast<uint32_t*>(lpb + bwd.levels + 1);
   const int32_t words = fwd.n / 32 + 1;  // a bit for every row and for the pad row (always clear)
-  const int32_t* __restrict__ tile_lp = reach.lp + (int64_t)blockIdx.x * (fwd.levels + 1);
+  const int32_t tile = tile_of_workgroup(xcds);
+  const int32_t* __restrict__ tile_lp = reach.lp + (int64_t)tile * (fwd.levels + 1);
   for (int32_t l = threadIdx.x; l <= fwd.levels; l += blockDim.x) lpf[l] = tile_lp[l];
   for (int32_t l = threadIdx.x; l <= bwd.levels; l += blockDim.x) lpb[l] = bwd.level_ptr[l];
   for (int32_t w = threadIdx.x; w < words; w += blockDim.x) live[w] = 0u;
@@ -361,7 +376,7 @@ __global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, Sweep
     atomicOr(&live[row >> 5], 1u << (row & 31));
   }
   __syncthreads();
-  const Tile<TW> g(ncols);
+  const Tile<TW> g(ncols, tile);
   const int32_t col = g.col;
   const int32_t plus = g.on ? rhs.plus[col] : -1;
   const int32_t minus = g.on ? rhs.minus[col] : -1;
@@ -465,14 +480,14 @@ template <int TW>
 __global__ void __launch_bounds__(1024) solve_complex_kernel(ComplexSweepDev fwd, ComplexSweepDev bwd,
                                                              const double* __restrict__ b, double* __restrict__ t,
                                                              double* __restrict__ x, int64_t ld, int32_t ncols,
-                                                             int32_t conjugate) {
+                                                             int32_t conjugate, int32_t xcds) {
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* lpf = reinterpret_cast<int32_t*>(smem);
   int32_t* lpb = lpf + fwd.levels + 1;
   for (int32_t l = threadIdx.x; l <= fwd.levels; l += blockDim.x) lpf[l] = fwd.level_ptr[l];
   for (int32_t l = threadIdx.x; l <= bwd.levels; l += blockDim.x) lpb[l] = bwd.level_ptr[l];
   __syncthreads();
-  const Tile<TW> g(ncols);
+  const Tile<TW> g(ncols, tile_of_workgroup(xcds));
   const int32_t col = g.col;
   complex_sweep<TW>(
       fwd, lpf, t, ld, g,
@@ -905,7 +920,7 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
                                                       double* __restrict__ x0, double* __restrict__ x1,
                                                       double* __restrict__ r, int64_t ld, int32_t ncols, double tol,
                                                       int32_t max_steps, int32_t* __restrict__ steps_out,
-                                                      double* __restrict__ ratio_out) {
+                                                      double* __restrict__ ratio_out, int32_t xcds) {
   __shared__ double scratch[1024];
   __shared__ double s_bnorm[TW], s_rnorm[TW], s_ratio[TW], s_best[TW];
   __shared__ int32_t s_best_buf[TW], s_steps[TW], s_active[TW];
@@ -914,7 +929,7 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
   int32_t* lpb = lpf + fwd.levels + 1;
   stage_levels(fwd, lpf);
   stage_levels(bwd, lpb);
-  const Tile<TW> g(ncols);
+  const Tile<TW> g(ncols, tile_of_workgroup(xcds));
 
   // ||b||inf per column
   double m = 0.0;
@@ -1662,6 +1677,17 @@ __global__ void scatter_negated_kernel(const double* __restrict__ acc, const int
 
 // ---- launchers ----
 
+// How many XCDs the one-dimensional wide launches spread runs of consecutive tiles over
+// (tile_of_workgroup): MI355X has 8. YBUS_HIP_XCDS=1 keeps workgroup i on tile i.
+static int32_t xcd_groups() {
+  static const int32_t groups = [] {
+    const char* e = std::getenv("YBUS_HIP_XCDS");
+    const int v = e ? std::atoi(e) : 8;
+    return (int32_t)(v >= 1 && v <= 64 ? v : 8);
+  }();
+  return groups;
+}
+
 #define DISPATCH_TILE(tile, F) \
   switch (tile) {              \
     case 1: F(1); break;       \
@@ -1686,7 +1712,7 @@ hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const Sweep
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
     if (e != hipSuccess) return e;                                                                               \
     hipLaunchKernelGGL((solve64_kernel<TWV, SPLIT>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, t, x, \
-                       ld, ncols);                                                                               \
+                       ld, ncols, xcd_groups());                                                                 \
   }
 #define L(TWV)         \
   if (split)           \
@@ -1789,7 +1815,7 @@ hipError_t launch_solve64_reach(hipStream_t stream, int tile, int threads, const
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
     if (e != hipSuccess) return e;                                                                              \
     hipLaunchKernelGGL((solve64_reach_kernel<TWV, SKIP>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, rhs, \
-                       reach, t, x, ld, ncols);                                                                 \
+                       reach, t, x, ld, ncols, xcd_groups());                                                   \
   }
 #define L(TWV)         \
   if (skip_loads)      \
@@ -1814,7 +1840,7 @@ hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepD
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
     if (e != hipSuccess) return e;                                                                               \
     hipLaunchKernelGGL((refine_kernel<TWV, SPLIT>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, d, \
-                       x0, x1, r, ld, ncols, tol, max_steps, steps, ratio);                                      \
+                       x0, x1, r, ld, ncols, tol, max_steps, steps, ratio, xcd_groups());                        \
   }
 #define L(TWV)         \
   if (split)           \
@@ -1840,7 +1866,7 @@ hipError_t launch_solve_complex(hipStream_t stream, int tile, int threads, const
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);              \
     if (e != hipSuccess) return e;                                                                           \
     hipLaunchKernelGGL((solve_complex_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, b, t, x, \
-                       ld, ncols, (int32_t)conjugate);                                                       \
+                       ld, ncols, (int32_t)conjugate, xcd_groups());                                         \
   }
   DISPATCH_TILE(tile, L)
 #undef L
