@@ -143,3 +143,22 @@ def test_autograd_recomputes_from_the_batch_size_up(gpu, monkeypatch):
         b.add_(1.0)
     with pytest.raises(RuntimeError, match="modified by an inplace operation"):
         x.sum().backward()
+
+
+def test_autograd_refuses_a_factor_changed_before_the_backward(gpu):
+    """A refactor between the passes bumps the factor's generation, and the backward raises
+    rather than pairing the forward's x with the new factor's transposed solve."""
+    from ybus_amd.autograd import solve
+
+    a = random_square_cuda(90, 4, 33)
+    lu = O.Lu(a, O.COLAMD)
+    factor = Y.GpuFactor.from_lu(gpu, lu)
+    factor.set_matrix(a)
+    start = factor.generation
+    b = cuda(right_hand_sides(90, 3, 4)).requires_grad_(True)
+    a_values = cuda(a.data).requires_grad_(True)
+    x = solve(factor, a_values, b)
+    assert factor.refactor(a.to_csc()) == "reused"
+    assert factor.generation == start + 1
+    with pytest.raises(RuntimeError, match="changed between the forward"):
+        x.sum().backward()

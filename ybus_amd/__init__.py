@@ -396,6 +396,14 @@ class GpuFactor:
         self._nnz_a = -1
         self._block: Optional[GpuBlock] = None
         self._lu_sizes = (0, 0)
+        self._generation = 0
+
+    @property
+    def generation(self) -> int:
+        """Counts the changes to the factor's values or its attached matrix (a refactor that
+        reused its pivots, update_lu_values, set_matrix): work that began before a change, such as
+        an autograd graph's forward solve, can tell that its backward would see other values."""
+        return self._generation
 
     @staticmethod
     def from_cholesky(gpu: Gpu, cholesky) -> "GpuFactor":
@@ -542,6 +550,8 @@ class GpuFactor:
         else:
             _check(library().ybus_hip_factor_refactor_lu(self._h, self._n, _u64(a.ptr), _u64(a.idx), _f64(a.data),
                                                          float(pivot_threshold), C.byref(outcome)))
+        if outcome.value == 0:
+            self._generation += 1
         return "reused" if outcome.value == 0 else "needs_pivoting"
 
     def update_lu_values(self, l_val, u_val):
@@ -552,6 +562,7 @@ class GpuFactor:
                                                                       _interleaved(u_val)))
         else:
             _check(library().ybus_hip_factor_update_lu_values(self._h, _f64(l_val), _f64(u_val)))
+        self._generation += 1
 
     def refactor_stats(self) -> dict:
         s = _RefactorStats()
@@ -572,6 +583,7 @@ class GpuFactor:
         _check(library().ybus_hip_factor_set_matrix(self._h, int(bool(a.row_major)), _u64(a.ptr), _u64(a.idx),
                                                     _f64(a.data)))
         self._nnz_a = int(np.asarray(a.ptr)[-1])
+        self._generation += 1
 
     def _release_block(self):
         if self._block is not None:

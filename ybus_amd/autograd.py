@@ -14,6 +14,12 @@ the passes: ``recompute=None`` does so from RECOMPUTE_VALUES values of x up (128
 True and False force either way. Every sum in the device solves runs in a fixed order, so the
 second solve gives the forward's bits and the gradients are the same either way.
 
+Two rules follow. Where the graph keeps b (recompute), an in-place change to b between the
+forward and the backward makes the backward raise (autograd's version check); where it keeps x,
+b is not needed again and may change. And the factor must not change between the passes: a
+refactor, update_lu_values or set_matrix in between would give a backward over other values
+than the forward's, so the backward raises instead (the factor's ``generation``).
+
 PyTorch ships its own HIP runtime: import torch before the first call that loads
 libybus_hip.so (creating a ybus_amd.Gpu does), so that the process has one HIP runtime.
 """
@@ -33,6 +39,7 @@ class _Solve(torch.autograd.Function):
         if recompute is None:
             recompute = x.numel() >= RECOMPUTE_VALUES
         ctx.factor = factor
+        ctx.generation = getattr(factor, "generation", 0)
         ctx.settings = (mode, tolerance, max_steps)
         ctx.recompute = recompute
         # b is the caller's own tensor (an in-place change to it after this call makes the
@@ -42,6 +49,9 @@ class _Solve(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, x_bar):
+        if getattr(ctx.factor, "generation", 0) != ctx.generation:
+            raise RuntimeError("the factor's values changed between the forward solve and its backward "
+                               "(a refactor, update_lu_values or set_matrix in between)")
         (kept,) = ctx.saved_tensors
         mode, tolerance, max_steps = ctx.settings
         if ctx.recompute:
@@ -58,5 +68,7 @@ def solve(factor, a_values: torch.Tensor, b: torch.Tensor, mode: int = FP64, tol
     """x = A^-1 b row by row, differentiable in A's stored values and in b (see the module). With
     mode=FP32_REFINED both the solve and the backward's transposed solve are FP32 sweeps refined in
     FP64 to `tolerance` (refine_normwise's rule); FP64 gives the CPU's bits. `recompute` chooses
-    whether the backward solves for x again rather than keeping it (None: for large batches)."""
+    whether the backward solves for x again rather than keeping it (None: from RECOMPUTE_VALUES
+    values of x up); then b must not be changed in place before the backward. The factor must not
+    change between the passes (the backward raises if it did)."""
     return _Solve.apply(factor, a_values, b, mode, tolerance, max_steps, recompute)
