@@ -242,6 +242,24 @@ int32_t ybus_hip_factor_solve_many_complex(ybus_hip_factor *f, const double *rhs
 int32_t ybus_hip_factor_solve_transpose_many_complex(ybus_hip_factor *f, const double *rhs,
                                                      uint64_t count, double *x);
 
+/* The complex matrix A of a complex factor (new), which FP32-refined complex solves need for their
+ * FP64 residual b - A x: rows x rows in CSR (row_major = 1) or CSC (row_major = 0) form, values as
+ * interleaved (re, im) doubles (Complex64). The refined solves take the residual as
+ * CsrMatrix::mul_vec does (product.rs:17-27), each row summed from zero in index order with
+ * num-complex's product. */
+int32_t ybus_hip_factor_set_matrix_complex(ybus_hip_factor *f, int32_t row_major, const uint64_t *ptr,
+                                           const uint64_t *idx, const double *val);
+
+/* Complex FP32 solves refined in FP64 (new; the reference's solve_refined_normwise over
+ * Complex64, accuracy.rs:83-122): the factor's values rounded to FP32 and swept with FP32
+ * arithmetic, the residual b - A x and the correction x + d in FP64, refine_normwise's stop rule
+ * with |z| the hypotenuse of z's parts (num-complex's norm); steps and ratio (either may be
+ * NULL) get each column's corrections and final relative residual. Right-hand sides and solutions
+ * as in ybus_hip_factor_solve_many_complex. Needs ybus_hip_factor_set_matrix_complex. */
+int32_t ybus_hip_factor_solve_many_refined_complex(ybus_hip_factor *f, const double *rhs, uint64_t count,
+                                                   double tolerance, uint64_t max_steps, double *x,
+                                                   uint64_t *steps, double *ratio);
+
 /* FP32 solves refined in FP64 (new): each right-hand side follows refine_normwise
  * (accuracy.rs:83-122): solve, then while ||b - A x||inf / ||b||inf > tolerance and at most
  * max_steps times, solve for the residual and add the correction, stopping early when a step
@@ -281,6 +299,14 @@ int32_t ybus_hip_factor_adjoint(ybus_hip_factor *f, int32_t mode, double toleran
 int32_t ybus_hip_factor_solve_device(ybus_hip_factor *f, int32_t mode, double tolerance,
                                      uint64_t max_steps, const double *b, uint64_t ld_b,
                                      uint64_t count, double *x, uint64_t ld_x);
+/* A complex factor's solves on device vectors (new): b and x hold `count` vectors of dim() complex
+ * values, interleaved (re, im) doubles, vector c at pointer + 2 c ld (ld counted in complex
+ * values, ld >= dim()). mode YBUS_HIP_FP64 and YBUS_HIP_TRANSPOSE give the bits of
+ * ybus_hip_factor_solve_many_complex / _solve_transpose_many_complex; YBUS_HIP_FP32_REFINED those of
+ * ybus_hip_factor_solve_many_refined_complex (to `tolerance` in at most max_steps corrections). */
+int32_t ybus_hip_factor_solve_device_complex(ybus_hip_factor *f, int32_t mode, double tolerance,
+                                             uint64_t max_steps, const double *b, uint64_t ld_b,
+                                             uint64_t count, double *x, uint64_t ld_x);
 /* ybus_hip_factor_solve_device in YBUS_HIP_FP64 or YBUS_HIP_TRANSPOSE that allocates no device
  * memory, on this or any later call. No counterpart in the reference: it is for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -91,9 +91,12 @@ def test_real_entry_points_reject_a_complex_factor_and_the_reverse(gpu, hermitia
     factor = Y.GpuFactor.from_cholesky_complex(gpu, chol)
     real_rhs = np.zeros((1, chol.n))
     with pytest.raises(Y.InvalidArgument):
-        factor.solve_many_refined(real_rhs)
+        factor.stage(real_rhs)
     with pytest.raises(Y.InvalidArgument):
         factor.pair_solves([Y.Pair(0, 1)])
+    # the complex refined solve needs the complex matrix first
+    with pytest.raises(Y.InvalidArgument, match="set_matrix_complex"):
+        factor.solve_many_refined(np.zeros((1, chol.n), complex))
     with pytest.raises(Y.InvalidArgument):
         Y.GpuBatch.new(gpu).add(factor)
     # a real factor refuses the complex solves
@@ -136,3 +139,134 @@ def test_one_unknown_and_a_pivoting_two_by_two(gpu):
     assert np.max(np.abs(a @ x.T - rhs.T)) <= 1e-14
     xt = lu.solve_transpose_many(rhs)
     assert np.max(np.abs(a.T @ xt.T - rhs.T)) <= 1e-14
+
+
+# ---- FP32 sweeps refined in FP64, and vectors in device memory ----
+
+
+def _sparse(a):
+    import scipy.sparse as sp
+
+    return a if sp.issparse(a) else sp.csr_matrix(a)
+
+
+def _column_ratios(a, x, b):
+    """Each column's ||b - A x||inf / ||b||inf (accuracy.rs:57-78), |z| the hypotenuse."""
+    r = b.T - _sparse(a) @ x.T
+    return np.max(np.abs(r), axis=0) / np.max(np.abs(b.T), axis=0)
+
+
+def _ordered_ratios(a, x, b):
+    """The same ratios with the residual formed as CsrMatrix::mul_vec forms it
+    (product.rs:17-27): each row summed from zero over its entries in ascending column order,
+    every complex product (re re - im im, re im + im re) and sum rounded on its own. That is the
+    device's order, so only the hypotenuses may round apart."""
+    m = _sparse(a).tocsr()
+    m.sort_indices()
+    ptr, idx, val = m.indptr, m.indices, m.data
+    lens = np.diff(ptr)
+    xr, xi = np.ascontiguousarray(x.real.T), np.ascontiguousarray(x.imag.T)
+    acc_r, acc_i = np.zeros_like(xr), np.zeros_like(xi)
+    for k in range(int(lens.max())):
+        rows = np.flatnonzero(lens > k)
+        at = ptr[rows] + k
+        vr, vi, c = val[at].real[:, None], val[at].imag[:, None], idx[at]
+        acc_r[rows] = acc_r[rows] + (vr * xr[c] - vi * xi[c])
+        acc_i[rows] = acc_i[rows] + (vr * xi[c] + vi * xr[c])
+    rn = np.max(np.hypot(b.real.T - acc_r, b.imag.T - acc_i), axis=0)
+    return rn / np.max(np.hypot(b.real.T, b.imag.T), axis=0)
+
+
+@pytest.mark.parametrize("case", ["cholesky", "lu"])
+def test_complex_refined_solves_reach_the_fp64_floor(gpu, hermitian, complex_lu, case):
+    """FP32 sweeps with FP64 residuals and corrections: every column reaches 1e-12 or, where no
+    FP64 solution can, twice the CPU FP64 solution's own residual; the reported ratio is the
+    residual of the returned x; and x is within a bound of the FP64 oracle's solution set by
+    the residuals (1e-9 relative here)."""
+    if case == "cholesky":
+        chol, a = hermitian
+        factor = Y.GpuFactor.from_cholesky_complex(gpu, chol)
+        cpu_solver = chol
+    else:
+        lu, a = complex_lu
+        factor = Y.GpuFactor.from_lu_complex(gpu, lu)
+        cpu_solver = lu
+    factor.set_matrix(_sparse(a).tocsc() if case == "lu" else _sparse(a))
+    b = complex_right_hand_sides(cpu_solver.n, 33, 17)
+    x, steps, ratio = factor.solve_many_refined(b, 1e-12, 10)
+    assert x.dtype == np.complex128 and x.shape == b.shape
+    cpu = cpu_solver.solve_many(b)
+    floor = _column_ratios(a, cpu, b)
+    got = _column_ratios(a, x, b)
+    assert np.all(ratio <= np.maximum(1e-12, 2.0 * floor))
+    # the reported ratio is the residual of the returned x, summed in the CPU's order (only the
+    # hypotenuses, libm's and the device's, may differ in their last bits)
+    ordered = _ordered_ratios(a, x, b)
+    assert np.max(np.abs(ratio - ordered) / ordered) <= 1e-14
+    assert np.all(got <= np.maximum(1e-12, 2.0 * floor) * 1.5)
+    assert np.all(steps >= 1) and np.all(steps <= 10)
+    scale = np.max(np.abs(cpu), axis=1)
+    assert np.max(np.max(np.abs(x - cpu), axis=1) / scale) <= 1e-9
+    # the first FP32 solve alone is far from that: the corrections did the work
+    x0, steps0, ratio0 = factor.solve_many_refined(b, 1e-12, 0)
+    assert np.all(steps0 == 0) and np.min(ratio0) > np.max(ratio)
+
+
+def test_complex_refined_solves_follow_a_refactor(gpu):
+    """A device refactor of a complex LU rewrites the FP32 copies of the sweeps too: the refined
+    solve then converges on the new matrix, to the dense solution of it."""
+    from tests.fixtures import random_square_cuda
+
+    class ComplexCsc:
+        def __init__(self, ptr, idx, data):
+            self.ptr, self.idx, self.data, self.row_major = ptr, idx, np.asarray(data, np.complex128), False
+
+    n, fill, seed = 120, 4, 11  # a case whose pivots pass the threshold (as in the refactor tests)
+    lu, a = complex_lu_case(n, fill, seed)
+    real = random_square_cuda(n, fill, seed)
+    cols = np.repeat(np.arange(n), np.diff(real.ptr))
+    moved = ComplexCsc(real.ptr, real.idx, a[real.idx, cols] * (1.0 + 0.01j))  # keeps every pivot
+    dense = np.zeros((n, n), complex)
+    dense[real.idx, cols] = moved.data
+    factor = Y.GpuFactor.from_lu_complex(gpu, lu)
+    assert factor.refactor(moved) == "reused"
+    factor.set_matrix(moved)
+    b = complex_right_hand_sides(n, 9, 3)
+    x, _, ratio = factor.solve_many_refined(b, 1e-12, 10)
+    assert np.all(ratio <= 1e-10)
+    assert np.max(_column_ratios(dense, x, b)) <= 1e-10
+    want = np.linalg.solve(dense, b.T).T
+    assert np.max(np.abs(x - want)) <= 1e-9 * np.max(np.abs(want))
+
+
+def test_complex_device_solves_match_the_host_solves(gpu, hermitian, complex_lu):
+    """complex128 tensors on the GPU: FP64 and transposed solves give the host calls' bits, with
+    contiguous and strided vectors; the refined solve gives the host refined solve's bits."""
+    import torch
+
+    for kind in ["cholesky", "lu"]:
+        if kind == "cholesky":
+            chol, a = hermitian
+            factor = Y.GpuFactor.from_cholesky_complex(gpu, chol)
+            n = chol.n
+        else:
+            lu, a = complex_lu
+            factor = Y.GpuFactor.from_lu_complex(gpu, lu)
+            n = lu.n
+        factor.set_matrix(_sparse(a))
+        b = complex_right_hand_sides(n, 21, 29)
+        tb = torch.tensor(b, dtype=torch.complex128, device="cuda")
+        assert factor.solve_device(tb).cpu().numpy().tobytes() == factor.solve_many(b).tobytes()
+        got_t = factor.solve_device(tb, transpose=True).cpu().numpy()
+        assert got_t.tobytes() == factor.solve_transpose_many(b).tobytes()
+        wide_in = torch.zeros((21, n + 5), dtype=torch.complex128, device="cuda")
+        wide_in[:, :n] = tb
+        wide_out = torch.full((21, n + 9), 7.0 + 1.0j, dtype=torch.complex128, device="cuda")
+        factor.solve_device(wide_in[:, :n], wide_out[:, :n])
+        assert wide_out[:, :n].cpu().numpy().tobytes() == factor.solve_many(b).tobytes()
+        assert torch.all(wide_out[:, n:] == 7.0 + 1.0j)
+        refined = factor.solve_device(tb, mode=Y.FP32_REFINED).cpu().numpy()
+        host, _, _ = factor.solve_many_refined(b, 1e-12, 10)
+        assert refined.tobytes() == host.tobytes(), kind
+        factor.set_block_width(4)  # several blocks
+        assert factor.solve_device(tb).cpu().numpy().tobytes() == factor.solve_many(b).tobytes()

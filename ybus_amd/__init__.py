@@ -131,6 +131,10 @@ def _declare(L):
         "ybus_hip_factor_is_complex": (i32, [P]),
         "ybus_hip_factor_solve_many_complex": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_transpose_many_complex": (i32, [P, f64p, U64, f64p]),
+        "ybus_hip_factor_set_matrix_complex": (i32, [P, i32, u64p, u64p, f64p]),
+        "ybus_hip_factor_solve_many_refined_complex": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
+        "ybus_hip_factor_solve_device_complex": (i32, [P, i32, C.c_double, U64, C.c_void_p, U64, U64, C.c_void_p,
+                                                       U64]),
         "ybus_hip_factor_free": (None, [P]),
         "ybus_hip_factor_dim": (U64, [P]),
         "ybus_hip_factor_block_width": (U64, [P]),
@@ -371,6 +375,21 @@ def _complex_rhs(rhs, n: int) -> np.ndarray:
     return b
 
 
+def _complex_compressed(a):
+    """(row_major, ptr, idx, complex data) of a complex matrix: a SciPy CSR/CSC matrix, or an
+    object with row_major, ptr, idx and complex data (or re and im)."""
+    if hasattr(a, "indptr"):
+        fmt = getattr(a, "format", "csr")
+        if fmt not in ("csr", "csc"):
+            a = a.tocsr()
+            fmt = "csr"
+        return fmt == "csr", np.asarray(a.indptr), np.asarray(a.indices), np.asarray(a.data, dtype=np.complex128)
+    data = getattr(a, "data", None)
+    if data is None or not np.iscomplexobj(data):
+        data = np.asarray(a.re) + 1j * np.asarray(a.im)
+    return bool(a.row_major), np.asarray(a.ptr), np.asarray(a.idx), np.asarray(data, dtype=np.complex128)
+
+
 def _interleaved(values) -> np.ndarray:
     """Complex values as interleaved (re, im) doubles, Complex64's layout."""
     return np.ascontiguousarray(values, dtype=np.complex128).view(np.float64)
@@ -579,10 +598,17 @@ class GpuFactor:
         return l_val, u_val
 
     def set_matrix(self, a):
-        """The matrix itself (CSR when ``a.row_major`` else CSC), for FP32_REFINED residuals."""
-        _check(library().ybus_hip_factor_set_matrix(self._h, int(bool(a.row_major)), _u64(a.ptr), _u64(a.idx),
-                                                    _f64(a.data)))
-        self._nnz_a = int(np.asarray(a.ptr)[-1])
+        """The matrix itself (CSR when ``a.row_major`` else CSC), for FP32_REFINED residuals. A
+        complex factor takes a complex matrix: a SciPy sparse CSR or CSC matrix, or an object with
+        ``row_major``, ``ptr``, ``idx`` and complex ``data``."""
+        if self._complex:
+            row_major, ptr, idx, data = _complex_compressed(a)
+            _check(library().ybus_hip_factor_set_matrix_complex(self._h, int(row_major), _u64(ptr), _u64(idx),
+                                                                _interleaved(data)))
+        else:
+            _check(library().ybus_hip_factor_set_matrix(self._h, int(bool(a.row_major)), _u64(a.ptr), _u64(a.idx),
+                                                        _f64(a.data)))
+        self._nnz_a = int(np.asarray(a.indptr if hasattr(a, "indptr") else a.ptr)[-1])
         self._generation += 1
 
     def _release_block(self):
@@ -624,7 +650,18 @@ class GpuFactor:
 
     def solve_many_refined(self, rhs, tolerance: float = 1e-12, max_steps: int = 10):
         """FP32 solves refined in FP64 to `tolerance` (refine_normwise's rule,
-        accuracy.rs:83-122): (solutions, steps per column, relative residual per column)."""
+        accuracy.rs:83-122): (solutions, steps per column, relative residual per column); complex
+        arrays for a complex factor (its complex matrix given to set_matrix)."""
+        if self._complex:
+            bz = _complex_rhs(rhs, self._n)
+            xz = _result_array(bz.shape, bz.dtype)
+            steps = np.zeros(bz.shape[0], np.uint64)
+            ratio = np.zeros(bz.shape[0])
+            if bz.shape[0]:
+                _check(library().ybus_hip_factor_solve_many_refined_complex(
+                    self._h, bz.view(np.float64), bz.shape[0], float(tolerance), int(max_steps), xz.view(np.float64),
+                    steps, ratio))
+            return xz, steps.astype(np.int64), ratio
         b = _rhs_array(rhs, self._n)
         self._release_block()
         x = _result_array(b.shape)
@@ -656,8 +693,9 @@ class GpuFactor:
     def _device_vectors(self, t, count=None):
         import torch
 
-        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda or t.dim() != 2:
-            raise InvalidArgument(INVALID_ARGUMENT, "a (count, dim) float64 tensor on the GPU")
+        want = torch.complex128 if self._complex else torch.float64
+        if not isinstance(t, torch.Tensor) or t.dtype != want or not t.is_cuda or t.dim() != 2:
+            raise InvalidArgument(INVALID_ARGUMENT, f"a (count, dim) {want} tensor on the GPU")
         if t.shape[1] != self._n:
             raise ShapeMismatch(self._n, t.shape[1])
         if count is not None and t.shape[0] != count:
@@ -681,8 +719,9 @@ class GpuFactor:
         torch.cuda.current_stream().synchronize()
         if mode is None:
             mode = TRANSPOSE if transpose else FP64
-        _check(library().ybus_hip_factor_solve_device(self._h, int(mode), float(tolerance), int(max_steps), pb,
-                                                        ldb, b.shape[0], px, ldx))
+        # a complex tensor's strides count complex values, as the complex entry point's do
+        fn = library().ybus_hip_factor_solve_device_complex if self._complex else library().ybus_hip_factor_solve_device
+        _check(fn(self._h, int(mode), float(tolerance), int(max_steps), pb, ldb, b.shape[0], px, ldx))
         return x
 
     def solve_device_noalloc(self, b, x, transpose: bool = False):

@@ -60,6 +60,12 @@ __host__ __device__ __forceinline__ size_t lds_vector_bytes(int32_t n, size_t el
 // A max that ignores NaN, as the reference's infinity-norm fold does (accuracy.rs:66-69)
 __device__ __forceinline__ double fold_max(double m, double v) { return v > m ? v : m; }
 
+// ||r||inf / ||b||inf as relative_residual forms it (accuracy.rs:57-78): 0 for a zero residual,
+// infinite for a nonzero one against a zero b
+__device__ __forceinline__ double ratio_of(double r_norm, double b_norm) {
+  return b_norm > 0.0 ? r_norm / b_norm : (r_norm > 0.0 ? __builtin_inf() : 0.0);
+}
+
 // Reduces each lane's value over the workgroup's slots into out[lane] (every thread must call).
 template <int TW>
 __device__ void reduce_max(double* scratch, double mine, double* out, const Tile<TW>& t) {
@@ -399,12 +405,14 @@ __global__ void __launch_bounds__(1024) solve64_reach_kernel(SweepDev fwd, Sweep
                          Identity(), AllLive(), Skip());
 }
 
-// ---- complex FP64 solves ----
+// ---- complex solves ----
 
 // Complex arithmetic rounded as num-complex 0.4 writes it (the reference's element type):
 // a * b = (a.re b.re - a.im b.im, a.re b.im + a.im b.re), and
 // a / b = ((a.re b.re + a.im b.im) / s, (a.im b.re - a.re b.im) / s) with s = b.re b.re + b.im b.im.
-// Each product and sum is rounded on its own (the file is built without contraction).
+// Each product and sum is rounded on its own (the file is built without contraction). The FP32
+// sweeps of a refined solve use the same formulas in float.
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -413,13 +421,44 @@ __device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
   const double s = b.x * b.x + b.y * b.y;
   return make_double2((a.x * b.x + a.y * b.y) / s, (a.y * b.x - a.x * b.y) / s);
 }
+__device__ __forceinline__ float2 zsub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 zmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 zdiv(float2 a, float2 b) {
+  const float s = b.x * b.x + b.y * b.y;
+  return make_float2((a.x * b.x + a.y * b.y) / s, (a.y * b.x - a.x * b.y) / s);
+}
+// |z| as num-complex's norm() takes it: the hypotenuse of its parts
+__device__ __forceinline__ double zabs(double2 v) { return hypot(v.x, v.y); }
+
+// A complex value of a block of E (double or float) values
+template <typename E>
+struct Cx;
+template <>
+struct Cx<double> {
+  using T = double2;
+  static __device__ __forceinline__ T make(double re, double im) { return make_double2(re, im); }
+  static __device__ __forceinline__ const T* val(const ComplexSweepDev& s) { return s.val; }
+  static __device__ __forceinline__ const T* diag(const ComplexSweepDev& s) { return s.diag; }
+};
+template <>
+struct Cx<float> {
+  using T = float2;
+  static __device__ __forceinline__ T make(float re, float im) { return make_float2(re, im); }
+  static __device__ __forceinline__ const T* val(const ComplexSweepDev& s) { return s.val32; }
+  static __device__ __forceinline__ const T* diag(const ComplexSweepDev& s) { return s.diag32; }
+};
 
 // Unknown `row` of column `col` in a complex block (real parts in row 2 row, imaginary in 2 row + 1)
-__device__ __forceinline__ double2 zload(const double* __restrict__ blk, int32_t row, int64_t ld, int32_t col) {
+template <typename E>
+__device__ __forceinline__ typename Cx<E>::T zload(const E* __restrict__ blk, int32_t row, int64_t ld, int32_t col) {
   const int64_t at = 2 * (int64_t)row * ld + col;
-  return make_double2(blk[at], blk[at + ld]);
+  return Cx<E>::make(blk[at], blk[at + ld]);
 }
-__device__ __forceinline__ void zstore(double* __restrict__ blk, int32_t row, int64_t ld, int32_t col, double2 v) {
+template <typename E>
+__device__ __forceinline__ void zstore(E* __restrict__ blk, int32_t row, int64_t ld, int32_t col,
+                                       typename Cx<E>::T v) {
   const int64_t at = 2 * (int64_t)row * ld + col;
   blk[at] = v.x;
   blk[at + ld] = v.y;
@@ -432,26 +471,29 @@ constexpr int COMPLEX_CHUNK = 4;
 // One complex sweep of a tile, row by row as forward_by_rows / back_by_rows and the LU
 // substitutions do (cholesky.rs:468-501, lu.rs:502-533,640-649,770-785): the row's right-hand
 // side, minus each term's value times its dependency in stored order, divided by the diagonal
-// unless it is unit. Lane (slot, c) takes rows lp[l] + slot + j slots of level l, loading the
-// next row's header while it works on one; a barrier ends each level, and every thread takes
-// every barrier.
-template <int TW, class Rhs, class Store>
-__device__ void complex_sweep(const ComplexSweepDev& s, const int32_t* lp, const double* __restrict__ dep_buf,
-                              int64_t ld, const Tile<TW>& g, Rhs rhs, Store store) {
+// unless it is unit. E is the block's (and the arithmetic's) precision: double for the FP64
+// solve, float for a refined solve's sweeps over the factor's FP32 copy. Lane (slot, c) takes
+// rows lp[l] + slot + j slots of level l when `act`, loading the next row's header while it
+// works on one; a barrier ends each level, and every thread takes every barrier.
+template <int TW, typename E, class Rhs, class Store>
+__device__ void complex_sweep(const ComplexSweepDev& s, const int32_t* lp, const E* __restrict__ dep_buf, int64_t ld,
+                              const Tile<TW>& g, bool act, Rhs rhs, Store store) {
+  using C = typename Cx<E>::T;
   const int4* __restrict__ head = s.head;
   const int32_t* __restrict__ dep = s.dep;
-  const double2* __restrict__ val = s.val;
+  const C* __restrict__ val = Cx<E>::val(s);
+  const C* __restrict__ dg = Cx<E>::diag(s);
   for (int32_t l = 0; l < s.levels; ++l) {
     const int32_t kb = lp[l] + g.slot, ke = lp[l + 1];
-    if (g.on && kb < ke) {
+    if (act && kb < ke) {
       int4 h = head[kb];
       for (int32_t k = kb; k < ke; k += g.slots) {
         const int4 next = head[k + g.slots < ke ? k + g.slots : k];
-        const double2 diag = s.diag ? s.diag[k] : make_double2(1.0, 0.0);
-        double2 acc = rhs(h.y);
+        const C diag = s.diag ? dg[k] : Cx<E>::make(1, 0);
+        C acc = rhs(h.y);
         for (int32_t p = h.z; p < h.w; p += COMPLEX_CHUNK) {
           int32_t d[COMPLEX_CHUNK];
-          double2 v[COMPLEX_CHUNK], x[COMPLEX_CHUNK];
+          C v[COMPLEX_CHUNK], x[COMPLEX_CHUNK];
 #pragma unroll
           for (int q = 0; q < COMPLEX_CHUNK; ++q) {
             const int32_t at = p + q < h.w ? p + q : p;  // past the row's end: reload its first, unused
@@ -462,7 +504,7 @@ __device__ void complex_sweep(const ComplexSweepDev& s, const int32_t* lp, const
           // still take the vector memory path's bandwidth
 #pragma unroll
           for (int q = 0; q < COMPLEX_CHUNK; ++q)
-            x[q] = p + q < h.w ? zload(dep_buf, d[q], ld, g.col) : make_double2(0.0, 0.0);
+            x[q] = p + q < h.w ? zload<E>(dep_buf, d[q], ld, g.col) : Cx<E>::make(0, 0);
 #pragma unroll
           for (int q = 0; q < COMPLEX_CHUNK; ++q)
             if (p + q < h.w) acc = zsub(acc, zmul(v[q], x[q]));
@@ -489,20 +531,147 @@ __global__ void __launch_bounds__(1024) solve_complex_kernel(ComplexSweepDev fwd
   __syncthreads();
   const Tile<TW> g(ncols, tile_of_workgroup(xcds));
   const int32_t col = g.col;
-  complex_sweep<TW>(
-      fwd, lpf, t, ld, g,
+  complex_sweep<TW, double>(
+      fwd, lpf, t, ld, g, g.on,
       [&](int32_t src) {
-        double2 v = zload(b, src, ld, col);
+        double2 v = zload<double>(b, src, ld, col);
         if (conjugate) v.y = -v.y;
         return v;
       },
-      [&](int32_t dst, double2 v) { zstore(t, dst, ld, col, v); });
-  complex_sweep<TW>(
-      bwd, lpb, x, ld, g, [&](int32_t src) { return zload(t, src, ld, col); },
-      [&](int32_t dst, double2 v) { zstore(x, dst, ld, col, v); });
+      [&](int32_t dst, double2 v) { zstore<double>(t, dst, ld, col, v); });
+  complex_sweep<TW, double>(
+      bwd, lpb, x, ld, g, g.on, [&](int32_t src) { return zload<double>(t, src, ld, col); },
+      [&](int32_t dst, double2 v) { zstore<double>(x, dst, ld, col, v); });
   // the backward sweep read its own results unconjugated; each lane now conjugates its column
   if (conjugate && g.on)
     for (int32_t i = g.slot; i < bwd.n; i += g.slots) x[(2 * (int64_t)i + 1) * ld + col] = -x[(2 * (int64_t)i + 1) * ld + col];
+}
+
+// ---- complex FP32 solves refined in FP64 ----
+
+// One FP32 solve of a tile's active columns: src (an FP64 complex block: the right-hand sides or
+// the residuals) rounded to float as its rows are read, forward into t, backward into d (both
+// FP32 complex blocks), the sweeps over the factor's FP32 values with FP32 accumulation
+template <int TW>
+__device__ void zsolve32(const ComplexSweepDev& fwd, const ComplexSweepDev& bwd, const int32_t* lpf,
+                         const int32_t* lpb, const double* __restrict__ src, float* __restrict__ t,
+                         float* __restrict__ d, int64_t ld, const Tile<TW>& g, bool act) {
+  const int32_t col = g.col;
+  complex_sweep<TW, float>(
+      fwd, lpf, t, ld, g, act,
+      [&](int32_t row) {
+        const double2 v = zload<double>(src, row, ld, col);
+        return make_float2((float)v.x, (float)v.y);
+      },
+      [&](int32_t dst, float2 v) { zstore<float>(t, dst, ld, col, v); });
+  complex_sweep<TW, float>(
+      bwd, lpb, d, ld, g, act, [&](int32_t row) { return zload<float>(t, row, ld, col); },
+      [&](int32_t dst, float2 v) { zstore<float>(d, dst, ld, col, v); });
+}
+
+// r = b - A x over the tile's active columns in FP64 (CsrMatrix::mul_vec's order: each row summed
+// from zero over its entries in index order, product.rs:17-27, then subtracted from b); returns
+// the thread's largest |r_i|
+template <int TW>
+__device__ double zresidual(const ZCsrDev& a, const double* __restrict__ b, const double* __restrict__ x,
+                            double* __restrict__ r, int64_t ld, const Tile<TW>& g, bool act) {
+  double m = 0.0;
+  if (!act) return m;
+  for (int32_t i = g.slot; i < a.rows; i += g.slots) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int32_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p) acc = zadd(acc, zmul(a.val[p], zload<double>(x, a.col[p], ld, g.col)));
+    const double2 ri = zsub(zload<double>(b, i, ld, g.col), acc);
+    zstore<double>(r, i, ld, g.col, ri);
+    m = fold_max(m, zabs(ri));
+  }
+  return m;
+}
+
+// FP32 complex sweeps refined in FP64 by refine_normwise's rule (accuracy.rs:83-122), one tile
+// of TW columns per workgroup: x0 = solve(b), then while a column is active, d = solve(r) and
+// x + d into the other iterate buffer; a column stays active while each step at least halves
+// its ratio ||b - A x||inf / ||b||inf (|z| its hypotenuse), and its best iterate ends in x0.
+template <int TW>
+__global__ void __launch_bounds__(1024) refine_complex_kernel(ComplexSweepDev fwd, ComplexSweepDev bwd, ZCsrDev a,
+                                                              const double* __restrict__ b, float* __restrict__ t,
+                                                              float* __restrict__ d, double* __restrict__ x0,
+                                                              double* __restrict__ x1, double* __restrict__ r,
+                                                              int64_t ld, int32_t ncols, double tol, int32_t max_steps,
+                                                              int32_t* __restrict__ steps_out,
+                                                              double* __restrict__ ratio_out, int32_t xcds) {
+  __shared__ double scratch[1024];
+  __shared__ double s_bnorm[TW], s_rnorm[TW], s_ratio[TW], s_best[TW];
+  __shared__ int32_t s_best_buf[TW], s_steps[TW], s_active[TW];
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem);
+  int32_t* lpb = lpf + fwd.levels + 1;
+  for (int32_t l = threadIdx.x; l <= fwd.levels; l += blockDim.x) lpf[l] = fwd.level_ptr[l];
+  for (int32_t l = threadIdx.x; l <= bwd.levels; l += blockDim.x) lpb[l] = bwd.level_ptr[l];
+  __syncthreads();
+  const Tile<TW> g(ncols, tile_of_workgroup(xcds));
+  const int32_t col = g.col;
+  const int32_t rows = 2 * a.rows;  // block rows: real and imaginary parts
+
+  double m = 0.0;
+  if (g.on)
+    for (int32_t i = g.slot; i < a.rows; i += g.slots) m = fold_max(m, zabs(zload<double>(b, i, ld, col)));
+  reduce_max<TW>(scratch, m, s_bnorm, g);
+
+  // x0 = solve(b), its residual and ratio
+  zsolve32<TW>(fwd, bwd, lpf, lpb, b, t, d, ld, g, g.on);
+  if (g.on)
+    for (int32_t k = g.slot; k < rows; k += g.slots) x0[k * ld + col] = (double)d[k * ld + col];
+  __syncthreads();
+  m = zresidual<TW>(a, b, x0, r, ld, g, g.on);
+  reduce_max<TW>(scratch, m, s_rnorm, g);
+  if (g.slot == 0) {
+    const double ratio = ratio_of(s_rnorm[g.lane], s_bnorm[g.lane]);
+    s_ratio[g.lane] = ratio;
+    s_best[g.lane] = ratio;
+    s_best_buf[g.lane] = 0;
+    s_steps[g.lane] = 0;
+    s_active[g.lane] = g.on && max_steps > 0 && !(ratio <= tol);
+  }
+  __syncthreads();
+
+  double* cur = x0;
+  double* nxt = x1;
+  for (int32_t step = 1; step <= max_steps; ++step) {
+    bool any = false;
+    for (int c = 0; c < TW; ++c) any = any || s_active[c];
+    if (!any) break;
+    const bool act = g.on && s_active[g.lane];
+    zsolve32<TW>(fwd, bwd, lpf, lpb, r, t, d, ld, g, act);
+    if (act)
+      for (int32_t k = g.slot; k < rows; k += g.slots) nxt[k * ld + col] = cur[k * ld + col] + (double)d[k * ld + col];
+    __syncthreads();
+    m = zresidual<TW>(a, b, nxt, r, ld, g, act);
+    reduce_max<TW>(scratch, m, s_rnorm, g);
+    if (g.slot == 0 && act) {
+      const double previous = s_ratio[g.lane];
+      const double ratio = ratio_of(s_rnorm[g.lane], s_bnorm[g.lane]);
+      s_ratio[g.lane] = ratio;
+      if (ratio < s_best[g.lane]) {
+        s_best[g.lane] = ratio;
+        s_best_buf[g.lane] = nxt == x1 ? 1 : 0;
+        s_steps[g.lane] = step;
+      }
+      const bool stop = ratio != ratio || ratio > 0.5 * previous || ratio <= tol || step == max_steps;
+      s_active[g.lane] = !stop;
+    }
+    __syncthreads();
+    double* swap = cur;
+    cur = nxt;
+    nxt = swap;
+  }
+
+  // the best iterate into x0
+  if (g.on && s_best_buf[g.lane] == 1)
+    for (int32_t k = g.slot; k < rows; k += g.slots) x0[k * ld + col] = x1[k * ld + col];
+  if (g.on && g.slot == 0) {
+    steps_out[col] = s_steps[g.lane];
+    ratio_out[col] = s_best[g.lane];
+  }
 }
 
 // ---- partitioned FP64 solves: a sweep's independent parts side by side ----
@@ -908,10 +1077,6 @@ __device__ double residual(const CsrDev& a, const double* __restrict__ b, const 
     p1 = n1;
   }
   return m;
-}
-
-__device__ __forceinline__ double ratio_of(double r_norm, double b_norm) {
-  return b_norm > 0.0 ? r_norm / b_norm : (r_norm > 0.0 ? __builtin_inf() : 0.0);
 }
 
 template <int TW, bool Split>
@@ -1873,6 +2038,28 @@ hipError_t launch_solve_complex(hipStream_t stream, int tile, int threads, const
   return hipGetLastError();
 }
 
+hipError_t launch_refine_complex(hipStream_t stream, int tile, int threads, const ComplexSweepDev& fwd,
+                                 const ComplexSweepDev& bwd, const ZCsrDev& a, const double* b, float* t, float* d,
+                                 double* x0, double* x1, double* r, int64_t ld, int32_t ncols, double tol,
+                                 int32_t max_steps, int32_t* steps, double* ratio) {
+  if (threads < tile || threads > 1024 || threads % 64 || (threads & (threads - 1)) || ld < ncols ||
+      !fwd.val32 || !bwd.val32)
+    return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
+  const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
+#define L(TWV)                                                                                                   \
+  {                                                                                                              \
+    hipError_t e = hipFuncSetAttribute((const void*)refine_complex_kernel<TWV>,                                   \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                  \
+    if (e != hipSuccess) return e;                                                                               \
+    hipLaunchKernelGGL((refine_complex_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, \
+                       d, x0, x1, r, ld, ncols, tol, max_steps, steps, ratio, xcd_groups());                     \
+  }
+  DISPATCH_TILE(tile, L)
+#undef L
+  return hipGetLastError();
+}
+
 hipError_t launch_to_block(hipStream_t stream, const double* vectors, double* block, int64_t n, int64_t count,
                            int64_t ld) {
   dim3 grid((unsigned)((n + TR - 1) / TR), (unsigned)((count + TR - 1) / TR));
@@ -2410,20 +2597,31 @@ __global__ void refresh_rows_kernel(RefreshDev r, const double* __restrict__ src
   }
 }
 
+// A complex sweep's values (and their FP32 copies, where kept) from the refactored ones
 __global__ void refresh_complex_kernel(int32_t n, int64_t entries, const int32_t* __restrict__ vpos,
                                        const int32_t* __restrict__ dpos, const double2* __restrict__ src,
-                                       double2* __restrict__ val, double2* __restrict__ diag) {
+                                       double2* __restrict__ val, double2* __restrict__ diag,
+                                       float2* __restrict__ val32, float2* __restrict__ diag32) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < entries) val[i] = src[vpos[i]];
-  if (dpos && i < n) diag[i] = src[dpos[i]];
+  if (i < entries) {
+    const double2 v = src[vpos[i]];
+    val[i] = v;
+    if (val32) val32[i] = make_float2((float)v.x, (float)v.y);
+  }
+  if (dpos && i < n) {
+    const double2 v = src[dpos[i]];
+    diag[i] = v;
+    if (diag32) diag32[i] = make_float2((float)v.x, (float)v.y);
+  }
 }
 
 hipError_t launch_refresh_complex(hipStream_t stream, int32_t n, int64_t entries, const int32_t* vpos,
-                                  const int32_t* dpos, const double2* src, double2* val, double2* diag) {
+                                  const int32_t* dpos, const double2* src, double2* val, double2* diag, float2* val32,
+                                  float2* diag32) {
   const int64_t count = std::max<int64_t>(entries, dpos ? n : 0);
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(refresh_complex_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, n, entries,
-                     vpos, dpos, src, val, diag);
+                     vpos, dpos, src, val, diag, val32, diag32);
   return hipGetLastError();
 }
 

@@ -60,6 +60,16 @@ struct ComplexSweepDev {
   const int32_t* dep = nullptr;
   const double2* val = nullptr;
   const double2* diag = nullptr;  // null: unit diagonal
+  const float2* val32 = nullptr;   // the values rounded to FP32 (a refined solve's sweeps)
+  const float2* diag32 = nullptr;
+};
+
+// A complex matrix in CSR form on the device ((re, im) values), for a refined solve's residual
+struct ZCsrDev {
+  int32_t rows = 0;
+  const int32_t* ptr = nullptr;
+  const int32_t* col = nullptr;
+  const double2* val = nullptr;
 };
 
 // A CSR matrix on the device.
@@ -234,6 +244,15 @@ hipError_t launch_solve_complex(hipStream_t stream, int tile, int threads, const
                                 const ComplexSweepDev& bwd, const double* b, double* t, double* x, int64_t ld,
                                 int32_t ncols, bool conjugate);
 
+// Complex FP32 sweeps (the sweeps' val32 / diag32) refined in FP64 against the complex matrix a,
+// one tile of `tile` columns per workgroup of `threads` (a power of two): b the right-hand sides,
+// t and d FP32 complex blocks, x0 (the result) and x1 the iterates, r the residuals, all complex
+// blocks of ld doubles a row; each column's corrections and final ratio into steps / ratio.
+hipError_t launch_refine_complex(hipStream_t stream, int tile, int threads, const ComplexSweepDev& fwd,
+                                 const ComplexSweepDev& bwd, const ZCsrDev& a, const double* b, float* t, float* d,
+                                 double* x0, double* x1, double* r, int64_t ld, int32_t ncols, double tol,
+                                 int32_t max_steps, int32_t* steps, double* ratio);
+
 hipError_t launch_to_block(hipStream_t stream, const double* vectors, double* block, int64_t n, int64_t count,
                            int64_t ld);
 hipError_t launch_from_block(hipStream_t stream, const double* block, double* vectors, int64_t n, int64_t count,
@@ -381,9 +400,11 @@ struct RefreshDev {
 };
 hipError_t launch_refresh(hipStream_t stream, const RefreshDev& r, const double* src);
 // A complex sweep's values (one (re, im) pair per entry, and per row unless its diagonal is unit)
-// from new factor values: val[p] = src[vpos[p]], diag[i] = src[dpos[i]]
+// from new factor values: val[p] = src[vpos[p]], diag[i] = src[dpos[i]], and their FP32 copies
+// val32 / diag32 rounded from them (either may be null)
 hipError_t launch_refresh_complex(hipStream_t stream, int32_t n, int64_t entries, const int32_t* vpos,
-                                  const int32_t* dpos, const double2* src, double2* val, double2* diag);
+                                  const int32_t* dpos, const double2* src, double2* val, double2* diag, float2* val32,
+                                  float2* diag32);
 
 // The workspace of the entry point that allocates nothing (ybus_hip_factor_solve_device_noalloc):
 // a file-scope device array of SCRATCH_BYTES in kernels.hip, which every device the library runs

@@ -357,6 +357,7 @@ struct DevSweepC {
   DevBuf<int32_t> level_ptr, src, dst, row_ptr, dep;
   DevBuf<int4> head;
   DevBuf<double2> val, diag;
+  DevBuf<float2> val32, diag32;  // the values rounded to FP32, for refined solves
   int32_t n = 0, levels = 0;
   bool unit = false;
   void upload(const Sweep& s, hipStream_t stream) {
@@ -366,6 +367,11 @@ struct DevSweepC {
     std::vector<double2> v(s.val.size()), d(unit ? 0 : (size_t)n);
     for (size_t p = 0; p < v.size(); ++p) v[p] = make_double2(s.val[p], s.val_im[p]);
     for (size_t k = 0; k < d.size(); ++k) d[k] = make_double2(s.diag[k], s.diag_im[k]);
+    std::vector<float2> v32(v.size()), d32(d.size());
+    for (size_t p = 0; p < v.size(); ++p) v32[p] = make_float2((float)v[p].x, (float)v[p].y);
+    for (size_t k = 0; k < d.size(); ++k) d32[k] = make_float2((float)d[k].x, (float)d[k].y);
+    val32.upload(v32, stream);
+    if (!unit) diag32.upload(d32, stream);
     std::vector<int4> h(n);
     for (int32_t k = 0; k < n; ++k) h[k] = make_int4(s.dst[k], s.src[k], s.row_ptr[k], s.row_ptr[k + 1]);
     head.upload(h, stream);
@@ -389,6 +395,8 @@ struct DevSweepC {
     c.dep = dep.p;
     c.val = val.p;
     c.diag = unit ? nullptr : diag.p;
+    c.val32 = val32.p;
+    c.diag32 = unit ? nullptr : diag32.p;
     return c;
   }
 };
@@ -614,6 +622,16 @@ struct ybus_hip_factor {
   bool complex = false;
   DevSweepC zfwd, zbwd, ztfwd, ztbwd;
   DevBuf<double> zb, zt, zx, zvec;
+  // a complex factor's refined solves: the second iterate, the residuals, the FP32 blocks, and
+  // the matrix the residuals are taken against (CSR by rows, (re, im) values)
+  DevBuf<double> zx1, zr;
+  DevBuf<float> zt32, zd32;
+  struct {
+    int32_t rows = 0;
+    size_t nnz = 0;
+    DevBuf<int32_t> ptr, col;
+    DevBuf<double2> val;
+  } za;
   // a real LU factor's pattern, and the plan for new values (built by the first refactor or
   // value update)
   LuPattern lu_host;
@@ -871,6 +889,57 @@ struct ybus_hip_factor {
     const uint64_t columns_bytes =
         pairs ? (uint64_t)ncols * ((uint64_t)n * 8 + 8) : (uint64_t)ncols * (uint64_t)n * 16;
     return factor + columns_bytes;
+  }
+
+  // A complex block of up to `width` columns (2n rows of ld doubles): b, t, x and the vectors'
+  // staging, and for refinement the second iterate, the residuals, the FP32 blocks and the
+  // per-column results
+  void reserve_complex(int64_t width, int64_t ld, bool refine) {
+    const size_t elems = (size_t)(2 * n) * (size_t)ld;
+    zb.reserve(elems);
+    zt.reserve(elems);
+    zx.reserve(elems);
+    zvec.reserve((size_t)(2 * n) * (size_t)width);
+    if (refine) {
+      zx1.reserve(elems);
+      zr.reserve(elems);
+      zt32.reserve(elems);
+      zd32.reserve(elems);
+      isteps.reserve((size_t)std::max<int64_t>(width, 16));
+      dratio.reserve((size_t)std::max<int64_t>(width, 16));
+    }
+  }
+
+  // Solves the complex block in zb (`chunk` columns, ld doubles a row) into zx: FP64, transposed
+  // (an LU factor's U^T, L^T sweeps; for a Cholesky factor conj(A^-1 conj(b))), or FP32 sweeps
+  // refined in FP64 against the attached complex matrix, each column's corrections and ratio into
+  // isteps / dratio
+  void zsolve(int mode, int64_t chunk, int64_t ld, double tol, int32_t max_steps) {
+    const int tile = tile_override ? tile_override : tile_width_for(chunk, ctx->cus);
+    const uint64_t nn = (uint64_t)n;
+    begin_kernel();
+    if (mode == YBUS_HIP_FP32_REFINED) {
+      int th = 64;
+      while (th * 2 <= threads_wide) th *= 2;
+      check(launch_refine_complex(stream(), tile, th, zfwd.view(), zbwd.view(),
+                                  ZCsrDev{za.rows, za.ptr.p, za.col.p, za.val.p}, zb.p, zt32.p, zd32.p, zx.p, zx1.p,
+                                  zr.p, ld, (int32_t)chunk, tol, max_steps, isteps.p, dratio.p),
+            "complex refine kernel");
+      // the FP32 factor's values (8 bytes) and indices once per solve, the residual's matrix once
+      // per residual (16-byte values, 4-byte indices), and each column's vectors; counted for one
+      // solve and residual plus three corrections, the kernel's usual work
+      const uint64_t solve = (uint64_t)nnz_factor * 12 + 8 * (nn + 1), resid = (uint64_t)za.nnz * 20 + 4 * (nn + 1);
+      end_kernel(4 * (solve + resid) + 32 * nn * (uint64_t)chunk * 4);
+      return;
+    }
+    const bool transpose = mode == YBUS_HIP_TRANSPOSE;
+    const bool lu_t = transpose && lu;
+    check(launch_solve_complex(stream(), tile, threads_wide, (lu_t ? ztfwd : zfwd).view(), (lu_t ? ztbwd : zbwd).view(),
+                               zb.p, zt.p, zx.p, ld, (int32_t)chunk, transpose && !lu),
+          "complex solve kernel");
+    // the factor's (re, im) values and 4-byte indices once, row pointers, and a 16-byte value
+    // in and out per unknown and column
+    end_kernel((uint64_t)nnz_factor * 20 + 8 * (nn + 1) + 32 * nn * (uint64_t)chunk);
   }
 
   // Whether the wide solves take the split sweeps (an LU factor with long rows)
@@ -1519,7 +1588,8 @@ static double refresh_sweeps(ybus_hip_factor* f, RefactorPlan& plan) {
     if (r.zsweep) {
       DevSweepC& z = *r.zsweep;
       check(launch_refresh_complex(stream, z.n, (int64_t)z.val.len, r.vpos.p, z.unit ? nullptr : r.dpos.p,
-                                   reinterpret_cast<const double2*>(plan.next.p), z.val.p, z.unit ? nullptr : z.diag.p),
+                                   reinterpret_cast<const double2*>(plan.next.p), z.val.p, z.unit ? nullptr : z.diag.p,
+                                   z.val32.p, z.unit ? nullptr : z.diag32.p),
             "refresh sweeps");
       continue;
     }
@@ -2094,25 +2164,33 @@ static int32_t solve_many_mode(ybus_hip_factor* f, const double* rhs, uint64_t c
 
 // Complex solves of `count` right-hand sides (n interleaved (re, im) values each), in blocks of
 // block_width() columns: each block is copied in, turned into a complex block (real and
-// imaginary rows), solved by one launch, and copied out. A Cholesky factor's transposed solve
-// is conj(A^-1 conj(b)), since A^T = conj(A) for a Hermitian A.
-static int32_t solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x, bool transpose) {
+// imaginary rows), solved by one launch (zsolve), and copied out.
+static void require_complex(const ybus_hip_factor* f) {
+  if (!f->complex) fail(YBUS_HIP_INVALID_ARGUMENT, "a real factor takes the real solves only");
+}
+
+static void require_complex_matrix(const ybus_hip_factor* f, int mode) {
+  if (mode != YBUS_HIP_FP64 && mode != YBUS_HIP_TRANSPOSE && mode != YBUS_HIP_FP32_REFINED)
+    fail(YBUS_HIP_INVALID_ARGUMENT, "unknown mode");
+  if (mode == YBUS_HIP_FP32_REFINED && !f->za.ptr.p)
+    fail(YBUS_HIP_INVALID_ARGUMENT, "FP32 refinement needs the matrix (ybus_hip_factor_set_matrix_complex)");
+}
+
+static int32_t solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x, int mode,
+                                  double tol, uint64_t max_steps, uint64_t* steps, double* ratio) {
   return guard([&] {
     if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
-    if (!f->complex) fail(YBUS_HIP_INVALID_ARGUMENT, "a real factor takes the real solves only");
+    require_complex(f);
+    require_complex_matrix(f, mode);
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
     if (count == 0 || f->n == 0) return;
     if (!rhs || !x) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
     check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    const bool refine = mode == YBUS_HIP_FP32_REFINED;
     const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
     const int64_t ld = round_up(width, 16);
     const int64_t rows = 2 * f->n;
-    const size_t elems = (size_t)rows * (size_t)ld;
-    f->zb.reserve(elems);
-    f->zt.reserve(elems);
-    f->zx.reserve(elems);
-    f->zvec.reserve((size_t)rows * (size_t)width);
-    const bool lu_t = transpose && f->lu;
-    const ComplexSweepDev fw = (lu_t ? f->ztfwd : f->zfwd).view(), bw = (lu_t ? f->ztbwd : f->zbwd).view();
+    f->reserve_complex(width, ld, refine);
     for (uint64_t start = 0; start < count; start += (uint64_t)width) {
       const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
       const size_t len = (size_t)chunk * (size_t)rows;
@@ -2121,14 +2199,8 @@ static int32_t solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_
       check(launch_to_block(f->stream(), f->zvec.p, f->zb.p, rows, chunk, ld), "to_block");
       check(hipStreamSynchronize(f->stream()), "upload");
       auto t1 = std::chrono::steady_clock::now();
-      f->begin_kernel();
-      check(launch_solve_complex(f->stream(), f->tile_override ? f->tile_override : tile_width_for(chunk, f->ctx->cus),
-                                 f->threads_wide, fw, bw, f->zb.p, f->zt.p, f->zx.p, ld, (int32_t)chunk,
-                                 transpose && !f->lu),
-            "complex solve kernel");
-      // the factor's (re, im) values and 4-byte indices once, row pointers, and a 16-byte value
-      // in and out per unknown and column
-      f->end_kernel((uint64_t)f->nnz_factor * 20 + 8 * (uint64_t)(f->n + 1) + 32 * (uint64_t)f->n * (uint64_t)chunk);
+      f->zsolve(mode, chunk, ld, tol, (int32_t)max_steps);
+      if (refine) read_refinement(f, chunk, steps ? steps + start : nullptr, ratio ? ratio + start : nullptr);
       auto t2 = std::chrono::steady_clock::now();
       check(launch_from_block(f->stream(), f->zx.p, f->zvec.p, rows, chunk, ld), "from_block");
       f->staging.to_host(f->stream(), x + start * (uint64_t)rows, f->zvec.p, len * sizeof(double));
@@ -2141,12 +2213,107 @@ static int32_t solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_
 }
 
 int32_t ybus_hip_factor_solve_many_complex(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x) {
-  return solve_many_complex(f, rhs, count, x, false);
+  return solve_many_complex(f, rhs, count, x, YBUS_HIP_FP64, 0.0, 0, nullptr, nullptr);
 }
 
 int32_t ybus_hip_factor_solve_transpose_many_complex(ybus_hip_factor* f, const double* rhs, uint64_t count,
                                                      double* x) {
-  return solve_many_complex(f, rhs, count, x, true);
+  return solve_many_complex(f, rhs, count, x, YBUS_HIP_TRANSPOSE, 0.0, 0, nullptr, nullptr);
+}
+
+int32_t ybus_hip_factor_solve_many_refined_complex(ybus_hip_factor* f, const double* rhs, uint64_t count,
+                                                   double tolerance, uint64_t max_steps, double* x, uint64_t* steps,
+                                                   double* ratio) {
+  return solve_many_complex(f, rhs, count, x, YBUS_HIP_FP32_REFINED, tolerance, max_steps, steps, ratio);
+}
+
+int32_t ybus_hip_factor_solve_device_complex(ybus_hip_factor* f, int32_t mode, double tolerance, uint64_t max_steps,
+                                             const double* b, uint64_t ld_b, uint64_t count, double* x,
+                                             uint64_t ld_x) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    require_complex(f);
+    require_complex_matrix(f, mode);
+    if (max_steps > 1000000) fail(YBUS_HIP_INVALID_ARGUMENT, "max_steps is out of range");
+    if (count == 0 || f->n == 0) return;
+    if (!b || !x) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (std::min(ld_b, ld_x) < (uint64_t)f->n) fail(YBUS_HIP_INVALID_ARGUMENT, "the vectors' stride is below dim()");
+    if (count > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "too many vectors");
+    check(hipSetDevice(f->ctx->device), "hipSetDevice");
+    const bool refine = mode == YBUS_HIP_FP32_REFINED;
+    const int64_t width = (int64_t)std::min<uint64_t>(f->width, count);
+    const int64_t ld = round_up(width, 16);
+    const int64_t rows = 2 * f->n;
+    const size_t row_bytes = (size_t)rows * sizeof(double);
+    f->reserve_complex(width, ld, refine);
+    for (uint64_t start = 0; start < count; start += (uint64_t)width) {
+      const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
+      // vector c of the chunk at b + 2 (start + c) ld_b doubles: packed into the staging block
+      // first unless the caller's vectors are already packed
+      const double* in = b + 2 * start * ld_b;
+      if (ld_b != (uint64_t)f->n) {
+        check(hipMemcpy2DAsync(f->zvec.p, row_bytes, in, 2 * ld_b * sizeof(double), row_bytes, (size_t)chunk,
+                               hipMemcpyDeviceToDevice, f->stream()),
+              "D2D");
+        in = f->zvec.p;
+      }
+      check(launch_to_block(f->stream(), in, f->zb.p, rows, chunk, ld), "to_block");
+      f->zsolve(mode, chunk, ld, tolerance, (int32_t)max_steps);
+      double* out = x + 2 * start * ld_x;
+      if (ld_x == (uint64_t)f->n) {
+        check(launch_from_block(f->stream(), f->zx.p, out, rows, chunk, ld), "from_block");
+      } else {
+        check(launch_from_block(f->stream(), f->zx.p, f->zvec.p, rows, chunk, ld), "from_block");
+        check(hipMemcpy2DAsync(out, 2 * ld_x * sizeof(double), f->zvec.p, row_bytes, row_bytes, (size_t)chunk,
+                               hipMemcpyDeviceToDevice, f->stream()),
+              "D2D");
+      }
+    }
+    check(hipStreamSynchronize(f->stream()), "solve");
+  });
+}
+
+int32_t ybus_hip_factor_set_matrix_complex(ybus_hip_factor* f, int32_t row_major, const uint64_t* ptr,
+                                           const uint64_t* idx, const double* val) {
+  return guard([&] {
+    if (!f || !ptr) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    require_complex(f);
+    const int64_t n = f->n;
+    std::string why;
+    if (!valid_compressed(n, n, ptr, idx, &why)) fail(YBUS_HIP_INVALID_ARGUMENT, "A: " + why);
+    const uint64_t nnz = ptr[n];
+    if (nnz && (!idx || !val)) fail(YBUS_HIP_INVALID_ARGUMENT, "null argument");
+    if (nnz > (uint64_t)INDEX_LIMIT) fail(YBUS_HIP_TOO_LARGE, "the matrix is too large for 32-bit indices");
+    std::vector<int32_t> rptr(n + 1), cols(nnz);
+    std::vector<double2> vals(nnz);
+    if (row_major) {
+      for (int64_t i = 0; i <= n; ++i) rptr[i] = (int32_t)ptr[i];
+      for (uint64_t p = 0; p < nnz; ++p) {
+        cols[p] = (int32_t)idx[p];
+        vals[p] = make_double2(val[2 * p], val[2 * p + 1]);
+      }
+    } else {
+      // CSC to CSR by a counting sort by row: each row's columns come out ascending, the order
+      // CsrMatrix::mul_vec sums them in (product.rs:17-27)
+      std::vector<int32_t> cnt(n + 1, 0);
+      for (uint64_t p = 0; p < nnz; ++p) cnt[idx[p] + 1] += 1;
+      for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+      rptr = cnt;
+      std::vector<int32_t> cursor(cnt.begin(), cnt.end() - 1);
+      for (int64_t j = 0; j < n; ++j)
+        for (uint64_t p = ptr[j]; p < ptr[j + 1]; ++p) {
+          const int32_t q = cursor[idx[p]]++;
+          cols[q] = (int32_t)j;
+          vals[q] = make_double2(val[2 * p], val[2 * p + 1]);
+        }
+    }
+    f->za.rows = (int32_t)n;
+    f->za.nnz = nnz;
+    f->za.ptr.upload(rptr, f->stream());
+    f->za.col.upload(cols, f->stream());
+    f->za.val.upload(vals, f->stream());
+    check(hipStreamSynchronize(f->stream()), "upload matrix");
+  });
 }
 
 int32_t ybus_hip_factor_solve_many(ybus_hip_factor* f, const double* rhs, uint64_t count, double* x) {
