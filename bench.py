@@ -791,8 +791,28 @@ def run_complex(gpu, args):
                kernel_ms=kernel_ms, bytes_per_launch=stats.bytes / stats.launches, achieved_gbs=achieved,
                hbm_frac=achieved / HBM_PEAK_GBS,
                bit_exact_sampled_columns=f"{len(sample)} of {len(sample)}" if exact else "MISMATCH")
+    # the same block as FP32 sweeps refined in FP64 against A (the complex matrix attached), and
+    # its distance from the CPU's FP64 solutions (the oracle), on the sampled columns
+    import scipy.sparse as sp
+
+    factor.set_matrix(sp.csr_matrix((re + 1j * im, np.asarray(bprime.idx), np.asarray(bprime.ptr)),
+                                    shape=bprime.shape))
+    factor.solve_many_refined(b, 1e-12, 10)
+    factor.reset_kernel_stats()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        xr, steps, ratio = factor.solve_many_refined(b, 1e-12, 10)
+    refined_s = (time.perf_counter() - t0) / reps
+    rstats = factor.kernel_stats()
+    cpu = chol.solve_many(b[sample])
+    scale = np.max(np.abs(cpu), axis=1)
+    out["fp32_refined"] = dict(kernel_ms=rstats.milliseconds / rstats.launches, step_ms=refined_s * 1e3,
+                               max_relative_residual=float(ratio.max()), mean_corrections=float(steps.mean()),
+                               max_rel_diff_vs_cpu_fp64=float(np.max(np.max(np.abs(xr[sample] - cpu), axis=1) / scale)),
+                               kernel="refine_complex_kernel")
     log(f"[complex] kernel {kernel_ms:.3f} ms ({achieved:.0f} GB/s), step with copies {step_s * 1e3:.3f} ms; "
-        f"bit-exact {out['bit_exact_sampled_columns']}")
+        f"bit-exact {out['bit_exact_sampled_columns']}; FP32-refined kernel "
+        f"{out['fp32_refined']['kernel_ms']:.3f} ms, residual <= {ratio.max():.2e}, {steps.mean():.2f} corrections")
     return out
 
 
