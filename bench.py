@@ -240,6 +240,28 @@ def run_precision(gpu, args):
             x64 = x
         else:
             x32 = x
+    # FP32-refined with the floor stop rule: the correction that cannot halve the ratio saved
+    factor.set_refine_stop(Y.STOP_AT_FLOOR)
+    for _ in range(max(1, args.warmup)):
+        factor.solve_staged(Y.FP32_REFINED, 1e-12, 10).free()
+    factor.reset_kernel_stats()
+    steps = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        block = factor.solve_staged(Y.FP32_REFINED, 1e-12, 10)
+        if i + 1 < steps:
+            block.free()
+    elapsed = (time.perf_counter() - t0) / steps
+    fstats = factor.kernel_stats()
+    x32f = block.solutions()
+    corr_f, ratio_f = block.refinement()
+    block.free()
+    factor.set_refine_stop(Y.STOP_NORMWISE)
+    out["fp32_refined_floor_stop"] = dict(seconds=elapsed, value=512 * nnz_lu / elapsed,
+                                          kernel_ms=fstats.milliseconds / fstats.launches,
+                                          max_relative_residual=float(np.max(ratio_f)),
+                                          mean_steps=float(corr_f.mean()))
+    log(f"[precision] fp32_refined, floor stop: {elapsed * 1e3:.3f} ms/solve, residual <= {np.max(ratio_f):.2e}")
     # the FP32 sweeps alone: no correction (max_steps = 0), so one FP32 solve and one residual
     for _ in range(max(1, args.warmup)):
         factor.solve_staged(Y.FP32_REFINED, 1e-12, 0).free()
@@ -262,6 +284,9 @@ def run_precision(gpu, args):
     out["fp64_vs_cpu_max_rel_diff"] = float(np.max(np.max(np.abs(x64 - cpu), axis=1) / scale))
     out["fp64_bit_exact_columns"] = int(sum(a.tobytes() == b.tobytes() for a, b in zip(x64, cpu)))
     out["fp32_refined_vs_cpu_fp64_max_rel_diff"] = float(np.max(np.max(np.abs(x32 - cpu), axis=1) / scale))
+    out["fp32_refined_floor_stop"]["vs_cpu_fp64_max_rel_diff"] = float(
+        np.max(np.max(np.abs(x32f - cpu), axis=1) / scale))
+    out["speedup_fp32_refined_floor_stop_over_fp64"] = out["fp32_refined_floor_stop"]["value"] / out["fp64"]["value"]
     # conditioning: the oracle's 1-norm estimate (accuracy.rs:207-279) and the infinity-norm one
     # that bounds ||x32 - x64|| by ||A^-1||inf times the residuals
     inv1 = configs.inverse_norm_1_estimate(lu.n, lu.solve, lu.solve_transpose)
@@ -385,6 +410,18 @@ def run_adjoint(gpu, args):
     torch.cuda.synchronize()
     seconds32 = (time.perf_counter() - t0) / reps
     stats32 = factor.kernel_stats()
+    # and with the floor stop rule (the correction that cannot halve the ratio saved)
+    factor.set_refine_stop(Y.STOP_AT_FLOOR)
+    step(FP32_REFINED)
+    torch.cuda.synchronize()
+    factor.reset_kernel_stats()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, theta32f, a_grad32f, b_grad32f = step(FP32_REFINED)
+    torch.cuda.synchronize()
+    seconds32f = (time.perf_counter() - t0) / reps
+    stats32f = factor.kernel_stats()
+    factor.set_refine_stop(Y.STOP_NORMWISE)
 
     def rel(got, want):
         return float((torch.max(torch.abs(got - want)) / torch.max(torch.abs(want))).detach())
@@ -399,6 +436,11 @@ def run_adjoint(gpu, args):
                fp32_refined=dict(ms_per_step=seconds32 * 1e3, library_kernel_ms_per_step=stats32.milliseconds / reps,
                                  theta_rel_err_vs_fp64=rel(theta32, theta), b_bar_rel_err_vs_fp64=rel(b_grad32, b_grad),
                                  a_bar_rel_err_vs_fp64=rel(a_grad32, a_grad)),
+               fp32_refined_floor_stop=dict(ms_per_step=seconds32f * 1e3,
+                                            library_kernel_ms_per_step=stats32f.milliseconds / reps,
+                                            theta_rel_err_vs_fp64=rel(theta32f, theta),
+                                            b_bar_rel_err_vs_fp64=rel(b_grad32f, b_grad),
+                                            a_bar_rel_err_vs_fp64=rel(a_grad32f, a_grad)),
                note="one step = forward solve, flows and loss, backward (the library's transposed solve and "
                     "SDDMM), all in device memory; library kernels = forward solve + transposed solve + SDDMM")
     log(f"[adjoint] {out['ms_per_step']:.2f} ms per device step (library kernels "
@@ -455,6 +497,18 @@ def run_batch(gpu, args):
     max_ratio = max(float(r.max()) for _, r in steps_ratio if len(r))
     mean_steps = float(np.mean(np.concatenate([s for s, _ in steps_ratio])))
     refined_err = max(float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for x, y in zip(refined, solved) if y.size)
+    # and with the floor stop rule
+    batch.set_refine_stop(Y.STOP_AT_FLOOR)
+    batch.solve_staged(Y.FP32_REFINED, 1e-12, 10)
+    batch.reset_kernel_stats()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.solve_staged(Y.FP32_REFINED, 1e-12, 10)
+    floor_s = (time.perf_counter() - t0) / reps
+    floor_stats = batch.kernel_stats()
+    floor_x = batch.solutions()
+    floor_ratio = max(float(r.max()) for _, r in batch.refinement() if len(r))
+    batch.set_refine_stop(Y.STOP_NORMWISE)
     # one launch per factor, the same blocks
     for f, b in zip(factors, rhs):
         f.stage(b)
@@ -472,6 +526,7 @@ def run_batch(gpu, args):
     cpu_x = [c.solve_many(b, cpu_threads()) for c, b in zip(cpu, rhs)]
     exact = sum(x.tobytes() == w.tobytes() for x, w in zip(solved, cpu_x))
     refined_vs_cpu = max(float(np.max(np.abs(x - w)) / np.max(np.abs(w))) for x, w in zip(refined, cpu_x) if w.size)
+    floor_vs_cpu = max(float(np.max(np.abs(x - w)) / np.max(np.abs(w))) for x, w in zip(floor_x, cpu_x) if w.size)
     out = dict(workload="256 grids, n_i log-uniform in [499, 19,999] (sum n = %d), r_i in 1..64 (sum r = %d), "
                         "AMD Cholesky, one non-bridge branch out per grid" % (int(sum(c.n for c in cpu)),
                                                                               int(counts.sum())),
@@ -482,6 +537,8 @@ def run_batch(gpu, args):
                fp32_refined=dict(batch_ms=refined_s * 1e3, batch_kernel_ms=refined_stats.milliseconds / reps,
                                  max_relative_residual=max_ratio, mean_corrections=mean_steps,
                                  max_rel_diff_vs_fp64=refined_err, max_rel_diff_vs_cpu_fp64=refined_vs_cpu),
+               fp32_refined_floor_stop=dict(batch_ms=floor_s * 1e3, batch_kernel_ms=floor_stats.milliseconds / reps,
+                                            max_relative_residual=floor_ratio, max_rel_diff_vs_cpu_fp64=floor_vs_cpu),
                bit_exact_factors=f"{exact} of {grids}")
     log(f"[batch] batch {out['batch_ms']:.3f} ms (kernel {out['batch_kernel_ms']:.3f}), one launch per factor "
         f"{out['one_launch_per_factor_ms']:.3f} ms; bit-exact {out['bit_exact_factors']}; FP32-refined "

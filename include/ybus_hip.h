@@ -242,6 +242,16 @@ int32_t ybus_hip_factor_solve_many_complex(ybus_hip_factor *f, const double *rhs
 int32_t ybus_hip_factor_solve_transpose_many_complex(ybus_hip_factor *f, const double *rhs,
                                                      uint64_t count, double *x);
 
+/* The stop rule of the factor's FP32-refined solves (new), each column's: YBUS_HIP_STOP_NORMWISE
+ * (the default) is refine_normwise's (accuracy.rs:99-120): stop at `tolerance`, when a correction
+ * fails to halve the ratio, on NaN, or after max_steps. YBUS_HIP_STOP_AT_FLOOR also stops as soon
+ * as the ratio is at most 2 u || |A| |x| ||inf / ||b||inf (u = 2^-53), the rounding floor of an
+ * FP64 residual, taken in the same residual pass: where `tolerance` is below the floor, the
+ * correction after it would not halve the ratio, so the floor rule saves that correction and
+ * keeps the same iterate. It applies to host, staged, device, complex and adjoint refinement. */
+enum { YBUS_HIP_STOP_NORMWISE = 0, YBUS_HIP_STOP_AT_FLOOR = 1 };
+int32_t ybus_hip_factor_set_refine_stop(ybus_hip_factor *f, int32_t rule);
+
 /* The complex matrix A of a complex factor (new), which FP32-refined complex solves need for their
  * FP64 residual b - A x: rows x rows in CSR (row_major = 1) or CSC (row_major = 0) form, values as
  * interleaved (re, im) doubles (Complex64). The refined solves take the residual as
@@ -449,6 +459,8 @@ int32_t ybus_hip_batch_create(ybus_hip_ctx *ctx, ybus_hip_batch **out);
 void ybus_hip_batch_free(ybus_hip_batch *b);
 int32_t ybus_hip_batch_add(ybus_hip_batch *b, ybus_hip_factor *f);
 uint64_t ybus_hip_batch_factors(const ybus_hip_batch *b);
+/* The stop rule of the batch's FP32-refined solves (ybus_hip_factor_set_refine_stop's rules). */
+int32_t ybus_hip_batch_set_refine_stop(ybus_hip_batch *b, int32_t rule);
 /* Copies the right-hand sides to the device: counts[i] vectors of factor i's dim(), for each
  * factor in the order added, all one after another. */
 int32_t ybus_hip_batch_stage(ybus_hip_batch *b, const double *rhs, const uint64_t *counts);

@@ -23,6 +23,8 @@ LIB_PATH = os.path.join(_HERE, "libybus_hip.so")
 
 OK, NO_DEVICE, INVALID_ARGUMENT, SHAPE_MISMATCH, TOO_LARGE, DEVICE_ERROR, OUT_OF_MEMORY, INTERNAL, PATTERN_MISMATCH = range(9)
 FP64, FP32_REFINED, TRANSPOSE = 0, 1, 2
+# refined solves' stop rules (ybus_hip_factor_set_refine_stop)
+STOP_NORMWISE, STOP_AT_FLOOR = 0, 1
 SCRATCH_BYTES = 64 << 20  # YBUS_HIP_SCRATCH_BYTES: the no-allocation workspace per device
 
 # ---- errors: CudaError (ybus_cuda/src/lib.rs:88-124) ----
@@ -132,6 +134,8 @@ def _declare(L):
         "ybus_hip_factor_solve_many_complex": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_solve_transpose_many_complex": (i32, [P, f64p, U64, f64p]),
         "ybus_hip_factor_set_matrix_complex": (i32, [P, i32, u64p, u64p, f64p]),
+        "ybus_hip_factor_set_refine_stop": (i32, [P, i32]),
+        "ybus_hip_batch_set_refine_stop": (i32, [P, i32]),
         "ybus_hip_factor_solve_many_refined_complex": (i32, [P, f64p, U64, C.c_double, U64, f64p, u64p, f64p]),
         "ybus_hip_factor_solve_device_complex": (i32, [P, i32, C.c_double, U64, C.c_void_p, U64, U64, C.c_void_p,
                                                        U64]),
@@ -503,6 +507,12 @@ class GpuFactor:
 
     def block_width(self) -> int:
         return int(library().ybus_hip_factor_block_width(self._h))
+
+    def set_refine_stop(self, rule: int):
+        """The FP32-refined solves' stop rule: STOP_NORMWISE (refine_normwise's, the default) or
+        STOP_AT_FLOOR (also stop once the ratio is within 2x of the FP64 residual's rounding
+        floor, 2 u || |A| |x| ||inf / ||b||inf, saving the correction that would not halve it)."""
+        _check(library().ybus_hip_factor_set_refine_stop(self._h, int(rule)))
 
     def set_block_width(self, width: int):
         library().ybus_hip_factor_set_block_width(self._h, int(width))
@@ -1018,6 +1028,10 @@ class GpuBatch:
 
     def factors(self) -> int:
         return int(library().ybus_hip_batch_factors(self._h))
+
+    def set_refine_stop(self, rule: int):
+        """The stop rule of the batch's FP32-refined solves (GpuFactor.set_refine_stop's rules)."""
+        _check(library().ybus_hip_batch_set_refine_stop(self._h, int(rule)))
 
     def stage(self, rhs):
         """rhs: one (count_i, dim_i) array per factor, in the order added."""

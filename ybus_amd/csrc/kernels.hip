@@ -66,6 +66,12 @@ __device__ __forceinline__ double ratio_of(double r_norm, double b_norm) {
   return b_norm > 0.0 ? r_norm / b_norm : (r_norm > 0.0 ? __builtin_inf() : 0.0);
 }
 
+// The stop rule's floor: twice the unit roundoff's share of ||A x|| that no FP64 residual gets
+// below, 2 u || |A| |x| ||inf / ||b||inf (0 without the floor rule, or for b = 0)
+__device__ __forceinline__ double floor_of(bool floor_rule, double ax_norm, double b_norm) {
+  return floor_rule && b_norm > 0.0 ? 2.0 * 0x1p-53 * (ax_norm / b_norm) : 0.0;
+}
+
 // Reduces each lane's value over the workgroup's slots into out[lane] (every thread must call).
 template <int TW>
 __device__ void reduce_max(double* scratch, double mine, double* out, const Tile<TW>& t) {
@@ -572,18 +578,27 @@ __device__ void zsolve32(const ComplexSweepDev& fwd, const ComplexSweepDev& bwd,
 // r = b - A x over the tile's active columns in FP64 (CsrMatrix::mul_vec's order: each row summed
 // from zero over its entries in index order, product.rs:17-27, then subtracted from b); returns
 // the thread's largest |r_i|
+// (ax non-null: also the thread's partial max of sum |a_ij| |x_j| into *ax)
 template <int TW>
 __device__ double zresidual(const ZCsrDev& a, const double* __restrict__ b, const double* __restrict__ x,
-                            double* __restrict__ r, int64_t ld, const Tile<TW>& g, bool act) {
-  double m = 0.0;
+                            double* __restrict__ r, int64_t ld, const Tile<TW>& g, bool act, double* ax = nullptr) {
+  double m = 0.0, axm = 0.0;
+  if (ax) *ax = 0.0;
   if (!act) return m;
   for (int32_t i = g.slot; i < a.rows; i += g.slots) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int32_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p) acc = zadd(acc, zmul(a.val[p], zload<double>(x, a.col[p], ld, g.col)));
+    double abs_sum = 0.0;
+    for (int32_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p) {
+      const double2 xj = zload<double>(x, a.col[p], ld, g.col);
+      acc = zadd(acc, zmul(a.val[p], xj));
+      if (ax) abs_sum = abs_sum + zabs(a.val[p]) * zabs(xj);
+    }
     const double2 ri = zsub(zload<double>(b, i, ld, g.col), acc);
     zstore<double>(r, i, ld, g.col, ri);
     m = fold_max(m, zabs(ri));
+    axm = fold_max(axm, abs_sum);
   }
+  if (ax) *ax = axm;
   return m;
 }
 
@@ -598,9 +613,10 @@ __global__ void __launch_bounds__(1024) refine_complex_kernel(ComplexSweepDev fw
                                                               double* __restrict__ x1, double* __restrict__ r,
                                                               int64_t ld, int32_t ncols, double tol, int32_t max_steps,
                                                               int32_t* __restrict__ steps_out,
-                                                              double* __restrict__ ratio_out, int32_t xcds) {
+                                                              double* __restrict__ ratio_out, int32_t xcds,
+                                                              int32_t floor_rule) {
   __shared__ double scratch[1024];
-  __shared__ double s_bnorm[TW], s_rnorm[TW], s_ratio[TW], s_best[TW];
+  __shared__ double s_bnorm[TW], s_rnorm[TW], s_ratio[TW], s_best[TW], s_axnorm[TW];
   __shared__ int32_t s_best_buf[TW], s_steps[TW], s_active[TW];
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* lpf = reinterpret_cast<int32_t*>(smem);
@@ -622,15 +638,18 @@ __global__ void __launch_bounds__(1024) refine_complex_kernel(ComplexSweepDev fw
   if (g.on)
     for (int32_t k = g.slot; k < rows; k += g.slots) x0[k * ld + col] = (double)d[k * ld + col];
   __syncthreads();
-  m = zresidual<TW>(a, b, x0, r, ld, g, g.on);
+  double ax = 0.0;
+  m = zresidual<TW>(a, b, x0, r, ld, g, g.on, floor_rule ? &ax : nullptr);
   reduce_max<TW>(scratch, m, s_rnorm, g);
+  if (floor_rule) reduce_max<TW>(scratch, ax, s_axnorm, g);
   if (g.slot == 0) {
     const double ratio = ratio_of(s_rnorm[g.lane], s_bnorm[g.lane]);
     s_ratio[g.lane] = ratio;
     s_best[g.lane] = ratio;
     s_best_buf[g.lane] = 0;
     s_steps[g.lane] = 0;
-    s_active[g.lane] = g.on && max_steps > 0 && !(ratio <= tol);
+    s_active[g.lane] = g.on && max_steps > 0 && !(ratio <= tol) &&
+                       !(ratio <= floor_of(floor_rule, s_axnorm[g.lane], s_bnorm[g.lane]));
   }
   __syncthreads();
 
@@ -645,8 +664,9 @@ __global__ void __launch_bounds__(1024) refine_complex_kernel(ComplexSweepDev fw
     if (act)
       for (int32_t k = g.slot; k < rows; k += g.slots) nxt[k * ld + col] = cur[k * ld + col] + (double)d[k * ld + col];
     __syncthreads();
-    m = zresidual<TW>(a, b, nxt, r, ld, g, act);
+    m = zresidual<TW>(a, b, nxt, r, ld, g, act, floor_rule ? &ax : nullptr);
     reduce_max<TW>(scratch, m, s_rnorm, g);
+    if (floor_rule) reduce_max<TW>(scratch, ax, s_axnorm, g);
     if (g.slot == 0 && act) {
       const double previous = s_ratio[g.lane];
       const double ratio = ratio_of(s_rnorm[g.lane], s_bnorm[g.lane]);
@@ -656,7 +676,8 @@ __global__ void __launch_bounds__(1024) refine_complex_kernel(ComplexSweepDev fw
         s_best_buf[g.lane] = nxt == x1 ? 1 : 0;
         s_steps[g.lane] = step;
       }
-      const bool stop = ratio != ratio || ratio > 0.5 * previous || ratio <= tol || step == max_steps;
+      const bool stop = ratio != ratio || ratio > 0.5 * previous || ratio <= tol || step == max_steps ||
+                        ratio <= floor_of(floor_rule, s_axnorm[g.lane], s_bnorm[g.lane]);
       s_active[g.lane] = !stop;
     }
     __syncthreads();
@@ -1041,10 +1062,13 @@ __device__ void combine(const double* __restrict__ xc, const float* __restrict__
 // taken RESIDUAL_CHUNK at a time: their columns and values loaded together, then their entries of
 // x together, then summed in order, so a row waits for two rounds of loads per chunk rather than
 // per term; the next row's bounds are loaded during this one.
+// With ax non-null (the floor stop rule) it also takes this thread's partial max of
+// sum |a_ij| |x_j| over its rows into *ax.
 template <int TW>
 __device__ double residual(const CsrDev& a, const double* __restrict__ b, const double* __restrict__ x,
-                           double* __restrict__ r, int64_t ld, const Tile<TW>& g, bool act) {
-  double m = 0.0;
+                           double* __restrict__ r, int64_t ld, const Tile<TW>& g, bool act, double* ax = nullptr) {
+  double m = 0.0, axm = 0.0;
+  if (ax) *ax = 0.0;
   if (!act) return m;
   constexpr int RESIDUAL_CHUNK = 4;
   int32_t i = g.slot;
@@ -1054,7 +1078,7 @@ __device__ double residual(const CsrDev& a, const double* __restrict__ b, const 
     const int32_t n0 = next < a.rows ? a.ptr[next] : 0, n1 = next < a.rows ? a.ptr[next + 1] : 0;
     const int64_t at = i * ld + g.col;
     const double bi = b[at];
-    double sum = 0.0;
+    double sum = 0.0, abs_sum = 0.0;
     for (int32_t p = p0; p < p1; p += RESIDUAL_CHUNK) {
       int32_t cc[RESIDUAL_CHUNK];
       double vv[RESIDUAL_CHUNK], xx[RESIDUAL_CHUNK];
@@ -1069,15 +1093,23 @@ __device__ double residual(const CsrDev& a, const double* __restrict__ b, const 
 #pragma unroll
       for (int q = 0; q < RESIDUAL_CHUNK; ++q)
         if (p + q < p1) sum = sum + vv[q] * xx[q];
+      if (ax) {
+#pragma unroll
+        for (int q = 0; q < RESIDUAL_CHUNK; ++q)
+          if (p + q < p1) abs_sum = abs_sum + fabs(vv[q]) * fabs(xx[q]);
+      }
     }
     const double v = bi - sum;
     r[at] = v;
     m = fold_max(m, fabs(v));
+    axm = fold_max(axm, abs_sum);
     p0 = n0;
     p1 = n1;
   }
+  if (ax) *ax = axm;
   return m;
 }
+
 
 template <int TW, bool Split>
 __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd, CsrDev a, const double* __restrict__ b,
@@ -1085,9 +1117,10 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
                                                       double* __restrict__ x0, double* __restrict__ x1,
                                                       double* __restrict__ r, int64_t ld, int32_t ncols, double tol,
                                                       int32_t max_steps, int32_t* __restrict__ steps_out,
-                                                      double* __restrict__ ratio_out, int32_t xcds) {
+                                                      double* __restrict__ ratio_out, int32_t xcds,
+                                                      int32_t floor_rule) {
   __shared__ double scratch[1024];
-  __shared__ double s_bnorm[TW], s_rnorm[TW], s_ratio[TW], s_best[TW];
+  __shared__ double s_bnorm[TW], s_rnorm[TW], s_ratio[TW], s_best[TW], s_axnorm[TW];
   __shared__ int32_t s_best_buf[TW], s_steps[TW], s_active[TW];
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* lpf = reinterpret_cast<int32_t*>(smem);
@@ -1110,15 +1143,18 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
     combine<TW>(nullptr, d, x0, a.rows, ld, g, g.on);
     __syncthreads();
   }
-  m = residual<TW>(a, b, x0, r, ld, g, g.on);
+  double ax = 0.0;
+  m = residual<TW>(a, b, x0, r, ld, g, g.on, floor_rule ? &ax : nullptr);
   reduce_max<TW>(scratch, m, s_rnorm, g);
+  if (floor_rule) reduce_max<TW>(scratch, ax, s_axnorm, g);
   if (g.slot == 0) {
     const double ratio = ratio_of(s_rnorm[g.lane], s_bnorm[g.lane]);
     s_ratio[g.lane] = ratio;
     s_best[g.lane] = ratio;
     s_best_buf[g.lane] = 0;
     s_steps[g.lane] = 0;
-    s_active[g.lane] = g.on && max_steps > 0 && !(ratio <= tol);
+    s_active[g.lane] = g.on && max_steps > 0 && !(ratio <= tol) &&
+                       !(ratio <= floor_of(floor_rule, s_axnorm[g.lane], s_bnorm[g.lane]));
   }
   __syncthreads();
 
@@ -1139,8 +1175,9 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
       combine<TW>(cur, d, nxt, a.rows, ld, g, act);
       __syncthreads();
     }
-    m = residual<TW>(a, b, nxt, r, ld, g, act);
+    m = residual<TW>(a, b, nxt, r, ld, g, act, floor_rule ? &ax : nullptr);
     reduce_max<TW>(scratch, m, s_rnorm, g);
+    if (floor_rule) reduce_max<TW>(scratch, ax, s_axnorm, g);
     if (g.slot == 0 && act) {
       const double previous = s_ratio[g.lane];
       const double ratio = ratio_of(s_rnorm[g.lane], s_bnorm[g.lane]);
@@ -1150,7 +1187,8 @@ __global__ void __launch_bounds__(1024) refine_kernel(SweepDev fwd, SweepDev bwd
         s_best_buf[g.lane] = nxt == x1 ? 1 : 0;
         s_steps[g.lane] = step;
       }
-      const bool stop = ratio != ratio || ratio > 0.5 * previous || ratio <= tol || step == max_steps;
+      const bool stop = ratio != ratio || ratio > 0.5 * previous || ratio <= tol || step == max_steps ||
+                        ratio <= floor_of(floor_rule, s_axnorm[g.lane], s_bnorm[g.lane]);
       s_active[g.lane] = !stop;
     }
     __syncthreads();
@@ -1320,17 +1358,22 @@ __device__ double block_max(double* scratch, double mine) {
 
 // r = b - A x for one column (A by user rows, each row summed from 0 in index order as
 // mul_vec's dot_kernel does); returns this thread's partial max |r|
+// (ax non-null: also this thread's partial max of sum |a_ij| |x_j| into *ax)
 __device__ double column_residual(const CsrDev& a, const double* __restrict__ b, const double* __restrict__ x,
-                                  double* __restrict__ r) {
-  double m = 0.0;
+                                  double* __restrict__ r, double* ax = nullptr) {
+  double m = 0.0, axm = 0.0;
   for (int32_t u = threadIdx.x; u < a.rows; u += blockDim.x) {
-    double sum = 0.0;
+    double sum = 0.0, abs_sum = 0.0;
     const int32_t pe = a.ptr[u + 1];
     for (int32_t p = a.ptr[u]; p < pe; ++p) sum = sum + a.val[p] * x[a.col[p]];
+    if (ax)
+      for (int32_t p = a.ptr[u]; p < pe; ++p) abs_sum = abs_sum + fabs(a.val[p]) * fabs(x[a.col[p]]);
     const double v = b[u] - sum;
     r[u] = v;
     m = fold_max(m, fabs(v));
+    axm = fold_max(axm, abs_sum);
   }
+  if (ax) *ax = axm;
   return m;
 }
 
@@ -1346,7 +1389,7 @@ __device__ void refine_column(const SweepDev& fwd, const SweepDev& bwd, const Cs
                               const int32_t* __restrict__ rhs_row, const int32_t* __restrict__ sol_of,
                               const double* __restrict__ b, double* __restrict__ xa, double* __restrict__ xb,
                               double* __restrict__ r, double tol, int32_t max_steps, int32_t* __restrict__ steps_out,
-                              double* __restrict__ ratio_out, unsigned char* smem) {
+                              double* __restrict__ ratio_out, unsigned char* smem, bool floor_rule) {
   double* scratch = reinterpret_cast<double*>(smem);
   const int32_t n = a.rows;
   float* xs = reinterpret_cast<float*>(smem + blockDim.x * sizeof(double));
@@ -1365,11 +1408,13 @@ __device__ void refine_column(const SweepDev& fwd, const SweepDev& bwd, const Cs
   lds_sweep<float>(bwd, lpb, xs);
   for (int32_t u = threadIdx.x; u < n; u += blockDim.x) xa[u] = (double)xs[sol_of[u]];
   __syncthreads();
-  double ratio = ratio_of(block_max(scratch, column_residual(a, b, xa, r)), bnorm);
+  double ax = 0.0;
+  double ratio = ratio_of(block_max(scratch, column_residual(a, b, xa, r, floor_rule ? &ax : nullptr)), bnorm);
+  double stop_at = floor_rule ? floor_of(true, block_max(scratch, ax), bnorm) : 0.0;
   double best = ratio;
   int32_t best_steps = 0;
   bool best_in_b = false;
-  bool active = max_steps > 0 && !(ratio <= tol);
+  bool active = max_steps > 0 && !(ratio <= tol) && !(ratio <= stop_at);
   double* cur = xa;
   double* nxt = xb;
   for (int32_t step = 1; active; ++step) {
@@ -1380,13 +1425,14 @@ __device__ void refine_column(const SweepDev& fwd, const SweepDev& bwd, const Cs
     for (int32_t u = threadIdx.x; u < n; u += blockDim.x) nxt[u] = cur[u] + (double)xs[sol_of[u]];
     __syncthreads();
     const double previous = ratio;
-    ratio = ratio_of(block_max(scratch, column_residual(a, b, nxt, r)), bnorm);
+    ratio = ratio_of(block_max(scratch, column_residual(a, b, nxt, r, floor_rule ? &ax : nullptr)), bnorm);
+    if (floor_rule) stop_at = floor_of(true, block_max(scratch, ax), bnorm);
     if (ratio < best) {
       best = ratio;
       best_steps = step;
       best_in_b = nxt == xb;
     }
-    active = !(ratio != ratio || ratio > 0.5 * previous || ratio <= tol || step == max_steps);
+    active = !(ratio != ratio || ratio > 0.5 * previous || ratio <= tol || step == max_steps || ratio <= stop_at);
     double* swap = cur;
     cur = nxt;
     nxt = swap;
@@ -1406,11 +1452,11 @@ __global__ void __launch_bounds__(1024) refine_lds_kernel(SweepDev fwd, SweepDev
                                                           double* __restrict__ x1b, double* __restrict__ rblock,
                                                           int64_t ld, double tol, int32_t max_steps,
                                                           int32_t* __restrict__ steps_out,
-                                                          double* __restrict__ ratio_out) {
+                                                          double* __restrict__ ratio_out, int32_t floor_rule) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int64_t col = blockIdx.x;
   refine_column(fwd, bwd, a, rhs_row, sol_of, bblock + col * ld, x0b + col * ld, x1b + col * ld, rblock + col * ld, tol,
-                max_steps, steps_out + col, ratio_out + col, smem);
+                max_steps, steps_out + col, ratio_out + col, smem, floor_rule != 0);
 }
 
 // ---- layout changes ----
@@ -1790,7 +1836,8 @@ __global__ void __launch_bounds__(1024) batch_refine_kernel(const BatchEntryDev*
                                                             int32_t* __restrict__ queue, const double* __restrict__ rhs,
                                                             double* __restrict__ x, double* __restrict__ x1,
                                                             double* __restrict__ r, double tol, int32_t max_steps,
-                                                            int32_t* __restrict__ steps, double* __restrict__ ratio) {
+                                                            int32_t* __restrict__ steps, double* __restrict__ ratio,
+                                                            int32_t floor_rule) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t s_item;
   for (;;) {
@@ -1804,7 +1851,7 @@ __global__ void __launch_bounds__(1024) batch_refine_kernel(const BatchEntryDev*
     const int64_t at = e.offset + (int64_t)w.y * e.n;
     const int64_t column = e.first_column + w.y;
     refine_column(e.fwd, e.bwd, e.a, e.rhs_row, e.sol_of, rhs + at, x + at, x1 + at, r + at, tol, max_steps,
-                  steps + column, ratio + column, smem);
+                  steps + column, ratio + column, smem, floor_rule != 0);
   }
 }
 
@@ -1995,7 +2042,8 @@ hipError_t launch_solve64_reach(hipStream_t stream, int tile, int threads, const
 
 hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                          const CsrDev& a, const double* b, float* t, float* d, double* x0, double* x1, double* r,
-                         int64_t ld, int32_t ncols, double tol, int32_t max_steps, int32_t* steps, double* ratio) {
+                         int64_t ld, int32_t ncols, double tol, int32_t max_steps, int32_t* steps, double* ratio,
+                         int32_t floor_rule) {
   if (threads < tile || threads > 1024 || threads % 64 || (threads & (threads - 1))) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
   const size_t bytes = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
@@ -2005,7 +2053,7 @@ hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepD
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
     if (e != hipSuccess) return e;                                                                               \
     hipLaunchKernelGGL((refine_kernel<TWV, SPLIT>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, d, \
-                       x0, x1, r, ld, ncols, tol, max_steps, steps, ratio, xcd_groups());                        \
+                       x0, x1, r, ld, ncols, tol, max_steps, steps, ratio, xcd_groups(), floor_rule);            \
   }
 #define L(TWV)         \
   if (split)           \
@@ -2041,7 +2089,7 @@ hipError_t launch_solve_complex(hipStream_t stream, int tile, int threads, const
 hipError_t launch_refine_complex(hipStream_t stream, int tile, int threads, const ComplexSweepDev& fwd,
                                  const ComplexSweepDev& bwd, const ZCsrDev& a, const double* b, float* t, float* d,
                                  double* x0, double* x1, double* r, int64_t ld, int32_t ncols, double tol,
-                                 int32_t max_steps, int32_t* steps, double* ratio) {
+                                 int32_t max_steps, int32_t* steps, double* ratio, int32_t floor_rule) {
   if (threads < tile || threads > 1024 || threads % 64 || (threads & (threads - 1)) || ld < ncols ||
       !fwd.val32 || !bwd.val32)
     return hipErrorInvalidValue;
@@ -2053,7 +2101,7 @@ hipError_t launch_refine_complex(hipStream_t stream, int tile, int threads, cons
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                  \
     if (e != hipSuccess) return e;                                                                               \
     hipLaunchKernelGGL((refine_complex_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, bwd, a, b, t, \
-                       d, x0, x1, r, ld, ncols, tol, max_steps, steps, ratio, xcd_groups());                     \
+                       d, x0, x1, r, ld, ncols, tol, max_steps, steps, ratio, xcd_groups(), floor_rule);         \
   }
   DISPATCH_TILE(tile, L)
 #undef L
@@ -2191,10 +2239,10 @@ hipError_t prepare_batch_refine(size_t lds_bytes) {
 hipError_t launch_batch_refine(hipStream_t stream, int grid, int threads, size_t lds_bytes,
                                const BatchEntryDev* entries, const int2* items, int32_t total, int32_t* queue,
                                const double* rhs, double* x, double* x1, double* r, double tol, int32_t max_steps,
-                               int32_t* steps, double* ratio) {
+                               int32_t* steps, double* ratio, int32_t floor_rule) {
   if (grid <= 0 || threads < 64 || threads > 1024 || (threads & (threads - 1))) return hipErrorInvalidValue;
   hipLaunchKernelGGL(batch_refine_kernel, dim3(grid), dim3(threads), lds_bytes, stream, entries, items, total, queue,
-                     rhs, x, x1, r, tol, max_steps, steps, ratio);
+                     rhs, x, x1, r, tol, max_steps, steps, ratio, floor_rule);
   return hipGetLastError();
 }
 
@@ -2271,14 +2319,14 @@ hipError_t launch_solve_lds(hipStream_t stream, int threads, bool fp32, const Sw
 hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fwd, const SweepDev& bwd,
                              const CsrDev& a, const int32_t* rhs_row, const int32_t* sol_of, const double* b,
                              double* x0, double* x1, double* r, int64_t ld, int32_t ncols, double tol,
-                             int32_t max_steps, int32_t* steps, double* ratio) {
+                             int32_t max_steps, int32_t* steps, double* ratio, int32_t floor_rule) {
   if (threads < 64 || threads > 1024 || (threads & (threads - 1))) return hipErrorInvalidValue;
   const size_t bytes = refine_lds_bytes(a.rows, threads, fwd.levels + bwd.levels);
   hipError_t e = hipFuncSetAttribute((const void*)refine_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)bytes);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(refine_lds_kernel, dim3(ncols), dim3(threads), bytes, stream, fwd, bwd, a, rhs_row, sol_of, b, x0,
-                     x1, r, ld, tol, max_steps, steps, ratio);
+                     x1, r, ld, tol, max_steps, steps, ratio, floor_rule);
   return hipGetLastError();
 }
 

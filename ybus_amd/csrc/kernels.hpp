@@ -126,7 +126,7 @@ hipError_t launch_solve_lds(hipStream_t stream, int threads, bool fp32, const Sw
 hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fwd, const SweepDev& bwd,
                              const CsrDev& a, const int32_t* rhs_row, const int32_t* sol_of, const double* b,
                              double* x0, double* x1, double* r, int64_t ld, int32_t ncols, double tol,
-                             int32_t max_steps, int32_t* steps, double* ratio);
+                             int32_t max_steps, int32_t* steps, double* ratio, int32_t floor_rule);
 
 // FP64 forward then backward sweep over column tiles of width `tile` (1..64, a power of two),
 // `threads` per workgroup (64..1024): t (packed) is the forward result, x the solution. t and x
@@ -152,11 +152,17 @@ hipError_t launch_solve64_reach(hipStream_t stream, int tile, int threads, const
                                 const RhsDev& rhs, const ReachDev& reach, double* t, double* x, int64_t ld,
                                 int32_t ncols, bool skip_loads);
 
+// The refinement launchers' floor_rule: 0 stops each column by refine_normwise's rule alone
+// (accuracy.rs:99-120); 1 stops it also once its ratio is at most 2 u || |A| |x| ||inf / ||b||inf,
+// the rounding floor of an FP64 residual, taken in the same residual pass (the correction after
+// it would not halve the ratio)
+
 // FP32 sweeps refined in FP64 to `tol` (refine_normwise's rule), one workgroup per column tile;
 // the best iterate ends in x0. steps/ratio get each column's corrections and relative residual.
 hipError_t launch_refine(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                          const CsrDev& a, const double* b, float* t, float* d, double* x0, double* x1, double* r,
-                         int64_t ld, int32_t ncols, double tol, int32_t max_steps, int32_t* steps, double* ratio);
+                         int64_t ld, int32_t ncols, double tol, int32_t max_steps, int32_t* steps, double* ratio,
+                         int32_t floor_rule);
 
 // Layout changes between count vectors of n values, one after another (the host's &[Vec<f64>])
 // and a row-interleaved block (n rows x ld, column c of row i at i * ld + c).
@@ -251,7 +257,7 @@ hipError_t launch_solve_complex(hipStream_t stream, int tile, int threads, const
 hipError_t launch_refine_complex(hipStream_t stream, int tile, int threads, const ComplexSweepDev& fwd,
                                  const ComplexSweepDev& bwd, const ZCsrDev& a, const double* b, float* t, float* d,
                                  double* x0, double* x1, double* r, int64_t ld, int32_t ncols, double tol,
-                                 int32_t max_steps, int32_t* steps, double* ratio);
+                                 int32_t max_steps, int32_t* steps, double* ratio, int32_t floor_rule);
 
 hipError_t launch_to_block(hipStream_t stream, const double* vectors, double* block, int64_t n, int64_t count,
                            int64_t ld);
@@ -320,7 +326,7 @@ hipError_t prepare_batch_refine(size_t lds_bytes);
 hipError_t launch_batch_refine(hipStream_t stream, int grid, int threads, size_t lds_bytes,
                                const BatchEntryDev* entries, const int2* items, int32_t total, int32_t* queue,
                                const double* rhs, double* x, double* x1, double* r, double tol, int32_t max_steps,
-                               int32_t* steps, double* ratio);
+                               int32_t* steps, double* ratio, int32_t floor_rule);
 // p[0..count) = value, as a kernel (so it can sit in a captured graph)
 hipError_t launch_fill(hipStream_t stream, int32_t* p, int32_t value, int32_t count);
 

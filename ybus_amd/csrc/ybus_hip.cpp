@@ -665,6 +665,8 @@ struct ybus_hip_factor {
   ybus_hip_kernel_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tile_override = 0, threads_wide = 512, threads_columns = 512;
+  // the refined solves' stop rule (YBUS_HIP_STOP_NORMWISE or YBUS_HIP_STOP_AT_FLOOR)
+  int32_t refine_stop = YBUS_HIP_STOP_NORMWISE;
   bool env_threads = false;  // YBUS_HIP_THREADS set: it overrides the tuned workgroup size
   // YBUS_HIP_TOP=tile[:threads]: the partitioned top kernel's configuration for blocks of vectors
   int top_override = 0, top_override_threads = 0;
@@ -923,7 +925,7 @@ struct ybus_hip_factor {
       while (th * 2 <= threads_wide) th *= 2;
       check(launch_refine_complex(stream(), tile, th, zfwd.view(), zbwd.view(),
                                   ZCsrDev{za.rows, za.ptr.p, za.col.p, za.val.p}, zb.p, zt32.p, zd32.p, zx.p, zx1.p,
-                                  zr.p, ld, (int32_t)chunk, tol, max_steps, isteps.p, dratio.p),
+                                  zr.p, ld, (int32_t)chunk, tol, max_steps, isteps.p, dratio.p, refine_stop),
             "complex refine kernel");
       // the FP32 factor's values (8 bytes) and indices once per solve, the residual's matrix once
       // per residual (16-byte values, 4-byte indices), and each column's vectors; counted for one
@@ -1015,7 +1017,7 @@ struct ybus_hip_factor {
       while (th * 2 <= column_threads(ncols, false)) th *= 2;
       check(launch_refine_lds(stream(), th, (t_sweeps ? ctfwd : cfwd).view(), (t_sweeps ? ctbwd : cbwd).view(), m,
                               (t_sweeps ? t_rhs_row : rhs_row).p, (t_sweeps ? t_sol_of : sol_of).p, b.p, x.p, x1.p,
-                              r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
+                              r.p, layout.ld, (int32_t)ncols, tol, max_steps, isteps.p, dratio.p, refine_stop),
             "refine kernel");
     } else {
       // the refinement sweeps one schedule: a configuration tuned for the partitioned solve does
@@ -1029,7 +1031,7 @@ struct ybus_hip_factor {
       const DevSweep& fs = sp ? (t_sweeps ? stfwd : sfwd) : (t_sweeps ? tfwd : fwd);
       const DevSweep& bs = sp ? (t_sweeps ? stbwd : sbwd) : (t_sweeps ? tbwd : bwd);
       check(launch_refine(stream(), tile, th, fs.view(), bs.view(), m, b.p, t32.p, d32.p, x.p, x1.p, r.p, layout.ld,
-                          (int32_t)ncols, tol, max_steps, isteps.p, dratio.p),
+                          (int32_t)ncols, tol, max_steps, isteps.p, dratio.p, refine_stop),
             "refine kernel");
     }
     // bytes: per column, its own corrections k_c; the factor and the matrix once per sweep and
@@ -2273,6 +2275,14 @@ int32_t ybus_hip_factor_solve_device_complex(ybus_hip_factor* f, int32_t mode, d
   });
 }
 
+int32_t ybus_hip_factor_set_refine_stop(ybus_hip_factor* f, int32_t rule) {
+  return guard([&] {
+    if (!f) fail(YBUS_HIP_INVALID_ARGUMENT, "factor is null");
+    if (rule != YBUS_HIP_STOP_NORMWISE && rule != YBUS_HIP_STOP_AT_FLOOR) fail(YBUS_HIP_INVALID_ARGUMENT, "unknown rule");
+    f->refine_stop = rule;
+  });
+}
+
 int32_t ybus_hip_factor_set_matrix_complex(ybus_hip_factor* f, int32_t row_major, const uint64_t* ptr,
                                            const uint64_t* idx, const double* val) {
   return guard([&] {
@@ -3162,6 +3172,7 @@ struct ybus_hip_batch {
     }
   };
   Plan plans[2];
+  int32_t refine_stop = YBUS_HIP_STOP_NORMWISE;  // the FP32-refined solves' stop rule
   Staging staging;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ybus_hip_kernel_stats stats{};
@@ -3304,7 +3315,7 @@ struct ybus_hip_batch {
       const int grid = std::max(1, std::min<int>(k.count, k.per_cu * ctx->cus));
       if (refined)
         check(launch_batch_refine(on, grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count,
-                                  queues.p + c, rhs.p, x.p, x1.p, r.p, tol, max_steps, steps.p, ratio.p),
+                                  queues.p + c, rhs.p, x.p, x1.p, r.p, tol, max_steps, steps.p, ratio.p, refine_stop),
               "batch kernel");
       else
         check(launch_batch(on, grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count, queues.p + c,
@@ -3333,6 +3344,15 @@ int32_t ybus_hip_batch_create(ybus_hip_ctx* ctx, ybus_hip_batch** out) {
 }
 
 void ybus_hip_batch_free(ybus_hip_batch* b) { delete b; }
+
+int32_t ybus_hip_batch_set_refine_stop(ybus_hip_batch* b, int32_t rule) {
+  return guard([&] {
+    if (!b) fail(YBUS_HIP_INVALID_ARGUMENT, "batch is null");
+    if (rule != YBUS_HIP_STOP_NORMWISE && rule != YBUS_HIP_STOP_AT_FLOOR) fail(YBUS_HIP_INVALID_ARGUMENT, "unknown rule");
+    if (b->refine_stop != rule) b->plans[1].drop_graph();  // the captured launches carry the rule
+    b->refine_stop = rule;
+  });
+}
 
 uint64_t ybus_hip_batch_factors(const ybus_hip_batch* b) { return b ? (uint64_t)b->factors.size() : 0; }
 
