@@ -970,6 +970,36 @@ def copy_peak_gbs(values: int = 1 << 28, repeats: int = 10) -> float:
     return 2 * values * 8 / (ms / 1e3) / 1e9
 
 
+def random_line_peaks_gbs(mib: int = 1024, row_bytes: int = 128, repeats: int = 5):
+    """What this GPU moves for the access pattern of the wide sweeps, whole 128-byte lines at
+    random rows of a large block, beside the plain copy: (gather, scatter) in GB/s of bytes read
+    plus written. The gather reads `mib` MiB of rows in a seeded random order and writes them in
+    order (torch.index_select); the scatter reads them in order and writes them to random rows
+    (Tensor.index_copy_). Timed with events on PyTorch's stream."""
+    import torch
+
+    width = row_bytes // 8
+    rows = mib * (1 << 20) // row_bytes
+    table = torch.ones((rows, width), dtype=torch.float64, device="cuda")
+    out = torch.empty_like(table)
+    order = torch.randperm(rows, generator=torch.Generator().manual_seed(SEED)).to("cuda")
+    rates = []
+    for op in (lambda: torch.index_select(table, 0, order, out=out), lambda: out.index_copy_(0, order, table)):
+        for _ in range(2):
+            op()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(repeats):
+            op()
+        end.record()
+        end.synchronize()
+        ms = start.elapsed_time(end) / repeats
+        rates.append(2 * rows * row_bytes / (ms / 1e3) / 1e9)
+    del table, out, order
+    torch.cuda.empty_cache()
+    return rates[0], rates[1]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1023,6 +1053,8 @@ def main():
     traffic, traffic_source = profiled_traffic(ptdf["kernel"])
     copy_gbs = copy_peak_gbs()
     log(f"[bench] device copy of 2 GiB: {copy_gbs:.0f} GB/s (read + written)")
+    gather_gbs, scatter_gbs = random_line_peaks_gbs()
+    log(f"[bench] 128-byte lines at random rows of 1 GiB: gather {gather_gbs:.0f} GB/s, scatter {scatter_gbs:.0f} GB/s")
 
     # parity: every column against the CPU's solve of its dense vector (bit for bit)
     from oracle import configs
@@ -1068,7 +1100,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": ptdf["kernel"], "kernel_ms": kernel_ms, "bytes_per_launch": bytes_per_launch,
-                     "copy_peak": copy_gbs, "frac_of_copy_peak": achieved / copy_gbs},
+                     "copy_peak": copy_gbs, "frac_of_copy_peak": achieved / copy_gbs,
+                     "random_line_gather_peak": gather_gbs, "random_line_scatter_peak": scatter_gbs,
+                     "counted_traffic_gbs": (traffic / (kernel_ms / 1e3) / 1e9) if traffic else None},
         "cpu_baseline": cpu_baseline,
         "end_to_end_ms": ptdf["end_to_end"] * 1e3,
         "parity": {"columns": rhs_count, "bit_exact_columns": bit_exact, "max_rel_diff_vs_cpu": max_rel},

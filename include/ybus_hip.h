@@ -256,7 +256,8 @@ int32_t ybus_hip_factor_set_refine_stop(ybus_hip_factor *f, int32_t rule);
  * FP64 residual b - A x: rows x rows in CSR (row_major = 1) or CSC (row_major = 0) form, values as
  * interleaved (re, im) doubles (Complex64). The refined solves take the residual as
  * CsrMatrix::mul_vec does (product.rs:17-27), each row summed from zero in index order with
- * num-complex's product. */
+ * num-complex's product. The matrix keeps its values until it is attached again: after a device
+ * refactor or value update, attach the new matrix before the next refined solve. */
 int32_t ybus_hip_factor_set_matrix_complex(ybus_hip_factor *f, int32_t row_major, const uint64_t *ptr,
                                            const uint64_t *idx, const double *val);
 
