@@ -1874,7 +1874,27 @@ __global__ void sddmm_kernel(const int32_t* __restrict__ row, const int32_t* __r
   const double* __restrict__ bi = bbar + (int64_t)row[q] * rs;
   const double* __restrict__ xj = x + (int64_t)col[q] * rs;
   double sum = first ? 0.0 : acc[q];
-  for (int32_t c = 0; c < ncols; ++c) sum = sum + bi[c * cs] * xj[c * cs];
+  int32_t c = 0;
+  if (cs == 1 && rs % 2 == 0) {
+    // a row-interleaved block (rows 16-byte aligned): eight columns of each row per round, their
+    // loads issued together so a row's line is fetched once rather than once per column, then
+    // summed in order
+    constexpr int RUN = 8;
+    for (; c + RUN <= ncols; c += RUN) {
+      double2 b2[RUN / 2], x2[RUN / 2];
+#pragma unroll
+      for (int k = 0; k < RUN / 2; ++k) {
+        b2[k] = *reinterpret_cast<const double2*>(bi + c + 2 * k);
+        x2[k] = *reinterpret_cast<const double2*>(xj + c + 2 * k);
+      }
+#pragma unroll
+      for (int k = 0; k < RUN / 2; ++k) {
+        sum = sum + b2[k].x * x2[k].x;
+        sum = sum + b2[k].y * x2[k].y;
+      }
+    }
+  }
+  for (; c < ncols; ++c) sum = sum + bi[c * cs] * xj[c * cs];
   acc[q] = sum;
 }
 
