@@ -355,6 +355,45 @@ def test_the_top_kernel_takes_its_own_configuration(gpu, monkeypatch, parts):
         assert factor.solve_many(dense).tobytes() == want.tobytes(), (split, tuned)
 
 
+@pytest.mark.parametrize("parts", ["8", "32"])
+def test_the_parts_swept_in_lds_slots(gpu, monkeypatch, parts):
+    """The partitioned solve's parts with their values in LDS slots (YBUS_HIP_PARTS_LDS=tile[:threads],
+    or the autotuner's third stage for blocks of vectors): a wavefront per column of the tile, the
+    records loaded rounds ahead, the backward parts reading the top's solutions as halo slots.
+    Every segment still subtracts its terms in order and divides as the sweep does, so a block of
+    vectors keeps the CPU's bits, over split sweeps and unsplit ones, at every tile width, with
+    the last tile ragged (200 columns) or the block narrower than the tile (3 columns), with the
+    top in LDS or in global memory."""
+    g = O.make_grid(3000, 778)
+    bp, _, _ = g.dc()
+    cpu = O.Cholesky(bp)
+    rng = np.random.default_rng(10)
+    blocks = [rng.standard_normal((k, bp.rows)) for k in (200, 3)]
+    wants = [cpu.solve_many(d, threads=4) for d in blocks]
+    monkeypatch.setenv("YBUS_HIP_PARTS", parts)
+    monkeypatch.setenv("YBUS_HIP_PATH", "wide")
+    for split in ["0", "4"]:
+        monkeypatch.setenv("YBUS_HIP_SPLIT", split)
+        for top_lds in ["0", "1"]:
+            monkeypatch.setenv("YBUS_HIP_TOP_LDS", top_lds)
+            for lds in ["1", "2:128", "4", "8:1024", "16", "0"]:
+                monkeypatch.setenv("YBUS_HIP_PARTS_LDS", lds)
+                factor = Y.GpuFactor.from_cholesky(gpu, cpu)
+                factor.set_block_width(200)
+                for dense, want in zip(blocks, wants):
+                    case = (split, top_lds, lds, len(dense))
+                    assert factor.solve_many(dense).tobytes() == want.tobytes(), case
+                    assert factor.solve_transpose_many(dense).tobytes() == want.tobytes(), case
+        monkeypatch.delenv("YBUS_HIP_PARTS_LDS")
+        monkeypatch.delenv("YBUS_HIP_TOP_LDS")
+        factor = Y.GpuFactor.from_cholesky(gpu, cpu)
+        factor.set_block_width(200)
+        tuned = factor.autotune(200)
+        assert tuned["parts_lds_tile"] in (0, 1, 2, 4, 8, 16), tuned
+        assert tuned["parts_lds_tile"] == 0 or tuned["parts_lds_threads"] >= 64 * tuned["parts_lds_tile"], tuned
+        assert factor.solve_many(blocks[0]).tobytes() == wants[0].tobytes(), (split, tuned)
+
+
 def test_outage_updates_and_flow_maxima_over_partitioned_sweeps(gpu, monkeypatch):
     """The N-1 screening chain (rank-one pair updates, the operator's product maxima, the angle
     maxima) gives the same bits over a partitioned factor as over one schedule."""

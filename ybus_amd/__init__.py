@@ -97,7 +97,8 @@ assert _COLUMN_MAX.itemsize == C.sizeof(_ColumnMax)
 
 class _Tuning(C.Structure):
     _fields_ = [("path", C.c_int32), ("threads", C.c_int32), ("tile", C.c_int32), ("milliseconds", C.c_double),
-                ("parts", C.c_int32), ("top_tile", C.c_int32), ("top_threads", C.c_int32)]
+                ("parts", C.c_int32), ("top_tile", C.c_int32), ("top_threads", C.c_int32),
+                ("parts_lds_tile", C.c_int32), ("parts_lds_threads", C.c_int32)]
 
 
 class _KernelStats(C.Structure):
@@ -535,8 +536,10 @@ class GpuFactor:
         """Times the kernel configurations for blocks of `columns` right-hand sides and keeps the
         fastest for this factor's blocks of that width and kind: pair blocks when `pairs` (True
         for synthetic pairs, or the caller's own list of Pair to time on), else blocks of vectors.
-        Returns {'path', 'threads', 'tile', 'parts', 'top_tile', 'top_threads', 'milliseconds'}
-        (top_*: the partitioned solve's top-of-tree kernel)."""
+        Returns {'path', 'threads', 'tile', 'parts', 'top_tile', 'top_threads', 'parts_lds_tile',
+        'parts_lds_threads', 'milliseconds'} (top_*: the partitioned solve's top-of-tree kernel;
+        parts_lds_*: its parts swept with their values in LDS, tile 0 where they stay in global
+        memory)."""
         self._release_block()
         t = _Tuning()
         plus = minus = None
@@ -548,6 +551,7 @@ class GpuFactor:
                                                   minus.ctypes.data if minus is not None else None, C.byref(t)))
         return dict(path="wide" if t.path == 1 else "columns", threads=int(t.threads), tile=int(t.tile),
                     parts=bool(t.parts), top_tile=int(t.top_tile), top_threads=int(t.top_threads),
+                    parts_lds_tile=int(t.parts_lds_tile), parts_lds_threads=int(t.parts_lds_threads),
                     milliseconds=float(t.milliseconds))
 
     def kernel_stats(self) -> KernelStats:

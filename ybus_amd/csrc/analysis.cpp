@@ -228,6 +228,62 @@ TopSlots top_slots(const Sweep& f, const Sweep& b) {
   return out;
 }
 
+PartSlots part_slots(const Sweep& f, const Sweep& b) {
+  const PartSlots bad;
+  if (f.parts < 1 || b.parts != f.parts || f.n != b.n) return bad;
+  PartSlots out;
+  out.parts = f.parts;
+  std::vector<int32_t> slot_of(f.n, -1), halo_of(f.n, -1);
+  // one sweep's parts; a term outside its part is a halo slot where `halo` allows it
+  auto build = [&](const Sweep& s, PartSlots::Dir& d, bool halo) {
+    d.seg_begin.assign(1, 0);
+    d.slot_begin.assign(1, 0);
+    d.halo_begin.assign(1, 0);
+    for (int32_t q = 0; q < s.parts; ++q) {
+      const int32_t k0 = s.level_ptr[s.part_level[q]], k1 = s.level_ptr[s.part_level[q + 1]];
+      if (k1 <= k0) return false;
+      const size_t first_row = d.out_row.size(), first_halo = d.halo_row.size();
+      int32_t rows = 0, halos = 0;
+      bool ok = true;
+      for (int32_t k = k0; k < k1 && ok; ++k) {
+        const int32_t r = s.dst[k];
+        if (slot_of[r] < 0) {
+          if (s.src[k] < 0) ok = false;  // a continuation before its row's first segment
+          slot_of[r] = rows++;
+          d.in_row.push_back(s.src[k]);
+          d.out_row.push_back(r);
+        }
+        d.slot.push_back(slot_of[r]);
+      }
+      for (int32_t p = s.row_ptr[k0]; p < s.row_ptr[k1] && ok; ++p) {
+        const int32_t r = s.dep[p];
+        if (slot_of[r] >= 0) {
+          d.dep.push_back(slot_of[r]);
+        } else if (!halo) {
+          ok = false;
+        } else {
+          if (halo_of[r] < 0) {
+            halo_of[r] = halos++;
+            d.halo_row.push_back(r);
+          }
+          d.dep.push_back(rows + halo_of[r]);
+        }
+      }
+      for (size_t i = first_row; i < d.out_row.size(); ++i) slot_of[d.out_row[i]] = -1;
+      for (size_t i = first_halo; i < d.halo_row.size(); ++i) halo_of[d.halo_row[i]] = -1;
+      if (!ok) return false;
+      d.seg_begin.push_back(k1 - s.level_ptr[s.part_level[0]]);
+      d.slot_begin.push_back((int32_t)d.out_row.size());
+      d.halo_begin.push_back((int32_t)d.halo_row.size());
+    }
+    return true;
+  };
+  // the forward parts read only their own rows; a backward part also reads top rows, final
+  // before the backward parts start
+  if (!build(f, out.fwd, false) || !build(b, out.bwd, true)) return bad;
+  return out;
+}
+
 std::vector<int32_t> tree_parts(const RowForm& lower, int32_t target, int32_t& parts) {
   parts = 0;
   const int64_t n = lower.n;

@@ -111,6 +111,27 @@ struct TopSlots {
 };
 TopSlots top_slots(const Sweep& fwd, const Sweep& bwd);
 
+// The independent parts of a partitioned forward sweep and of its backward sweep, each part
+// renumbered for a solve that holds its values in LDS: the part's rows take slots in the order
+// their first segments come in, and (backward only) each top row a part's terms read takes a
+// halo slot after them. A first segment starts from its row's source row (in_row: the forward's
+// right-hand side row, the backward's forward value), a continuing one from the partial value in
+// its slot; each row's result goes to out_row. Per part q: segments seg_begin[q] ..
+// seg_begin[q + 1] of the sweep (its own segment order), slots slot_begin[q] .. (in_row,
+// out_row), halo slots halo_begin[q] .. (halo_row: the backward block's row each holds).
+// parts = 0 when the sweeps do not have this form.
+struct PartSlots {
+  struct Dir {
+    std::vector<int32_t> seg_begin, slot_begin, halo_begin;  // parts + 1 each
+    std::vector<int32_t> slot;                               // per segment: its row's slot in its part
+    std::vector<int32_t> dep;                                // per term of those segments: its slot
+    std::vector<int32_t> in_row, out_row, halo_row;
+  };
+  int32_t parts = 0;
+  Dir fwd, bwd;
+};
+PartSlots part_slots(const Sweep& fwd, const Sweep& bwd);
+
 // Each row's level: one more than the deepest of its dependencies (0 for none).
 std::vector<int32_t> levels_of(const RowForm& form);
 

@@ -858,9 +858,11 @@ __device__ __forceinline__ TopSeg top_seg(const TopStage& st, int32_t k) {
   return TopSeg{st.rec[k], st.diag[k]};
 }
 
-// One segment of column `col`: its slot's value minus its terms in order, over its diagonal
+// One segment of column `col`: its slot's value minus its terms in order (its first four in the
+// record, the rest from ov_val / ov_dep), over its diagonal
 template <int TW>
-__device__ __forceinline__ void top_segment(const TopStage& st, const TopSeg& g, bool unit, double* v, int32_t col) {
+__device__ __forceinline__ void top_segment(const double* ov_val, const int32_t* ov_dep, const TopSeg& g, bool unit,
+                                            double* v, int32_t col) {
   const int32_t slot = g.r.head.x, terms = g.r.head.y;
   const double x0 = v[g.r.dep.x * TW + col], x1 = v[g.r.dep.y * TW + col], x2 = v[g.r.dep.z * TW + col],
                x3 = v[g.r.dep.w * TW + col];
@@ -870,7 +872,7 @@ __device__ __forceinline__ void top_segment(const TopStage& st, const TopSeg& g,
   if (terms > 2) a = a - g.r.v23.x * x2;
   if (terms > 3) a = a - g.r.v23.y * x3;
   for (int32_t q = g.r.head.z, end = g.r.head.z + terms - RECORD_ENTRIES; q < end; ++q)
-    a = a - st.ov_val[q] * v[st.ov_dep[q] * TW + col];
+    a = a - ov_val[q] * v[ov_dep[q] * TW + col];
   v[slot * TW + col] = unit ? a : a / g.diag;
 }
 
@@ -886,7 +888,7 @@ __device__ void top_slots_sweep(const TopStage& st, bool unit, double* v, int32_
   for (int32_t l = 0; l < st.levels; ++l) {
     const int32_t next_hi = st.lp[l + 2 <= st.levels ? l + 2 : st.levels];
     for (int32_t k = lo + first; k < hi;) {
-      top_segment<TW>(st, g, unit, v, col);
+      top_segment<TW>(st.ov_val, st.ov_dep, g, unit, v, col);
       k += step;
       if (k < hi) g = top_seg(st, k);
     }
@@ -996,6 +998,128 @@ __global__ void __launch_bounds__(1024) parts_backward_kernel(SweepDev bwd, Reac
   } else {
     auto rhs_x = [&](int32_t src) { return Split && src < 0 ? x[(~src) * ld + col] : t[src * ld + col]; };
     wide_sweep<double, TW>(s, lp, x, ld, g, g.on, rhs_x, store_x, Identity(), AllLive(), std::true_type());
+  }
+}
+
+// Phases 1 and 3 with a part's values in LDS (PartSlotsDev): workgroup w takes part w / tiles
+// and tile w % tiles, consecutive w on one XCD, so a part's records are fetched into that XCD's
+// L2 once. It loads the part's slots (forward: the right-hand sides; backward: the forward values
+// from src and the halo, the top's solutions, from dst), its overflow terms and its rounds into
+// LDS. Then each column of the tile is swept by one wavefront alone, as the LDS top is: a lane
+// takes one segment a round, the levels need no workgroup barrier, and a wavefront fence after
+// each round keeps the compiler from moving the next round's reads above its writes. A part's
+// records (64 bytes and a diagonal a segment) are too many for LDS beside the values, so they
+// stay in global memory: each lane loads its segments PARTS_LOOKAHEAD rounds before it sweeps
+// them, which records allow since they do not depend on the values. Each segment subtracts its
+// terms in the sweep's order and divides as the sweep does, so the bits are those of the parts
+// kernels over the block. Then the workgroup stores the part's results into dst.
+template <int TW, bool Backward>
+__global__ void __launch_bounds__(1024) parts_lds_kernel(PartSlotsDev ps, int32_t unit, RhsDev rhs,
+                                                         const double* __restrict__ src, double* dst, int64_t ld,
+                                                         int32_t ncols, int32_t tiles, int32_t xcds) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int U = 8;  // items a thread stages or stores at once
+  constexpr int D = PARTS_LOOKAHEAD;
+  const PartsLdsLayout o = parts_lds_layout(ps, TW);
+  double* v = reinterpret_cast<double*>(smem);
+  double* ov_val = reinterpret_cast<double*>(smem + o.ov_val);
+  int32_t* ov_dep = reinterpret_cast<int32_t*>(smem + o.ov_dep);
+  int4* rt = reinterpret_cast<int4*>(smem + o.rounds);
+  const int32_t w = tile_of_workgroup(xcds);
+  const PartDesc d = ps.desc[w / tiles];
+  const int32_t col0 = (w % tiles) * TW;
+  const int32_t tid = threadIdx.x, threads = blockDim.x;
+  // the slots' values: item i is slot i / TW of column i % TW, so a tile's columns of a row are
+  // neighbouring lanes. Every thread first loads its U items' rows, then their values, so a batch
+  // waits for two rounds of loads (an item with no row loads element 0 of its block and drops it)
+  const int32_t items = (d.rows + d.halo) * TW;
+  {
+    const double* __restrict__ from = Backward ? src : rhs.block;  // a pair block: from plus, minus
+    for (int32_t base = 0; base < items; base += U * threads) {
+      int32_t row[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t i = base + u * threads + tid, sl = i / TW;
+        row[u] = i >= items || col0 + i % TW >= ncols ? -1
+                 : sl < d.rows                         ? ps.in_row[d.slot0 + sl]
+                                                       : ps.halo_row[d.halo0 + sl - d.rows];
+      }
+      double r[U];
+      if (from) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t i = base + u * threads + tid;
+          const double* p = Backward && i / TW >= d.rows ? dst : from;
+          const double value = p[row[u] >= 0 ? (int64_t)row[u] * ld + col0 + i % TW : 0];
+          r[u] = row[u] >= 0 ? value : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t i = base + u * threads + tid;
+          r[u] = row[u] >= 0 ? forward_rhs(rhs, ld, col0 + i % TW, true)(row[u]) : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t i = base + u * threads + tid;
+        if (i < items) v[i] = r[u];
+      }
+    }
+  }
+  for (int32_t i = tid; i < d.ov; i += threads) {
+    ov_val[i] = ps.ov_val[d.ov0 + i];
+    ov_dep[i] = ps.ov_dep[d.ov0 + i];
+  }
+  for (int32_t i = tid; i < d.rounds + D; i += threads)
+    rt[i] = i < d.rounds ? ps.rounds[d.round0 + i] : make_int4(0, 0, 0, 0);
+  __syncthreads();
+  const int32_t wave = tid / 64, lane = tid % 64;
+  if (wave < TW) {
+    const TopRecord* __restrict__ rec = ps.rec + d.seg0;
+    const double* __restrict__ diag = ps.diag + d.seg0;
+    // this lane's segment of round R (an empty round or lane: the part's first, with head.w 0 so
+    // it is skipped; a fetched segment has head.w 1)
+    auto fetch = [&](int32_t R) {
+      const int4 e = rt[R];
+      const bool on = lane < e.y;
+      TopSeg g{rec[on ? e.x + lane : 0], diag[on ? e.x + lane : 0]};
+      g.r.head.w = on ? 1 : 0;
+      return g;
+    };
+    // the fetches stay in round order (a scheduling barrier after each): the loop's first wait
+    // must hold for the first pass too, so a first fetch issued last would make every pass wait
+    // for all its loads
+    TopSeg g[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      g[u] = fetch(u);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int32_t R0 = 0; R0 < d.rounds; R0 += D) {
+#pragma unroll
+      for (int u = 0; u < D; ++u) {
+        if (g[u].r.head.w) top_segment<TW>(ov_val, ov_dep, g[u], unit != 0, v, wave);
+        g[u] = fetch(R0 + u + D);
+        __builtin_amdgcn_sched_barrier(0);
+        wavefront_order();
+      }
+    }
+  }
+  __syncthreads();
+  // the part's results, U items a thread at once: their rows, then their stores
+  for (int32_t base = 0; base < d.rows * TW; base += U * threads) {
+    int32_t row[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t i = base + u * threads + tid;
+      row[u] = i < d.rows * TW && col0 + i % TW < ncols ? ps.out_row[d.slot0 + i / TW] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t i = base + u * threads + tid;
+      if (row[u] >= 0) dst[(int64_t)row[u] * ld + col0 + i % TW] = v[i];
+    }
   }
 }
 
@@ -1960,13 +2084,20 @@ hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const Sweep
 
 hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, int top_tile, int top_threads,
                                 const SweepDev& fwd, const SweepDev& bwd, const PartLevels& levels, const RhsDev& rhs,
-                                const ReachDev& reach, const TopSlotsDev* top_lds, double* t, double* x, int64_t ld,
-                                int32_t ncols) {
+                                const ReachDev& reach, const TopSlotsDev* top_lds, const PartsLds& parts_lds,
+                                double* t, double* x, int64_t ld, int32_t ncols) {
   auto valid = [](int tw, int th) {
     return tw >= 1 && tw <= 64 && (tw & (tw - 1)) == 0 && th >= tw && th <= 1024 && th % 64 == 0;
   };
   if (!valid(tile, threads) || !valid(top_tile, top_threads) || fwd.parts < 1 || bwd.parts < 1)
     return hipErrorInvalidValue;
+  // the LDS parts: a wavefront per column of the tile, both sweeps' parts, no reach
+  const int lds_tile = parts_lds.tile;
+  if (lds_tile && (!parts_lds.fwd || !parts_lds.bwd || reach.lp || lds_tile > 16 || !valid(lds_tile, parts_lds.threads) ||
+                   parts_lds.threads < 64 * lds_tile || parts_lds.fwd->parts != fwd.parts ||
+                   parts_lds.bwd->parts != bwd.parts))
+    return hipErrorInvalidValue;
+  const int32_t lds_tiles = lds_tile ? (ncols + lds_tile - 1) / lds_tile : 0;
   // split sweeps come with blocks of vectors only (a pair block's reach is over the unsplit
   // ones), and a pair block's reach bitmaps are per tile, so its top kernel keeps the parts' tile
   const bool split = fwd.split || bwd.split;
@@ -1997,6 +2128,18 @@ hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, int t
     hipLaunchKernelGGL((top_lds_kernel<TWV>), dim3(top_tiles), dim3(top_threads), bytes, stream, fwd, bwd,       \
                        *top_lds, rhs, t, x, ld, ncols);                                                         \
   }
+#define LAUNCH_PARTS(TWV, BACKWARD, PS, UNIT, SRC, DST)                                                          \
+  {                                                                                                             \
+    const size_t bytes = parts_lds_layout(PS, TWV).total;                                                       \
+    const hipError_t e = hipFuncSetAttribute((const void*)parts_lds_kernel<TWV, BACKWARD>,                       \
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);           \
+    if (e != hipSuccess) return e;                                                                              \
+    hipLaunchKernelGGL((parts_lds_kernel<TWV, BACKWARD>), dim3((unsigned)(lds_tiles * (PS).parts)),              \
+                       dim3(parts_lds.threads), bytes, stream, PS, UNIT, rhs, SRC, DST, ld, ncols, lds_tiles,    \
+                       xcd_groups());                                                                           \
+  }
+#define FWD_LDS(TWV) LAUNCH_PARTS(TWV, false, *parts_lds.fwd, (int32_t)fwd.unit, nullptr, t)
+#define BWD_LDS(TWV) LAUNCH_PARTS(TWV, true, *parts_lds.bwd, (int32_t)bwd.unit, t, x)
 #define FWD(TWV)                                                                                                 \
   if (split)                                                                                                    \
     LAUNCH(parts_forward_kernel, TWV, true, b1, dim3(tiles, (unsigned)fwd.parts), threads, fwd, rhs, reach, t,  \
@@ -2020,12 +2163,34 @@ hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, int t
   else                                                                                                          \
     LAUNCH(parts_backward_kernel, TWV, false, b3, dim3(tiles, (unsigned)bwd.parts), threads, bwd, reach, t, x,  \
            ld, ncols, wp)
-  DISPATCH_TILE(tile, FWD)
+  // the LDS parts take tiles of up to 16 columns (a wavefront each)
+#define DISPATCH_LDS_TILE(F)             \
+  switch (lds_tile) {                    \
+    case 1: F(1); break;                 \
+    case 2: F(2); break;                 \
+    case 4: F(4); break;                 \
+    case 8: F(8); break;                 \
+    case 16: F(16); break;               \
+    default: return hipErrorInvalidValue; \
+  }
+  if (lds_tile) {
+    DISPATCH_LDS_TILE(FWD_LDS)
+  } else {
+    DISPATCH_TILE(tile, FWD)
+  }
   DISPATCH_TILE(top_tile, TOP)
-  DISPATCH_TILE(tile, BWD)
+  if (lds_tile) {
+    DISPATCH_LDS_TILE(BWD_LDS)
+  } else {
+    DISPATCH_TILE(tile, BWD)
+  }
 #undef BWD
 #undef TOP
 #undef FWD
+#undef DISPATCH_LDS_TILE
+#undef BWD_LDS
+#undef FWD_LDS
+#undef LAUNCH_PARTS
 #undef LAUNCH_LDS
 #undef LAUNCH
   return hipGetLastError();

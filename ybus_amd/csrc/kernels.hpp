@@ -236,12 +236,61 @@ __host__ __device__ inline TopLdsLayout top_lds_layout(const TopSlotsDev& top, i
   return o;
 }
 
+// The independent parts of one sweep (forward or backward) in LDS slots, for the LDS parts
+// kernel: part q's segments are records rec[d.seg0 ..] with their diagonals (a continuing
+// segment's is 1), slot- and term-numbered within the part; its slots' source rows in_row and
+// result rows out_row from d.slot0, the halo's rows (backward: top rows of the solution block)
+// from d.halo0; its rounds (one wavefront's pass over up to 64 of a level's segments:
+// (first segment in the part, segments, 1 where the round ends its level, 0)) from d.round0, a
+// multiple of PARTS_LOOKAHEAD of them (the last ones empty); its terms past a segment's fourth
+// (overflow) from d.ov0, the records' head.z counting from there.
+constexpr int PARTS_LOOKAHEAD = 4;  // rounds whose records are loaded ahead of the one swept
+struct PartDesc {
+  int32_t seg0, segs, slot0, rows, halo0, halo, round0, rounds, ov0, ov, pad[2];
+};
+struct PartSlotsDev {
+  int32_t parts = 0;
+  const PartDesc* desc = nullptr;
+  const TopRecord* rec = nullptr;
+  const double* diag = nullptr;
+  const int32_t* in_row = nullptr;
+  const int32_t* out_row = nullptr;
+  const int32_t* halo_row = nullptr;
+  const int4* rounds = nullptr;
+  const double* ov_val = nullptr;
+  const int32_t* ov_dep = nullptr;
+  int32_t most_slots = 0, most_ov = 0, most_rounds = 0;  // the largest over the parts
+};
+// Byte offsets in LDS for a tile of `tile` columns: the slots' values (slot-major, the tile's
+// columns side by side), the overflow terms' values and slots, and the rounds (with
+// PARTS_LOOKAHEAD empty ones after the last)
+struct PartsLdsLayout {
+  size_t values = 0, ov_val = 0, ov_dep = 0, rounds = 0, total = 0;
+};
+__host__ __device__ inline PartsLdsLayout parts_lds_layout(const PartSlotsDev& p, int tile) {
+  PartsLdsLayout o;
+  auto up = [](size_t bytes) { return (bytes + 15) / 16 * 16; };
+  o.ov_val = up((size_t)p.most_slots * (size_t)tile * sizeof(double));
+  o.ov_dep = o.ov_val + up((size_t)p.most_ov * sizeof(double));
+  o.rounds = o.ov_dep + up((size_t)p.most_ov * sizeof(int32_t));
+  o.total = o.rounds + (size_t)(p.most_rounds + PARTS_LOOKAHEAD) * sizeof(int4);
+  return o;
+}
+// The LDS parts kernels' configuration: tile 0 takes the parts kernels over the block in global
+// memory
+struct PartsLds {
+  const PartSlotsDev* fwd = nullptr;
+  const PartSlotsDev* bwd = nullptr;
+  int tile = 0, threads = 0;
+};
+
 // top_lds: the top kernel over TopSlotsDev in LDS (null: the top kernel over the block in global
-// memory). Blocks of vectors, and pair blocks without a reach, only.
+// memory). parts_lds: the parts swept in LDS slots (tile 0: over the block in global memory).
+// Blocks of vectors, and pair blocks without a reach, only, for either.
 hipError_t launch_solve64_parts(hipStream_t stream, int tile, int threads, int top_tile, int top_threads,
                                 const SweepDev& fwd, const SweepDev& bwd, const PartLevels& levels, const RhsDev& rhs,
-                                const ReachDev& reach, const TopSlotsDev* top_lds, double* t, double* x, int64_t ld,
-                                int32_t ncols);
+                                const ReachDev& reach, const TopSlotsDev* top_lds, const PartsLds& parts_lds,
+                                double* t, double* x, int64_t ld, int32_t ncols);
 
 // Complex FP64 solve of a block: forward sweep b -> t, backward t -> x. A complex block holds
 // unknown i's real parts in row 2i and its imaginary parts in row 2i + 1 (ld doubles a row).

@@ -351,6 +351,114 @@ struct DevTopSlots {
   }
 };
 
+// One sweep's independent parts renumbered into LDS slots (part_slots), as the LDS parts kernel
+// reads them: per part a descriptor, the segments' records and diagonals, the slots' rows, the
+// halo's rows, the rounds (a wavefront's pass over up to 64 of a level's segments, padded to a
+// multiple of PARTS_LOOKAHEAD with empty ones) and the terms past each segment's fourth
+struct DevPartSlots {
+  DevBuf<PartDesc> desc;
+  DevBuf<TopRecord> rec;
+  DevBuf<double> diag, ov_val;
+  DevBuf<int32_t> in_row, out_row, halo_row, ov_dep;
+  DevBuf<int4> rounds;
+  PartSlotsDev dev;
+  // false (nothing uploaded) where an index would leave its part's slots, terms or segments
+  bool upload(const PartSlots::Dir& h, int32_t parts, const Sweep& s, hipStream_t stream) {
+    constexpr int32_t lanes = 64;
+    std::vector<PartDesc> descs(parts);
+    std::vector<TopRecord> recs(h.slot.size());
+    std::vector<double> diags(h.slot.size()), ovv;
+    std::vector<int32_t> ovd;
+    std::vector<int4> rnd;
+    PartSlotsDev d;
+    d.parts = parts;
+    for (int32_t q = 0; q < parts; ++q) {
+      PartDesc& p = descs[q];
+      p = PartDesc{};
+      p.seg0 = h.seg_begin[q];
+      p.segs = h.seg_begin[q + 1] - h.seg_begin[q];
+      p.slot0 = h.slot_begin[q];
+      p.rows = h.slot_begin[q + 1] - h.slot_begin[q];
+      p.halo0 = h.halo_begin[q];
+      p.halo = h.halo_begin[q + 1] - h.halo_begin[q];
+      p.round0 = (int32_t)rnd.size();
+      p.ov0 = (int32_t)ovv.size();
+      for (int32_t l = s.part_level[q]; l < s.part_level[q + 1]; ++l) {
+        const int32_t lo = s.level_ptr[l] - p.seg0, width = s.level_ptr[l + 1] - s.level_ptr[l];
+        for (int32_t j = 0; j < width; j += lanes)
+          rnd.push_back(make_int4(lo + j, std::min(lanes, width - j), j + lanes >= width ? 1 : 0, 0));
+      }
+      while ((rnd.size() - (size_t)p.round0) % PARTS_LOOKAHEAD) rnd.push_back(make_int4(0, 0, 0, 0));
+      p.rounds = (int32_t)rnd.size() - p.round0;
+      for (int32_t k = h.seg_begin[q]; k < h.seg_begin[q + 1]; ++k) {
+        const int32_t q0 = s.row_ptr[k], q1 = s.row_ptr[k + 1];
+        TopRecord& r = recs[k];
+        int32_t dep[RECORD_ENTRIES];
+        double val[RECORD_ENTRIES];
+        for (int e = 0; e < RECORD_ENTRIES; ++e) {
+          const bool inside = q0 + e < q1;
+          dep[e] = inside ? h.dep[q0 + e] : h.slot[k];
+          val[e] = inside ? s.val[q0 + e] : 0.0;
+        }
+        r.head = make_int4(h.slot[k], q1 - q0, (int32_t)ovv.size() - p.ov0, 0);
+        r.dep = make_int4(dep[0], dep[1], dep[2], dep[3]);
+        r.v01 = make_double2(val[0], val[1]);
+        r.v23 = make_double2(val[2], val[3]);
+        for (int32_t e = q0 + RECORD_ENTRIES; e < q1; ++e) {
+          ovv.push_back(s.val[e]);
+          ovd.push_back(h.dep[e]);
+        }
+        diags[k] = s.unit ? 1.0 : s.diag[k];
+      }
+      p.ov = (int32_t)ovv.size() - p.ov0;
+      // every slot a record or overflow term names is one of the part's, and every round's
+      // segments are the part's
+      const int32_t segs = h.seg_begin[q + 1] - h.seg_begin[q], slots = p.rows + p.halo;
+      for (int32_t k = h.seg_begin[q]; k < h.seg_begin[q + 1]; ++k) {
+        const TopRecord& r = recs[k];
+        for (int32_t x : {r.head.x, r.dep.x, r.dep.y, r.dep.z, r.dep.w})
+          if (x < 0 || x >= slots) return false;
+        if (r.head.x >= p.rows || r.head.y < 0 || r.head.z < 0 ||
+            r.head.z + std::max(0, r.head.y - RECORD_ENTRIES) > p.ov)
+          return false;
+      }
+      for (int32_t e = p.ov0; e < p.ov0 + p.ov; ++e)
+        if (ovd[e] < 0 || ovd[e] >= slots) return false;
+      for (int32_t e = p.round0; e < p.round0 + p.rounds; ++e)
+        if (rnd[e].x < 0 || rnd[e].y < 0 || rnd[e].x + rnd[e].y > segs) return false;
+      if (segs < 1 || p.rows < 1) return false;
+      for (int32_t e = p.slot0; e < p.slot0 + p.rows; ++e)
+        if (h.in_row[e] < 0 || h.in_row[e] >= s.n || h.out_row[e] < 0 || h.out_row[e] >= s.n) return false;
+      for (int32_t e = p.halo0; e < p.halo0 + p.halo; ++e)
+        if (h.halo_row[e] < 0 || h.halo_row[e] >= s.n) return false;
+      d.most_slots = std::max(d.most_slots, p.rows + p.halo);
+      d.most_ov = std::max(d.most_ov, p.ov);
+      d.most_rounds = std::max(d.most_rounds, p.rounds);
+    }
+    desc.upload(descs, stream);
+    rec.upload(recs, stream);
+    diag.upload(diags, stream);
+    ov_val.upload(ovv, stream);
+    ov_dep.upload(ovd, stream);
+    in_row.upload(h.in_row, stream);
+    out_row.upload(h.out_row, stream);
+    halo_row.upload(h.halo_row, stream);
+    rounds.upload(rnd, stream);
+    check(hipStreamSynchronize(stream), "upload part slots");
+    d.desc = desc.p;
+    d.rec = rec.p;
+    d.diag = diag.p;
+    d.in_row = in_row.p;
+    d.out_row = out_row.p;
+    d.halo_row = halo_row.p;
+    d.rounds = rounds.p;
+    d.ov_val = ov_val.p;
+    d.ov_dep = ov_dep.p;
+    dev = d;
+    return true;
+  }
+};
+
 // A complex factor's sweep on the device: level pointers, the rows' CSR arrays, and the values
 // as (re, im) pairs
 struct DevSweepC {
@@ -612,6 +720,14 @@ struct ybus_hip_factor {
   // YBUS_HIP_TOP_LDS=0 keeps the top kernel over the block in global memory
   DevTopSlots top_slots;
   bool use_top_lds = true;
+  // the same sweeps' independent parts in LDS slots (parts_lds.parts 0: none), which blocks of
+  // vectors take at the tuned tile width (Tuned::parts_lds_tile); YBUS_HIP_PARTS_LDS=tile[:threads]
+  // sets it for every block of vectors, 0 keeps the parts kernels over the block in global memory
+  struct {
+    int32_t parts = 0;
+    DevPartSlots fwd, bwd;
+  } parts_lds;
+  int parts_lds_env = -1, parts_lds_env_threads = 0;
   // column path: schedules in factor order, and each unknown's factor row for the right-hand
   // side (rhs_row) and the solution (sol_of), for the solve and the transposed solve
   DevSweep cfwd, cbwd, ctfwd, ctbwd;
@@ -677,6 +793,7 @@ struct ybus_hip_factor {
     int threads = 512, tile = 0;
     int parts = -1;  // the partitioned sweeps: 1 on, 0 off, -1 by the default rule
     int top_tile = 0, top_threads = 0;  // the partitioned top kernel's own (0: the parts')
+    int parts_lds_tile = 0, parts_lds_threads = 0;  // the parts in LDS slots (0: over the block)
   };
   std::map<int, Tuned> tuned;
   const Tuned* trial = nullptr;  // a configuration being timed by the autotuner
@@ -699,6 +816,28 @@ struct ybus_hip_factor {
   int column_threads(int64_t ncols, bool pairs) const {
     const Tuned* t = tuned_for(ncols, pairs);
     return t && t->columns && !env_threads ? t->threads : threads_columns;
+  }
+  // The LDS parts' configuration a partitioned block of vectors takes (tile 0: the parts over the
+  // block): YBUS_HIP_PARTS_LDS's, else the tuned one, where both sweeps' parts fit LDS
+  PartsLds lds_parts_for(int64_t ncols, bool pairs) const {
+    PartsLds pl;
+    if (pairs || parts_lds.parts == 0) return pl;
+    int tile = 0, threads = 0;
+    if (parts_lds_env >= 0) {
+      tile = parts_lds_env;
+      threads = parts_lds_env_threads ? parts_lds_env_threads : std::max(256, 64 * tile);
+    } else if (const Tuned* c = tuned_for(ncols, pairs)) {
+      tile = c->parts_lds_tile;
+      threads = c->parts_lds_threads;
+    }
+    if (tile <= 0 || parts_lds_layout(parts_lds.fwd.dev, tile).total > lds_cap ||
+        parts_lds_layout(parts_lds.bwd.dev, tile).total > lds_cap)
+      return pl;
+    pl.fwd = &parts_lds.fwd.dev;
+    pl.bwd = &parts_lds.bwd.dev;
+    pl.tile = tile;
+    pl.threads = threads;
+    return pl;
   }
 
   ~ybus_hip_factor() {
@@ -987,7 +1126,8 @@ struct ybus_hip_factor {
                              top_lds_layout(top_slots.dev, top_tile).total <= lds_cap;
         check(launch_solve64_parts(stream(), tile, threads, top_tile, top_threads, (sp ? spfwd : pfwd).view(),
                                    (sp ? spbwd : pbwd).view(), part_levels, rhs, reach,
-                                   lds_top ? &top_slots.dev : nullptr, t.p, x.p, layout.ld, (int32_t)ncols),
+                                   lds_top ? &top_slots.dev : nullptr, lds_parts_for(ncols, pairs), t.p, x.p,
+                                   layout.ld, (int32_t)ncols),
               "solve kernel");
       } else if (pairs && !transpose && reach_tile == tile && !reach_parts) {
         check(launch_solve64_reach(stream(), tile, wide_threads(ncols, pairs), f.view(), bw.view(), rhs,
@@ -1174,6 +1314,10 @@ static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
   const char* tile = std::getenv("YBUS_HIP_TILE");
   if (tile) f->tile_override = std::atoi(tile);
   if (const char* lds = std::getenv("YBUS_HIP_TOP_LDS")) f->use_top_lds = std::string(lds) != "0";
+  if (const char* pl = std::getenv("YBUS_HIP_PARTS_LDS")) {
+    f->parts_lds_env = std::max(0, std::atoi(pl));
+    if (const char* colon = std::strchr(pl, ':')) f->parts_lds_env_threads = std::atoi(colon + 1);
+  }
   if (const char* top = std::getenv("YBUS_HIP_TOP")) {
     f->top_override = std::atoi(top);
     if (const char* colon = std::strchr(top, ':')) f->top_override_threads = std::atoi(colon + 1);
@@ -1355,6 +1499,11 @@ int32_t ybus_hip_factor_upload_cholesky(ybus_hip_ctx* ctx, uint64_t n, const uin
         const Sweep& tb = f->parts_split ? f->spbwd.host : pb;
         const TopSlots ts = top_slots(tf, tb);
         if (ts.rows > 0) f->top_slots.upload(ts, tf, tb, ctx->stream);
+        // and their parts, in LDS slots too
+        const PartSlots ps = part_slots(tf, tb);
+        if (ps.parts > 0 && f->parts_lds.fwd.upload(ps.fwd, ps.parts, tf, ctx->stream) &&
+            f->parts_lds.bwd.upload(ps.bwd, ps.parts, tb, ctx->stream))
+          f->parts_lds.parts = ps.parts;
         f->pfwd.upload(std::move(pf), ctx->stream, true);
         f->pbwd.upload(std::move(pb), ctx->stream, true);
         f->part_levels = PartLevels{f->pfwd.most_part_levels(), f->pfwd.top_levels(), f->pbwd.top_levels(),
@@ -2697,6 +2846,25 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
               best_ms = ms;
             }
           }
+        // then the parts in LDS slots, at the tile widths whose slots fit, the top's kept
+        if (f->parts_lds.parts > 0) {
+          const ybus_hip_factor::Tuned top_best = best;
+          for (int tile : {1, 2, 4, 8, 16})
+            for (int th : {256, 512, 1024}) {
+              if (th < 64 * tile || (tile != 1 && tile / 2 >= k) ||
+                  parts_lds_layout(f->parts_lds.fwd.dev, tile).total > f->lds_cap ||
+                  parts_lds_layout(f->parts_lds.bwd.dev, tile).total > f->lds_cap)
+                continue;
+              ybus_hip_factor::Tuned c = top_best;
+              c.parts_lds_tile = tile;
+              c.parts_lds_threads = th;
+              const double ms = time_config(c);
+              if (ms < best_ms) {
+                best = c;
+                best_ms = ms;
+              }
+            }
+        }
       }
     } catch (...) {
       f->trial = nullptr;
@@ -2717,7 +2885,9 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
                              best_ms,
                              best.columns ? 0 : std::max(best.parts, 0),
                              best.top_tile ? best.top_tile : (best.columns ? 0 : best.tile),
-                             best.top_tile ? best.top_threads : (best.columns ? 0 : best.threads)};
+                             best.top_tile ? best.top_threads : (best.columns ? 0 : best.threads),
+                             best.parts_lds_tile,
+                             best.parts_lds_threads};
   });
 }
 
