@@ -87,6 +87,34 @@ __device__ void reduce_max(double* scratch, double mine, double* out, const Tile
 
 // ---- wide tiles ----
 
+// Element k of an array in global memory, loaded as global memory. A pointer the compiler cannot
+// trace to a kernel argument (one read from a struct in memory, as a batch entry's) is loaded
+// with flat instructions, which count against the LDS counter too: every wait on an LDS read
+// would then also wait for all the record fetches in flight. Every array these kernels read
+// this way is device memory.
+template <typename T>
+struct NativeOf {
+  using type = T;
+};
+template <>
+struct NativeOf<int4> {
+  typedef int type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct NativeOf<float4> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct NativeOf<double2> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <typename T>
+__device__ __forceinline__ T global_at(const T* p, int64_t k) {
+  using N = typename NativeOf<T>::type;
+  const N v = ((const __attribute__((address_space(1))) N*)p)[k];
+  return __builtin_bit_cast(T, v);
+}
+
 // One row's record in registers
 template <typename T>
 struct Record {
@@ -96,31 +124,38 @@ struct Record {
   T diag;
 };
 
-template <typename T>
-__device__ __forceinline__ void fetch_values(const SweepDev& s, int32_t k, Record<T>& r);
-template <>
-__device__ __forceinline__ void fetch_values<double>(const SweepDev& s, int32_t k, Record<double>& r) {
-  const double2 a = s.rec.val01[k], b = s.rec.val23[k];
-  r.val[0] = a.x;
-  r.val[1] = a.y;
-  r.val[2] = b.x;
-  r.val[3] = b.y;
-  r.diag = s.rec.diag64[k];
-}
-template <>
-__device__ __forceinline__ void fetch_values<float>(const SweepDev& s, int32_t k, Record<float>& r) {
-  const float4 a = s.rec.val32[k];
-  r.val[0] = a.x;
-  r.val[1] = a.y;
-  r.val[2] = a.z;
-  r.val[3] = a.w;
-  r.diag = s.rec.diag32[k];
+// p[k], as a global load where G (see global_at), else as the compiler chooses
+template <bool G, typename P>
+__device__ __forceinline__ P load_at(const P* p, int64_t k) {
+  if constexpr (G)
+    return global_at(p, k);
+  else
+    return p[k];
 }
 
-// Issues the (independent, coalesced) loads of position k's record
-template <typename T>
+template <typename T, bool G = false>
+__device__ __forceinline__ void fetch_values(const SweepDev& s, int32_t k, Record<T>& r) {
+  if constexpr (std::is_same<T, double>::value) {
+    const double2 a = load_at<G>(s.rec.val01, k), b = load_at<G>(s.rec.val23, k);
+    r.val[0] = a.x;
+    r.val[1] = a.y;
+    r.val[2] = b.x;
+    r.val[3] = b.y;
+    r.diag = load_at<G>(s.rec.diag64, k);
+  } else {
+    const float4 a = load_at<G>(s.rec.val32, k);
+    r.val[0] = a.x;
+    r.val[1] = a.y;
+    r.val[2] = a.z;
+    r.val[3] = a.w;
+    r.diag = load_at<G>(s.rec.diag32, k);
+  }
+}
+
+// Issues the (independent, coalesced) loads of position k's record (G: as global loads)
+template <typename T, bool G = false>
 __device__ __forceinline__ void fetch(const SweepDev& s, int32_t k, Record<T>& r) {
-  const int4 h = s.rec.head[k], d = s.rec.dep[k];
+  const int4 h = load_at<G>(s.rec.head, k), d = load_at<G>(s.rec.dep, k);
   r.dst = h.x;
   r.src = h.y;
   r.p0 = h.z;
@@ -129,7 +164,7 @@ __device__ __forceinline__ void fetch(const SweepDev& s, int32_t k, Record<T>& r
   r.dep[1] = d.y;
   r.dep[2] = d.z;
   r.dep[3] = d.w;
-  fetch_values<T>(s, k, r);
+  fetch_values<T, G>(s, k, r);
 }
 
 // Waits until every field of r has arrived. A record slot is refilled while the loads of its
@@ -1347,7 +1382,7 @@ __device__ __forceinline__ void lds_barrier() {
 
 // One row in place: xs[dst] = (xs[dst] - its terms) / diag, the terms subtracted in the row's
 // packed order, the inline four first, then the rest from the sweep's arrays.
-template <typename T>
+template <typename T, bool G = false>
 __device__ __forceinline__ void lds_row(const SweepDev& s, const T* __restrict__ val, T* xs, const Record<T>& r) {
   const int32_t len = r.p1 - r.p0;
   T acc = xs[r.dst];
@@ -1363,8 +1398,8 @@ __device__ __forceinline__ void lds_row(const SweepDev& s, const T* __restrict__
 #pragma unroll
     for (int q = 0; q < OVERFLOW_CHUNK; ++q) {
       const int32_t pq = p + q < r.p1 ? p + q : r.p1 - 1;
-      dd[q] = s.dep[pq];
-      vv[q] = val[pq];
+      dd[q] = load_at<G>(s.dep, pq);
+      vv[q] = load_at<G>(val, pq);
     }
 #pragma unroll
     for (int q = 0; q < OVERFLOW_CHUNK; ++q) xx[q] = xs[dd[q]];
@@ -1386,7 +1421,7 @@ constexpr int LDS_AHEAD = 4;
 // last), so the number of loads in flight, and the waits the compiler places, are the same on
 // every path. A narrow level (the common case) then waits only on LDS and its arithmetic; the
 // further rows of a wide level are fetched as they come.
-template <typename T>
+template <typename T, bool G = false>
 __device__ void lds_sweep(const SweepDev& s, const int32_t* lp, T* xs) {
   const T* __restrict__ val = values_of<T>(s);
   const int32_t slot = threadIdx.x, slots = blockDim.x;
@@ -1395,18 +1430,18 @@ __device__ void lds_sweep(const SweepDev& s, const int32_t* lp, T* xs) {
   const int32_t last = lp[levels] - 1;
   auto ahead = [&](int32_t m, Record<T>& r) {
     const int32_t k = m < levels ? lp[m] + slot : last;
-    fetch<T>(s, k < last ? k : last, r);
+    fetch<T, G>(s, k < last ? k : last, r);
   };
   // level l from the record slot holding it; `refill` then fetches level l + LDS_AHEAD into it
   auto level = [&](int32_t l, Record<T>& r, bool refill) {
     const int32_t end = lp[l + 1];
     int32_t k = lp[l] + slot;
     if (k < end) {
-      lds_row<T>(s, val, xs, r);
+      lds_row<T, G>(s, val, xs, r);
       for (k += slots; k < end; k += slots) {
         Record<T> extra;
-        fetch<T>(s, k, extra);
-        lds_row<T>(s, val, xs, extra);
+        fetch<T, G>(s, k, extra);
+        lds_row<T, G>(s, val, xs, extra);
       }
     }
     if (refill) {
@@ -1914,6 +1949,7 @@ __global__ void operator_columns_kernel(ProductEntry e, int32_t ncols, double* _
 // One right-hand side b (contiguous, by unknown) of one factor solved with its solution in LDS:
 // smem holds the vector in factor order, then both sweeps' level pointers; out gets x by unknown.
 // Every thread must call; the caller's barrier must separate it from the next use of smem.
+template <bool G>
 __device__ void column_solve(const SweepDev& fwd, const SweepDev& bwd, int32_t n, const int32_t* __restrict__ rhs_row,
                              const int32_t* __restrict__ sol_of, const double* __restrict__ b,
                              double* __restrict__ out, unsigned char* smem) {
@@ -1924,16 +1960,20 @@ __device__ void column_solve(const SweepDev& fwd, const SweepDev& bwd, int32_t n
   stage_level_ptr(bwd, lpb);
   for (int32_t u = threadIdx.x; u < n; u += blockDim.x) xs[rhs_row[u]] = b[u];
   __syncthreads();
-  lds_sweep<double>(fwd, lpf, xs);
-  lds_sweep<double>(bwd, lpb, xs);
+  lds_sweep<double, G>(fwd, lpf, xs);
+  lds_sweep<double, G>(bwd, lpb, xs);
   for (int32_t u = threadIdx.x; u < n; u += blockDim.x) out[u] = xs[sol_of[u]];
 }
 
 // A persistent kernel over a queue of (factor, column) items: each workgroup takes the next item
 // (one atomic add on the queue's counter by one thread), solves it, and takes another, until the
 // queue is empty; every workgroup then sees an item past the end and returns. The items are
-// ordered longest first, so the last ones to start are the shortest.
-__global__ void __launch_bounds__(1024) batch_kernel(const BatchEntryDev* __restrict__ entries,
+// ordered longest first, so the last ones to start are the shortest. G: the records are loaded
+// as global memory (global_at), which keeps more of them in flight and so more registers: up to
+// 512 threads, for the classes of one or two workgroups a compute unit; the others keep the loads
+// the compiler chooses, in up to 128 registers, so up to 16 small workgroups share a unit.
+template <bool G>
+__global__ void __launch_bounds__(G ? 512 : 1024) batch_kernel(const BatchEntryDev* __restrict__ entries,
                                                      const int2* __restrict__ items, int32_t total,
                                                      int32_t* __restrict__ queue, const double* __restrict__ rhs,
                                                      double* __restrict__ x) {
@@ -1948,7 +1988,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(const BatchEntryDev* __rest
     const int2 w = items[it];
     const BatchEntryDev e = entries[w.x];
     const int64_t at = e.offset + (int64_t)w.y * e.n;
-    column_solve(e.fwd, e.bwd, e.n, e.rhs_row, e.sol_of, rhs + at, x + at, smem);
+    column_solve<G>(e.fwd, e.bwd, e.n, e.rhs_row, e.sol_of, rhs + at, x + at, smem);
   }
 }
 
@@ -2405,14 +2445,25 @@ hipError_t launch_product_max(hipStream_t stream, const CsrDev& m, const double*
                     pb, best, best_row, not_finite);
 }
 
-hipError_t prepare_batch(size_t lds_bytes) {
-  return hipFuncSetAttribute((const void*)batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+// The batch kernel a class takes: global record loads for one or two workgroups a compute unit
+static const void* batch_kernel_for(int per_cu) {
+  return per_cu <= 2 ? (const void*)batch_kernel<true> : (const void*)batch_kernel<false>;
 }
 
-hipError_t launch_batch(hipStream_t stream, int grid, int threads, size_t lds_bytes, const BatchEntryDev* entries,
-                        const int2* items, int32_t total, int32_t* queue, const double* rhs, double* x) {
-  if (grid <= 0 || threads < 64 || threads > 1024 || threads % 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(threads), lds_bytes, stream, entries, items, total, queue, rhs, x);
+hipError_t prepare_batch(size_t lds_bytes, int per_cu) {
+  return hipFuncSetAttribute(batch_kernel_for(per_cu), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+hipError_t launch_batch(hipStream_t stream, int grid, int threads, size_t lds_bytes, int per_cu,
+                        const BatchEntryDev* entries, const int2* items, int32_t total, int32_t* queue,
+                        const double* rhs, double* x) {
+  if (grid <= 0 || threads < 64 || threads > (per_cu <= 2 ? 512 : 1024) || threads % 64) return hipErrorInvalidValue;
+  if (per_cu <= 2)
+    hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(threads), lds_bytes, stream, entries, items, total, queue,
+                       rhs, x);
+  else
+    hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(threads), lds_bytes, stream, entries, items, total, queue,
+                       rhs, x);
   return hipGetLastError();
 }
 

@@ -365,10 +365,12 @@ hipError_t launch_scatter_negated(hipStream_t stream, const double* acc, const i
 
 // Batches: a persistent launch of `grid` workgroups that take (factor, column) items from the
 // queue counter (which must hold 0 at the launch) until `total` are taken. prepare_batch allows
-// the kernel lds_bytes of LDS (call it before a launch or a graph capture).
-hipError_t prepare_batch(size_t lds_bytes);
-hipError_t launch_batch(hipStream_t stream, int grid, int threads, size_t lds_bytes, const BatchEntryDev* entries,
-                        const int2* items, int32_t total, int32_t* queue, const double* rhs, double* x);
+// the kernel lds_bytes of LDS (call it before a launch or a graph capture). per_cu: the class's
+// workgroups a compute unit (one or two: workgroups of at most 512 threads, else 1,024).
+hipError_t prepare_batch(size_t lds_bytes, int per_cu);
+hipError_t launch_batch(hipStream_t stream, int grid, int threads, size_t lds_bytes, int per_cu,
+                        const BatchEntryDev* entries, const int2* items, int32_t total, int32_t* queue,
+                        const double* rhs, double* x);
 // The FP32-refined batch: the same queue, each column refined in FP64 against its factor's matrix
 // (x1 and r: scratch laid out like x; steps and ratio: one per column of the batch)
 hipError_t prepare_batch_refine(size_t lds_bytes);

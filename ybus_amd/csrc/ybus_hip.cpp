@@ -3403,6 +3403,20 @@ struct ybus_hip_batch {
     return shapes;
   }
 
+  // The class (workgroups per CU) and LDS bytes of a factor's columns whose sweeps have `levels`
+  // levels between them (0: none fits)
+  static int class_of(const ybus_hip_factor* f, bool refined, int32_t levels, size_t& lds) {
+    const int32_t n = (int32_t)std::max<int64_t>(f->n, 1);
+    for (const auto& kt : class_shapes(refined)) {
+      const size_t bytes = refined ? refine_lds_bytes(n, kt.second, levels) : solve_lds_bytes(n, false, levels);
+      if ((size_t)kt.first * bytes <= f->lds_cap) {
+        lds = bytes;
+        return kt.first;
+      }
+    }
+    return 0;
+  }
+
   // Items in classes for a mode: each factor's columns go to the class of the most workgroups
   // per compute unit whose LDS they fit; longest first within a class (a column's time is set by
   // its levels, then by its rows)
@@ -3419,19 +3433,9 @@ struct ybus_hip_batch {
     for (size_t i = 0; i < factors.size(); ++i) {
       ybus_hip_factor* f = factors[i];
       if (!counts[i]) continue;
-      const int32_t n = (int32_t)std::max<int64_t>(f->n, 1);
       const int32_t levels = f->cfwd.host.levels + f->cbwd.host.levels;
-      int per_cu = 0;
       size_t lds = 0;
-      for (const auto& kt : class_shapes(refined)) {
-        const int k = kt.first, threads = kt.second;
-        if (per_cu) break;
-        const size_t bytes = refined ? refine_lds_bytes(n, threads, levels) : solve_lds_bytes(n, false, levels);
-        if ((size_t)k * bytes <= f->lds_cap) {
-          per_cu = k;
-          lds = bytes;
-        }
-      }
+      const int per_cu = class_of(f, refined, levels, lds);
       if (!per_cu)
         fail(YBUS_HIP_INVALID_ARGUMENT,
              "factor " + std::to_string(i) + " (dim " + std::to_string(f->n) + ") does not fit in LDS for this mode");
@@ -3450,6 +3454,8 @@ struct ybus_hip_batch {
         c.per_cu = all[k].per_cu;
         for (const auto& kt : class_shapes(refined))
           if (kt.first == c.per_cu) c.threads = kt.second;
+        // the FP64 kernel of one or two workgroups a compute unit takes at most 512 threads
+        if (!refined && c.per_cu <= 2) c.threads = std::min(c.threads, 512);
         c.first = (int32_t)k;
         plan.classes.push_back(c);
       }
@@ -3488,8 +3494,8 @@ struct ybus_hip_batch {
                                   queues.p + c, rhs.p, x.p, x1.p, r.p, tol, max_steps, steps.p, ratio.p, refine_stop),
               "batch kernel");
       else
-        check(launch_batch(on, grid, k.threads, k.lds, entries.p, plan.items.p + k.first, k.count, queues.p + c,
-                           rhs.p, x.p),
+        check(launch_batch(on, grid, k.threads, k.lds, k.per_cu, entries.p, plan.items.p + k.first, k.count,
+                           queues.p + c, rhs.p, x.p),
               "batch kernel");
       if (forked) check(hipEventRecord(join[c - 1], on), "hipEventRecord");
     }
@@ -3602,7 +3608,7 @@ int32_t ybus_hip_batch_solve_staged(ybus_hip_batch* b, int32_t mode, double tole
     if (plan.exec && refined && (plan.tol != tolerance || plan.max_steps != (int32_t)max_steps)) plan.drop_graph();
     if (!plan.exec) {
       for (const ybus_hip_batch::Class& c : plan.classes)
-        check(refined ? prepare_batch_refine(c.lds) : prepare_batch(c.lds), "batch kernel attribute");
+        check(refined ? prepare_batch_refine(c.lds) : prepare_batch(c.lds, c.per_cu), "batch kernel attribute");
       check(hipStreamBeginCapture(b->stream(), hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       try {
         b->enqueue(refined, tolerance, (int32_t)max_steps);
