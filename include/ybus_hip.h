@@ -418,8 +418,9 @@ int32_t ybus_hip_factor_path(const ybus_hip_factor *f, uint64_t columns, int32_t
  * whether the wide path sweeps the factor's partitioned form (1: a Cholesky factor's independent
  * elimination subtrees side by side, then the top of the tree; 0: one schedule), and for a
  * partitioned block of vectors the top of the tree's own tile width and workgroup size (the
- * parts' own where the top takes the same), and the tile width and workgroup size of the parts
- * swept with their values in LDS (0: the parts over the block in global memory). */
+ * parts' own where the top takes the same), the tile width and workgroup size of the parts
+ * swept with their values in LDS (0: the parts over the block in global memory), and whether a
+ * one-schedule wide block keeps its last positions' values in an LDS ring (window 1). */
 typedef struct {
   int32_t path;
   int32_t threads;
@@ -430,6 +431,7 @@ typedef struct {
   int32_t top_threads;
   int32_t parts_lds_tile;
   int32_t parts_lds_threads;
+  int32_t window;
 } ybus_hip_tuning;
 
 /* Autotuner (new): times every kernel configuration a block of `columns` right-hand sides can
@@ -438,7 +440,8 @@ typedef struct {
  * where it has one, partitioned schedule), each on a synthetic FP64 block
  * of that width, then, when a partitioned block of vectors wins, the same tile widths and sizes
  * for its top-of-tree kernel alone, then its parts swept in LDS (tiles of 1 to 16 columns at 256,
- * 512 and 1024 threads, where they fit), and keeps the fastest for this factor's later blocks of that width (up to the
+ * 512 and 1024 threads, where they fit) or, when one schedule wins, the same solve with its last positions in LDS at
+ * each tile width and size, and keeps the fastest for this factor's later blocks of that width (up to the
  * next power of two) and kind: pair blocks (pairs != 0; pair_solves, pair_updates, staged pairs)
  * or blocks of vectors (solve_many, staged vectors, refinement). A pair block is timed on the
  * caller's own `columns` pairs when plus and minus are given (the pairs' reach sets the forward

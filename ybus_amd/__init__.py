@@ -98,7 +98,7 @@ assert _COLUMN_MAX.itemsize == C.sizeof(_ColumnMax)
 class _Tuning(C.Structure):
     _fields_ = [("path", C.c_int32), ("threads", C.c_int32), ("tile", C.c_int32), ("milliseconds", C.c_double),
                 ("parts", C.c_int32), ("top_tile", C.c_int32), ("top_threads", C.c_int32),
-                ("parts_lds_tile", C.c_int32), ("parts_lds_threads", C.c_int32)]
+                ("parts_lds_tile", C.c_int32), ("parts_lds_threads", C.c_int32), ("window", C.c_int32)]
 
 
 class _KernelStats(C.Structure):
@@ -537,9 +537,9 @@ class GpuFactor:
         fastest for this factor's blocks of that width and kind: pair blocks when `pairs` (True
         for synthetic pairs, or the caller's own list of Pair to time on), else blocks of vectors.
         Returns {'path', 'threads', 'tile', 'parts', 'top_tile', 'top_threads', 'parts_lds_tile',
-        'parts_lds_threads', 'milliseconds'} (top_*: the partitioned solve's top-of-tree kernel;
-        parts_lds_*: its parts swept with their values in LDS, tile 0 where they stay in global
-        memory)."""
+        'parts_lds_threads', 'window', 'milliseconds'} (top_*: the partitioned solve's top-of-tree
+        kernel; parts_lds_*: its parts swept with their values in LDS, tile 0 where they stay in
+        global memory; window: a one-schedule wide solve with its last positions in LDS)."""
         self._release_block()
         t = _Tuning()
         plus = minus = None
@@ -552,7 +552,7 @@ class GpuFactor:
         return dict(path="wide" if t.path == 1 else "columns", threads=int(t.threads), tile=int(t.tile),
                     parts=bool(t.parts), top_tile=int(t.top_tile), top_threads=int(t.top_threads),
                     parts_lds_tile=int(t.parts_lds_tile), parts_lds_threads=int(t.parts_lds_threads),
-                    milliseconds=float(t.milliseconds))
+                    window=bool(t.window), milliseconds=float(t.milliseconds))
 
     def kernel_stats(self) -> KernelStats:
         s = _KernelStats()

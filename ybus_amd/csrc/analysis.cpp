@@ -284,6 +284,37 @@ PartSlots part_slots(const Sweep& f, const Sweep& b) {
   return out;
 }
 
+WindowCode window_code(const Sweep& s, int32_t cap) {
+  WindowCode w;
+  w.cap = cap;
+  const int32_t positions = s.positions(), n = s.n;
+  if (cap < 1) return w;
+  std::vector<int32_t> level_of(positions), written(n + 1, -1);
+  for (int32_t l = 0; l < s.levels; ++l)
+    for (int32_t k = s.level_ptr[l]; k < s.level_ptr[l + 1]; ++k) level_of[k] = l;
+  w.src.resize(positions);
+  w.dep.resize(s.dep.size());
+  auto code = [&](int32_t row, int32_t l, int32_t end) -> int32_t {
+    const int32_t q = row >= 0 && row <= n ? written[row] : -1;
+    if (q < 0) return row;
+    if (q >= end - cap) return ~(q % cap);
+    if (level_of[q] <= l - WINDOW_AHEAD - 1) return row;
+    return row + n + 1;
+  };
+  for (int32_t k = 0; k < positions; ++k) {
+    const int32_t l = level_of[k], end = s.level_ptr[l + 1];
+    if (s.src[k] >= 0) {
+      w.src[k] = s.src[k];
+    } else {
+      const int32_t c = code(s.dst[k], l, end);
+      w.src[k] = c < 0 ? c : WINDOW_PARTIAL;
+    }
+    for (int32_t p = s.row_ptr[k]; p < s.row_ptr[k + 1]; ++p) w.dep[p] = code(s.dep[p], l, end);
+    written[s.dst[k]] = k;
+  }
+  return w;
+}
+
 std::vector<int32_t> tree_parts(const RowForm& lower, int32_t target, int32_t& parts) {
   parts = 0;
   const int64_t n = lower.n;

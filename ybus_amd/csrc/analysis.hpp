@@ -5,6 +5,7 @@
 // tile-width choice for a block of right-hand sides.
 #pragma once
 
+#include <climits>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -131,6 +132,26 @@ struct PartSlots {
   Dir fwd, bwd;
 };
 PartSlots part_slots(const Sweep& fwd, const Sweep& bwd);
+
+// A sweep's sources encoded for a solve that keeps its last `cap` positions' values in an LDS ring
+// (position q in slot q mod cap) and loads other values WINDOW_AHEAD levels before their row's
+// level. For a position at level l, a term's row d (written last at position q) is:
+//   ~(q mod cap) (negative), when q is still in the ring as level l runs (q >= the level's end
+//     - cap);
+//   d, when q is at level l - WINDOW_AHEAD - 1 or earlier, or d is never written by the sweep
+//     (the pad row): its value is final by the start of level l - WINDOW_AHEAD, so it can be
+//     loaded then;
+//   d + n + 1, otherwise: loaded when the row is swept.
+// src: a first segment's right-hand side row; a continuing segment's own row by the same rule,
+// as ~(q mod cap), or WINDOW_PARTIAL (its partial value read from the block at dst).
+constexpr int32_t WINDOW_PARTIAL = INT32_MIN;
+constexpr int32_t WINDOW_AHEAD = 2;
+struct WindowCode {
+  int32_t cap = 0;
+  std::vector<int32_t> src;  // per position
+  std::vector<int32_t> dep;  // per term
+};
+WindowCode window_code(const Sweep& s, int32_t cap);
 
 // Each row's level: one more than the deepest of its dependencies (0 for none).
 std::vector<int32_t> levels_of(const RowForm& form);

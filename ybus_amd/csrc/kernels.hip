@@ -395,6 +395,176 @@ __global__ void __launch_bounds__(1024) solve64_kernel(SweepDev fwd, SweepDev bw
                          [&](int32_t dst, double v) { x[dst * ld + col] = v; });
 }
 
+// ---- FP64 solve with the last positions' values in LDS ----
+
+// A source as window_code encodes it (the pad row, a row that is never written, is old)
+constexpr int32_t WINDOW_PARTIAL_SOURCE = INT32_MIN;
+
+// A thread's position of a level for the windowed sweep (k < 0: none there): its window head and
+// inline sources, the sweep's values and diagonal, and the values loaded ahead for it (its
+// right-hand side, then its inline terms that are old)
+struct WinRow {
+  int32_t k;
+  int4 h, d;
+  double2 v01, v23;
+  double diag;
+  double pre[RECORD_ENTRIES + 1];
+};
+
+// Position k's record if k < end (else position 0's, and k = -1)
+__device__ __forceinline__ void win_fetch(const SweepDev& s, const WindowDev& w, int32_t k, int32_t end, WinRow& x) {
+  const bool on = k < end;
+  const int32_t at = on ? k : 0;
+  x.k = on ? k : -1;
+  x.h = w.head[at];
+  x.d = w.dep4[at];
+  x.v01 = s.rec.val01[at];
+  x.v23 = s.rec.val23[at];
+  x.diag = s.rec.diag64[at];
+}
+
+// Loads x's right-hand side and its old inline terms (their rows are final by the start of the
+// level before x's), where the thread has the position and its lane a column
+template <int TW, class Rhs>
+__device__ __forceinline__ void win_load_ahead(WinRow& x, const double* __restrict__ dep_buf, int64_t ld, int32_t n,
+                                               const Tile<TW>& g, Rhs& rhs) {
+  const bool go = x.k >= 0 && g.on;
+  x.pre[0] = go && x.h.y >= 0 ? rhs(x.h.y) : 0.0;
+  const int32_t dd[RECORD_ENTRIES] = {x.d.x, x.d.y, x.d.z, x.d.w};
+#pragma unroll
+  for (int i = 0; i < RECORD_ENTRIES; ++i) {
+    const bool old = go && dd[i] >= 0 && dd[i] <= n;
+    const double v = dep_buf[(int64_t)(old ? dd[i] : n) * ld + g.col];
+    x.pre[1 + i] = old ? v : 0.0;
+  }
+}
+
+// x's value: its source minus its terms in order, over its diagonal, each value from the ring,
+// from what was loaded ahead (ahead: x's old terms and right-hand side were), or from the block.
+// The block's loads (a first row's rare late terms or partial value; every old one of a row not
+// loaded ahead) are issued and waited for inside their own branch: a wait placed after it would
+// also wait for the loads issued ahead for the next levels, which come after x's in the queue.
+template <int TW, class Rhs>
+__device__ __forceinline__ double win_value(const SweepDev& s, const WindowDev& w, const WinRow& x, bool ahead,
+                                            const double* ring, const double* __restrict__ dep_buf, int64_t ld,
+                                            int32_t n, const Tile<TW>& g, Rhs& rhs) {
+  const int32_t len = x.h.w - x.h.z;
+  const int32_t dd[RECORD_ENTRIES] = {x.d.x, x.d.y, x.d.z, x.d.w};
+  const double vv[RECORD_ENTRIES] = {x.v01.x, x.v01.y, x.v23.x, x.v23.y};
+  double a = x.h.y >= 0 ? (ahead ? x.pre[0] : 0.0) : (x.h.y != WINDOW_PARTIAL_SOURCE ? ring[(~x.h.y) * TW + g.lane] : 0.0);
+  double xv[RECORD_ENTRIES];
+  bool block = x.h.y == WINDOW_PARTIAL_SOURCE || (x.h.y >= 0 && !ahead);
+#pragma unroll
+  for (int i = 0; i < RECORD_ENTRIES; ++i) {
+    xv[i] = dd[i] < 0 ? ring[(~dd[i]) * TW + g.lane] : (ahead && dd[i] <= n ? x.pre[1 + i] : 0.0);
+    block = block || (len > i && dd[i] >= 0 && (dd[i] > n || !ahead));
+  }
+  if (block) {
+    if (x.h.y >= 0 && !ahead) a = rhs(x.h.y);
+    if (x.h.y == WINDOW_PARTIAL_SOURCE) a = dep_buf[(int64_t)x.h.x * ld + g.col];
+#pragma unroll
+    for (int i = 0; i < RECORD_ENTRIES; ++i)
+      if (len > i && dd[i] >= 0 && (dd[i] > n || !ahead))
+        xv[i] = dep_buf[(int64_t)(dd[i] > n ? dd[i] - n - 1 : dd[i]) * ld + g.col];
+    __asm__ volatile("" ::"v"(a), "v"(xv[0]), "v"(xv[1]), "v"(xv[2]), "v"(xv[3]));
+  }
+#pragma unroll
+  for (int i = 0; i < RECORD_ENTRIES; ++i)
+    if (len > i) a = a - vv[i] * xv[i];
+  for (int32_t p = x.h.z + RECORD_ENTRIES; p < x.h.w; ++p) {
+    const int32_t e = w.dep[p];
+    a = a - s.val64[p] * (e < 0 ? ring[(~e) * TW + g.lane] : dep_buf[(int64_t)(e > n ? e - n - 1 : e) * ld + g.col]);
+  }
+  return s.unit ? a : a / x.diag;
+}
+
+// One sweep of a tile with its last w.cap positions' values in the LDS ring (slot q mod cap, the
+// tile's TW columns side by side). Level l: the thread fetches its first position of level l + 3,
+// loads the old values of its first position of level l + 2 (final since the barrier before
+// level l: the encoding's old values are three levels back), then sweeps its positions of level
+// l (the first from the values loaded at level l - 2; the second fetched at the level's start,
+// before the loads ahead, so waiting for it does not wait for them; any further ones after),
+// storing each value into the block and its ring slot, and a barrier ends the level. Four row
+// sets take the four roles in turn (the loop unrolled by four), so no register copy waits on
+// loads in flight. A ring slot is written by position k only after every reader of the position
+// it held (k - cap) is done: a reader of position q at level l has q >= the level's end - cap >
+// k - cap for every k it sweeps beside. Every row is computed as wide_level computes it, so the
+// bits are the same.
+template <int TW, class Rhs, class Store>
+__device__ void window_sweep(const SweepDev& s, const WindowDev& w, const int32_t* lp, double* ring,
+                             const double* __restrict__ dep_buf, int64_t ld, const Tile<TW>& g, bool act, Rhs rhs,
+                             Store store) {
+  const int32_t levels = s.levels, n = s.n, cap = w.cap, slots = g.slots;
+  if (levels == 0 || lp[levels] == 0) return;
+  auto end_of = [&](int32_t l) { return l < levels ? lp[l + 1] : 0; };
+  auto first_of = [&](int32_t l) { return l < levels ? lp[l] + g.slot : 0; };
+  auto sweep_row = [&](const WinRow& x, bool ahead) {
+    const double v = win_value<TW>(s, w, x, ahead, ring, dep_buf, ld, n, g, rhs);
+    store(x.h.x, v);
+    ring[(x.k % cap) * TW + g.lane] = v;
+  };
+  auto level = [&](int32_t l, WinRow& a, WinRow& c, WinRow& d) {
+    const int32_t end = end_of(l);
+    WinRow e;
+    win_fetch(s, w, a.k >= 0 ? a.k + slots : end, end, e);
+    win_fetch(s, w, first_of(l + 3), end_of(l + 3), d);
+    win_load_ahead<TW>(c, dep_buf, ld, n, g, rhs);
+    if (act && a.k >= 0) {
+      sweep_row(a, true);
+      if (e.k >= 0) {
+        sweep_row(e, false);
+        for (int32_t k = e.k + slots; k < end; k += slots) {
+          WinRow f;
+          win_fetch(s, w, k, end, f);
+          sweep_row(f, false);
+        }
+      }
+    }
+    __syncthreads();
+  };
+  // a: level 0 (its values loaded now), b: level 1 (loaded now), c: level 2, d: fetched at level 0
+  WinRow a, b, c, d;
+  win_fetch(s, w, first_of(0), end_of(0), a);
+  win_fetch(s, w, first_of(1), end_of(1), b);
+  win_fetch(s, w, first_of(2), end_of(2), c);
+  win_load_ahead<TW>(a, dep_buf, ld, n, g, rhs);
+  win_load_ahead<TW>(b, dep_buf, ld, n, g, rhs);
+  for (int32_t l = 0; l < levels; l += 4) {
+    level(l, a, c, d);
+    if (l + 1 >= levels) break;
+    level(l + 1, b, d, a);
+    if (l + 2 >= levels) break;
+    level(l + 2, c, a, b);
+    if (l + 3 >= levels) break;
+    level(l + 3, d, b, c);
+  }
+}
+
+template <int TW>
+__global__ void __launch_bounds__(512) solve64_window_kernel(SweepDev fwd, WindowDev wf, SweepDev bwd, WindowDev wb,
+                                                              RhsDev rhs, double* __restrict__ t,
+                                                              double* __restrict__ x, int64_t ld, int32_t ncols,
+                                                              int32_t xcds) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* ring = reinterpret_cast<double*>(smem);
+  int32_t* lpf = reinterpret_cast<int32_t*>(smem + (size_t)wf.cap * TW * sizeof(double));
+  int32_t* lpb = lpf + fwd.levels + 1;
+  for (int32_t l = threadIdx.x; l <= fwd.levels; l += blockDim.x) lpf[l] = fwd.level_ptr[l];
+  stage_levels(bwd, lpb);
+  const Tile<TW> g(ncols, tile_of_workgroup(xcds));
+  const int32_t col = g.col;
+  const int32_t plus = !rhs.block && g.on ? rhs.plus[col] : -1;
+  const int32_t minus = !rhs.block && g.on ? rhs.minus[col] : -1;
+  auto from_rhs = [&](int32_t src) -> double {
+    if (rhs.block) return rhs.block[(int64_t)src * ld + col];
+    return (src == plus ? 1.0 : 0.0) - (src == minus ? 1.0 : 0.0);
+  };
+  window_sweep<TW>(fwd, wf, lpf, ring, t, ld, g, g.on, from_rhs,
+                   [&](int32_t dst, double v) { t[(int64_t)dst * ld + col] = v; });
+  window_sweep<TW>(bwd, wb, lpb, ring, x, ld, g, g.on, [&](int32_t src) { return t[(int64_t)src * ld + col]; },
+                   [&](int32_t dst, double v) { x[(int64_t)dst * ld + col] = v; });
+}
+
 // A pair block whose forward sweep visits only each tile's reach: the rows of the forward
 // sweep that depend, directly or through other rows, on one of the tile's plus or minus
 // unknowns. Every other forward row of the tile is exactly +0.0 in the CPU's solve (its
@@ -2119,6 +2289,36 @@ hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const Sweep
   DISPATCH_TILE(tile, L)
 #undef L
 #undef S
+  return hipGetLastError();
+}
+
+size_t window_lds_bytes(const SweepDev& fwd, const SweepDev& bwd, int tile, int32_t cap) {
+  return (size_t)cap * (size_t)tile * sizeof(double) + (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
+}
+
+int32_t window_cap_for(const SweepDev& fwd, const SweepDev& bwd, int tile, size_t lds_cap) {
+  const size_t levels = (size_t)(fwd.levels + bwd.levels + 2) * sizeof(int32_t);
+  if (tile < 1 || lds_cap <= levels) return 0;
+  return (int32_t)((lds_cap - levels) / ((size_t)tile * sizeof(double)));
+}
+
+hipError_t launch_solve64_window(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const WindowDev& wf,
+                                 const SweepDev& bwd, const WindowDev& wb, const RhsDev& rhs, double* t, double* x,
+                                 int64_t ld, int32_t ncols) {
+  if (threads < tile || threads > 512 || threads % 64 || wf.cap < 1 || wb.cap != wf.cap || !wf.head || !wb.head)
+    return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)((ncols + tile - 1) / tile);
+  const size_t bytes = window_lds_bytes(fwd, bwd, tile, wf.cap);
+#define L(TWV)                                                                                                  \
+  {                                                                                                             \
+    hipError_t e = hipFuncSetAttribute((const void*)solve64_window_kernel<TWV>,                                 \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                \
+    if (e != hipSuccess) return e;                                                                              \
+    hipLaunchKernelGGL((solve64_window_kernel<TWV>), dim3(grid), dim3(threads), bytes, stream, fwd, wf, bwd, wb, \
+                       rhs, t, x, ld, ncols, xcd_groups());                                                     \
+  }
+  DISPATCH_TILE(tile, L)
+#undef L
   return hipGetLastError();
 }
 

@@ -134,6 +134,27 @@ hipError_t launch_refine_lds(hipStream_t stream, int threads, const SweepDev& fw
 hipError_t launch_solve64(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const SweepDev& bwd,
                           const RhsDev& rhs, double* t, double* x, int64_t ld, int32_t ncols);
 
+// A sweep's sources for the windowed solve (window_code in analysis.hpp: a value in the LDS ring
+// of the last `cap` positions, a row final by the level before the one before, or a row written
+// by the level before): per position its head (dst, source, p0, p1) and its first four terms'
+// sources (unused ones: the pad row), and every term's source. Values and diagonals come from the
+// sweep's own records.
+struct WindowDev {
+  const int4* head = nullptr;
+  const int4* dep4 = nullptr;
+  const int32_t* dep = nullptr;
+  int32_t cap = 0;
+};
+// The FP64 solve with each tile's last `cap` positions' values in an LDS ring (both sweeps' cap
+// the same): a term in the ring is read from LDS, a term final three levels back is loaded two
+// levels ahead, and only the others go to the block when their row is swept. LDS bytes for a tile width,
+// the largest ring a compute unit's LDS allows, and the launch (at most 512 threads).
+size_t window_lds_bytes(const SweepDev& fwd, const SweepDev& bwd, int tile, int32_t cap);
+int32_t window_cap_for(const SweepDev& fwd, const SweepDev& bwd, int tile, size_t lds_cap);
+hipError_t launch_solve64_window(hipStream_t stream, int tile, int threads, const SweepDev& fwd, const WindowDev& wf,
+                                 const SweepDev& bwd, const WindowDev& wb, const RhsDev& rhs, double* t, double* x,
+                                 int64_t ld, int32_t ncols);
+
 // The reach of a pair block's tiles: tile b's forward rows are the positions pos[lp[b (levels +
 // 1) + l] ..) of each level l, in level order (levels = the forward sweep's).
 struct ReachDev {

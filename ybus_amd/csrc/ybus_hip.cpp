@@ -182,8 +182,42 @@ struct Staging {
 
 // A sweep on the device: its level pointers, its rows' records (analysis order, field by
 // field), and the CSR arrays holding every entry (the records inline the first few of each row)
+// A sweep's sources encoded for the windowed solve with a ring of `cap` positions (window_code)
+struct DevWindow {
+  DevBuf<int4> head, dep4;
+  DevBuf<int32_t> dep;
+  WindowDev dev;
+  void build(const Sweep& s, int32_t cap, hipStream_t stream) {
+    const WindowCode c = window_code(s, cap);
+    const int32_t count = s.positions();
+    std::vector<int4> hd(std::max(count, 1)), d4(std::max(count, 1));
+    for (int32_t k = 0; k < count; ++k) {
+      const int32_t p0 = s.row_ptr[k], p1 = s.row_ptr[k + 1];
+      hd[k] = make_int4(s.dst[k], c.src[k], p0, p1);
+      int32_t d[RECORD_ENTRIES];
+      for (int q = 0; q < RECORD_ENTRIES; ++q) d[q] = p0 + q < p1 ? c.dep[p0 + q] : s.n;  // unused: the pad row
+      d4[k] = make_int4(d[0], d[1], d[2], d[3]);
+    }
+    head.upload(hd, stream);
+    dep4.upload(d4, stream);
+    dep.upload(c.dep, stream);
+    check(hipStreamSynchronize(stream), "upload window");
+    dev = WindowDev{head.p, dep4.p, dep.p, cap};
+  }
+};
+
 struct DevSweep {
   DevBuf<int32_t> level_ptr, src, dst, row_ptr, dep, part_level;
+  // the windowed solve's encodings of this sweep, by ring size (built on first use)
+  std::map<int32_t, std::unique_ptr<DevWindow>> windows;
+  const DevWindow* window(int32_t cap, hipStream_t stream, bool build) {
+    auto it = windows.find(cap);
+    if (it != windows.end()) return it->second.get();
+    if (!build) return nullptr;
+    auto w = std::make_unique<DevWindow>();
+    w->build(host, cap, stream);
+    return (windows[cap] = std::move(w)).get();
+  }
   DevBuf<double> val64, diag64;
   DevBuf<float> val32, diag32;
   DevBuf<int4> head, dep4;
@@ -728,6 +762,12 @@ struct ybus_hip_factor {
     DevPartSlots fwd, bwd;
   } parts_lds;
   int parts_lds_env = -1, parts_lds_env_threads = 0;
+  // the one-schedule wide FP64 solve with each tile's last positions in an LDS ring
+  // (solve64_window_kernel), where tuned or YBUS_HIP_WINDOW=1 (0: never)
+  int window_env = -1;
+  // inside ybus_hip_factor_solve_device_noalloc: nothing may be allocated, so a ring encoding not
+  // yet built is not built (the barrier solve runs instead)
+  bool no_alloc = false;
   // column path: schedules in factor order, and each unknown's factor row for the right-hand
   // side (rhs_row) and the solution (sol_of), for the solve and the transposed solve
   DevSweep cfwd, cbwd, ctfwd, ctbwd;
@@ -794,6 +834,7 @@ struct ybus_hip_factor {
     int parts = -1;  // the partitioned sweeps: 1 on, 0 off, -1 by the default rule
     int top_tile = 0, top_threads = 0;  // the partitioned top kernel's own (0: the parts')
     int parts_lds_tile = 0, parts_lds_threads = 0;  // the parts in LDS slots (0: over the block)
+    bool window = false;  // the one-schedule wide solve with the last positions in LDS
   };
   std::map<int, Tuned> tuned;
   const Tuned* trial = nullptr;  // a configuration being timed by the autotuner
@@ -816,6 +857,12 @@ struct ybus_hip_factor {
   int column_threads(int64_t ncols, bool pairs) const {
     const Tuned* t = tuned_for(ncols, pairs);
     return t && t->columns && !env_threads ? t->threads : threads_columns;
+  }
+  // Whether a one-schedule wide block of `ncols` columns takes the windowed solve
+  bool window_for(int64_t ncols, bool pairs) const {
+    if (window_env >= 0) return window_env == 1;
+    const Tuned* t = tuned_for(ncols, pairs);
+    return t && t->window;
   }
   // The LDS parts' configuration a partitioned block of vectors takes (tile 0: the parts over the
   // block): YBUS_HIP_PARTS_LDS's, else the tuned one, where both sweeps' parts fit LDS
@@ -1136,11 +1183,25 @@ struct ybus_hip_factor {
       } else {
         // an LU factor's split sweeps, where built
         const bool sp = split_sweeps();
-        const DevSweep& fs = sp ? (transpose ? stfwd : sfwd) : f;
-        const DevSweep& bs = sp ? (transpose ? stbwd : sbwd) : bw;
-        check(launch_solve64(stream(), tile, wide_threads(ncols, pairs), fs.view(), bs.view(), rhs, t.p, x.p, layout.ld,
-                             (int32_t)ncols),
-              "solve kernel");
+        DevSweep& fs = const_cast<DevSweep&>(sp ? (transpose ? stfwd : sfwd) : f);
+        DevSweep& bs = const_cast<DevSweep&>(sp ? (transpose ? stbwd : sbwd) : bw);
+        const DevWindow* wf = nullptr;
+        const DevWindow* wb = nullptr;
+        if (window_for(ncols, pairs) && wide_threads(ncols, pairs) <= 512) {
+          const int32_t cap = window_cap_for(fs.view(), bs.view(), tile, lds_cap);
+          if (cap >= 1) {
+            wf = fs.window(cap, stream(), !no_alloc);
+            wb = bs.window(cap, stream(), !no_alloc);
+          }
+        }
+        if (wf && wb)
+          check(launch_solve64_window(stream(), tile, wide_threads(ncols, pairs), fs.view(), wf->dev, bs.view(),
+                                      wb->dev, rhs, t.p, x.p, layout.ld, (int32_t)ncols),
+                "solve kernel");
+        else
+          check(launch_solve64(stream(), tile, wide_threads(ncols, pairs), fs.view(), bs.view(), rhs, t.p, x.p,
+                               layout.ld, (int32_t)ncols),
+                "solve kernel");
       }
     }
     end_kernel(bytes64(ncols, pairs));
@@ -1318,6 +1379,7 @@ static void begin_factor(ybus_hip_factor* f, ybus_hip_ctx* ctx, int64_t n) {
     f->parts_lds_env = std::max(0, std::atoi(pl));
     if (const char* colon = std::strchr(pl, ':')) f->parts_lds_env_threads = std::atoi(colon + 1);
   }
+  if (const char* win = std::getenv("YBUS_HIP_WINDOW")) f->window_env = std::string(win) != "0" ? 1 : 0;
   if (const char* top = std::getenv("YBUS_HIP_TOP")) {
     f->top_override = std::atoi(top);
     if (const char* colon = std::strchr(top, ':')) f->top_override_threads = std::atoi(colon + 1);
@@ -2644,9 +2706,13 @@ int32_t ybus_hip_factor_solve_device_noalloc(ybus_hip_factor* f, int32_t mode, c
     struct Restore {
       ybus_hip_factor* f;
       Layout layout;
-      ~Restore() { f->layout = layout; }
+      ~Restore() {
+        f->layout = layout;
+        f->no_alloc = false;
+      }
     } restore{f, own};
     f->layout = layout;
+    f->no_alloc = true;
     for (uint64_t start = 0; start < count; start += (uint64_t)width) {
       const int64_t chunk = (int64_t)std::min<uint64_t>((uint64_t)width, count - start);
       copy_in_device(f, b + start * ld_b, (int64_t)ld_b, chunk, f->b.p);
@@ -2866,6 +2932,25 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
             }
         }
       }
+      // a one-schedule wide block: the solve with the last positions in LDS, at each tile width
+      // and workgroup size
+      if (!best.columns && best.parts != 1 && f->window_env < 0) {
+        const ybus_hip_factor::Tuned wide_best = best;
+        for (int th : {256, 512})
+          for (int tile : {1, 2, 4, 8, 16}) {
+            if (th < tile || (tile != 1 && tile / 2 >= k)) continue;
+            ybus_hip_factor::Tuned c = wide_best;
+            c.window = true;
+            c.tile = tile;
+            c.threads = th;
+            c.parts = 0;
+            const double ms = time_config(c);
+            if (ms < best_ms) {
+              best = c;
+              best_ms = ms;
+            }
+          }
+      }
     } catch (...) {
       f->trial = nullptr;
       (void)hipEventDestroy(e0);
@@ -2887,7 +2972,8 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
                              best.top_tile ? best.top_tile : (best.columns ? 0 : best.tile),
                              best.top_tile ? best.top_threads : (best.columns ? 0 : best.threads),
                              best.parts_lds_tile,
-                             best.parts_lds_threads};
+                             best.parts_lds_threads,
+                             best.window ? 1 : 0};
   });
 }
 
