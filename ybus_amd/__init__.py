@@ -77,6 +77,14 @@ def library() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ybus_amd.build` "
                                "(the GPU path has no CPU fallback)")
+        # One HIP runtime per process. PyTorch ships its own libamdhip64 (soname libamdhip64.so.7,
+        # needed by torch as libamdhip64.so): loaded first, it is the one the library binds to;
+        # loaded after the library's, torch would map a second runtime beside it, which then finds
+        # no GPU. So torch, where installed, is imported before the library is loaded.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = C.CDLL(LIB_PATH)
         _declare(_lib)
     return _lib
