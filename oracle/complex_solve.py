@@ -12,12 +12,15 @@ Python floats or as float64 numpy arrays over many right-hand sides at once (num
 elementwise operation on its own, so the two agree bit for bit).
 
 Parity. The reference's complex tests check tolerances only (accuracy_test.rs:105-115,
-sparse_rhs_test.rs:239), so no golden vector pins the rounding of a complex product or quotient:
-the bit-level agreement with num-complex is "parity unpinned" for these functions. What pins them
-(tests/test_oracle_complex.py): the factor satisfies L L^H = P A P^T, the solves agree with a
-dense LAPACK solve to 1e-12, the transposed solves with one of A^T (plain, not conjugated), the
-Cholesky transposed solve is the conjugate identity, and ``zmul``/``zdiv`` agree with exact
-rational arithmetic to within their rounding.
+sparse_rhs_test.rs:239), so no golden vector pins the rounding of a complex product or quotient.
+What pins them: the factor satisfies L L^H = P A P^T, the solves agree with a dense LAPACK solve
+to 1e-12, the transposed solves with one of A^T (plain, not conjugated), and the Cholesky
+transposed solve is the conjugate identity (tests/test_oracle_complex.py). Bit for bit, where the
+mathematics fixes the bits (tests/test_oracle_complex_rounding.py): products whose real products
+cancel only when each is rounded on its own (no fused multiply-add), quotients of Gaussian
+integers, which num-complex's formula rounds correctly and a scaled (Smith) or reciprocal
+division does not, and 2 x 2 LU and Cholesky solves that round only in their last division, whose
+results must be the correctly rounded exact solutions.
 
 Factors. ``ComplexCholesky`` restates the up-looking numeric factorization
 (SymbolicCholesky::factor, cholesky.rs:206-311) over the symbolic analysis (ordering and pattern)

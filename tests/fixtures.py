@@ -266,3 +266,99 @@ def complex_relative_residual(a, x: np.ndarray, b: np.ndarray) -> float:
     """The worst column's ||b - A x||inf / ||b||inf (accuracy.rs:57-78), A dense or scipy sparse."""
     r = b.T - a @ x.T
     return float(np.max(np.max(np.abs(r), axis=0) / np.max(np.abs(b.T), axis=0)))
+
+
+def _gauss(z):
+    """A complex number's parts as exact Fractions."""
+    from fractions import Fraction
+    return Fraction(float(z.real)), Fraction(float(z.imag))
+
+
+def _gauss_mul(a, b):
+    return a[0] * b[0] - a[1] * b[1], a[0] * b[1] + a[1] * b[0]
+
+
+def _gauss_div(a, b):
+    s = b[0] * b[0] + b[1] * b[1]
+    return (a[0] * b[0] + a[1] * b[1]) / s, (a[1] * b[0] - a[0] * b[1]) / s
+
+
+def _gauss_sub(a, b):
+    return a[0] - b[0], a[1] - b[1]
+
+
+def exact_solve_2x2(a: np.ndarray, b: np.ndarray):
+    """The exact solution of the 2 x 2 complex system a x = b (Cramer's rule in Fractions), for each
+    row of the (count, 2) complex b: a list of [(re, im), (re, im)] pairs of Fractions."""
+    m = [[_gauss(a[i, j]) for j in range(2)] for i in range(2)]
+    det = _gauss_sub(_gauss_mul(m[0][0], m[1][1]), _gauss_mul(m[0][1], m[1][0]))
+    out = []
+    for row in b:
+        b0, b1 = _gauss(row[0]), _gauss(row[1])
+        x0 = _gauss_div(_gauss_sub(_gauss_mul(b0, m[1][1]), _gauss_mul(m[0][1], b1)), det)
+        x1 = _gauss_div(_gauss_sub(_gauss_mul(m[0][0], b1), _gauss_mul(m[1][0], b0)), det)
+        out.append([x0, x1])
+    return out
+
+
+def rounded(exact) -> np.ndarray:
+    """exact_solve_2x2's solutions rounded to the nearest doubles, part by part, as (count, 2)
+    complex128 (float() of a Fraction rounds correctly)."""
+    x = np.empty((len(exact), 2), dtype=np.complex128)
+    for k, (x0, x1) in enumerate(exact):
+        x[k, 0] = complex(float(x0[0]), float(x0[1]))
+        x[k, 1] = complex(float(x1[0]), float(x1[1]))
+    return x
+
+
+def gaussian_closed_forms(count: int = 16, seed: int = 5):
+    """Three 2 x 2 complex factors in natural order with Gaussian-integer entries, each solved so
+    that only its last division rounds: every product and sum before it is of small integers or
+    half-integers, so it is exact, and a quotient whose parts' numerators and the divisor's squared
+    magnitude are exact is the correctly rounded exact quotient. Any solve in num-complex's
+    arithmetic, in the reference's operation order, must return the correctly rounded exact
+    solution; a scaled (Smith) division or a product with the reciprocal returns other bits for
+    most of these right-hand sides.
+    - "lu": L = [1, 0; 2 - i, 1], U = [3 + i, 1 + 2i; 0, 1 + i], solving A x = b: x1 = y1 / (1 + i)
+      is a half-integer, x0 = (y0 - (1 + 2i) x1) / (3 + i) rounds once.
+    - "lu_transpose": L = I, U = [1 + i, 2 - 3i; 0, 3 + i], solving A^T x = b: x0 = b0 / (1 + i)
+      is a half-integer, x1 = (b1 - (2 - 3i) x0) / (3 + i) rounds once.
+    - "cholesky": L = [3, 0; 2 - i, 1] (a real diagonal), solving L L^H x = b with b0's parts
+      multiples of 3: y0 = b0 / 3 and x1 = y1 are integers, x0 = (y0 - (2 + i) x1) / 3 rounds once.
+    Each case: the factor's arrays as the uploads take them, the dense A (or A^T) the solve
+    inverts, the (count, 2) right-hand sides, the exact solutions, the divisor of the rounded
+    component and that component's index."""
+    rng = np.random.default_rng(seed)
+
+    def gaussian(k, scale=1):
+        parts = rng.integers(-9, 10, (k, 2))
+        return scale * (parts[:, 0] + 1j * parts[:, 1]).astype(np.complex128)
+
+    cases = {}
+    l10, u00, u01, u11 = 2 - 1j, 3 + 1j, 1 + 2j, 1 + 1j
+    b = np.stack([gaussian(count), gaussian(count)], axis=1)
+    lo = np.array([[1, 0], [l10, 1]], dtype=np.complex128)
+    up = np.array([[u00, u01], [0, u11]], dtype=np.complex128)
+    cases["lu"] = dict(
+        arrays=dict(l_ptr=[0, 2, 3], l_idx=[0, 1, 1], l_val=np.array([1, l10, 1], np.complex128),
+                    u_ptr=[0, 1, 3], u_idx=[0, 0, 1], u_val=np.array([u00, u01, u11], np.complex128),
+                    row_perm=[0, 1], col_perm=[0, 1]),
+        system=lo @ up, b=b, divisor=u00, last=0)
+    u00, u01, u11 = 1 + 1j, 2 - 3j, 3 + 1j
+    b = np.stack([gaussian(count), gaussian(count)], axis=1)
+    up = np.array([[u00, u01], [0, u11]], dtype=np.complex128)
+    cases["lu_transpose"] = dict(
+        arrays=dict(l_ptr=[0, 1, 2], l_idx=[0, 1], l_val=np.array([1, 1], np.complex128),
+                    u_ptr=[0, 1, 3], u_idx=[0, 0, 1], u_val=np.array([u00, u01, u11], np.complex128),
+                    row_perm=[0, 1], col_perm=[0, 1]),
+        system=up.T, b=b, divisor=u11, last=1)
+    l00, l10, l11 = 3 + 0j, 2 - 1j, 1 + 0j
+    b = np.stack([gaussian(count, 3), gaussian(count)], axis=1)
+    lo = np.array([[l00, 0], [l10, l11]], dtype=np.complex128)
+    cases["cholesky"] = dict(
+        arrays=dict(l_ptr=[0, 1, 3], l_idx=[0, 0, 1], l_val=np.array([l00, l10, l11], np.complex128),
+                    perm=[0, 1]),
+        system=lo @ lo.conj().T, b=b, divisor=l00, last=0)
+    for case in cases.values():
+        case["exact"] = exact_solve_2x2(case["system"], case["b"])
+    return cases

@@ -270,3 +270,24 @@ def test_complex_device_solves_match_the_host_solves(gpu, hermitian, complex_lu)
         assert refined.tobytes() == host.tobytes(), kind
         factor.set_block_width(4)  # several blocks
         assert factor.solve_device(tb).cpu().numpy().tobytes() == factor.solve_many(b).tobytes()
+
+
+@pytest.mark.parametrize("name", ["lu", "lu_transpose", "cholesky"])
+def test_closed_form_solves_return_the_correctly_rounded_exact_solutions(gpu, name):
+    """2 x 2 factors whose solves round only in their last division (num-complex's quotient of
+    exact parts is the correctly rounded exact quotient): the device must return the correctly
+    rounded exact solutions of Cramer's rule in rational arithmetic, for every right-hand side."""
+    from tests.fixtures import gaussian_closed_forms, rounded
+    case = gaussian_closed_forms()[name]
+    arrays = case["arrays"]
+    if name == "cholesky":
+        factor = Y.GpuFactor.from_cholesky_complex_arrays(gpu, 2, arrays["l_ptr"], arrays["l_idx"],
+                                                          arrays["l_val"], arrays["perm"])
+    else:
+        factor = Y.GpuFactor.from_lu_complex_arrays(gpu, 2, arrays["l_ptr"], arrays["l_idx"], arrays["l_val"],
+                                                    arrays["u_ptr"], arrays["u_idx"], arrays["u_val"],
+                                                    arrays["row_perm"], arrays["col_perm"])
+    got = factor.solve_transpose_many(case["b"]) if name == "lu_transpose" else factor.solve_many(case["b"])
+    want = rounded(case["exact"])
+    # equal bit for bit, a zero's sign aside
+    assert (got + 0.0).tobytes() == (want + 0.0).tobytes()
