@@ -1387,53 +1387,98 @@ __device__ void combine(const double* __restrict__ xc, const float* __restrict__
 }
 
 // r = b - A x for the tile's columns (each row summed from 0 in index order, as mul_vec's
-// dot_kernel does, product.rs:17-27); returns this thread's partial max |r|. A row's terms are
-// taken RESIDUAL_CHUNK at a time: their columns and values loaded together, then their entries of
-// x together, then summed in order, so a row waits for two rounds of loads per chunk rather than
-// per term; the next row's bounds are loaded during this one.
+// dot_kernel does, product.rs:17-27); returns this thread's partial max |r|. A thread's rows are
+// pipelined: while one row's entries of x are in flight, the next row's first RESIDUAL_CHUNK
+// columns and values and the bounds of the row after that are loaded, so a row of up to
+// RESIDUAL_CHUNK terms costs one round of dependent loads. Two register sets alternate between
+// the row being summed and the next one, so no value in flight is copied. The matrix's column and
+// value arrays carry RESIDUAL_CHUNK padding entries past its last (column 0, value 0), so a chunk
+// is loaded whole from any position up to nnz; a chunk's terms past its row's end are not summed,
+// and their x loads repeat the row's first column's.
 // With ax non-null (the floor stop rule) it also takes this thread's partial max of
 // sum |a_ij| |x_j| over its rows into *ax.
+struct ResidualRow {
+  int32_t p0, p1;
+  int32_t c[RESIDUAL_CHUNK];
+  double v[RESIDUAL_CHUNK];
+};
+
 template <int TW>
 __device__ double residual(const CsrDev& a, const double* __restrict__ b, const double* __restrict__ x,
                            double* __restrict__ r, int64_t ld, const Tile<TW>& g, bool act, double* ax = nullptr) {
+  constexpr int C = RESIDUAL_CHUNK;
   double m = 0.0, axm = 0.0;
   if (ax) *ax = 0.0;
-  if (!act) return m;
-  constexpr int RESIDUAL_CHUNK = 4;
-  int32_t i = g.slot;
-  int32_t p0 = i < a.rows ? a.ptr[i] : 0, p1 = i < a.rows ? a.ptr[i + 1] : 0;
-  for (; i < a.rows; i += g.slots) {
-    const int32_t next = i + g.slots;
-    const int32_t n0 = next < a.rows ? a.ptr[next] : 0, n1 = next < a.rows ? a.ptr[next + 1] : 0;
-    const int64_t at = i * ld + g.col;
-    const double bi = b[at];
-    double sum = 0.0, abs_sum = 0.0;
-    for (int32_t p = p0; p < p1; p += RESIDUAL_CHUNK) {
-      int32_t cc[RESIDUAL_CHUNK];
-      double vv[RESIDUAL_CHUNK], xx[RESIDUAL_CHUNK];
+  const int32_t rows = a.rows;
+  if (!act || g.slot >= rows) return m;
+  auto bounds = [&](int32_t i, ResidualRow& w) {
+    const int32_t k = i < rows ? i : rows;
+    w.p0 = a.ptr[k];
+    w.p1 = a.ptr[k < rows ? k + 1 : rows];
+  };
+  auto chunk = [&](ResidualRow& w) {
 #pragma unroll
-      for (int q = 0; q < RESIDUAL_CHUNK; ++q) {
-        const int32_t pq = p + q < p1 ? p + q : p1 - 1;
-        cc[q] = a.col[pq];
-        vv[q] = a.val[pq];
+    for (int q = 0; q < C; ++q) {
+      w.c[q] = a.col[w.p0 + q];
+      w.v[q] = a.val[w.p0 + q];
+    }
+  };
+  // row i from `cur` (its bounds and first chunk loaded); `nxt` holds row i + slots's bounds and
+  // gets its first chunk, and `cur` then the bounds of row i + 2 slots
+  auto row = [&](int32_t i, ResidualRow& cur, ResidualRow& nxt) {
+    const int32_t p0 = cur.p0, p1 = cur.p1;
+    const int64_t at = (int64_t)i * ld + g.col;
+    double xx[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) xx[q] = x[(int64_t)(p0 + q < p1 ? cur.c[q] : cur.c[0]) * ld + g.col];
+    const double bi = b[at];
+    chunk(nxt);
+    bounds(i + 2 * g.slots, cur);
+    double sum = 0.0, abs_sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      if (p0 + q < p1) sum = sum + cur.v[q] * xx[q];
+    if (ax) {
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+        if (p0 + q < p1) abs_sum = abs_sum + fabs(cur.v[q]) * fabs(xx[q]);
+    }
+    // the rest of a long row, a chunk at a time
+    for (int32_t p = p0 + C; p < p1; p += C) {
+      int32_t cc[C];
+      double v2[C], x2[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        cc[q] = a.col[p + q];
+        v2[q] = a.val[p + q];
       }
 #pragma unroll
-      for (int q = 0; q < RESIDUAL_CHUNK; ++q) xx[q] = x[cc[q] * ld + g.col];
+      for (int q = 0; q < C; ++q) x2[q] = x[(int64_t)(p + q < p1 ? cc[q] : cc[0]) * ld + g.col];
 #pragma unroll
-      for (int q = 0; q < RESIDUAL_CHUNK; ++q)
-        if (p + q < p1) sum = sum + vv[q] * xx[q];
+      for (int q = 0; q < C; ++q)
+        if (p + q < p1) sum = sum + v2[q] * x2[q];
       if (ax) {
 #pragma unroll
-        for (int q = 0; q < RESIDUAL_CHUNK; ++q)
-          if (p + q < p1) abs_sum = abs_sum + fabs(vv[q]) * fabs(xx[q]);
+        for (int q = 0; q < C; ++q)
+          if (p + q < p1) abs_sum = abs_sum + fabs(v2[q]) * fabs(x2[q]);
       }
     }
     const double v = bi - sum;
     r[at] = v;
     m = fold_max(m, fabs(v));
     axm = fold_max(axm, abs_sum);
-    p0 = n0;
-    p1 = n1;
+  };
+  ResidualRow s0, s1;
+  bounds(g.slot, s0);
+  chunk(s0);
+  bounds(g.slot + g.slots, s1);
+  for (int32_t i = g.slot;;) {
+    row(i, s0, s1);
+    i += g.slots;
+    if (i >= rows) break;
+    row(i, s1, s0);
+    i += g.slots;
+    if (i >= rows) break;
   }
   if (ax) *ax = axm;
   return m;

@@ -73,6 +73,10 @@ struct ZCsrDev {
 };
 
 // A CSR matrix on the device.
+// The FP64 residual loads a row's entries this many at a time from any position up to nnz, so a
+// matrix uploaded for it carries this many padding entries past its last (column 0, value 0)
+constexpr int RESIDUAL_CHUNK = 8;
+
 struct CsrDev {
   int32_t rows = 0, cols = 0;
   const int32_t* ptr = nullptr;

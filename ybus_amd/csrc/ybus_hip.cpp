@@ -565,6 +565,14 @@ constexpr int64_t INDEX_LIMIT = 0x7fffffff;
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// v with `pad` value-initialized entries after its last
+template <typename T>
+std::vector<T> padded(const std::vector<T>& v, size_t pad) {
+  std::vector<T> out(v);
+  out.resize(v.size() + pad, T());
+  return out;
+}
+
 // The most branches one outage set may hold (its system is solved densely on the host)
 constexpr int32_t MAX_OUTAGE_SET = 64;
 
@@ -2229,8 +2237,9 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
     f->a.rows = f->a.cols = (int32_t)n;
     f->a.nnz = nnz;
     f->a.ptr.upload(rptr, f->stream());
-    f->a.col.upload(cols, f->stream());
-    f->a.val.upload(vals, f->stream());
+    // the residual's padding entries (column 0, value 0) past the last
+    f->a.col.upload(padded(cols, RESIDUAL_CHUNK), f->stream());
+    f->a.val.upload(padded(vals, RESIDUAL_CHUNK), f->stream());
     f->a.row.upload(rows, f->stream());
     // A^T by rows: a counting sort of A's entries by column, rows ascending within each
     {
@@ -2249,8 +2258,8 @@ int32_t ybus_hip_factor_set_matrix(ybus_hip_factor* f, int32_t row_major, const 
       f->at.rows = f->at.cols = (int32_t)n;
       f->at.nnz = nnz;
       f->at.ptr.upload(tptr, f->stream());
-      f->at.col.upload(tcol, f->stream());
-      f->at.val.upload(tval, f->stream());
+      f->at.col.upload(padded(tcol, RESIDUAL_CHUNK), f->stream());
+      f->at.val.upload(padded(tval, RESIDUAL_CHUNK), f->stream());
       f->at.t_order.upload(tord, f->stream());
     }
     f->a.src_row_major = row_major != 0;
