@@ -2942,8 +2942,11 @@ int32_t ybus_hip_factor_autotune(ybus_hip_factor* f, uint64_t columns, int32_t p
         }
       }
       // a one-schedule wide block: the solve with the last positions in LDS, at each tile width
-      // and workgroup size
-      if (!best.columns && best.parts != 1 && f->window_env < 0) {
+      // and workgroup size (not for a pair block whose reach is staged: it takes the reach
+      // kernel at every tile width, so a windowed candidate would time that kernel again)
+      const bool reach_staged =
+          pairs && f->use_reach && reach_lds_bytes(f->fwd.view(), f->bwd.view()) <= f->lds_cap;
+      if (!best.columns && best.parts != 1 && f->window_env < 0 && !reach_staged) {
         const ybus_hip_factor::Tuned wide_best = best;
         for (int th : {256, 512})
           for (int tile : {1, 2, 4, 8, 16}) {
